@@ -1,0 +1,132 @@
+"""Exact diagonalisation oracle for small S=1/2 XXZ lattices.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/ (and nothing in the product).
+
+Restates the thermal averages of the reference's "diagonalization" worker
+(diag.C:159-293 static/dynamic averages, diag.C:313-488 observables) for
+H = sum_b [ Jz Sz_i Sz_j + Jxy/2 (S+_i S-_j + S-_i S+_j) ] on the chain and
+square lattices with ALPS bond conventions.  The Hamiltonian is block-diagonal
+in total Sz, so each sector is diagonalised densely with numpy.linalg.eigh.
+Susceptibilities use the Kubo formula of diag.C:252-293,
+chi_O = (1/(Z V)) sum_ij |<i|O|j>|^2 w_ij with w_ij = (e^{-b E_i} - e^{-b E_j})/(E_j - E_i),
+and w_ii = b e^{-b E_i}.
+"""
+
+import itertools
+import math
+
+import numpy as np
+
+
+def lattice_bonds(kind, L):
+    """(src, tgt) lists and gauge with ALPS conventions (chain: bond i = (i, i+1 mod L);
+    square: site x*L+y, bond 2s = (s, s+y^), 2s+1 = (s, s+x^))."""
+    if kind == "chain":
+        N = L
+        bonds = [(i, (i + 1) % L) for i in range(L)]
+        gauge = [1 if s % 2 == 0 else -1 for s in range(N)]
+    elif kind == "square":
+        N = L * L
+        bonds = []
+        for s in range(N):
+            x, y = divmod(s, L)
+            bonds.append((s, x * L + (y + 1) % L))
+            bonds.append((s, ((x + 1) % L) * L + y))
+        gauge = [1 if (s // L + s % L) % 2 == 0 else -1 for s in range(N)]
+    else:
+        raise ValueError(kind)
+    return N, bonds, np.array(gauge, dtype=float)
+
+
+def _sector_states(N, nup):
+    states = []
+    for ups in itertools.combinations(range(N), nup):
+        b = 0
+        for u in ups:
+            b |= 1 << u
+        states.append(b)
+    return np.array(sorted(states), dtype=np.int64)
+
+
+def _sector_hamiltonian(states, bonds, jxy, jz):
+    dim = len(states)
+    H = np.zeros((dim, dim))
+    cols = np.arange(dim)
+    for (a, b) in bonds:
+        sa = ((states >> a) & 1) - 0.5
+        sb = ((states >> b) & 1) - 0.5
+        H[cols, cols] += jz * sa * sb
+        hop = sa != sb
+        t = states[hop] ^ (1 << a) ^ (1 << b)
+        rows = np.searchsorted(states, t)
+        np.add.at(H, (rows, cols[hop]), 0.5 * jxy)
+    return H
+
+
+def diagonalize(kind, L, T, jxy=1.0, jz=1.0):
+    """Thermal averages keyed by the reference's observable names."""
+    beta = 1.0 / T
+    N, bonds, gauge = lattice_bonds(kind, L)
+    V = float(N)
+    sectors = []
+    for nup in range(N + 1):
+        states = _sector_states(N, nup)
+        H = _sector_hamiltonian(states, bonds, jxy, jz)
+        E, U = np.linalg.eigh(H)
+        sz = np.array([[0.5 if (int(s) >> i) & 1 else -0.5 for i in range(N)] for s in states])
+        mu = sz.sum(axis=1)
+        ms = sz @ gauge
+        sectors.append((E, U, mu, ms))
+    e0 = min(sec[0].min() for sec in sectors)
+    Z = sum(np.exp(-beta * (sec[0] - e0)).sum() for sec in sectors)
+    res = {}
+    ene = sum((np.exp(-beta * (E - e0)) * E).sum() for (E, _, _, _) in sectors) / Z
+    ene2 = sum((np.exp(-beta * (E - e0)) * E * E).sum() for (E, _, _, _) in sectors) / Z
+    res["Energy"] = ene
+    res["Energy Density"] = ene / V
+    res["Specific Heat"] = beta * beta * (ene2 - ene * ene) / V
+    res["Ground State Energy"] = e0
+
+    def static(diag_of):
+        v2 = v4 = va = 0.0
+        for (E, U, mu, ms) in sectors:
+            d = diag_of(mu, ms)
+            w = np.exp(-beta * (E - e0))
+            p = U * U  # |<n|basis>|^2, columns = eigenvectors
+            v2 += (w * (p.T @ (d ** 2))).sum()
+            v4 += (w * (p.T @ (d ** 4))).sum()
+            va += (w * (p.T @ np.abs(d))).sum()
+        return v2 / Z, v4 / Z, va / Z
+
+    def kubo(diag_of):
+        tot = 0.0
+        for (E, U, mu, ms) in sectors:
+            d = diag_of(mu, ms)
+            A = U.T @ (d[:, None] * U)
+            w = np.exp(-beta * (E - e0))
+            dE = E[:, None] - E[None, :]
+            with np.errstate(divide="ignore", invalid="ignore"):
+                wij = -(w[:, None] - w[None, :]) / dE
+            same = np.abs(dE) <= 1e-12
+            wij = np.where(same, beta * w[:, None] * np.ones_like(dE), wij)
+            tot += (A * A * wij).sum()
+        return tot / Z / V
+
+    m2, m4, ma = static(lambda mu, ms: mu)
+    res["Magnetization^2"] = m2
+    res["Magnetization^4"] = m4
+    res["|Magnetization|"] = ma
+    res["Susceptibility"] = kubo(lambda mu, ms: mu)
+    s2, s4, sa = static(lambda mu, ms: ms)
+    res["Staggered Magnetization^2"] = s2
+    res["Staggered Magnetization^4"] = s4
+    res["|Staggered Magnetization|"] = sa
+    res["Staggered Susceptibility"] = kubo(lambda mu, ms: ms)
+    res["Staggered Structure Factor"] = s2 / V
+    return res
+
+
+if __name__ == "__main__":
+    r = diagonalize("chain", 4, 0.25)
+    for k in sorted(r):
+        print(k, r[k])

@@ -1,0 +1,1163 @@
+// CPU oracle for the continuous-imaginary-time loop update (S=1/2 XXZ).
+//
+// TEST INFRASTRUCTURE ONLY.  Nothing in the product library links or calls
+// this file; only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+// leg load it (through ctypes), and only as the checker / the timed CPU
+// baseline.
+//
+// Three restatements live here, all plain serial C++:
+//
+//  1. "looper mode": the serial path of ALPS/looper's "loop; path integral"
+//     worker for bond-only S=1/2 XXZ models without field and sign
+//     (path_integral.C:380-864, the #else branches), with looper's union-find
+//     semantics (looper/union_find.h:57-82,163-172,242-284,307-350), cut rules
+//     and leg accessors (looper/graph_impl.h:255-328,489-494), weights
+//     (looper/weight_impl.h:165-188), improved estimators
+//     (looper/susceptibility.h:96-256, looper/stiffness.h:82-130,
+//     looper/energy.h:55-86).  RNG: std::mt19937 + std distributions and a
+//     Walker alias table of our own; the worker's ALPS RNG stream is not
+//     reproduced (parity for this mode is statistical).
+//
+//  2. "standalone mode": standalone/loop.C:39-180 with its own union-find
+//     variant (standalone/union_find.h:112-143), consuming std::mt19937(29833)
+//     exactly as that program does, so its printed averages can be compared
+//     digit for digit with standalone/loop.op and with the reference program
+//     built under oracle/_ref.
+//
+//  3. "philox mode": the counter-based algorithm the GPU kernels implement
+//     (per-bond Poisson streams from Philox4x32-10, 2^-52 integer time grid,
+//     world-line segments as union-find nodes, min-index roots), written
+//     sequentially so GPU and CPU trajectories can be compared bit for bit.
+//
+// Plus partition oracles on an explicit leg graph: looper's fragment
+// algorithm, a breadth-first search, and canonicalisation of labels to the
+// minimum element index.
+//
+// Build: see oracle/Makefile (g++ -O2 -ffp-contract=off; contraction must stay
+// off so the deterministic logarithm matches the device's bit for bit).
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <deque>
+#include <random>
+#include <vector>
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as easy
+// as 1, 2, 3", SC'11): 10 rounds of a 4x32 multiply-xor network with a Weyl
+// key schedule.
+// ---------------------------------------------------------------------------
+struct u32x4 { uint32_t v[4]; };
+
+inline u32x4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                           uint32_t k0, uint32_t k1) {
+  const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u;
+  const uint32_t W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+  for (int r = 0; r < 10; ++r) {
+    uint64_t p0 = (uint64_t)M0 * c0;
+    uint64_t p1 = (uint64_t)M1 * c2;
+    uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+    uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+    uint32_t n0 = hi1 ^ c1 ^ k0;
+    uint32_t n2 = hi0 ^ c3 ^ k1;
+    c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+    k0 += W0; k1 += W1;
+  }
+  u32x4 out = {{c0, c1, c2, c3}};
+  return out;
+}
+
+// Counter layout shared with the device: (sweep, stage, element, sub-index),
+// key = (seed, chain id).
+enum { STAGE_DIAG = 1, STAGE_REGRAPH = 2, STAGE_FLIP = 3 };
+
+// ---------------------------------------------------------------------------
+// Deterministic natural logarithm built only from IEEE-754 +,-,*,/ (no fused
+// operations), so that x86-64 and gfx950 produce identical bits.
+// log(x) = e*ln2 + 2*atanh(s), s = (m-1)/(m+1), m in (sqrt(1/2), sqrt(2)],
+// atanh by its Taylor series s + s^3/3 + s^5/5 + ... (|s| <= 0.1716).
+// ---------------------------------------------------------------------------
+inline double det_log(double x) {
+  uint64_t bits;
+  std::memcpy(&bits, &x, 8);
+  int e = (int)((bits >> 52) & 0x7ff) - 1023;
+  uint64_t mb = (bits & 0x000fffffffffffffull) | 0x3ff0000000000000ull;
+  double m;
+  std::memcpy(&m, &mb, 8);
+  if (m > 1.4142135623730951) { m = m * 0.5; e += 1; }
+  double f = m - 1.0;
+  double s = f / (2.0 + f);
+  double z = s * s;
+  double p = 1.0 / 25.0;
+  p = p * z + 1.0 / 23.0;
+  p = p * z + 1.0 / 21.0;
+  p = p * z + 1.0 / 19.0;
+  p = p * z + 1.0 / 17.0;
+  p = p * z + 1.0 / 15.0;
+  p = p * z + 1.0 / 13.0;
+  p = p * z + 1.0 / 11.0;
+  p = p * z + 1.0 / 9.0;
+  p = p * z + 1.0 / 7.0;
+  p = p * z + 1.0 / 5.0;
+  p = p * z + 1.0 / 3.0;
+  double logm = 2.0 * s + 2.0 * s * (z * p);
+  return (double)e * 0.69314718055994530942 + logm;
+}
+
+const uint64_t TGRID = 1ull << 52;  // imaginary time tau = T * 2^-52, T in [0, 2^52)
+
+inline uint64_t exp_gap(const u32x4& x, double scale) {
+  uint64_t u53 = ((uint64_t)x.v[0] << 21) | (uint64_t)(x.v[1] >> 11);
+  double u = (double)(u53 + 1) * (1.0 / 9007199254740992.0);  // (0, 1]
+  double g = -det_log(u) * scale;
+  if (!(g < 4503599627370496.0)) return TGRID;
+  return (uint64_t)g;
+}
+
+// ---------------------------------------------------------------------------
+// Lattices with ALPS conventions (pinned by test/spinmodel_helper.op,
+// test/virtual_lattice.op): chain bond i = (i, i+1 mod L); square site
+// s = x*L + y, bond 2s = (s, s+y^), bond 2s+1 = (s, s+x^); gauge = +1 on the
+// sublattice of site 0.
+// ---------------------------------------------------------------------------
+struct Lattice {
+  int kind = 0, L = 0, N = 0, Nb = 0, dim = 1;
+  std::vector<int> src, tgt, dir, gauge;
+  std::vector<std::vector<int>> site_bonds;  // bonds incident to each site
+  void init(int k, int l) {
+    kind = k; L = l;
+    if (kind == 0) {
+      N = L; Nb = L; dim = 1;
+      src.resize(Nb); tgt.resize(Nb); dir.assign(Nb, 0);
+      for (int b = 0; b < Nb; ++b) { src[b] = b; tgt[b] = (b + 1) % L; }
+      gauge.resize(N);
+      for (int s = 0; s < N; ++s) gauge[s] = (s % 2 == 0) ? 1 : -1;
+    } else {
+      N = L * L; Nb = 2 * N; dim = 2;
+      src.resize(Nb); tgt.resize(Nb); dir.resize(Nb);
+      for (int s = 0; s < N; ++s) {
+        int x = s / L, y = s % L;
+        src[2 * s] = s; tgt[2 * s] = x * L + (y + 1) % L; dir[2 * s] = 1;
+        src[2 * s + 1] = s; tgt[2 * s + 1] = ((x + 1) % L) * L + y; dir[2 * s + 1] = 0;
+      }
+      gauge.resize(N);
+      for (int s = 0; s < N; ++s) gauge[s] = ((s / L + s % L) % 2 == 0) ? 1 : -1;
+    }
+    site_bonds.assign(N, std::vector<int>());
+    for (int b = 0; b < Nb; ++b) { site_bonds[src[b]].push_back(b); site_bonds[tgt[b]].push_back(b); }
+  }
+  // relative bond vector component d (ALPS bond_vector_relative, unit lattice constant)
+  int vr(int b, int d) const { return (kind == 0) ? 1 : (dir[b] == d ? 1 : 0); }
+};
+
+// ---------------------------------------------------------------------------
+// XXZ bond weights, looper/weight_impl.h:165-188 (standard and "ergodic"
+// solutions; FORCE_SCATTER = a).
+// ---------------------------------------------------------------------------
+struct Weights { double v[4]; double offset; int sign; };
+
+inline double crop_0(double x) { return x > 0 ? x : 0; }
+
+Weights xxz_weights(double jxy_in, double jz, double a) {
+  Weights w;
+  w.sign = (jxy_in <= 0 ? 1 : -1);
+  double jxy = std::fabs(jxy_in);
+  if (a < 0) a = 0;
+  if (a > 1) a = 1;
+  if (jxy + std::fabs(jz) > 1e-10) {
+    if (jxy - jz > 2 * a * jxy) {
+      w.v[0] = crop_0(std::min(jxy / 2, (jxy + jz) / 4));
+      w.v[1] = crop_0(std::min(jxy / 2, (jxy - jz) / 4));
+      w.v[2] = crop_0(-(jxy - jz) / 2.0);
+      w.v[3] = crop_0(-(jxy + jz) / 2);
+    } else {
+      w.v[0] = (1 - a) * jxy / 2;
+      w.v[1] = a * jxy / 2;
+      w.v[2] = -((1 - 2 * a) * jxy - jz) / 2;
+      w.v[3] = 0;
+    }
+  } else {
+    w.v[0] = w.v[1] = w.v[2] = w.v[3] = 0;
+  }
+  w.offset = (w.v[0] + w.v[1] + w.v[2] + w.v[3]) / 2;
+  return w;
+}
+
+// Integer thresholds shared by the philox mode and the device:
+// graph g is chosen for 32-bit draw x when thr[g-1] <= x < thr[g].
+struct Thresholds { uint64_t thr[4]; uint64_t toff; };
+
+Thresholds make_thresholds(const Weights& w) {
+  Thresholds t;
+  double W = w.v[0] + w.v[1] + w.v[2] + w.v[3];
+  int last = -1;
+  for (int g = 0; g < 4; ++g) if (w.v[g] > 0) last = g;
+  double cum = 0;
+  for (int g = 0; g < 4; ++g) {
+    cum += w.v[g];
+    if (last < 0 || g >= last) t.thr[g] = 1ull << 32;
+    else t.thr[g] = (uint64_t)std::floor(std::ldexp(cum / W, 32));
+  }
+  double s01 = w.v[0] + w.v[1];
+  t.toff = (s01 > 0) ? (uint64_t)std::floor(std::ldexp(w.v[0] / s01, 32)) : (1ull << 32);
+  return t;
+}
+
+inline int choose_graph(uint32_t x, const Thresholds& t) {
+  int g = 0;
+  while (g < 3 && (uint64_t)x >= t.thr[g]) ++g;
+  return g;
+}
+
+// ---------------------------------------------------------------------------
+// looper's union-find node and operations, serial branch
+// (looper/union_find.h:57-82 node, 163-172 root_index_ph, 242-284 unify,
+// 307-350 set_id/copy_id).
+// ---------------------------------------------------------------------------
+struct UFNode {
+  int parent_ = -1;
+  int id_ = 0;
+  bool is_root() const { return parent_ <= 0; }
+  void set_parent(int p) { parent_ = p + 1; }
+  int parent() const { return parent_ - 1; }
+  void set_weight(int w) { parent_ = -w; }
+  int weight() const { return -parent_; }
+};
+
+int uf_root_index(const std::vector<UFNode>& v, int g) {
+  while (!v[g].is_root()) g = v[g].parent();
+  return g;
+}
+
+int uf_root_index_ph(std::vector<UFNode>& v, int g) {
+  if (v[g].is_root()) return g;
+  while (true) {
+    int p = v[g].parent();
+    if (v[p].is_root()) return p;
+    v[g].set_parent(v[p].parent());
+    g = p;
+  }
+}
+
+int uf_unify(std::vector<UFNode>& v, int g0, int g1) {
+  int r0 = uf_root_index_ph(v, g0);
+  int r1 = uf_root_index_ph(v, g1);
+  if (r0 != r1) {
+    if (v[r0].weight() < v[r1].weight()) std::swap(r0, r1);
+    v[r0].set_weight(v[r0].weight() + v[r1].weight());
+    v[r1].set_parent(r0);
+  }
+  return r0;
+}
+
+// Variant with full compression along both paths (looper's unify_compress,
+// union_find.h:204-222; same roots and weights as uf_unify).
+int uf_unify_compress(std::vector<UFNode>& v, int g0, int g1) {
+  int r0 = uf_root_index(v, g0);
+  int r1 = uf_root_index(v, g1);
+  if (r0 != r1) {
+    if (v[r0].weight() < v[r1].weight()) std::swap(r0, r1);
+    v[r0].set_weight(v[r0].weight() + v[r1].weight());
+    v[r1].set_parent(r0);
+  }
+  for (int g : {g0, g1}) {
+    while (g != r0) { int p = v[g].parent(); v[g].set_parent(r0); g = p; }
+  }
+  return r0;
+}
+
+int uf_set_id(std::vector<UFNode>& v, int start, int n, int nc) {
+  for (int i = start; i < start + n; ++i) if (v[i].is_root()) v[i].id_ = nc++;
+  return nc;
+}
+void uf_copy_id(std::vector<UFNode>& v, int start, int n) {
+  for (int i = start; i < start + n; ++i) v[i].id_ = v[uf_root_index(v, i)].id_;
+}
+
+// Cut rules and reconnect (looper/graph_impl.h:255-295) and leg accessors
+// (graph_impl.h:489-494).
+inline bool is_compatible(int g, int c0, int c1) { return ((g & 1) ^ c0 ^ c1) != 0; }
+inline bool is_frozen(int g) { return (g & 2) == 2; }
+
+struct Reconnect { int fid, curr0, curr1, loop0, loop1; };
+Reconnect reconnect(std::vector<UFNode>& frag, int fid, int g, int curr0, int curr1) {
+  Reconnect r;
+  if ((g & 2) == 2) {
+    int u = uf_unify(frag, curr0, curr1);
+    r.loop0 = r.loop1 = r.curr0 = r.curr1 = u;
+  } else if (g == 0) {
+    frag[fid] = UFNode();
+    r.loop0 = uf_unify(frag, curr0, curr1);
+    r.loop1 = r.curr0 = r.curr1 = fid;
+    ++fid;
+  } else {
+    r.loop0 = curr0; r.loop1 = curr1;
+    r.curr0 = curr1; r.curr1 = curr0;
+  }
+  r.fid = fid;
+  return r;
+}
+inline int loop_l0(int, int l0, int) { return l0; }
+inline int loop_l1(int g, int l0, int l1) { return (g == 0 || g == 5) ? l0 : l1; }
+inline int loop_u0(int, int, int l1) { return l1; }
+inline int loop_u1(int g, int l0, int l1) { return (g == 0 || g == 5) ? l1 : l0; }
+
+// ---------------------------------------------------------------------------
+// Sweep record layout (shared with the device library's lg_sweep_record).
+// ---------------------------------------------------------------------------
+enum {
+  R_NOP = 0, R_NC, R_ENERGY,
+  R_UMAG0, R_USIZE2, R_UMAG2, R_USIZE4, R_UMAG4, R_USIZE, R_UMAG,
+  R_SMAG0, R_SSIZE2, R_SMAG2, R_SSIZE4, R_SMAG4, R_SSIZE, R_SMAG,
+  R_W2, R_COUNT
+};
+
+struct Estimate {   // susceptibility + stiffness improved estimate (per cluster)
+  double usize0 = 0, umag0 = 0, usize = 0, umag = 0;
+  double ssize0 = 0, smag0 = 0, ssize = 0, smag = 0;
+  double winding[3] = {0, 0, 0};
+};
+
+inline double p2(double x) { return x * x; }
+inline double p4(double x) { return p2(p2(x)); }
+
+void collect(const Estimate& e, double* rec) {
+  rec[R_UMAG0] += e.umag0;
+  rec[R_USIZE2] += p2(e.usize0);
+  rec[R_UMAG2] += p2(e.umag0);
+  rec[R_USIZE4] += p4(e.usize0);
+  rec[R_UMAG4] += p4(e.umag0);
+  rec[R_USIZE] += p2(e.usize);
+  rec[R_UMAG] += p2(e.umag);
+  rec[R_SMAG0] += e.smag0;
+  rec[R_SSIZE2] += p2(e.ssize0);
+  rec[R_SMAG2] += p2(e.smag0);
+  rec[R_SSIZE4] += p4(e.ssize0);
+  rec[R_SMAG4] += p4(e.smag0);
+  rec[R_SSIZE] += p2(e.ssize);
+  rec[R_SMAG] += p2(e.smag);
+  for (int d = 0; d < 3; ++d) rec[R_W2] += p2(0.5 * e.winding[d]);
+}
+
+// end_s / begin_s / start_bottom of susceptibility.h:113-147
+inline void est_end(Estimate& e, double t, int gs, int c) {
+  e.usize += t * 0.5;
+  e.umag += t * (0.5 - c);
+  e.ssize += gs * t * 0.5;
+  e.smag += gs * t * (0.5 - c);
+}
+inline void est_begin(Estimate& e, double t, int gs, int c) { est_end(e, -t, gs, c); }
+inline void est_start_bottom(Estimate& e, int gs, int c) {
+  est_begin(e, 0.0, gs, c);
+  e.usize0 += 0.5;
+  e.umag0 += (0.5 - c);
+  e.ssize0 += gs * 0.5;
+  e.smag0 += gs * (0.5 - c);
+}
+
+// ---------------------------------------------------------------------------
+// Walker alias table (Vose's construction), our own.
+// ---------------------------------------------------------------------------
+struct Alias {
+  std::vector<double> prob;
+  std::vector<int> alias;
+  void init(const std::vector<double>& w) {
+    int n = (int)w.size();
+    prob.assign(n, 0); alias.assign(n, 0);
+    double sum = 0;
+    for (double x : w) sum += x;
+    std::vector<double> q(n);
+    std::vector<int> small, large;
+    for (int i = 0; i < n; ++i) {
+      q[i] = w[i] * n / sum;
+      (q[i] < 1 ? small : large).push_back(i);
+    }
+    while (!small.empty() && !large.empty()) {
+      int s = small.back(); small.pop_back();
+      int l = large.back(); large.pop_back();
+      prob[s] = q[s]; alias[s] = l;
+      q[l] = (q[l] + q[s]) - 1;
+      (q[l] < 1 ? small : large).push_back(l);
+    }
+    for (int i : large) { prob[i] = 1; alias[i] = i; }
+    for (int i : small) { prob[i] = 1; alias[i] = i; }
+  }
+  template <class RNG>
+  int operator()(RNG& u01) const {
+    int n = (int)prob.size();
+    double x = u01() * n;
+    int i = (int)x;
+    if (i >= n) i = n - 1;
+    return (x - i < prob[i]) ? i : alias[i];
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Looper-mode worker: path_integral.C:380-864, serial, FIELD=SIGN=false,
+// IMPROVE=true, S=1/2 (max_virtual_sites == 1).
+// ---------------------------------------------------------------------------
+struct LooperOp {
+  int bond;
+  int type;     // bit0 off-diagonal, graph << 2 (operator.h:52-117)
+  double time;
+  int loop0, loop1;
+};
+
+struct LooperChain {
+  Lattice lat;
+  Weights w;
+  double beta = 1, offset_total = 0, graph_weight = 0, p_off = 1;
+  std::mt19937 eng;
+  std::uniform_real_distribution<double> u01{0.0, 1.0};
+  Alias alias;
+  std::vector<std::pair<int, int>> graph_of;  // alias index -> (bond, g)
+  std::vector<int> spins, spins_c, current;
+  std::vector<LooperOp> ops, ops_p;
+  std::vector<double> times;
+  std::vector<UFNode> frag;
+  std::vector<Estimate> est;
+  std::vector<char> to_flip;
+  uint64_t mcs = 0;
+
+  void init(int kind, int L, double b, double jxy, double jz, double fs, uint32_t seed) {
+    lat.init(kind, L);
+    w = xxz_weights(jxy, jz, fs);
+    beta = b;
+    eng.seed(seed);
+    std::vector<double> wl;
+    for (int bd = 0; bd < lat.Nb; ++bd)
+      for (int g = 0; g < 4; ++g)
+        if (w.v[g] > 1e-10) { wl.push_back(w.v[g]); graph_of.push_back({bd, g}); graph_weight += w.v[g]; }
+    if (!wl.empty()) alias.init(wl);
+    if (graph_weight < 1e-20) graph_weight = 1e-20;
+    offset_total = lat.Nb * w.offset;
+    p_off = (w.v[0] + w.v[1] > 1e-10) ? w.v[0] / (w.v[0] + w.v[1]) : 1;
+    spins.assign(lat.N, 0);
+    spins_c.assign(lat.N, 0);
+    current.assign(lat.N, 0);
+  }
+
+  void sweep(double* rec) {
+    const int N = lat.N;
+    auto rnd = [&]() { return u01(eng); };
+    std::swap(ops, ops_p);
+    ops.clear();
+    LooperOp sentinel{0, 0, 1.0, 0, 0};
+    ops_p.push_back(sentinel);
+    spins_c = spins;
+    // fill times (path_integral.C:403-425)
+    std::exponential_distribution<double> expdist(beta * graph_weight);
+    times.clear();
+    double t = 0;
+    while (t < 1) { t += expdist(eng); times.push_back(t); }
+    // fragments (427-453)
+    frag.assign(N + ops_p.size() + times.size(), UFNode());
+    for (int s = 0; s < N; ++s) current[s] = s;
+    int fid = N;
+    size_t tmi = 0;
+    for (size_t opi = 0; opi < ops_p.size();) {
+      if (times[tmi] < ops_p[opi].time) {
+        auto bg = graph_of[alias(rnd)];
+        int b = bg.first, g = bg.second;
+        if (is_compatible(g, spins_c[lat.src[b]], spins_c[lat.tgt[b]])) {
+          ops.push_back(LooperOp{b, g << 2, times[tmi], 0, 0});
+          ++tmi;
+        } else {
+          ++tmi;
+          continue;
+        }
+      } else {
+        if ((ops_p[opi].type & 1) == 0) { ++opi; continue; }
+        ops.push_back(ops_p[opi]);
+        ++opi;
+      }
+      LooperOp& oi = ops.back();
+      int b = oi.bond, s0 = lat.src[b], s1 = lat.tgt[b];
+      if (oi.type & 1) {
+        int g = (rnd() < p_off) ? 0 : 1;    // choose_offdiagonal, graph_impl.h:323-327
+        oi.type = (oi.type & 1) | (g << 2);
+        spins_c[s0] ^= 1;
+        spins_c[s1] ^= 1;
+      }
+      Reconnect r = reconnect(frag, fid, oi.type >> 2, current[s0], current[s1]);
+      fid = r.fid; current[s0] = r.curr0; current[s1] = r.curr1;
+      oi.loop0 = r.loop0; oi.loop1 = r.loop1;
+    }
+    // symmetrize (583-588)
+    for (int s = 0; s < N; ++s) uf_unify(frag, s, current[s]);
+    // cluster ids (626-648)
+    int nc = uf_set_id(frag, 0, fid, 0);
+    uf_copy_id(frag, 0, fid);
+    // accumulate (651-737)
+    est.assign(nc, Estimate());
+    spins_c = spins;
+    for (int s = 0; s < N; ++s) est_start_bottom(est[frag[s].id_], lat.gauge[s], spins_c[s]);
+    for (auto& op : ops) {
+      int g = op.type >> 2;
+      if (is_frozen(g)) continue;
+      int b = op.bond, s0 = lat.src[b], s1 = lat.tgt[b];
+      double tt = op.time;
+      Estimate& el0 = est[frag[loop_l0(g, op.loop0, op.loop1)].id_];
+      Estimate& el1 = est[frag[loop_l1(g, op.loop0, op.loop1)].id_];
+      est_end(el0, tt, lat.gauge[s0], spins_c[s0]);
+      for (int d = 0; d < lat.dim; ++d) el0.winding[d] += (1 - 2 * spins_c[s0]) * lat.vr(b, d);
+      est_end(el1, tt, lat.gauge[s1], spins_c[s1]);
+      if (op.type & 1) { spins_c[s0] ^= 1; spins_c[s1] ^= 1; }
+      Estimate& eu0 = est[frag[loop_u0(g, op.loop0, op.loop1)].id_];
+      Estimate& eu1 = est[frag[loop_u1(g, op.loop0, op.loop1)].id_];
+      est_begin(eu0, tt, lat.gauge[s0], spins_c[s0]);
+      for (int d = 0; d < lat.dim; ++d) eu0.winding[d] -= (1 - 2 * spins_c[s0]) * lat.vr(b, d);
+      est_begin(eu1, tt, lat.gauge[s1], spins_c[s1]);
+    }
+    for (int s = 0; s < N; ++s) est_end(est[frag[current[s]].id_], 1.0, lat.gauge[s], spins_c[s]);
+    // collect (740-778)
+    for (int i = 0; i < R_COUNT; ++i) rec[i] = 0;
+    for (int c = 0; c < nc; ++c) collect(est[c], rec);
+    rec[R_NOP] = (double)ops.size();
+    rec[R_NC] = nc;
+    rec[R_ENERGY] = offset_total - ops.size() / beta;
+    // determine flip (781-803), FIELD=false: (2r-1) < 0
+    to_flip.assign(nc, 0);
+    for (int c = 0; c < nc; ++c) to_flip[c] = (2 * rnd() - 1) < 0;
+    // flip operators & spins (806-825)
+    for (auto& op : ops)
+      if ((to_flip[frag[op.loop0].id_] ^ to_flip[frag[op.loop1].id_]) & 1) op.type ^= 1;
+    for (int s = 0; s < N; ++s) if (to_flip[frag[s].id_]) spins[s] ^= 1;
+    ++mcs;
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Standalone mode: standalone/loop.C:39-180 consuming the RNG the same way.
+// ---------------------------------------------------------------------------
+struct SANode {  // standalone/union_find.h:44-57
+  int parent_ = -1;
+  int id_ = 0;
+  bool is_root() const { return parent_ < 0; }
+  int weight() const { return -parent_; }
+};
+int sa_root(const std::vector<SANode>& v, int g) { while (!v[g].is_root()) g = v[g].parent_; return g; }
+int sa_unify(std::vector<SANode>& v, int g0, int g1) {  // unify_weight, :112-126
+  int r0 = sa_root(v, g0), r1 = sa_root(v, g1);
+  if (r0 != r1) {
+    if (v[r0].weight() < v[r1].weight()) std::swap(r0, r1);
+    v[r0].parent_ += v[r1].parent_;
+    v[r1].parent_ = r0;
+  }
+  for (int g : {g0, g1}) { while (g != r0) { int p = v[g].parent_; v[g].parent_ = r0; g = p; } }
+  return r0;
+}
+
+struct RunStat {  // standalone/observable.h
+  double sum = 0, esq = 0; unsigned count = 0;
+  void add(double x) { sum += x; esq += x * x; ++count; }
+  double mean() const { return count ? sum / count : 0; }
+  double error() const { return count > 1 ? std::sqrt((esq / count - mean() * mean()) / (count - 1)) : 0; }
+};
+
+void standalone_run(unsigned L, double T, unsigned sweeps, unsigned therm, double* mean5, double* err5) {
+  const unsigned nsites = L, nbonds = L;
+  const double beta = 1 / T;
+  std::mt19937 eng(29833);
+  std::uniform_real_distribution<> d_uniform;
+  std::exponential_distribution<> d_time(beta * nbonds / 2);
+  struct Op { int type; unsigned bond; unsigned upper, lower; double time; };
+  std::vector<Op> ops, ops_p;
+  std::vector<int> spins(nsites, 0), current(nsites);
+  std::vector<SANode> frag;
+  std::vector<char> to_flip;
+  struct Est { double mag = 0, size = 0, length = 0; };
+  std::vector<Est> est;
+  RunStat o[5];
+  auto left = [&](unsigned b) { return (int)b; };
+  auto right = [&](unsigned b) { return (b == nbonds - 1) ? 0 : (int)b + 1; };
+  for (unsigned mcs = 0; mcs < therm + sweeps; ++mcs) {
+    std::swap(ops, ops_p); ops.clear();
+    frag.clear(); frag.resize(nsites);
+    for (unsigned s = 0; s < nsites; ++s) current[s] = s;
+    double t = d_time(eng);
+    for (size_t opi = 0; t < 1 || opi != ops_p.size();) {
+      if (opi == ops_p.size() || t < ops_p[opi].time) {
+        const int b = static_cast<int>(nbonds * d_uniform(eng));
+        if (spins[left(b)] != spins[right(b)]) {
+          ops.push_back(Op{0, (unsigned)b, 0, 0, t});
+          t += d_time(eng);
+        } else {
+          t += d_time(eng);
+          continue;
+        }
+      } else {
+        if (ops_p[opi].type == 0) { ++opi; continue; }
+        ops.push_back(ops_p[opi]);
+        ++opi;
+      }
+      Op& oi = ops.back();
+      const int s0 = left(oi.bond), s1 = right(oi.bond);
+      if (oi.type == 1) { spins[s0] ^= 1; spins[s1] ^= 1; }
+      oi.lower = sa_unify(frag, current[s0], current[s1]);
+      frag.push_back(SANode());
+      oi.upper = current[s0] = current[s1] = (unsigned)frag.size() - 1;
+    }
+    for (unsigned s = 0; s < nsites; ++s) sa_unify(frag, s, current[s]);
+    int nc = 0;
+    for (auto& f : frag) if (f.is_root()) f.id_ = nc++;
+    for (auto& f : frag) f.id_ = frag[sa_root(frag, (int)(&f - &frag[0]))].id_;
+    to_flip.resize(nc);
+    est.clear(); est.resize(nc);
+    for (auto& op : ops) {
+      est[frag[op.lower].id_].length += 2 * op.time;
+      est[frag[op.upper].id_].length -= 2 * op.time;
+    }
+    for (unsigned s = 0; s < nsites; ++s) {
+      int id = frag[s].id_;
+      est[id].mag += 1 - 2 * spins[s];
+      est[id].size += 1;
+      est[id].length += 1;
+    }
+    double usus = 0, smag = 0, ssus = 0;
+    for (auto& e : est) { usus += e.mag * e.mag; smag += e.size * e.size; ssus += e.length * e.length; }
+    for (int c = 0; c < nc; ++c) to_flip[c] = (d_uniform(eng) < 0.5);
+    for (auto& op : ops)
+      if (to_flip[frag[op.lower].id_] ^ to_flip[frag[op.upper].id_]) op.type ^= 1;
+    for (unsigned s = 0; s < nsites; ++s) if (to_flip[frag[s].id_]) spins[s] ^= 1;
+    if (mcs >= therm) {
+      o[0].add(nc);
+      o[1].add((0.25 * nbonds - ops.size() / beta) / nsites);
+      o[2].add(0.25 * beta * usus / nsites);
+      o[3].add(0.25 * smag);
+      o[4].add(0.25 * beta * ssus / nsites);
+    }
+  }
+  for (int i = 0; i < 5; ++i) { mean5[i] = o[i].mean(); err5[i] = o[i].error(); }
+}
+
+// ---------------------------------------------------------------------------
+// Canonical configuration and partitions.
+//
+// A configuration is spins0[N] plus operators in canonical order (bond
+// ascending, then time ascending; equal times keep their given order).
+// Elements: 0..N-1 the world-line boundary at tau = 0 of each site,
+// N + 4k + {0:l0, 1:l1, 2:u0, 3:u1} the legs of operator k; l0/u0 sit on the
+// bond's source, l1/u1 on its target.  Events on a site are ordered by
+// (time, bond, rank in bond).
+// ---------------------------------------------------------------------------
+struct SiteEvents {
+  std::vector<int> off;           // per site: start in ev (size N+1)
+  std::vector<int> ev_op;         // operator index of each event
+  std::vector<int> ev_side;       // 0 = source side, 1 = target side
+  std::vector<int> pos0, pos1;    // per op: position of its event on source / target site
+};
+
+SiteEvents build_site_events(const Lattice& lat, int nops, const int* bond, const double* time) {
+  SiteEvents se;
+  std::vector<std::vector<std::pair<int, int>>> per(lat.N);  // (op, side)
+  for (int k = 0; k < nops; ++k) {
+    per[lat.src[bond[k]]].push_back({k, 0});
+    per[lat.tgt[bond[k]]].push_back({k, 1});
+  }
+  se.off.assign(lat.N + 1, 0);
+  se.pos0.assign(nops, -1);
+  se.pos1.assign(nops, -1);
+  for (int s = 0; s < lat.N; ++s) {
+    auto& v = per[s];
+    std::stable_sort(v.begin(), v.end(), [&](const std::pair<int, int>& a, const std::pair<int, int>& b) {
+      if (time[a.first] != time[b.first]) return time[a.first] < time[b.first];
+      if (bond[a.first] != bond[b.first]) return bond[a.first] < bond[b.first];
+      return a.first < b.first;
+    });
+    se.off[s + 1] = se.off[s] + (int)v.size();
+    for (size_t j = 0; j < v.size(); ++j) {
+      se.ev_op.push_back(v[j].first);
+      se.ev_side.push_back(v[j].second);
+      if (v[j].second == 0) se.pos0[v[j].first] = (int)j; else se.pos1[v[j].first] = (int)j;
+    }
+  }
+  return se;
+}
+
+// Leg graph edges (intra-operator cut edges + world-line edges).
+std::vector<std::pair<int, int>> leg_graph_edges(const Lattice& lat, int nops, const int* bond,
+                                                 const double* time, const int* type) {
+  int N = lat.N;
+  std::vector<std::pair<int, int>> E;
+  for (int k = 0; k < nops; ++k) {
+    int g = type[k] >> 2;
+    int l0 = N + 4 * k, l1 = l0 + 1, u0 = l0 + 2, u1 = l0 + 3;
+    if (g & 2) { E.push_back({l0, l1}); E.push_back({u0, u1}); E.push_back({l0, u0}); }
+    else if (g == 0) { E.push_back({l0, l1}); E.push_back({u0, u1}); }
+    else { E.push_back({l0, u1}); E.push_back({l1, u0}); }
+  }
+  SiteEvents se = build_site_events(lat, nops, bond, time);
+  for (int s = 0; s < N; ++s) {
+    int a = se.off[s], b = se.off[s + 1];
+    if (a == b) continue;
+    auto lower = [&](int j) { return N + 4 * se.ev_op[j] + (se.ev_side[j] == 0 ? 0 : 1); };
+    auto upper = [&](int j) { return N + 4 * se.ev_op[j] + (se.ev_side[j] == 0 ? 2 : 3); };
+    E.push_back({s, lower(a)});
+    for (int j = a; j + 1 < b; ++j) E.push_back({upper(j), lower(j + 1)});
+    E.push_back({upper(b - 1), s});
+  }
+  return E;
+}
+
+void canonicalise(std::vector<int>& comp, int n) {
+  // comp[i] = arbitrary component id -> min element index of the component
+  std::vector<int> mn(n, INT32_MAX);
+  for (int i = 0; i < n; ++i) mn[comp[i]] = std::min(mn[comp[i]], i);
+  for (int i = 0; i < n; ++i) comp[i] = mn[comp[i]];
+}
+
+void bfs_labels(const Lattice& lat, int nops, const int* bond, const double* time, const int* type,
+                int* labels) {
+  int n = lat.N + 4 * nops;
+  auto E = leg_graph_edges(lat, nops, bond, time, type);
+  std::vector<std::vector<int>> adj(n);
+  for (auto& e : E) { adj[e.first].push_back(e.second); adj[e.second].push_back(e.first); }
+  std::vector<int> comp(n, -1);
+  for (int i = 0; i < n; ++i) {
+    if (comp[i] >= 0) continue;
+    std::deque<int> q{i};
+    comp[i] = i;
+    while (!q.empty()) {
+      int x = q.front(); q.pop_front();
+      for (int y : adj[x]) if (comp[y] < 0) { comp[y] = i; q.push_back(y); }
+    }
+  }
+  for (int i = 0; i < n; ++i) labels[i] = comp[i];  // BFS from increasing i -> min index already
+}
+
+// looper's fragment algorithm on a given configuration: operators visited in
+// tau order ((time, bond, rank) as on a site), reconnect + unify, symmetrize,
+// set_id/copy_id; labels mapped to canonical elements through loop0/loop1 and
+// the leg accessors.
+void looper_labels(const Lattice& lat, int nops, const int* bond, const double* time, const int* type,
+                   int* labels, int compress_variant) {
+  int N = lat.N;
+  std::vector<int> order(nops);
+  for (int k = 0; k < nops; ++k) order[k] = k;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    if (time[a] != time[b]) return time[a] < time[b];
+    if (bond[a] != bond[b]) return bond[a] < bond[b];
+    return a < b;
+  });
+  std::vector<UFNode> frag(N + nops);
+  std::vector<int> current(N), loop0(nops), loop1(nops);
+  for (int s = 0; s < N; ++s) current[s] = s;
+  int fid = N;
+  for (int k : order) {
+    int g = type[k] >> 2, b = bond[k], s0 = lat.src[b], s1 = lat.tgt[b];
+    if (compress_variant && g == 0) {
+      frag[fid] = UFNode();
+      loop0[k] = uf_unify_compress(frag, current[s0], current[s1]);
+      loop1[k] = current[s0] = current[s1] = fid++;
+      continue;
+    }
+    if (compress_variant && (g & 2)) {
+      int u = uf_unify_compress(frag, current[s0], current[s1]);
+      loop0[k] = loop1[k] = current[s0] = current[s1] = u;
+      continue;
+    }
+    Reconnect r = reconnect(frag, fid, g, current[s0], current[s1]);
+    fid = r.fid; current[s0] = r.curr0; current[s1] = r.curr1;
+    loop0[k] = r.loop0; loop1[k] = r.loop1;
+  }
+  for (int s = 0; s < N; ++s) {
+    if (compress_variant) uf_unify_compress(frag, s, current[s]);
+    else uf_unify(frag, s, current[s]);
+  }
+  int nc = uf_set_id(frag, 0, fid, 0);
+  (void)nc;
+  uf_copy_id(frag, 0, fid);
+  int n = N + 4 * nops;
+  std::vector<int> comp(n);
+  for (int s = 0; s < N; ++s) comp[s] = frag[s].id_;
+  for (int k = 0; k < nops; ++k) {
+    int g = type[k] >> 2;
+    comp[N + 4 * k + 0] = frag[loop_l0(g, loop0[k], loop1[k])].id_;
+    comp[N + 4 * k + 1] = frag[loop_l1(g, loop0[k], loop1[k])].id_;
+    comp[N + 4 * k + 2] = frag[loop_u0(g, loop0[k], loop1[k])].id_;
+    comp[N + 4 * k + 3] = frag[loop_u1(g, loop0[k], loop1[k])].id_;
+  }
+  canonicalise(comp, n);
+  for (int i = 0; i < n; ++i) labels[i] = comp[i];
+}
+
+// ---------------------------------------------------------------------------
+// Philox mode: the device algorithm, sequential.
+// ---------------------------------------------------------------------------
+struct PhiloxChain {
+  Lattice lat;
+  Weights w;
+  Thresholds th;
+  double beta = 1, scale = 0, offset_total = 0, W = 0;
+  uint32_t key0 = 0, key1 = 0;
+  uint32_t sweep = 0;
+  std::vector<int> spins;                 // tau = 0 spins
+  // operators, bond CSR in canonical order
+  std::vector<int> boff;                  // Nb+1
+  std::vector<double> otime;
+  std::vector<int> otype;                 // offdiag bit | g << 2
+  // previous configuration's site events (post-flip segment spins)
+  std::vector<int> psoff;                 // N+1, offsets in segment space
+  std::vector<int> pm;                    // number of events per site
+  std::vector<double> pev_time;           // indexed by segment node (event j's time at psoff[s]+j)
+  std::vector<int> pseg_spin;             // post-flip spin of each segment
+  // last sweep's cluster data (for inspection)
+  std::vector<int> last_root;             // per segment node (chain-local)
+  std::vector<int> last_l0, last_l1;      // per op
+  int last_nodes = 0;
+
+  void init(int kind, int L, double b, double jxy, double jz, double fs, uint32_t seed, uint32_t chain) {
+    lat.init(kind, L);
+    w = xxz_weights(jxy, jz, fs);
+    th = make_thresholds(w);
+    beta = b;
+    W = w.v[0] + w.v[1] + w.v[2] + w.v[3];
+    scale = (W > 0) ? std::ldexp(1.0, 52) / (beta * W) : 0;
+    offset_total = lat.Nb * w.offset;
+    key0 = seed; key1 = chain;
+    spins.assign(lat.N, 0);
+    boff.assign(lat.Nb + 1, 0);
+    otime.clear(); otype.clear();
+    reset_prev_from_ops();
+  }
+
+  // Rebuild the "previous configuration" site arrays from the current
+  // operators and spins (used at start and after loading a configuration).
+  void reset_prev_from_ops() {
+    int nops = (int)otime.size();
+    std::vector<int> bond(nops);
+    for (int b = 0; b < lat.Nb; ++b) for (int k = boff[b]; k < boff[b + 1]; ++k) bond[k] = b;
+    SiteEvents se = build_site_events(lat, nops, bond.data(), otime.data());
+    psoff.assign(lat.N + 1, 0);
+    pm.assign(lat.N, 0);
+    for (int s = 0; s < lat.N; ++s) {
+      pm[s] = se.off[s + 1] - se.off[s];
+      psoff[s + 1] = psoff[s] + std::max(pm[s], 1);
+    }
+    pev_time.assign(psoff[lat.N], 0);
+    pseg_spin.assign(psoff[lat.N], 0);
+    for (int s = 0; s < lat.N; ++s) {
+      int c = spins[s];
+      if (pm[s] == 0) { pseg_spin[psoff[s]] = c; continue; }
+      for (int j = 0; j < pm[s]; ++j) {
+        int k = se.ev_op[se.off[s] + j];
+        pev_time[psoff[s] + j] = otime[k];
+        pseg_spin[psoff[s] + j] = c;
+        c ^= (otype[k] & 1);
+      }
+    }
+  }
+
+  // spin at tau on site s in the previous configuration; returns -1 on a tie
+  int spin_at(int s, double tau) const {
+    int m = pm[s], base = psoff[s];
+    if (m == 0) return pseg_spin[base];
+    const double* t = &pev_time[base];
+    int j = (int)(std::lower_bound(t, t + m, tau) - t);
+    if (j < m && t[j] == tau) return -1;
+    return pseg_spin[base + (j == m ? 0 : j)];
+  }
+
+  void sweep_once(double* rec) {
+    const int N = lat.N, Nb = lat.Nb;
+    // (a) diagonal update, per bond
+    std::vector<int> nboff(Nb + 1, 0);
+    std::vector<double> ntime;
+    std::vector<int> ntype;
+    ntime.reserve(otime.size() + 16);
+    ntype.reserve(otype.size() + 16);
+    std::vector<double> acc_t;
+    std::vector<int> acc_g;
+    for (int b = 0; b < Nb; ++b) {
+      int s0 = lat.src[b], s1 = lat.tgt[b];
+      acc_t.clear(); acc_g.clear();
+      if (W > 0) {
+        uint64_t T = 0;
+        for (uint32_t k = 0;; ++k) {
+          u32x4 x = philox4x32_10(sweep, STAGE_DIAG, (uint32_t)b, k, key0, key1);
+          T += exp_gap(x, scale);
+          if (T >= TGRID) break;
+          double tau = (double)T * (1.0 / 4503599627370496.0);
+          int g = choose_graph(x.v[2], th);
+          int c0 = spin_at(s0, tau), c1 = spin_at(s1, tau);
+          if (c0 < 0 || c1 < 0) continue;
+          if (is_compatible(g, c0, c1)) { acc_t.push_back(tau); acc_g.push_back(g); }
+        }
+      }
+      // merge kept (regraphed) off-diagonal ops with accepted candidates
+      int kept_idx = 0;
+      size_t ai = 0;
+      for (int r = boff[b]; r < boff[b + 1]; ++r) {
+        if ((otype[r] & 1) == 0) continue;
+        u32x4 x = philox4x32_10(sweep, STAGE_REGRAPH, (uint32_t)b, (uint32_t)kept_idx, key0, key1);
+        int g = ((uint64_t)x.v[0] < th.toff) ? 0 : 1;
+        ++kept_idx;
+        while (ai < acc_t.size() && acc_t[ai] < otime[r]) {
+          ntime.push_back(acc_t[ai]); ntype.push_back(acc_g[ai] << 2); ++ai;
+        }
+        ntime.push_back(otime[r]); ntype.push_back(1 | (g << 2));
+      }
+      while (ai < acc_t.size()) { ntime.push_back(acc_t[ai]); ntype.push_back(acc_g[ai] << 2); ++ai; }
+      nboff[b + 1] = (int)ntime.size();
+    }
+    boff.swap(nboff);
+    otime.swap(ntime);
+    otype.swap(ntype);
+    int nops = (int)otime.size();
+    std::vector<int> bond(nops);
+    for (int b = 0; b < Nb; ++b) for (int k = boff[b]; k < boff[b + 1]; ++k) bond[k] = b;
+
+    // (b) world-line linking: segments as nodes
+    SiteEvents se = build_site_events(lat, nops, bond.data(), otime.data());
+    std::vector<int> soff(N + 1, 0), m(N);
+    for (int s = 0; s < N; ++s) { m[s] = se.off[s + 1] - se.off[s]; soff[s + 1] = soff[s] + std::max(m[s], 1); }
+    int M = soff[N];
+    std::vector<int> seg_spin(M), seg_site(M);
+    std::vector<double> ev_time(M, 0);
+    for (int s = 0; s < N; ++s) {
+      int c = spins[s];
+      if (m[s] == 0) { seg_spin[soff[s]] = c; seg_site[soff[s]] = s; continue; }
+      for (int j = 0; j < m[s]; ++j) {
+        int k = se.ev_op[se.off[s] + j];
+        ev_time[soff[s] + j] = otime[k];
+        seg_spin[soff[s] + j] = c;
+        seg_site[soff[s] + j] = s;
+        c ^= (otype[k] & 1);
+      }
+    }
+    auto node = [&](int s, int j) { return soff[s] + j; };
+    auto next = [&](int s, int i) { return (i + 1 == soff[s + 1]) ? soff[s] : i + 1; };
+    last_l0.assign(nops, 0); last_l1.assign(nops, 0);
+    // (c) union-find with min-index roots (any correct algorithm gives the
+    // same partition; roots are the minimum node of each cluster)
+    std::vector<int> parent(M);
+    for (int i = 0; i < M; ++i) parent[i] = i;
+    auto find = [&](int x) { while (parent[x] != x) { parent[x] = parent[parent[x]]; x = parent[x]; } return x; };
+    auto unite = [&](int a, int b) {
+      a = find(a); b = find(b);
+      if (a == b) return;
+      if (a < b) parent[b] = a; else parent[a] = b;
+    };
+    for (int k = 0; k < nops; ++k) {
+      int b = bond[k], s0 = lat.src[b], s1 = lat.tgt[b];
+      int l0 = node(s0, se.pos0[k]), l1 = node(s1, se.pos1[k]);
+      int u0 = next(s0, l0), u1 = next(s1, l1);
+      last_l0[k] = l0; last_l1[k] = l1;
+      int g = otype[k] >> 2;
+      if (g & 2) { unite(l0, l1); unite(u0, u1); unite(l0, u0); }
+      else if (g == 0) { unite(l0, l1); unite(u0, u1); }
+      else { unite(l0, u1); unite(l1, u0); }
+    }
+    for (int i = 0; i < M; ++i) parent[i] = find(i);
+    last_root = parent;
+    last_nodes = M;
+    // (e) improved estimators per segment
+    std::vector<Estimate> acc(M);
+    for (int s = 0; s < N; ++s) {
+      int gs = lat.gauge[s];
+      if (m[s] == 0) {
+        int i = soff[s];
+        Estimate& e = acc[parent[i]];
+        int c = seg_spin[i];
+        est_start_bottom(e, gs, c);
+        est_end(e, 1.0, gs, c);
+        continue;
+      }
+      for (int j = 0; j < m[s]; ++j) {
+        int i = soff[s] + j;
+        Estimate& e = acc[parent[i]];
+        int c = seg_spin[i];
+        double len = (j == 0) ? (ev_time[soff[s]] + 1.0 - ev_time[soff[s] + m[s] - 1])
+                              : (ev_time[i] - ev_time[i - 1]);
+        if (j == 0) {
+          e.usize0 += 0.5; e.umag0 += 0.5 - c; e.ssize0 += gs * 0.5; e.smag0 += gs * (0.5 - c);
+        }
+        e.usize += 0.5 * len; e.umag += (0.5 - c) * len;
+        e.ssize += gs * 0.5 * len; e.smag += gs * (0.5 - c) * len;
+        // winding: this segment is the lower leg of event j and the upper leg of event j-1
+        int ka = se.ev_op[se.off[s] + j], sa = se.ev_side[se.off[s] + j];
+        int jb = (j == 0) ? m[s] - 1 : j - 1;
+        int kb = se.ev_op[se.off[s] + jb], sb = se.ev_side[se.off[s] + jb];
+        if (sa == 0) for (int d = 0; d < lat.dim; ++d) e.winding[d] += (1 - 2 * c) * lat.vr(bond[ka], d);
+        if (sb == 0) for (int d = 0; d < lat.dim; ++d) e.winding[d] -= (1 - 2 * c) * lat.vr(bond[kb], d);
+      }
+    }
+    for (int i = 0; i < R_COUNT; ++i) rec[i] = 0;
+    int nc = 0;
+    for (int i = 0; i < M; ++i) if (parent[i] == i) { collect(acc[i], rec); ++nc; }
+    rec[R_NOP] = nops;
+    rec[R_NC] = nc;
+    rec[R_ENERGY] = offset_total - nops / beta;
+    // (d) flip: one bit per root
+    std::vector<char> fb(M, 0);
+    for (int i = 0; i < M; ++i)
+      if (parent[i] == i) fb[i] = philox4x32_10(sweep, STAGE_FLIP, (uint32_t)i, 0, key0, key1).v[0] & 1;
+    for (int k = 0; k < nops; ++k) {
+      int b = bond[k], s0 = lat.src[b];
+      int l0 = last_l0[k], u0 = next(s0, l0);
+      if (fb[parent[l0]] ^ fb[parent[u0]]) otype[k] ^= 1;
+    }
+    for (int s = 0; s < N; ++s) if (fb[parent[soff[s]]]) spins[s] ^= 1;
+    for (int i = 0; i < M; ++i) seg_spin[i] ^= fb[parent[i]];
+    // the flipped configuration becomes "previous" for the next sweep
+    psoff = soff; pm = m; pev_time = ev_time; pseg_spin = seg_spin;
+    ++sweep;
+  }
+};
+
+}  // namespace
+
+// ===========================================================================
+// C ABI for ctypes
+// ===========================================================================
+extern "C" {
+
+int or_record_size() { return R_COUNT; }
+
+void or_philox4x32_10(const uint32_t* ctr, const uint32_t* key, uint32_t* out) {
+  u32x4 x = philox4x32_10(ctr[0], ctr[1], ctr[2], ctr[3], key[0], key[1]);
+  for (int i = 0; i < 4; ++i) out[i] = x.v[i];
+}
+
+void or_det_log(const double* x, double* y, int n) { for (int i = 0; i < n; ++i) y[i] = det_log(x[i]); }
+
+void or_exp_gap(const uint32_t* x4, double scale, uint64_t* gap) {
+  u32x4 x = {{x4[0], x4[1], x4[2], x4[3]}};
+  *gap = exp_gap(x, scale);
+}
+
+void or_xxz_weights(double jxy, double jz, double force_scatter, double* v4, double* offset, int* sign) {
+  Weights w = xxz_weights(jxy, jz, force_scatter);
+  for (int g = 0; g < 4; ++g) v4[g] = w.v[g];
+  *offset = w.offset;
+  *sign = w.sign;
+}
+
+void or_thresholds(double jxy, double jz, double force_scatter, uint64_t* thr4, uint64_t* toff) {
+  Thresholds t = make_thresholds(xxz_weights(jxy, jz, force_scatter));
+  for (int g = 0; g < 4; ++g) thr4[g] = t.thr[g];
+  *toff = t.toff;
+}
+
+int or_lattice_info(int kind, int L, int* N, int* Nb, int* dim) {
+  if ((kind != 0 && kind != 1) || L < 2) return -1;
+  Lattice lat; lat.init(kind, L);
+  *N = lat.N; *Nb = lat.Nb; *dim = lat.dim;
+  return 0;
+}
+int or_lattice_bonds(int kind, int L, int* src, int* tgt, int* gauge) {
+  Lattice lat; lat.init(kind, L);
+  for (int b = 0; b < lat.Nb; ++b) { src[b] = lat.src[b]; tgt[b] = lat.tgt[b]; }
+  for (int s = 0; s < lat.N; ++s) gauge[s] = lat.gauge[s];
+  return 0;
+}
+
+// union-find on an explicit pair list (test/union_find.C semantics):
+// returns parent (-1 for roots), root, weight (roots only) per node.
+void or_uf_pairs(int n, int npairs, const int* pairs, int variant, int* parent, int* root, int* weight) {
+  std::vector<UFNode> v(n);
+  for (int i = 0; i < npairs; ++i) {
+    if (variant == 0) uf_unify(v, pairs[2 * i], pairs[2 * i + 1]);
+    else uf_unify_compress(v, pairs[2 * i], pairs[2 * i + 1]);
+  }
+  for (int i = 0; i < n; ++i) {
+    parent[i] = v[i].is_root() ? -1 : v[i].parent();
+    root[i] = uf_root_index(v, i);
+    weight[i] = v[i].is_root() ? v[i].weight() : 0;
+  }
+}
+
+// ---- looper mode ---------------------------------------------------------
+void* or_looper_create(int kind, int L, double beta, double jxy, double jz, double fs, uint32_t seed) {
+  LooperChain* c = new LooperChain();
+  c->init(kind, L, beta, jxy, jz, fs, seed);
+  return c;
+}
+void or_looper_sweep(void* h, int n, double* rec) {
+  LooperChain* c = (LooperChain*)h;
+  for (int i = 0; i < n; ++i) c->sweep(rec + (size_t)i * R_COUNT);
+}
+void or_looper_destroy(void* h) { delete (LooperChain*)h; }
+
+// ---- standalone mode -----------------------------------------------------
+void or_standalone_run(unsigned L, double T, unsigned sweeps, unsigned therm, double* mean5, double* err5) {
+  standalone_run(L, T, sweeps, therm, mean5, err5);
+}
+
+// ---- partitions ------------------------------------------------------------
+int or_bfs_labels(int kind, int L, int nops, const int* bond, const double* time, const int* type,
+                  int* labels) {
+  Lattice lat; lat.init(kind, L);
+  bfs_labels(lat, nops, bond, time, type, labels);
+  return 0;
+}
+int or_looper_labels(int kind, int L, int nops, const int* bond, const double* time, const int* type,
+                     int compress_variant, int* labels) {
+  Lattice lat; lat.init(kind, L);
+  looper_labels(lat, nops, bond, time, type, labels, compress_variant);
+  return 0;
+}
+// Segment-node index of each element (boundary s -> its segment 0; leg ->
+// the world-line segment it touches) for a canonical configuration.
+int or_segment_of_elements(int kind, int L, int nops, const int* bond, const double* time, int* seg) {
+  Lattice lat; lat.init(kind, L);
+  SiteEvents se = build_site_events(lat, nops, bond, time);
+  std::vector<int> soff(lat.N + 1, 0);
+  for (int s = 0; s < lat.N; ++s) soff[s + 1] = soff[s] + std::max(se.off[s + 1] - se.off[s], 1);
+  for (int s = 0; s < lat.N; ++s) seg[s] = soff[s];
+  for (int k = 0; k < nops; ++k) {
+    int s0 = lat.src[bond[k]], s1 = lat.tgt[bond[k]];
+    int l0 = soff[s0] + se.pos0[k], l1 = soff[s1] + se.pos1[k];
+    int u0 = (l0 + 1 == soff[s0 + 1]) ? soff[s0] : l0 + 1;
+    int u1 = (l1 + 1 == soff[s1 + 1]) ? soff[s1] : l1 + 1;
+    seg[lat.N + 4 * k + 0] = l0; seg[lat.N + 4 * k + 1] = l1;
+    seg[lat.N + 4 * k + 2] = u0; seg[lat.N + 4 * k + 3] = u1;
+  }
+  return soff[lat.N];
+}
+
+// ---- philox mode -------------------------------------------------------------
+void* or_philox_create(int kind, int L, double beta, double jxy, double jz, double fs, uint32_t seed,
+                       uint32_t chain) {
+  PhiloxChain* c = new PhiloxChain();
+  c->init(kind, L, beta, jxy, jz, fs, seed, chain);
+  return c;
+}
+void or_philox_sweep(void* h, int n, double* rec) {
+  PhiloxChain* c = (PhiloxChain*)h;
+  for (int i = 0; i < n; ++i) c->sweep_once(rec + (size_t)i * R_COUNT);
+}
+int or_philox_num_ops(void* h) { return (int)((PhiloxChain*)h)->otime.size(); }
+int or_philox_num_nodes(void* h) { return ((PhiloxChain*)h)->last_nodes; }
+// state after the last sweep: spins[N], bond offsets[Nb+1], times[nops], types[nops]
+void or_philox_state(void* h, int* spins, int* boff, double* time, int* type) {
+  PhiloxChain* c = (PhiloxChain*)h;
+  for (int s = 0; s < c->lat.N; ++s) spins[s] = c->spins[s];
+  for (int b = 0; b <= c->lat.Nb; ++b) boff[b] = c->boff[b];
+  for (size_t k = 0; k < c->otime.size(); ++k) { time[k] = c->otime[k]; type[k] = c->otype[k]; }
+}
+// roots (chain-local segment nodes) of the last sweep, before the flip
+void or_philox_roots(void* h, int* root) {
+  PhiloxChain* c = (PhiloxChain*)h;
+  for (int i = 0; i < c->last_nodes; ++i) root[i] = c->last_root[i];
+}
+// load spins + operators (canonical order) and the sweep counter
+void or_philox_load(void* h, const int* spins, const int* boff, const double* time, const int* type,
+                    uint32_t sweep) {
+  PhiloxChain* c = (PhiloxChain*)h;
+  for (int s = 0; s < c->lat.N; ++s) c->spins[s] = spins[s];
+  for (int b = 0; b <= c->lat.Nb; ++b) c->boff[b] = boff[b];
+  int n = boff[c->lat.Nb];
+  c->otime.assign(time, time + n);
+  c->otype.assign(type, type + n);
+  c->sweep = sweep;
+  c->reset_prev_from_ops();
+}
+void or_philox_destroy(void* h) { delete (PhiloxChain*)h; }
+
+}  // extern "C"
