@@ -1,0 +1,742 @@
+// Host side of libloopgpu: batches of chains resident in HBM, sweep launches
+// (plain or HIP-graph replay), configuration upload/download and the C-ABI of
+// include/loopgpu.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/loopgpu.h"
+#include "lg_common.h"
+#include "lg_host.h"
+#include "lg_kernels.h"
+
+namespace lg {
+
+thread_local std::string g_last_error;
+
+int set_error(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+#define LG_HIP(call)                                                                   \
+  do {                                                                                 \
+    hipError_t e_ = (call);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      return set_error(LG_ERR_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// model setup (host): XXZ weights of weight_impl.h:165-188
+// ---------------------------------------------------------------------------
+static void xxz_weights(double jxy_in, double jz, double a, double v[4], double* offset) {
+  double jxy = std::fabs(jxy_in);
+  a = std::min(1.0, std::max(0.0, a));
+  auto crop0 = [](double x) { return x > 0 ? x : 0.0; };
+  if (jxy + std::fabs(jz) > 1e-10) {
+    if (jxy - jz > 2 * a * jxy) {
+      v[0] = crop0(std::min(jxy / 2, (jxy + jz) / 4));
+      v[1] = crop0(std::min(jxy / 2, (jxy - jz) / 4));
+      v[2] = crop0(-(jxy - jz) / 2.0);
+      v[3] = crop0(-(jxy + jz) / 2);
+    } else {
+      v[0] = (1 - a) * jxy / 2;
+      v[1] = a * jxy / 2;
+      v[2] = -((1 - 2 * a) * jxy - jz) / 2;
+      v[3] = 0;
+    }
+  } else {
+    v[0] = v[1] = v[2] = v[3] = 0;
+  }
+  *offset = (v[0] + v[1] + v[2] + v[3]) / 2;
+}
+
+int check_params(const lg_params& p) {
+  if (p.lattice != LG_LATTICE_CHAIN && p.lattice != LG_LATTICE_SQUARE)
+    return set_error(LG_ERR_ARG, "lattice must be chain (0) or square (1)");
+  if (p.L < 2 || (p.L & 1))
+    return set_error(LG_ERR_ARG, "L must be even and >= 2 (bipartite, sign-free lattice)");
+  if (p.lattice == LG_LATTICE_SQUARE && (long long)p.L * p.L * 2 > (1ll << 27))
+    return set_error(LG_ERR_ARG, "square lattice too large");
+  if (!(p.beta > 0) || !std::isfinite(p.beta))
+    return set_error(LG_ERR_ARG, "beta must be positive (negative temperature)");
+  if (!std::isfinite(p.jxy) || !std::isfinite(p.jz))
+    return set_error(LG_ERR_ARG, "couplings must be finite");
+  return LG_OK;
+}
+
+ChainDev make_chain_dev(const lg_params& p) {
+  ChainDev d;
+  std::memset(&d, 0, sizeof(d));
+  d.kind = p.lattice;
+  d.L = p.L;
+  d.N = (p.lattice == LG_LATTICE_CHAIN) ? p.L : p.L * p.L;
+  d.Nb = (p.lattice == LG_LATTICE_CHAIN) ? p.L : 2 * p.L * p.L;
+  d.dim = (p.lattice == LG_LATTICE_CHAIN) ? 1 : 2;
+  double v[4], off;
+  xxz_weights(p.jxy, p.jz, p.force_scatter, v, &off);
+  double W = v[0] + v[1] + v[2] + v[3];
+  d.beta = p.beta;
+  d.has_weight = W > 0;
+  d.scale = (W > 0) ? std::ldexp(1.0, 52) / (p.beta * W) : 0.0;
+  d.offset_total = d.Nb * off;
+  int last = -1;
+  for (int g = 0; g < 4; ++g) if (v[g] > 0) last = g;
+  double cum = 0;
+  for (int g = 0; g < 3; ++g) {
+    cum += v[g];
+    if (last < 0 || g >= last) d.thr[g] = 1ull << 32;
+    else d.thr[g] = (uint64_t)std::floor(std::ldexp(cum / W, 32));
+  }
+  double s01 = v[0] + v[1];
+  d.toff = (s01 > 0) ? (uint64_t)std::floor(std::ldexp(v[0] / s01, 32)) : (1ull << 32);
+  d.key0 = p.seed;
+  d.key1 = p.chain_id;
+  double lam = p.beta * W;
+  d.cand_cap = (int)std::ceil(lam + 12.0 * std::sqrt(lam) + 16.0);
+  return d;
+}
+
+// upper bound of the mean operator count: beta * Nb * (offset_b + |Jz|/4 + |Jxy|/2)
+static long long op_capacity(const lg_params& p, const ChainDev& d) {
+  double v[4], off;
+  xxz_weights(p.jxy, p.jz, p.force_scatter, v, &off);
+  double bound = p.beta * d.Nb * (off + std::fabs(p.jz) / 4 + std::fabs(p.jxy) / 2);
+  return (long long)std::ceil(bound + 10.0 * std::sqrt(bound) + 1024.0);
+}
+
+// ---------------------------------------------------------------------------
+// Batch
+// ---------------------------------------------------------------------------
+Batch::~Batch() { release(); }
+
+void Batch::release() {
+  for (int i = 0; i < 2; ++i)
+    if (graph_exec[i]) { hipGraphExecDestroy(graph_exec[i]); graph_exec[i] = nullptr; }
+  for (void* p : allocations) hipFree(p);
+  allocations.clear();
+  if (stream) { hipStreamDestroy(stream); stream = nullptr; }
+}
+
+template <class T>
+int Batch::alloc(T*& ptr, size_t n) {
+  void* p = nullptr;
+  size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
+  hipError_t e = hipMalloc(&p, bytes);
+  if (e != hipSuccess) {
+    ptr = nullptr;
+    return set_error(LG_ERR_DEVICE, "hipMalloc of " + std::to_string(bytes) + " bytes: " +
+                                        hipGetErrorString(e));
+  }
+  allocations.push_back(p);
+  ptr = (T*)p;
+  return LG_OK;
+}
+
+int Batch::init_for_config(const lg_params* p, long long nops) {
+  min_op_cap = nops + 1024;
+  return init(p, 1);
+}
+
+int Batch::init(const lg_params* p, int n) {
+  if (n <= 0) return set_error(LG_ERR_ARG, "need at least one chain");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+    return set_error(LG_ERR_DEVICE, "no HIP device available");
+  params.assign(p, p + n);
+  chains.resize(n);
+  long long sites = 0, bonds = 0, cand = 0, ops = 0;
+  for (int c = 0; c < n; ++c) {
+    int rc = check_params(p[c]);
+    if (rc) return rc;
+    ChainDev d = make_chain_dev(p[c]);
+    d.site_base = d.old_site_base = (int)sites;
+    d.bond_base = d.old_bond_base = (int)bonds;
+    d.cand_base = cand;
+    sites += d.N;
+    bonds += d.Nb;
+    cand += (long long)d.Nb * d.cand_cap;
+    ops += op_capacity(p[c], d);
+    chains[c] = d;
+  }
+  if (sites >= (1ll << 30) || bonds >= (1ll << 28) || ops >= (1ll << 30))
+    return set_error(LG_ERR_ARG, "batch too large for 32-bit indexing");
+  nsites = (int)sites;
+  nbonds = (int)bonds;
+  cand_total = cand;
+  ops = std::max(ops, min_op_cap);
+  op_cap = (int)ops;
+  long long ncap = 2 * ops + sites;
+  if (ncap >= (1ll << 31) - 1) return set_error(LG_ERR_ARG, "node capacity exceeds 32-bit indexing");
+  node_cap = (int)ncap;
+  LG_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  int rc = allocate();
+  if (rc) return rc;
+  return reset_state();
+}
+
+int Batch::allocate() {
+  int rc = 0;
+  for (int i = 0; i < 2; ++i) {
+    State& s = st[i];
+    if ((rc = alloc(s.spins, nsites))) return rc;
+    if ((rc = alloc(s.bo, nbonds + 1))) return rc;
+    if ((rc = alloc(s.ot, op_cap))) return rc;
+    if ((rc = alloc(s.oi, op_cap))) return rc;
+    if ((rc = alloc(s.soff, nsites + 1))) return rc;
+    if ((rc = alloc(s.evt, node_cap))) return rc;
+    if ((rc = alloc(s.segspin, node_cap))) return rc;
+  }
+  if ((rc = alloc(d_chains, chains.size()))) return rc;
+  if ((rc = alloc(bond_chain, nbonds))) return rc;
+  if ((rc = alloc(site_chain, nsites))) return rc;
+  if ((rc = alloc(counts, nbonds))) return rc;
+  if ((rc = alloc(acc_count, nbonds))) return rc;
+  if ((rc = alloc(kpre, op_cap))) return rc;
+  if ((rc = alloc(cand_t, cand_total))) return rc;
+  if ((rc = alloc(cand_g, cand_total))) return rc;
+  if ((rc = alloc(kc_new, op_cap))) return rc;
+  if ((rc = alloc(l0, op_cap))) return rc;
+  if ((rc = alloc(l1, op_cap))) return rc;
+  if ((rc = alloc(evflags, node_cap))) return rc;
+  if ((rc = alloc(node_site, node_cap))) return rc;
+  if ((rc = alloc(parent, node_cap))) return rc;
+  if ((rc = alloc(fbit, node_cap))) return rc;
+  if ((rc = alloc(acc, node_cap))) return rc;
+  scan_nparts = (nbonds + 2047) / 2048 + (nsites + 2047) / 2048;
+  if ((rc = alloc(scan_part, scan_nparts))) return rc;
+  if ((rc = alloc(totals, 2))) return rc;
+  if ((rc = alloc(flags, 1))) return rc;
+  if ((rc = alloc(d_sweep, chains.size()))) return rc;
+  if ((rc = alloc(rec_acc, chains.size() * R_COUNT))) return rc;
+  ring_size = (int)std::max<long long>(16, std::min<long long>(4096, (64ll << 20) / (chains.size() * R_COUNT * 8)));
+  if ((rc = alloc(rec_ring, (size_t)ring_size * chains.size() * R_COUNT))) return rc;
+  if ((rc = alloc(ring_pos, 1))) return rc;
+  // static tables
+  std::vector<int> bc(nbonds), sc(nsites);
+  for (size_t c = 0; c < chains.size(); ++c) {
+    for (int b = 0; b < chains[c].Nb; ++b) bc[chains[c].bond_base + b] = (int)c;
+    for (int s = 0; s < chains[c].N; ++s) sc[chains[c].site_base + s] = (int)c;
+  }
+  LG_HIP(hipMemcpy(d_chains, chains.data(), chains.size() * sizeof(ChainDev), hipMemcpyHostToDevice));
+  LG_HIP(hipMemcpy(bond_chain, bc.data(), nbonds * sizeof(int), hipMemcpyHostToDevice));
+  LG_HIP(hipMemcpy(site_chain, sc.data(), nsites * sizeof(int), hipMemcpyHostToDevice));
+  LG_HIP(hipMemset(rec_acc, 0, chains.size() * R_COUNT * sizeof(double)));
+  LG_HIP(hipMemset(flags, 0, sizeof(int)));
+  LG_HIP(hipMemset(ring_pos, 0, sizeof(int)));
+  LG_HIP(hipMemset(totals, 0, 2 * sizeof(int)));
+  // launch geometry
+  int dev = 0, cus = 256;
+  hipGetDevice(&dev);
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, dev) == hipSuccess) cus = prop.multiProcessorCount;
+  cfg.blocks_op = std::max(64, cus * 8);
+  cfg.blocks_node = std::max(64, cus * 8);
+  double lam_max = 0, ops_per_bond = 0;
+  for (size_t c = 0; c < chains.size(); ++c) {
+    double lam = chains[c].has_weight ? std::ldexp(1.0, 52) / chains[c].scale : 0.0;
+    lam_max = std::max(lam_max, lam);
+  }
+  ops_per_bond = lam_max;
+  int G = 4;
+  while (G < 64 && G < ops_per_bond) G *= 2;
+  cfg.group = G;
+  return LG_OK;
+}
+
+// all-up spins, empty operator string (path_integral.C:225)
+int Batch::reset_state() {
+  cur = 0;
+  State& s = st[cur];
+  std::vector<int> zeros(nbonds + 1, 0), soff(nsites + 1);
+  for (int i = 0; i <= nsites; ++i) soff[i] = i;
+  std::vector<double> evt(nsites, EMPTY_TIME);
+  LG_HIP(hipMemset(s.spins, 0, nsites));
+  LG_HIP(hipMemcpy(s.bo, zeros.data(), (nbonds + 1) * sizeof(int), hipMemcpyHostToDevice));
+  LG_HIP(hipMemcpy(s.soff, soff.data(), (nsites + 1) * sizeof(int), hipMemcpyHostToDevice));
+  LG_HIP(hipMemcpy(s.evt, evt.data(), nsites * sizeof(double), hipMemcpyHostToDevice));
+  LG_HIP(hipMemset(s.segspin, 0, nsites));
+  LG_HIP(hipMemset(d_sweep, 0, chains.size() * sizeof(uint32_t)));
+  host_sweep.assign(chains.size(), 0);
+  invalidate_graphs();
+  return LG_OK;
+}
+
+void Batch::invalidate_graphs() {
+  for (int i = 0; i < 2; ++i)
+    if (graph_exec[i]) { hipGraphExecDestroy(graph_exec[i]); graph_exec[i] = nullptr; }
+}
+
+DevBatch Batch::dev(int from) const {
+  const State& o = st[from];
+  const State& w = st[1 - from];
+  DevBatch B;
+  B.nchains = (int)chains.size();
+  B.nsites = nsites;
+  B.nbonds = nbonds;
+  B.chains = d_chains;
+  B.bond_chain = bond_chain;
+  B.site_chain = site_chain;
+  B.spins_old = o.spins; B.bo_old = o.bo; B.ot_old = o.ot; B.oi_old = o.oi;
+  B.soff_old = o.soff; B.evt_old = o.evt; B.segspin_old = o.segspin;
+  B.spins_new = w.spins; B.bo_new = w.bo; B.ot_new = w.ot; B.oi_new = w.oi;
+  B.soff_new = w.soff; B.evt_new = w.evt; B.segspin_new = w.segspin;
+  B.counts = counts; B.acc_count = acc_count; B.kpre = kpre;
+  B.cand_t = cand_t; B.cand_g = cand_g; B.kc_new = kc_new;
+  B.l0 = l0; B.l1 = l1; B.evflags = evflags; B.node_site = node_site;
+  B.parent = parent; B.fbit = fbit; B.acc = acc;
+  B.scan_part = scan_part; B.totals = totals; B.flags = flags;
+  B.sweep = d_sweep; B.rec_acc = rec_acc; B.rec_ring = rec_ring;
+  B.ring_pos = ring_pos; B.ring_size = ring_size;
+  B.op_cap = op_cap; B.node_cap = node_cap;
+  return B;
+}
+
+int Batch::capture(int from) {
+  hipGraph_t g = nullptr;
+  LG_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+  launch_sweep(dev(from), cfg, stream);
+  LG_HIP(hipStreamEndCapture(stream, &g));
+  hipError_t e = hipGraphInstantiate(&graph_exec[from], g, nullptr, nullptr, 0);
+  hipGraphDestroy(g);
+  if (e != hipSuccess) return set_error(LG_ERR_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+  return LG_OK;
+}
+
+int Batch::check_flags() {
+  int f = 0;
+  LG_HIP(hipMemcpy(&f, flags, sizeof(int), hipMemcpyDeviceToHost));
+  if (f) {
+    std::string m = "device capacity exceeded:";
+    if (f & FLAG_CAND_OVERFLOW) m += " candidate slots";
+    if (f & FLAG_OP_OVERFLOW) m += " operators";
+    if (f & FLAG_NODE_OVERFLOW) m += " segments";
+    if (f & FLAG_RING_OVERFLOW) m += " record ring";
+    return set_error(LG_ERR_CAPACITY, m);
+  }
+  return LG_OK;
+}
+
+int Batch::enqueue_sweep() {
+  if (use_graph) {
+    if (!graph_exec[cur]) {
+      int rc = capture(cur);
+      if (rc) return rc;
+    }
+    LG_HIP(hipGraphLaunch(graph_exec[cur], stream));
+  } else {
+    launch_sweep(dev(cur), cfg, stream);
+    LG_HIP(hipGetLastError());
+  }
+  cur ^= 1;
+  return LG_OK;
+}
+
+int Batch::run(uint32_t n, lg_sweep_record* out) {
+  int rc = check_flags();
+  if (rc) return rc;
+  const size_t nc = chains.size();
+  std::vector<double> buf;
+  uint32_t done = 0;
+  while (done < n) {
+    uint32_t chunk = std::min<uint32_t>(n - done, (uint32_t)ring_size);
+    LG_HIP(hipMemsetAsync(ring_pos, 0, sizeof(int), stream));
+    for (uint32_t i = 0; i < chunk; ++i) {
+      rc = enqueue_sweep();
+      if (rc) return rc;
+    }
+    if (out) {
+      buf.resize((size_t)chunk * nc * R_COUNT);
+      LG_HIP(hipMemcpyAsync(buf.data(), rec_ring, buf.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
+    }
+    LG_HIP(hipStreamSynchronize(stream));
+    rc = check_flags();
+    if (rc) return rc;
+    if (out) {
+      static_assert(sizeof(lg_sweep_record) == R_COUNT * sizeof(double), "record layout");
+      std::memcpy(out + (size_t)done * nc, buf.data(), buf.size() * sizeof(double));
+    }
+    for (size_t c = 0; c < nc; ++c) host_sweep[c] += chunk;
+    done += chunk;
+  }
+  return LG_OK;
+}
+
+// Autotuner: time candidate lane-group sizes of the diagonal update and grid
+// sizes of the per-operator / per-node kernels on the live configuration.
+int Batch::tune(uint32_t sweeps_per_trial) {
+  if (sweeps_per_trial == 0) sweeps_per_trial = 8;
+  int rc = run(sweeps_per_trial, nullptr);  // warm state and code
+  if (rc) return rc;
+  int dev = 0, cus = 256;
+  hipGetDevice(&dev);
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, dev) == hipSuccess) cus = prop.multiProcessorCount;
+  LaunchCfg best = cfg;
+  double best_t = 1e300;
+  const int groups[] = {4, 8, 16, 32, 64};
+  const int mults[] = {2, 4, 8, 16, 32};
+  // group size first with the current grids, then the grids with the best group
+  for (int pass = 0; pass < 2; ++pass) {
+    for (int i = 0; i < 5; ++i) {
+      LaunchCfg trial = best;
+      if (pass == 0) trial.group = groups[i];
+      else trial.blocks_op = trial.blocks_node = cus * mults[i];
+      cfg = trial;
+      invalidate_graphs();
+      rc = synchronize();
+      if (rc) return rc;
+      auto t0 = std::chrono::steady_clock::now();
+      rc = run(sweeps_per_trial, nullptr);
+      if (rc) return rc;
+      double t = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+      if (t < best_t) { best_t = t; best = trial; }
+    }
+  }
+  cfg = best;
+  invalidate_graphs();
+  return LG_OK;
+}
+
+int Batch::synchronize() {
+  LG_HIP(hipStreamSynchronize(stream));
+  return LG_OK;
+}
+
+// ---- configuration transfer (single-chain helpers) ------------------------
+// Builds the per-site world-line arrays of a configuration on the host:
+// events ordered by (time, bond, rank), segment offsets max(m,1), event times
+// and the spin below each event.
+void host_site_arrays(const ChainDev& d, const std::vector<int>& spins, const std::vector<int>& bo,
+                      const std::vector<double>& ot, const std::vector<int>& offd,
+                      std::vector<int>& soff, std::vector<double>& evt, std::vector<uint8_t>& segspin) {
+  const int N = d.N;
+  std::vector<std::vector<int>> ev(N);
+  for (int b = 0; b < d.Nb; ++b) {
+    int s0, s1;
+    bond_sites(d, b, s0, s1);
+    for (int k = bo[b]; k < bo[b + 1]; ++k) { ev[s0].push_back(k); ev[s1].push_back(k); }
+  }
+  std::vector<int> bond_of(ot.size());
+  for (int b = 0; b < d.Nb; ++b) for (int k = bo[b]; k < bo[b + 1]; ++k) bond_of[k] = b;
+  soff.assign(N + 1, 0);
+  for (int s = 0; s < N; ++s) {
+    std::sort(ev[s].begin(), ev[s].end(), [&](int a, int b) {
+      if (ot[a] != ot[b]) return ot[a] < ot[b];
+      if (bond_of[a] != bond_of[b]) return bond_of[a] < bond_of[b];
+      return a < b;
+    });
+    soff[s + 1] = soff[s] + std::max<int>((int)ev[s].size(), 1);
+  }
+  evt.assign(soff[N], EMPTY_TIME);
+  segspin.assign(soff[N], 0);
+  for (int s = 0; s < N; ++s) {
+    int c = spins[s];
+    if (ev[s].empty()) { segspin[soff[s]] = (uint8_t)c; continue; }
+    for (size_t j = 0; j < ev[s].size(); ++j) {
+      int k = ev[s][j];
+      evt[soff[s] + j] = ot[k];
+      segspin[soff[s] + j] = (uint8_t)c;
+      c ^= offd[k];
+    }
+  }
+}
+
+int validate_config(const ChainDev& d, const int32_t* bond_offsets, const double* times,
+                    const int32_t* types, long long& nops) {
+  if (bond_offsets[0] != 0) return set_error(LG_ERR_ARG, "bond_offsets[0] must be 0");
+  for (int b = 0; b < d.Nb; ++b)
+    if (bond_offsets[b + 1] < bond_offsets[b]) return set_error(LG_ERR_ARG, "bond_offsets not monotone");
+  nops = bond_offsets[d.Nb];
+  for (int b = 0; b < d.Nb; ++b)
+    for (int k = bond_offsets[b]; k < bond_offsets[b + 1]; ++k) {
+      if (!(times[k] >= 0 && times[k] < 1)) return set_error(LG_ERR_ARG, "operator time outside [0,1)");
+      if (k > bond_offsets[b] && times[k] < times[k - 1])
+        return set_error(LG_ERR_ARG, "operator times not ascending within a bond");
+      if (types[k] < 0 || (types[k] >> 2) > 3) return set_error(LG_ERR_ARG, "invalid operator type");
+    }
+  return LG_OK;
+}
+
+int Batch::set_config(int c, const int32_t* spins_in, const int32_t* bond_offsets, const double* times,
+                      const int32_t* types, uint64_t sweep) {
+  if (chains.size() != 1) return set_error(LG_ERR_UNSUPPORTED, "set_config needs a single-chain handle");
+  const ChainDev& d = chains[c];
+  long long nops = 0;
+  int rc = validate_config(d, bond_offsets, times, types, nops);
+  if (rc) return rc;
+  if (nops > op_cap) return set_error(LG_ERR_CAPACITY, "configuration exceeds operator capacity");
+  std::vector<int> spins(spins_in, spins_in + d.N), bo(bond_offsets, bond_offsets + d.Nb + 1), offd(nops);
+  std::vector<double> ot(times, times + nops);
+  std::vector<int> oi(nops);
+  for (int b = 0; b < d.Nb; ++b)
+    for (int k = bo[b]; k < bo[b + 1]; ++k) {
+      offd[k] = types[k] & 1;
+      oi[k] = ((d.bond_base + b) << 3) | ((types[k] >> 2) << 1) | (types[k] & 1);
+    }
+  std::vector<int> soff;
+  std::vector<double> evt;
+  std::vector<uint8_t> segspin;
+  host_site_arrays(d, spins, bo, ot, offd, soff, evt, segspin);
+  if ((long long)soff[d.N] > node_cap) return set_error(LG_ERR_CAPACITY, "configuration exceeds segment capacity");
+  std::vector<uint8_t> sp(d.N);
+  for (int s = 0; s < d.N; ++s) sp[s] = (uint8_t)(spins[s] & 1);
+  State& s = st[cur];
+  LG_HIP(hipMemcpy(s.spins, sp.data(), d.N, hipMemcpyHostToDevice));
+  LG_HIP(hipMemcpy(s.bo, bo.data(), (d.Nb + 1) * sizeof(int), hipMemcpyHostToDevice));
+  if (nops) {
+    LG_HIP(hipMemcpy(s.ot, ot.data(), nops * sizeof(double), hipMemcpyHostToDevice));
+    LG_HIP(hipMemcpy(s.oi, oi.data(), nops * sizeof(int), hipMemcpyHostToDevice));
+  }
+  LG_HIP(hipMemcpy(s.soff, soff.data(), (d.N + 1) * sizeof(int), hipMemcpyHostToDevice));
+  LG_HIP(hipMemcpy(s.evt, evt.data(), evt.size() * sizeof(double), hipMemcpyHostToDevice));
+  LG_HIP(hipMemcpy(s.segspin, segspin.data(), segspin.size(), hipMemcpyHostToDevice));
+  uint32_t sw = (uint32_t)sweep;
+  LG_HIP(hipMemcpy(d_sweep, &sw, sizeof(uint32_t), hipMemcpyHostToDevice));
+  host_sweep[c] = sweep;
+  return LG_OK;
+}
+
+int Batch::get_config(int c, int32_t* spins, int32_t* bond_offsets, double* times, int32_t* types) {
+  if (chains.size() != 1) return set_error(LG_ERR_UNSUPPORTED, "get_config needs a single-chain handle");
+  const ChainDev& d = chains[c];
+  const State& s = st[cur];
+  std::vector<uint8_t> sp(d.N);
+  std::vector<int> bo(d.Nb + 1);
+  LG_HIP(hipMemcpy(sp.data(), s.spins, d.N, hipMemcpyDeviceToHost));
+  LG_HIP(hipMemcpy(bo.data(), s.bo, (d.Nb + 1) * sizeof(int), hipMemcpyDeviceToHost));
+  int nops = bo[d.Nb] - bo[0];
+  std::vector<int> oi(nops);
+  if (spins) for (int i = 0; i < d.N; ++i) spins[i] = sp[i];
+  if (bond_offsets) for (int b = 0; b <= d.Nb; ++b) bond_offsets[b] = bo[b] - bo[0];
+  if (nops && times) LG_HIP(hipMemcpy(times, s.ot + bo[0], nops * sizeof(double), hipMemcpyDeviceToHost));
+  if (nops && types) {
+    LG_HIP(hipMemcpy(oi.data(), s.oi + bo[0], nops * sizeof(int), hipMemcpyDeviceToHost));
+    for (int k = 0; k < nops; ++k) types[k] = (oi[k] & 1) | (((oi[k] >> 1) & 3) << 2);
+  }
+  return LG_OK;
+}
+
+int Batch::num_ops(int c, long long* n) {
+  const ChainDev& d = chains[c];
+  int a = 0, b = 0;
+  LG_HIP(hipMemcpy(&a, st[cur].bo + d.bond_base, sizeof(int), hipMemcpyDeviceToHost));
+  LG_HIP(hipMemcpy(&b, st[cur].bo + d.bond_base + d.Nb, sizeof(int), hipMemcpyDeviceToHost));
+  *n = b - a;
+  return LG_OK;
+}
+
+// Cluster stage on a given configuration, written into the "new" buffers as
+// if the diagonal update had produced it.
+int Batch::cluster_only(const int32_t* spins_in, const int32_t* bond_offsets, const double* times,
+                        const int32_t* types, uint64_t sweep, int32_t* labels, int32_t* spins_out,
+                        int32_t* types_out, int64_t* n_clusters) {
+  const ChainDev& d = chains[0];
+  long long nops = 0;
+  int rc = validate_config(d, bond_offsets, times, types, nops);
+  if (rc) return rc;
+  if (nops > op_cap) return set_error(LG_ERR_CAPACITY, "configuration exceeds operator capacity");
+  std::vector<int> cnt(d.Nb), oi(nops), kc(nops);
+  for (int b = 0; b < d.Nb; ++b) {
+    cnt[b] = bond_offsets[b + 1] - bond_offsets[b];
+    int k_off = 0;
+    for (int k = bond_offsets[b]; k < bond_offsets[b + 1]; ++k) {
+      oi[k] = (b << 3) | ((types[k] >> 2) << 1) | (types[k] & 1);
+      kc[k] = k_off;
+      k_off += types[k] & 1;
+    }
+  }
+  std::vector<uint8_t> sp(d.N);
+  for (int s = 0; s < d.N; ++s) sp[s] = (uint8_t)(spins_in[s] & 1);
+  const int from = cur;
+  State& o = st[from];
+  State& w = st[1 - from];
+  LG_HIP(hipMemcpy(o.spins, sp.data(), d.N, hipMemcpyHostToDevice));
+  LG_HIP(hipMemcpy(counts, cnt.data(), d.Nb * sizeof(int), hipMemcpyHostToDevice));
+  if (nops) {
+    LG_HIP(hipMemcpy(w.ot, times, nops * sizeof(double), hipMemcpyHostToDevice));
+    LG_HIP(hipMemcpy(w.oi, oi.data(), nops * sizeof(int), hipMemcpyHostToDevice));
+    LG_HIP(hipMemcpy(kc_new, kc.data(), nops * sizeof(int), hipMemcpyHostToDevice));
+  }
+  uint32_t sw = (uint32_t)sweep;
+  LG_HIP(hipMemcpy(d_sweep, &sw, sizeof(uint32_t), hipMemcpyHostToDevice));
+  LG_HIP(hipMemset(ring_pos, 0, sizeof(int)));
+  DevBatch B = dev(from);
+  launch_site_scan(B, stream);
+  launch_cluster_stage(B, cfg, stream);
+  LG_HIP(hipGetLastError());
+  LG_HIP(hipStreamSynchronize(stream));
+  rc = check_flags();
+  if (rc) return rc;
+  const int nelem = d.N + 4 * (int)nops;
+  int* canon = nullptr;
+  int* lab = nullptr;
+  int* roots = nullptr;
+  LG_HIP(hipMalloc(&canon, sizeof(int) * std::max(node_cap, 1)));
+  LG_HIP(hipMalloc(&lab, sizeof(int) * std::max(nelem, 1)));
+  LG_HIP(hipMalloc(&roots, sizeof(int) * std::max(nelem, 1)));
+  std::vector<int> big(node_cap, 0x7fffffff);
+  hipError_t e = hipMemcpy(canon, big.data(), sizeof(int) * node_cap, hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    launch_canon(B, 0, nelem, canon, lab, roots, stream);
+    e = hipStreamSynchronize(stream);
+  }
+  std::vector<int> labv(nelem);
+  if (e == hipSuccess) e = hipMemcpy(labv.data(), lab, sizeof(int) * nelem, hipMemcpyDeviceToHost);
+  hipFree(canon);
+  hipFree(lab);
+  hipFree(roots);
+  if (e != hipSuccess) return set_error(LG_ERR_DEVICE, std::string("canonical labels: ") + hipGetErrorString(e));
+  if (labels) std::memcpy(labels, labv.data(), sizeof(int) * nelem);
+  std::vector<uint8_t> spo(d.N);
+  LG_HIP(hipMemcpy(spo.data(), w.spins, d.N, hipMemcpyDeviceToHost));
+  if (spins_out) for (int s = 0; s < d.N; ++s) spins_out[s] = spo[s];
+  if (types_out && nops) {
+    LG_HIP(hipMemcpy(oi.data(), w.oi, nops * sizeof(int), hipMemcpyDeviceToHost));
+    for (long long k = 0; k < nops; ++k) types_out[k] = (oi[k] & 1) | (((oi[k] >> 1) & 3) << 2);
+  }
+  if (n_clusters) {
+    std::vector<double> rec(R_COUNT);
+    LG_HIP(hipMemcpy(rec.data(), rec_ring, R_COUNT * sizeof(double), hipMemcpyDeviceToHost));
+    *n_clusters = (int64_t)rec[R_NC];
+  }
+  return LG_OK;
+}
+
+}  // namespace lg
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+using lg::Batch;
+using lg::set_error;
+
+struct lg_chain { Batch b; };
+struct lg_batch { Batch b; };
+
+extern "C" {
+
+int lg_last_error(char* buf, size_t len) {
+  if (!buf || len == 0) return LG_ERR_ARG;
+  std::snprintf(buf, len, "%s", lg::g_last_error.c_str());
+  return LG_OK;
+}
+
+int lg_device_count(int* n) {
+  if (!n) return set_error(LG_ERR_ARG, "null pointer");
+  int c = 0;
+  if (hipGetDeviceCount(&c) != hipSuccess) c = 0;
+  *n = c;
+  return LG_OK;
+}
+
+int lg_chain_create(const lg_params* p, lg_chain** out) {
+  if (!p || !out) return set_error(LG_ERR_ARG, "null pointer");
+  *out = nullptr;
+  lg_chain* h = new lg_chain();
+  int rc = h->b.init(p, 1);
+  if (rc) { delete h; return rc; }
+  *out = h;
+  return LG_OK;
+}
+
+int lg_chain_sweep(lg_chain* h, uint32_t n, lg_sweep_record* rec) {
+  if (!h) return set_error(LG_ERR_ARG, "null handle");
+  return h->b.run(n, rec);
+}
+
+int lg_chain_info(lg_chain* h, int32_t* n_sites, int32_t* n_bonds, int64_t* n_ops, uint64_t* sweep) {
+  if (!h) return set_error(LG_ERR_ARG, "null handle");
+  if (n_sites) *n_sites = h->b.chains[0].N;
+  if (n_bonds) *n_bonds = h->b.chains[0].Nb;
+  if (sweep) *sweep = h->b.host_sweep[0];
+  if (n_ops) {
+    long long n = 0;
+    int rc = h->b.num_ops(0, &n);
+    if (rc) return rc;
+    *n_ops = n;
+  }
+  return LG_OK;
+}
+
+int lg_chain_get_config(lg_chain* h, int32_t* spins, int32_t* bond_offsets, double* times, int32_t* types) {
+  if (!h) return set_error(LG_ERR_ARG, "null handle");
+  return h->b.get_config(0, spins, bond_offsets, times, types);
+}
+
+int lg_chain_set_config(lg_chain* h, const int32_t* spins, const int32_t* bond_offsets, const double* times,
+                        const int32_t* types, uint64_t sweep) {
+  if (!h || !spins || !bond_offsets) return set_error(LG_ERR_ARG, "null pointer");
+  return h->b.set_config(0, spins, bond_offsets, times, types, sweep);
+}
+
+int lg_chain_destroy(lg_chain* h) {
+  delete h;
+  return LG_OK;
+}
+
+int lg_cluster_config(const lg_params* p, const int32_t* spins, const int32_t* bond_offsets, const double* times,
+                      const int32_t* types, uint64_t sweep, int32_t* labels, int32_t* spins_out,
+                      int32_t* types_out, int64_t* n_clusters) {
+  if (!p || !spins || !bond_offsets) return set_error(LG_ERR_ARG, "null pointer");
+  lg_params q = *p;
+  // capacity from the configuration itself
+  lg::ChainDev d = lg::make_chain_dev(q);
+  long long nops = bond_offsets[d.Nb];
+  Batch b;
+  int rc = b.init_for_config(&q, nops);
+  if (rc) return rc;
+  return b.cluster_only(spins, bond_offsets, times, types, sweep, labels, spins_out, types_out, n_clusters);
+}
+
+int lg_batch_create(const lg_params* p, int32_t n_chains, lg_batch** out) {
+  if (!p || !out) return set_error(LG_ERR_ARG, "null pointer");
+  *out = nullptr;
+  lg_batch* h = new lg_batch();
+  int rc = h->b.init(p, n_chains);
+  if (rc) { delete h; return rc; }
+  *out = h;
+  return LG_OK;
+}
+
+int lg_batch_run(lg_batch* h, uint32_t n, lg_sweep_record* rec) {
+  if (!h) return set_error(LG_ERR_ARG, "null handle");
+  return h->b.run(n, rec);
+}
+
+int lg_batch_set_graph(lg_batch* h, int enable) {
+  if (!h) return set_error(LG_ERR_ARG, "null handle");
+  h->b.use_graph = enable != 0;
+  return LG_OK;
+}
+
+int lg_batch_tune(lg_batch* h, uint32_t sweeps_per_trial) {
+  if (!h) return set_error(LG_ERR_ARG, "null handle");
+  return h->b.tune(sweeps_per_trial);
+}
+
+int lg_batch_chain_info(lg_batch* h, int32_t chain, int64_t* n_ops, uint64_t* sweep) {
+  if (!h) return set_error(LG_ERR_ARG, "null handle");
+  if (chain < 0 || chain >= (int)h->b.chains.size()) return set_error(LG_ERR_ARG, "chain index out of range");
+  if (sweep) *sweep = h->b.host_sweep[chain];
+  if (n_ops) {
+    long long n = 0;
+    int rc = h->b.num_ops(chain, &n);
+    if (rc) return rc;
+    *n_ops = n;
+  }
+  return LG_OK;
+}
+
+int lg_batch_destroy(lg_batch* h) {
+  delete h;
+  return LG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,186 @@
+// Device/host shared definitions for the gfx950 loop-update kernels.
+//
+// Layout and conventions (shared by every kernel):
+//  * A batch holds independent chains.  Their sites, bonds, operators and
+//    world-line segments live in global arrays; chain c owns contiguous
+//    ranges [site_base, site_base + N), [bond_base, bond_base + Nb), and the
+//    operators / segments that the per-sweep scans place there.
+//  * Operators are kept in bond-major order (bond ascending, imaginary time
+//    ascending) -- the operator string of path_integral.C:57-195 stored as
+//    CSR by bond.  An operator is (time f64, info i32) with
+//    info = bond << 3 | graph << 1 | offdiagonal  (cf. operator.h:52-117).
+//  * Imaginary time tau lives on the grid k * 2^-52, k in [0, 2^52), and is
+//    stored as an exact FP64 value.
+//  * Union-find nodes are world-line segments: for site s with m events
+//    (operator legs touching s, ordered by (tau, bond, rank)), node soff[s]+j
+//    is the segment just below event j; node soff[s] is the wrapping segment
+//    that also holds the tau = 0 boundary.  A site without events owns one
+//    node.  Roots are the minimum node index of each cluster.
+#pragma once
+
+#include <stdint.h>
+
+#ifdef __HIPCC__
+#define LG_HD __host__ __device__ __forceinline__
+#else
+#define LG_HD inline
+#endif
+
+namespace lg {
+
+constexpr uint64_t TGRID = 1ull << 52;
+constexpr double TGRID_INV = 1.0 / 4503599627370496.0;
+constexpr double EMPTY_TIME = 2.0;  // ev_time of the single node of an event-free site
+
+enum : uint32_t { STAGE_DIAG = 1, STAGE_REGRAPH = 2, STAGE_FLIP = 3 };
+
+// ev_flags bits
+enum : uint8_t { EV_OFFDIAG = 1, EV_SRC = 2, EV_DIR1 = 4, EV_EMPTY = 8 };
+
+// Per-chain descriptor (device resident, rebuilt by the host when the set of
+// chains in a batch changes).
+struct ChainDev {
+  int kind;            // 0 chain, 1 square
+  int L, N, Nb, dim;
+  int site_base, bond_base;          // layout of the configuration being built
+  int old_site_base, old_bond_base;  // layout of the previous configuration
+  int cand_cap;                      // candidate slots per bond
+  long long cand_base;               // first candidate slot of bond 0
+  double beta, scale;                // scale = 2^52 / (beta * sum_g v_g)
+  double offset_total;               // sum_b offset_b (energy.h / weight_impl.h)
+  uint64_t thr[3];                   // graph thresholds: g = first g with x < thr[g], else 3
+  uint64_t toff;                     // off-diagonal regraph threshold
+  uint32_t key0, key1;               // Philox key = (seed, chain id)
+  int has_weight;
+};
+
+// Philox4x32-10 (Salmon et al., SC'11): 10 rounds, multipliers D2511F53 /
+// CD9E8D57, Weyl increments 9E3779B9 / BB67AE85.
+struct U4 { uint32_t x, y, z, w; };
+
+LG_HD U4 philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0, uint32_t k1) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+    uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+    uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+    uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+    c1 = (uint32_t)p1;
+    c3 = (uint32_t)p0;
+    c0 = n0;
+    c2 = n2;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+  return U4{c0, c1, c2, c3};
+}
+
+// Natural logarithm from IEEE +,-,*,/ only (no contraction), identical on
+// host and device: log x = e ln2 + 2 atanh(s), s = (m-1)/(m+1),
+// m in (sqrt(1/2), sqrt(2)], atanh by its Taylor series.
+LG_HD double det_log(double x) {
+#ifdef __HIPCC__
+#pragma clang fp contract(off)
+#endif
+  uint64_t bits = *reinterpret_cast<uint64_t*>(&x);
+  int e = (int)((bits >> 52) & 0x7ff) - 1023;
+  uint64_t mb = (bits & 0x000fffffffffffffull) | 0x3ff0000000000000ull;
+  double m = *reinterpret_cast<double*>(&mb);
+  if (m > 1.4142135623730951) { m = m * 0.5; e += 1; }
+  double f = m - 1.0;
+  double s = f / (2.0 + f);
+  double z = s * s;
+  double p = 1.0 / 25.0;
+  p = p * z + 1.0 / 23.0;
+  p = p * z + 1.0 / 21.0;
+  p = p * z + 1.0 / 19.0;
+  p = p * z + 1.0 / 17.0;
+  p = p * z + 1.0 / 15.0;
+  p = p * z + 1.0 / 13.0;
+  p = p * z + 1.0 / 11.0;
+  p = p * z + 1.0 / 9.0;
+  p = p * z + 1.0 / 7.0;
+  p = p * z + 1.0 / 5.0;
+  p = p * z + 1.0 / 3.0;
+  double logm = 2.0 * s + 2.0 * s * (z * p);
+  return (double)e * 0.69314718055994530942 + logm;
+}
+
+// Exponential gap on the 2^-52 grid for rate lambda (scale = 2^52/lambda),
+// from the first 53 bits of a Philox output; saturates at 2^52.
+LG_HD uint64_t exp_gap(const U4& r, double scale) {
+#ifdef __HIPCC__
+#pragma clang fp contract(off)
+#endif
+  uint64_t u53 = ((uint64_t)r.x << 21) | (uint64_t)(r.y >> 11);
+  double u = (double)(u53 + 1) * (1.0 / 9007199254740992.0);
+  double g = -det_log(u) * scale;
+  if (!(g < 4503599627370496.0)) return TGRID;
+  return (uint64_t)g;
+}
+
+// graph g has probability v_g / sum v; thr[g] = floor(2^32 (v_0+..+v_g)/sum v),
+// and 2^32 from the last graph of nonzero weight on
+LG_HD int choose_graph(uint32_t x, const ChainDev& c) {
+  int g = 0;
+  while (g < 3 && (uint64_t)x >= c.thr[g]) ++g;
+  return g;
+}
+
+// ---- lattice geometry (ALPS conventions) ----------------------------------
+// chain: bond b = (b, b+1 mod L); site s has bonds {s (source), s-1 (target)}
+// square: site s = x*L + y; bond 2s = (s, s+y^) [dir 1], 2s+1 = (s, s+x^) [dir 0];
+//         site s has bonds {2s, 2s+1 (source), 2(s-y^), 2(s-x^)+1 (target)}
+LG_HD void bond_sites(const ChainDev& c, int b, int& s0, int& s1) {
+  if (c.kind == 0) {
+    s0 = b;
+    s1 = (b + 1 == c.L) ? 0 : b + 1;
+  } else {
+    int s = b >> 1, x = s / c.L, y = s - x * c.L;
+    s0 = s;
+    if ((b & 1) == 0) s1 = x * c.L + ((y + 1 == c.L) ? 0 : y + 1);
+    else s1 = ((x + 1 == c.L) ? 0 : x + 1) * c.L + y;
+  }
+}
+
+LG_HD int bond_dir(const ChainDev& c, int b) { return (c.kind == 0) ? 0 : ((b & 1) ? 0 : 1); }
+
+// bonds incident to site s; returns their count (2 or 4)
+LG_HD int site_bonds(const ChainDev& c, int s, int* bl) {
+  if (c.kind == 0) {
+    bl[0] = s;
+    bl[1] = (s == 0) ? c.L - 1 : s - 1;
+    return 2;
+  }
+  int x = s / c.L, y = s - x * c.L;
+  int ym = (y == 0) ? c.L - 1 : y - 1;
+  int xm = (x == 0) ? c.L - 1 : x - 1;
+  bl[0] = 2 * s;
+  bl[1] = 2 * s + 1;
+  bl[2] = 2 * (x * c.L + ym);
+  bl[3] = 2 * (xm * c.L + y) + 1;
+  return 4;
+}
+
+LG_HD int gauge(const ChainDev& c, int s) {
+  if (c.kind == 0) return (s & 1) ? -1 : 1;
+  int x = s / c.L, y = s - x * c.L;
+  return ((x + y) & 1) ? -1 : 1;
+}
+
+// Sweep record layout (identical to lg_sweep_record in include/loopgpu.h).
+enum {
+  R_NOP = 0, R_NC, R_ENERGY,
+  R_UMAG0, R_USIZE2, R_UMAG2, R_USIZE4, R_UMAG4, R_USIZE, R_UMAG,
+  R_SMAG0, R_SSIZE2, R_SMAG2, R_SSIZE4, R_SMAG4, R_SSIZE, R_SMAG,
+  R_W2, R_BETA, R_SWEEP, R_COUNT
+};
+
+// Per-cluster accumulator, one 64-byte record per root node.
+struct ClusterAcc {
+  double len, lmag, slen, smag;   // sum of segment lengths: plain, spin-, gauge-, gauge*spin-weighted
+  int n0, m0, sn0, sm0;           // tau = 0 boundary: sites, sum sigma, sum gauge, sum gauge*sigma
+  int w0, w1, pad0, pad1;         // winding numbers (x, y)
+};
+
+}  // namespace lg

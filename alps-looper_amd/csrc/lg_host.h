@@ -1,0 +1,101 @@
+// Host-side batch of chains resident in device memory.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/loopgpu.h"
+#include "lg_common.h"
+#include "lg_kernels.h"
+
+namespace lg {
+
+int set_error(int code, const std::string& msg);
+int check_params(const lg_params& p);
+ChainDev make_chain_dev(const lg_params& p);
+
+// One configuration: tau = 0 spins, operators (bond CSR) and the world-line
+// segment arrays built from them.
+struct State {
+  uint8_t* spins = nullptr;
+  int* bo = nullptr;
+  double* ot = nullptr;
+  int* oi = nullptr;
+  int* soff = nullptr;
+  double* evt = nullptr;
+  uint8_t* segspin = nullptr;
+};
+
+class Batch {
+ public:
+  ~Batch();
+  int init(const lg_params* p, int n);
+  int init_for_config(const lg_params* p, long long nops);
+  int run(uint32_t n, lg_sweep_record* out);
+  int tune(uint32_t sweeps_per_trial);
+  int synchronize();
+  int set_config(int c, const int32_t* spins, const int32_t* bond_offsets, const double* times,
+                 const int32_t* types, uint64_t sweep);
+  int get_config(int c, int32_t* spins, int32_t* bond_offsets, double* times, int32_t* types);
+  int num_ops(int c, long long* n);
+  int cluster_only(const int32_t* spins, const int32_t* bond_offsets, const double* times,
+                   const int32_t* types, uint64_t sweep, int32_t* labels, int32_t* spins_out,
+                   int32_t* types_out, int64_t* n_clusters);
+  DevBatch dev(int from) const;
+  int enqueue_sweep();
+
+  std::vector<lg_params> params;
+  std::vector<ChainDev> chains;
+  std::vector<uint64_t> host_sweep;
+  int nsites = 0, nbonds = 0;
+  long long cand_total = 0;
+  int op_cap = 0, node_cap = 0;
+  long long min_op_cap = 0;
+  State st[2];
+  int cur = 0;                 // st[cur] holds the latest configuration
+  LaunchCfg cfg;
+  bool use_graph = false;
+  hipStream_t stream = nullptr;
+
+ private:
+  template <class T>
+  int alloc(T*& ptr, size_t n);
+  int allocate();
+  int reset_state();
+  void release();
+  void invalidate_graphs();
+  int capture(int from);
+  int check_flags();
+
+  std::vector<void*> allocations;
+  hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
+  ChainDev* d_chains = nullptr;
+  int* bond_chain = nullptr;
+  int* site_chain = nullptr;
+  int* counts = nullptr;
+  int* acc_count = nullptr;
+  int* kpre = nullptr;
+  double* cand_t = nullptr;
+  uint8_t* cand_g = nullptr;
+  int* kc_new = nullptr;
+  int* l0 = nullptr;
+  int* l1 = nullptr;
+  uint8_t* evflags = nullptr;
+  int* node_site = nullptr;
+  int* parent = nullptr;
+  uint8_t* fbit = nullptr;
+  ClusterAcc* acc = nullptr;
+  int* scan_part = nullptr;
+  int scan_nparts = 0;
+  int* totals = nullptr;
+  int* flags = nullptr;
+  uint32_t* d_sweep = nullptr;
+  double* rec_acc = nullptr;
+  double* rec_ring = nullptr;
+  int* ring_pos = nullptr;
+  int ring_size = 0;
+};
+
+}  // namespace lg

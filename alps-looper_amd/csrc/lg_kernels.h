@@ -1,0 +1,75 @@
+// Device buffer bundle and launch interface of the sweep kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "lg_common.h"
+
+namespace lg {
+
+enum : int { FLAG_CAND_OVERFLOW = 1, FLAG_OP_OVERFLOW = 2, FLAG_NODE_OVERFLOW = 4, FLAG_RING_OVERFLOW = 8 };
+
+// Pointers of one sweep: "old" = previous configuration (read), "new" =
+// configuration being built (written).  Passed by value to every kernel.
+struct DevBatch {
+  int nchains;
+  int nsites, nbonds;          // totals of the new layout
+  const ChainDev* chains;
+  const int* bond_chain;       // [nbonds]
+  const int* site_chain;       // [nsites]
+  // previous configuration
+  const uint8_t* spins_old;    // tau = 0 spins, old site index
+  const int* bo_old;           // bond offsets, old bond index
+  const double* ot_old;
+  const int* oi_old;
+  const int* soff_old;         // segment offsets, old site index
+  const double* evt_old;       // event time per segment node (EMPTY_TIME for event-free sites)
+  const uint8_t* segspin_old;  // post-flip spin per segment node
+  // new configuration
+  uint8_t* spins_new;
+  int* bo_new;
+  double* ot_new;
+  int* oi_new;
+  int* soff_new;
+  double* evt_new;
+  uint8_t* segspin_new;
+  // per-sweep scratch
+  int* counts;                 // [nbonds] new operators per bond
+  int* acc_count;              // [nbonds] accepted candidates per bond
+  int* kpre;                   // [old ops] off-diagonal operators before, within the bond
+  double* cand_t;              // candidate slots
+  uint8_t* cand_g;
+  int* kc_new;                 // [new ops] off-diagonal operators before, within the bond
+  int* l0;                     // [new ops] lower-leg node on the source site
+  int* l1;                     // [new ops] lower-leg node on the target site
+  uint8_t* evflags;            // [nodes]
+  int* node_site;              // [nodes]
+  int* parent;                 // [nodes]
+  uint8_t* fbit;               // [nodes] flip bit of the node's cluster
+  ClusterAcc* acc;             // [nodes] (roots only)
+  int* scan_part;
+  int* totals;                 // [0] operators, [1] nodes of the new configuration
+  int* flags;
+  uint32_t* sweep;             // [nchains]
+  double* rec_acc;             // [nchains][R_COUNT]
+  double* rec_ring;            // [ring][nchains][R_COUNT]
+  int* ring_pos;
+  int ring_size;
+  int op_cap, node_cap;
+};
+
+struct LaunchCfg {
+  int group = 64;              // lanes per bond in the diagonal update
+  int max_blocks_bond = 1 << 20;
+  int blocks_op = 1024;        // grid of the per-operator kernels (grid-stride)
+  int blocks_node = 1024;      // grid of the per-node kernels (grid-stride)
+};
+
+void launch_sweep(const DevBatch& B, const LaunchCfg& L, hipStream_t st);
+void launch_cluster_stage(const DevBatch& B, const LaunchCfg& L, hipStream_t st);
+void launch_site_scan(const DevBatch& B, hipStream_t st);
+void launch_canon(const DevBatch& B, int chain, int nelem, int* canon, int* labels, int* roots,
+                  hipStream_t st);
+int scan_blocks(const DevBatch& B);
+
+}  // namespace lg

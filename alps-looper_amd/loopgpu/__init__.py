@@ -1,0 +1,201 @@
+"""ctypes binding of libloopgpu.so (the C-ABI declared in include/loopgpu.h).
+
+The library is the product: everything here only marshals arguments.  If the
+shared object has not been built, importing this module raises -- there is no
+fallback path.
+"""
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "libloopgpu.so")
+
+CHAIN = 0
+SQUARE = 1
+
+REC_FIELDS = [
+    "n_op", "n_clusters", "energy", "umag0", "usize2", "umag2", "usize4", "umag4", "usize", "umag",
+    "smag0", "ssize2", "smag2", "ssize4", "smag4", "ssize", "smag", "w2", "beta", "sweep",
+]
+REC_SIZE = len(REC_FIELDS)
+
+EXPORTED = [
+    "lg_last_error", "lg_device_count",
+    "lg_chain_create", "lg_chain_sweep", "lg_chain_info", "lg_chain_get_config",
+    "lg_chain_set_config", "lg_chain_destroy", "lg_cluster_config",
+    "lg_batch_create", "lg_batch_run", "lg_batch_tune", "lg_batch_set_graph",
+    "lg_batch_chain_info", "lg_batch_destroy",
+]
+
+
+class LGParams(ctypes.Structure):
+    _fields_ = [
+        ("lattice", ctypes.c_int32), ("L", ctypes.c_int32), ("beta", ctypes.c_double),
+        ("jxy", ctypes.c_double), ("jz", ctypes.c_double), ("force_scatter", ctypes.c_double),
+        ("seed", ctypes.c_uint32), ("chain_id", ctypes.c_uint32),
+    ]
+
+
+class LoopGPUError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("loopgpu error %d: %s" % (code, msg))
+        self.code = code
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError("libloopgpu.so not built at %s (run __graft_entry__.build())" % LIB_PATH)
+    lib = ctypes.CDLL(LIB_PATH)
+    P = ctypes.POINTER
+    vp = ctypes.c_void_p
+    lib.lg_last_error.argtypes = [ctypes.c_char_p, ctypes.c_size_t]
+    lib.lg_device_count.argtypes = [P(ctypes.c_int)]
+    lib.lg_chain_create.argtypes = [P(LGParams), P(vp)]
+    lib.lg_chain_sweep.argtypes = [vp, ctypes.c_uint32, vp]
+    lib.lg_chain_info.argtypes = [vp, P(ctypes.c_int32), P(ctypes.c_int32), P(ctypes.c_int64), P(ctypes.c_uint64)]
+    lib.lg_chain_get_config.argtypes = [vp, vp, vp, vp, vp]
+    lib.lg_chain_set_config.argtypes = [vp, vp, vp, vp, vp, ctypes.c_uint64]
+    lib.lg_chain_destroy.argtypes = [vp]
+    lib.lg_cluster_config.argtypes = [P(LGParams), vp, vp, vp, vp, ctypes.c_uint64, vp, vp, vp, P(ctypes.c_int64)]
+    lib.lg_batch_create.argtypes = [P(LGParams), ctypes.c_int32, P(vp)]
+    lib.lg_batch_run.argtypes = [vp, ctypes.c_uint32, vp]
+    lib.lg_batch_tune.argtypes = [vp, ctypes.c_uint32]
+    lib.lg_batch_set_graph.argtypes = [vp, ctypes.c_int]
+    lib.lg_batch_chain_info.argtypes = [vp, ctypes.c_int32, P(ctypes.c_int64), P(ctypes.c_uint64)]
+    lib.lg_batch_destroy.argtypes = [vp]
+    return lib
+
+
+lib = _load()
+
+
+def _check(rc):
+    if rc != 0:
+        buf = ctypes.create_string_buffer(1024)
+        lib.lg_last_error(buf, 1024)
+        raise LoopGPUError(rc, buf.value.decode(errors="replace"))
+
+
+def device_count():
+    n = ctypes.c_int(0)
+    _check(lib.lg_device_count(ctypes.byref(n)))
+    return n.value
+
+
+def params(lattice, L, beta, jxy=1.0, jz=None, seed=23125, chain_id=0, force_scatter=0.0):
+    if jz is None:
+        jz = jxy
+    return LGParams(int(lattice), int(L), float(beta), float(jxy), float(jz), float(force_scatter),
+                    int(seed), int(chain_id))
+
+
+def lattice_size(lattice, L):
+    return (L, L) if lattice == CHAIN else (L * L, 2 * L * L)
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+class Chain:
+    """One chain on the plain one-chain-per-launch path."""
+
+    def __init__(self, p):
+        self.p = p
+        self.h = ctypes.c_void_p()
+        _check(lib.lg_chain_create(ctypes.byref(p), ctypes.byref(self.h)))
+        self.n_sites, self.n_bonds = lattice_size(p.lattice, p.L)
+
+    def sweep(self, n, records=True):
+        rec = np.zeros((n, REC_SIZE)) if records else None
+        _check(lib.lg_chain_sweep(self.h, n, _ptr(rec)))
+        return rec
+
+    def info(self):
+        ns, nb, no, sw = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int64(), ctypes.c_uint64()
+        _check(lib.lg_chain_info(self.h, ctypes.byref(ns), ctypes.byref(nb), ctypes.byref(no), ctypes.byref(sw)))
+        return ns.value, nb.value, no.value, sw.value
+
+    def get_config(self):
+        _, nb, nops, _ = self.info()
+        spins = np.zeros(self.n_sites, np.int32)
+        bo = np.zeros(nb + 1, np.int32)
+        t = np.zeros(max(nops, 1), np.float64)
+        ty = np.zeros(max(nops, 1), np.int32)
+        _check(lib.lg_chain_get_config(self.h, _ptr(spins), _ptr(bo), _ptr(t), _ptr(ty)))
+        return spins, bo, t[:nops], ty[:nops]
+
+    def set_config(self, spins, bo, times, types, sweep):
+        spins = np.ascontiguousarray(spins, np.int32)
+        bo = np.ascontiguousarray(bo, np.int32)
+        times = np.ascontiguousarray(times, np.float64)
+        types = np.ascontiguousarray(types, np.int32)
+        _check(lib.lg_chain_set_config(self.h, _ptr(spins), _ptr(bo), _ptr(times), _ptr(types), int(sweep)))
+
+    def close(self):
+        if self.h:
+            lib.lg_chain_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Many chains packed into one launch per sweep stage."""
+
+    def __init__(self, plist):
+        arr = (LGParams * len(plist))(*plist)
+        self.n = len(plist)
+        self.h = ctypes.c_void_p()
+        _check(lib.lg_batch_create(arr, self.n, ctypes.byref(self.h)))
+
+    def run(self, n, records=True):
+        rec = np.zeros((n, self.n, REC_SIZE)) if records else None
+        _check(lib.lg_batch_run(self.h, n, _ptr(rec)))
+        return rec
+
+    def tune(self, sweeps_per_trial=8):
+        _check(lib.lg_batch_tune(self.h, sweeps_per_trial))
+
+    def set_graph(self, enable):
+        _check(lib.lg_batch_set_graph(self.h, 1 if enable else 0))
+
+    def chain_info(self, c):
+        no, sw = ctypes.c_int64(), ctypes.c_uint64()
+        _check(lib.lg_batch_chain_info(self.h, c, ctypes.byref(no), ctypes.byref(sw)))
+        return no.value, sw.value
+
+    def close(self):
+        if self.h:
+            lib.lg_batch_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def cluster_config(p, spins, bo, times, types, sweep):
+    """Cluster stage on a given configuration -> (labels, spins_out, types_out, n_clusters)."""
+    spins = np.ascontiguousarray(spins, np.int32)
+    bo = np.ascontiguousarray(bo, np.int32)
+    times = np.ascontiguousarray(times, np.float64)
+    types = np.ascontiguousarray(types, np.int32)
+    n_sites = len(spins)
+    nops = int(bo[-1])
+    labels = np.zeros(n_sites + 4 * nops, np.int32)
+    spo = np.zeros(n_sites, np.int32)
+    tyo = np.zeros(max(nops, 1), np.int32)
+    nc = ctypes.c_int64()
+    _check(lib.lg_cluster_config(ctypes.byref(p), _ptr(spins), _ptr(bo), _ptr(times), _ptr(types),
+                                 int(sweep), _ptr(labels), _ptr(spo), _ptr(tyo), ctypes.byref(nc)))
+    return labels, spo, tyo[:nops], nc.value

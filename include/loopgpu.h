@@ -1,0 +1,129 @@
+/*
+ * loopgpu: gfx950 (MI355X) continuous-imaginary-time loop-cluster update for
+ * S=1/2 Heisenberg / XXZ models -- the C-ABI of libloopgpu.so.
+ *
+ * It replaces the hot path of ALPS/looper's "loop; path integral" worker
+ * (path_integral.C:323-864): one call to lg_chain_sweep(h, n, rec) performs n
+ * calls of loop_worker::run() (path_integral.C:323-351) and returns, per
+ * sweep, the raw collector sums that loop_worker::dispatch hands to
+ * coll.commit() (path_integral.C:846-862; energy.h:76-80, susceptibility.h:
+ * 200-256, stiffness.h:123-128).  Everything below the worker -- diagonal
+ * update and world-line linking (path_integral.C:380-588), cluster
+ * identification (looper/union_find.h:242-350), cluster flip
+ * (path_integral.C:781-825) and improved estimators (path_integral.C:651-778)
+ * -- runs on the device.
+ *
+ * Conventions: plain C types only; every function returns LG_OK (0) or a
+ * negative status and never throws; lg_last_error() gives the message of the
+ * last failure on the calling thread.  Device memory is owned by the library;
+ * caller buffers are host memory.  A handle is used from one thread at a time.
+ */
+#ifndef LOOPGPU_H
+#define LOOPGPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  LG_OK = 0,
+  LG_ERR_ARG = -1,       /* invalid argument (reference: std::invalid_argument) */
+  LG_ERR_DEVICE = -2,    /* HIP runtime failure or no device */
+  LG_ERR_CAPACITY = -3,  /* device buffer capacity exceeded */
+  LG_ERR_STATE = -4,     /* call not valid in the handle's current state */
+  LG_ERR_UNSUPPORTED = -5
+};
+
+enum { LG_LATTICE_CHAIN = 0, LG_LATTICE_SQUARE = 1 };
+
+/* Parameters of one Markov chain.  Mirrors the worker's parameters:
+ * LATTICE ("chain lattice" / "square lattice"), L, T or BETA, J or Jxy/Jz,
+ * SEED, FORCE_SCATTER (loop_config.h, lattice.h:304-307, temperature.h:43-79,
+ * weight_impl.h:165-188); the spin model is S=1/2 XXZ,
+ * H = sum_b [Jz Sz Sz + Jxy/2 (S+S- + S-S+)].
+ * chain_id selects an independent random stream for the same seed. */
+typedef struct lg_params {
+  int32_t lattice;       /* LG_LATTICE_CHAIN or LG_LATTICE_SQUARE */
+  int32_t L;             /* linear size (even, >= 2); chain: L sites, square: L*L */
+  double beta;           /* inverse temperature 1/T */
+  double jxy;            /* transverse coupling Jxy (J for the Heisenberg model) */
+  double jz;             /* longitudinal coupling Jz */
+  double force_scatter;  /* FORCE_SCATTER in [0,1] (0 = standard weights) */
+  uint32_t seed;         /* SEED */
+  uint32_t chain_id;     /* independent stream index */
+} lg_params;
+
+/* Per-sweep collector sums (all improved estimators) -- see the collector of
+ * susceptibility.h:177-198 and stiffness.h:117-121. */
+typedef struct lg_sweep_record {
+  double n_op;          /* operators after the diagonal update ("nop") */
+  double n_clusters;    /* Number of Clusters */
+  double energy;        /* energy_offset - nop/beta (path_integral.C:851) */
+  double umag0;         /* sum_c umag0_c            (tau = 0 estimates) */
+  double usize2, umag2, usize4, umag4;   /* sum_c usize0^2, umag0^2, usize0^4, umag0^4 */
+  double usize, umag;   /* sum_c usize_c^2, umag_c^2 (tau-integrated) */
+  double smag0;
+  double ssize2, smag2, ssize4, smag4;
+  double ssize, smag;
+  double w2;            /* sum_c sum_d (w_{c,d}/2)^2 */
+  double beta;
+  double sweep;         /* sweep index (mcs) of this record */
+} lg_sweep_record;
+
+typedef struct lg_chain lg_chain;   /* one chain, plain one-chain-per-launch path */
+typedef struct lg_batch lg_batch;   /* many chains packed into each launch */
+
+/* ---- library ------------------------------------------------------------ */
+int lg_last_error(char* buf, size_t len);
+int lg_device_count(int* n);
+
+/* ---- plain path: one chain per launch ------------------------------------ */
+/* reference: loop_worker::loop_worker(alps::Parameters const&)
+ * (path_integral.C:202-305); all-up spins, empty operator string. */
+int lg_chain_create(const lg_params* p, lg_chain** out);
+/* reference: n x loop_worker::run(obs) (path_integral.C:323-351); rec may be
+ * NULL, else it receives n records. */
+int lg_chain_sweep(lg_chain* h, uint32_t n, lg_sweep_record* rec);
+/* sizes of the current configuration */
+int lg_chain_info(lg_chain* h, int32_t* n_sites, int32_t* n_bonds, int64_t* n_ops, uint64_t* sweep);
+/* configuration (checkpoint content of path_integral.C:111-124): spins at
+ * tau=0 [n_sites] (0 = up), operators in bond-major order with bond offsets
+ * [n_bonds+1], times [n_ops] and type (offdiagonal | graph << 2, operator.h
+ * layout) [n_ops]. */
+int lg_chain_get_config(lg_chain* h, int32_t* spins, int32_t* bond_offsets, double* times,
+                        int32_t* types);
+int lg_chain_set_config(lg_chain* h, const int32_t* spins, const int32_t* bond_offsets,
+                        const double* times, const int32_t* types, uint64_t sweep);
+int lg_chain_destroy(lg_chain* h);
+
+/* Cluster stage only, on a given configuration (stages 7-15 of
+ * path_integral.C without the diagonal update): links world lines, identifies
+ * clusters, draws flip bits for sweep index `sweep` and flips.  Outputs:
+ * labels[n_sites + 4*n_ops] canonical cluster label of every element
+ * (element s = tau=0 boundary of site s, n_sites + 4k + {0,1,2,3} = legs
+ * l0,l1,u0,u1 of operator k), label = minimum element index of the cluster;
+ * spins_out / types_out the flipped configuration; n_clusters. */
+int lg_cluster_config(const lg_params* p, const int32_t* spins, const int32_t* bond_offsets,
+                      const double* times, const int32_t* types, uint64_t sweep, int32_t* labels,
+                      int32_t* spins_out, int32_t* types_out, int64_t* n_clusters);
+
+/* ---- batched form: many chains per launch --------------------------------- */
+int lg_batch_create(const lg_params* p, int32_t n_chains, lg_batch** out);
+/* n sweeps of every chain; rec receives n * n_chains records, sweep-major */
+int lg_batch_run(lg_batch* h, uint32_t n, lg_sweep_record* rec);
+/* choose lane-group and grid sizes by timing candidate settings for a few
+ * sweeps (the configuration advances by those sweeps) */
+int lg_batch_tune(lg_batch* h, uint32_t sweeps_per_trial);
+/* use a captured HIP graph for the sweep (1) or plain launches (0) */
+int lg_batch_set_graph(lg_batch* h, int enable);
+int lg_batch_chain_info(lg_batch* h, int32_t chain, int64_t* n_ops, uint64_t* sweep);
+int lg_batch_destroy(lg_batch* h);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LOOPGPU_H */

@@ -1,0 +1,223 @@
+"""ctypes access to the CPU oracle (oracle/liboracle.so) -- test infrastructure."""
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+LIB = os.path.join(ORACLE_DIR, "liboracle.so")
+
+
+def build_oracle():
+    subprocess.run(["make", "-s", "-C", ORACLE_DIR, "liboracle.so"], check=True)
+
+
+if not os.path.exists(LIB):
+    build_oracle()
+
+lib = ctypes.CDLL(LIB)
+vp = ctypes.c_void_p
+D = ctypes.c_double
+I = ctypes.c_int
+U = ctypes.c_uint32
+lib.or_record_size.restype = I
+lib.or_philox4x32_10.argtypes = [vp, vp, vp]
+lib.or_det_log.argtypes = [vp, vp, I]
+lib.or_exp_gap.argtypes = [vp, D, vp]
+lib.or_xxz_weights.argtypes = [D, D, D, vp, vp, vp]
+lib.or_thresholds.argtypes = [D, D, D, vp, vp]
+lib.or_lattice_info.argtypes = [I, I, vp, vp, vp]
+lib.or_lattice_bonds.argtypes = [I, I, vp, vp, vp]
+lib.or_uf_pairs.argtypes = [I, I, vp, I, vp, vp, vp]
+lib.or_looper_create.restype = vp
+lib.or_looper_create.argtypes = [I, I, D, D, D, D, U]
+lib.or_looper_sweep.argtypes = [vp, I, vp]
+lib.or_looper_destroy.argtypes = [vp]
+lib.or_standalone_run.argtypes = [U, D, U, U, vp, vp]
+lib.or_bfs_labels.argtypes = [I, I, I, vp, vp, vp, vp]
+lib.or_looper_labels.argtypes = [I, I, I, vp, vp, vp, I, vp]
+lib.or_segment_of_elements.argtypes = [I, I, I, vp, vp, vp]
+lib.or_segment_of_elements.restype = I
+lib.or_philox_create.restype = vp
+lib.or_philox_create.argtypes = [I, I, D, D, D, D, U, U]
+lib.or_philox_sweep.argtypes = [vp, I, vp]
+lib.or_philox_num_ops.argtypes = [vp]
+lib.or_philox_num_nodes.argtypes = [vp]
+lib.or_philox_state.argtypes = [vp, vp, vp, vp, vp]
+lib.or_philox_roots.argtypes = [vp, vp]
+lib.or_philox_load.argtypes = [vp, vp, vp, vp, vp, U]
+lib.or_philox_destroy.argtypes = [vp]
+
+R = lib.or_record_size()
+CHAIN, SQUARE = 0, 1
+STAGE_DIAG, STAGE_REGRAPH, STAGE_FLIP = 1, 2, 3
+
+
+def _p(a):
+    return a.ctypes.data
+
+
+def philox(ctr, key):
+    c = np.asarray(ctr, np.uint32)
+    k = np.asarray(key, np.uint32)
+    out = np.zeros(4, np.uint32)
+    lib.or_philox4x32_10(_p(c), _p(k), _p(out))
+    return out
+
+
+def det_log(x):
+    x = np.ascontiguousarray(x, np.float64)
+    y = np.zeros_like(x)
+    lib.or_det_log(_p(x), _p(y), len(x))
+    return y
+
+
+def lattice(kind, L):
+    N, Nb, dim = I(), I(), I()
+    lib.or_lattice_info(kind, L, ctypes.byref(N), ctypes.byref(Nb), ctypes.byref(dim))
+    src = np.zeros(Nb.value, np.int32)
+    tgt = np.zeros(Nb.value, np.int32)
+    gauge = np.zeros(N.value, np.int32)
+    lib.or_lattice_bonds(kind, L, _p(src), _p(tgt), _p(gauge))
+    return N.value, Nb.value, src, tgt, gauge
+
+
+def xxz_weights(jxy, jz, fs=0.0):
+    v = np.zeros(4)
+    off = D()
+    sign = I()
+    lib.or_xxz_weights(jxy, jz, fs, _p(v), ctypes.byref(off), ctypes.byref(sign))
+    return v, off.value, sign.value
+
+
+class LooperChain:
+    def __init__(self, kind, L, beta, jxy=1.0, jz=None, seed=23125, fs=0.0):
+        jz = jxy if jz is None else jz
+        self.h = lib.or_looper_create(kind, L, beta, jxy, jz, fs, seed)
+
+    def sweep(self, n):
+        rec = np.zeros((n, R))
+        lib.or_looper_sweep(self.h, n, _p(rec))
+        return rec
+
+    def __del__(self):
+        lib.or_looper_destroy(self.h)
+
+
+class PhiloxChain:
+    def __init__(self, kind, L, beta, jxy=1.0, jz=None, seed=23125, chain_id=0, fs=0.0):
+        jz = jxy if jz is None else jz
+        self.kind, self.L = kind, L
+        self.N, self.Nb, _, _, _ = lattice(kind, L)
+        self.h = lib.or_philox_create(kind, L, beta, jxy, jz, fs, seed, chain_id)
+
+    def sweep(self, n):
+        rec = np.zeros((n, R))
+        lib.or_philox_sweep(self.h, n, _p(rec))
+        return rec
+
+    def state(self):
+        n = lib.or_philox_num_ops(self.h)
+        spins = np.zeros(self.N, np.int32)
+        bo = np.zeros(self.Nb + 1, np.int32)
+        t = np.zeros(max(n, 1))
+        ty = np.zeros(max(n, 1), np.int32)
+        lib.or_philox_state(self.h, _p(spins), _p(bo), _p(t), _p(ty))
+        return spins, bo, t[:n], ty[:n]
+
+    def roots(self):
+        m = lib.or_philox_num_nodes(self.h)
+        r = np.zeros(max(m, 1), np.int32)
+        lib.or_philox_roots(self.h, _p(r))
+        return r[:m]
+
+    def load(self, spins, bo, times, types, sweep):
+        spins = np.ascontiguousarray(spins, np.int32)
+        bo = np.ascontiguousarray(bo, np.int32)
+        times = np.ascontiguousarray(times, np.float64)
+        types = np.ascontiguousarray(types, np.int32)
+        lib.or_philox_load(self.h, _p(spins), _p(bo), _p(times), _p(types), sweep)
+
+    def __del__(self):
+        lib.or_philox_destroy(self.h)
+
+
+def bond_of_ops(bo):
+    return np.repeat(np.arange(len(bo) - 1, dtype=np.int32), np.diff(bo))
+
+
+def bfs_labels(kind, L, bo, times, types):
+    bond = bond_of_ops(bo)
+    N = lattice(kind, L)[0]
+    n = len(times)
+    lab = np.zeros(N + 4 * n, np.int32)
+    t = np.ascontiguousarray(times, np.float64)
+    ty = np.ascontiguousarray(types, np.int32)
+    lib.or_bfs_labels(kind, L, n, _p(bond), _p(t), _p(ty), _p(lab))
+    return lab
+
+
+def looper_labels(kind, L, bo, times, types, compress=0):
+    bond = bond_of_ops(bo)
+    N = lattice(kind, L)[0]
+    n = len(times)
+    lab = np.zeros(N + 4 * n, np.int32)
+    t = np.ascontiguousarray(times, np.float64)
+    ty = np.ascontiguousarray(types, np.int32)
+    lib.or_looper_labels(kind, L, n, _p(bond), _p(t), _p(ty), compress, _p(lab))
+    return lab
+
+
+def segment_of_elements(kind, L, bo, times):
+    bond = bond_of_ops(bo)
+    N = lattice(kind, L)[0]
+    n = len(times)
+    seg = np.zeros(N + 4 * n, np.int32)
+    t = np.ascontiguousarray(times, np.float64)
+    M = lib.or_segment_of_elements(kind, L, n, _p(bond), _p(t), _p(seg))
+    return seg, M
+
+
+def expected_flip(kind, L, seed, chain_id, sweep, spins, bo, times, types, labels):
+    """Flipped spins/types for canonical labels: flip bit of a cluster =
+    Philox(sweep, FLIP, min segment node of the cluster, 0).x & 1."""
+    N = len(spins)
+    n = len(times)
+    seg, M = segment_of_elements(kind, L, bo, times)
+    # min segment node per cluster (cluster keyed by canonical label)
+    minseg = {}
+    for e in range(N + 4 * n):
+        c = labels[e]
+        s = seg[e]
+        if c not in minseg or s < minseg[c]:
+            minseg[c] = s
+    fbit = {c: int(philox([sweep, STAGE_FLIP, s, 0], [seed, chain_id])[0] & 1) for c, s in minseg.items()}
+    spins_out = np.array([spins[s] ^ fbit[labels[s]] for s in range(N)], np.int32)
+    types_out = np.array([types[k] ^ (fbit[labels[N + 4 * k]] ^ fbit[labels[N + 4 * k + 2]]) for k in range(n)],
+                         np.int32)
+    return spins_out, types_out, len(minseg)
+
+
+def random_config(kind, L, nops, rng, graphs=(0, 1, 2, 3), time_grid=False):
+    """Random operator/cut configuration in canonical (bond, time) order.
+    Any assignment of graphs defines a valid leg graph for the partition."""
+    N, Nb, _, _, _ = lattice(kind, L)
+    bond = np.sort(rng.integers(0, Nb, size=nops)).astype(np.int32)
+    if time_grid:
+        t = rng.integers(0, 16, size=nops) / 16.0  # forces equal times on a site
+    else:
+        t = rng.integers(0, 1 << 52, size=nops, dtype=np.int64).astype(np.float64) * 2.0 ** -52
+    order = np.lexsort((t, bond))
+    bond = bond[order]
+    t = t[order]
+    g = rng.choice(np.array(graphs), size=nops)
+    off = rng.integers(0, 2, size=nops)
+    types = (off | (g << 2)).astype(np.int32)
+    bo = np.zeros(Nb + 1, np.int32)
+    np.add.at(bo, bond + 1, 1)
+    bo = np.cumsum(bo).astype(np.int32)
+    spins = rng.integers(0, 2, size=N).astype(np.int32)
+    return spins, bo, t, types

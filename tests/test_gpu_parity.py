@@ -1,0 +1,131 @@
+"""GPU parity: the HIP path against the CPU oracle, through the C-ABI.
+
+* cluster stage on given operator/cut configurations: canonical cluster labels
+  bit-exact with looper's union-find (oracle restatement of
+  looper/union_find.h + graph_impl.h reconnect, pinned by test/union_find.op),
+  flipped spins and operator types bit-exact;
+* whole sweeps: operator strings, spins, operator counts and cluster counts
+  bit-exact with the oracle's counter-based (philox) mode; FP64 estimator sums
+  within rtol 1e-9 (atomic accumulation order differs);
+* batched launches: every chain identical to the same chain run alone.
+"""
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-9
+EXACT = [0, 1]          # n_op, n_clusters
+FLOAT = list(range(2, 18))
+
+
+def _cmp_records(gpu, cpu, where=""):
+    assert gpu.shape[0] == cpu.shape[0]
+    np.testing.assert_array_equal(gpu[:, EXACT], cpu[:, EXACT], err_msg="n_op/n_clusters " + where)
+    np.testing.assert_allclose(gpu[:, FLOAT], cpu[:, FLOAT], rtol=RTOL, atol=1e-9, err_msg=where)
+
+
+CONFIGS = [
+    # kind, L, nops, graphs, time_grid
+    (O.CHAIN, 2, 0, (0,), False),
+    (O.CHAIN, 4, 7, (0, 1, 2, 3), False),
+    (O.CHAIN, 8, 40, (0,), False),
+    (O.CHAIN, 16, 300, (0, 1), False),
+    (O.CHAIN, 16, 300, (0, 1, 2, 3), True),     # many equal times
+    (O.SQUARE, 2, 30, (0, 1, 2, 3), False),
+    (O.SQUARE, 4, 200, (0, 1), False),
+    (O.SQUARE, 8, 2000, (0, 1, 2, 3), False),
+    (O.SQUARE, 16, 20000, (0,), False),
+    (O.CHAIN, 64, 5000, (0, 1, 2, 3), True),
+]
+
+
+@pytest.mark.parametrize("cfg", CONFIGS, ids=lambda c: "k%d_L%d_n%d" % (c[0], c[1], c[2]))
+def test_cluster_labels_match_looper_union_find(loopgpu, cfg):
+    kind, L, nops, graphs, grid = cfg
+    rng = np.random.default_rng(1000 + L * 7 + nops)
+    spins, bo, t, ty = O.random_config(kind, L, nops, rng, graphs, grid)
+    seed, chain_id, sweep = 4242, 3, 17
+    p = loopgpu.params(kind, L, 1.0, 1.0, 1.0, seed=seed, chain_id=chain_id)
+    print("cluster parity", cfg, flush=True)
+    labels, spo, tyo, nc = loopgpu.cluster_config(p, spins, bo, t, ty, sweep)
+    ref = O.looper_labels(kind, L, bo, t, ty)
+    np.testing.assert_array_equal(labels, ref)
+    exp_sp, exp_ty, exp_nc = O.expected_flip(kind, L, seed, chain_id, sweep, spins, bo, t, ty, ref)
+    np.testing.assert_array_equal(spo, exp_sp)
+    np.testing.assert_array_equal(tyo, exp_ty)
+    assert nc == exp_nc
+
+
+TRAJ = [
+    # kind, L, beta, jxy, jz, sweeps
+    (O.CHAIN, 16, 4.0, 1.0, 1.0, 60),
+    (O.CHAIN, 12, 10.0, 1.0, 0.6, 60),       # XY regime: cross graphs
+    (O.CHAIN, 10, 3.0, 1.0, 1.7, 60),        # Ising-like: frozen graphs
+    (O.SQUARE, 4, 2.0, 1.0, 1.0, 60),
+    (O.SQUARE, 8, 6.0, 1.0, 0.8, 40),
+    (O.CHAIN, 64, 10.0, 1.0, 1.0, 40),       # C1 shape
+]
+
+
+@pytest.mark.parametrize("cfg", TRAJ, ids=lambda c: "k%d_L%d_b%g_jxy%g_jz%g" % c[:5])
+def test_sweeps_match_oracle_bit_exact(loopgpu, cfg):
+    kind, L, beta, jxy, jz, n = cfg
+    print("trajectory", cfg, flush=True)
+    g = loopgpu.Chain(loopgpu.params(kind, L, beta, jxy, jz, seed=777, chain_id=5))
+    c = O.PhiloxChain(kind, L, beta, jxy, jz, seed=777, chain_id=5)
+    rg = g.sweep(n)
+    rc = c.sweep(n)
+    _cmp_records(rg[:, :18], rc, str(cfg))
+    gs, gbo, gt, gty = g.get_config()
+    cs, cbo, ct, cty = c.state()
+    np.testing.assert_array_equal(gs, cs)
+    np.testing.assert_array_equal(gbo, cbo)
+    np.testing.assert_array_equal(gt, ct)
+    np.testing.assert_array_equal(gty, cty)
+    assert np.all(rg[:, 19] == np.arange(n))
+
+
+def test_batch_equals_single_chains(loopgpu):
+    plist = [
+        loopgpu.params(O.CHAIN, 8, 2.0, 1.0, 1.0, seed=11, chain_id=0),
+        loopgpu.params(O.SQUARE, 4, 3.0, 1.0, 0.7, seed=11, chain_id=1),
+        loopgpu.params(O.CHAIN, 32, 8.0, 1.0, 1.0, seed=11, chain_id=2),
+        loopgpu.params(O.SQUARE, 6, 1.0, 1.0, 1.4, seed=11, chain_id=3),
+    ]
+    n = 30
+    b = loopgpu.Batch(plist)
+    rb = b.run(n)
+    for i, p in enumerate(plist):
+        print("batch chain", i, flush=True)
+        rs = loopgpu.Chain(p).sweep(n)
+        np.testing.assert_array_equal(rb[:, i, EXACT], rs[:, EXACT])
+        np.testing.assert_allclose(rb[:, i, FLOAT], rs[:, FLOAT], rtol=RTOL, atol=1e-9)
+
+
+def test_graph_replay_matches_plain_launches(loopgpu):
+    plist = [loopgpu.params(O.SQUARE, 8, 4.0, 1.0, 1.0, seed=5, chain_id=i) for i in range(3)]
+    a = loopgpu.Batch(plist)
+    b = loopgpu.Batch(plist)
+    b.set_graph(True)
+    ra = a.run(25)
+    rb = b.run(25)
+    np.testing.assert_array_equal(ra[:, :, EXACT], rb[:, :, EXACT])
+    np.testing.assert_allclose(ra[:, :, FLOAT], rb[:, :, FLOAT], rtol=RTOL, atol=1e-9)
+
+
+def test_set_config_roundtrip_and_resume(loopgpu):
+    p = loopgpu.params(O.CHAIN, 16, 5.0, 1.0, 1.0, seed=9)
+    a = loopgpu.Chain(p)
+    a.sweep(20)
+    cfg = a.get_config()
+    _, _, _, sw = a.info()
+    ra = a.sweep(15)
+    b = loopgpu.Chain(p)
+    b.set_config(*cfg, sweep=sw)
+    rb = b.sweep(15)
+    np.testing.assert_array_equal(ra[:, EXACT], rb[:, EXACT])
+    np.testing.assert_allclose(ra[:, FLOAT], rb[:, FLOAT], rtol=RTOL, atol=1e-9)
