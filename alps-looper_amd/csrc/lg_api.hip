@@ -211,6 +211,7 @@ int Batch::allocate() {
   if ((rc = alloc(acc, node_cap))) return rc;
   scan_nparts = (nbonds + 2047) / 2048 + (nsites + 2047) / 2048;
   if ((rc = alloc(scan_part, scan_nparts))) return rc;
+  if ((rc = alloc(col_part, (size_t)col_chunks(node_cap) * 16))) return rc;
   if ((rc = alloc(totals, 2))) return rc;
   if ((rc = alloc(flags, 1))) return rc;
   if ((rc = alloc(d_sweep, chains.size()))) return rc;
@@ -218,6 +219,8 @@ int Batch::allocate() {
   ring_size = (int)std::max<long long>(16, std::min<long long>(4096, (64ll << 20) / (chains.size() * R_COUNT * 8)));
   if ((rc = alloc(rec_ring, (size_t)ring_size * chains.size() * R_COUNT))) return rc;
   if ((rc = alloc(ring_pos, 1))) return rc;
+  if ((rc = alloc(end_count, 1))) return rc;
+  LG_HIP(hipMemset(end_count, 0, sizeof(int)));
   // static tables
   std::vector<int> bc(nbonds), sc(nsites);
   for (size_t c = 0; c < chains.size(); ++c) {
@@ -291,9 +294,9 @@ DevBatch Batch::dev(int from) const {
   B.cand_t = cand_t; B.cand_g = cand_g; B.kc_new = kc_new;
   B.l0 = l0; B.l1 = l1; B.evflags = evflags; B.node_site = node_site;
   B.parent = parent; B.fbit = fbit; B.acc = acc;
-  B.scan_part = scan_part; B.totals = totals; B.flags = flags;
+  B.scan_part = scan_part; B.col_part = col_part; B.totals = totals; B.flags = flags;
   B.sweep = d_sweep; B.rec_acc = rec_acc; B.rec_ring = rec_ring;
-  B.ring_pos = ring_pos; B.ring_size = ring_size;
+  B.ring_pos = ring_pos; B.end_count = end_count; B.ring_size = ring_size;
   B.op_cap = op_cap; B.node_cap = node_cap;
   return B;
 }
@@ -402,6 +405,38 @@ int Batch::tune(uint32_t sweeps_per_trial) {
   cfg = best;
   invalidate_graphs();
   return LG_OK;
+}
+
+// n sweeps with HIP events between the kernel launches on the batch's stream;
+// ms[i] = average duration of kernel i per sweep (kKernelNames order).
+int Batch::profile(uint32_t n, double* ms) {
+  int rc = check_flags();
+  if (rc) return rc;
+  const int K = NUM_SWEEP_KERNELS;
+  std::vector<hipEvent_t> ev((size_t)(K + 1) * std::max<uint32_t>(n, 1));
+  for (auto& e : ev) LG_HIP(hipEventCreate(&e));
+  for (int i = 0; i < K; ++i) ms[i] = 0;
+  uint32_t done = 0;
+  while (done < n) {
+    uint32_t chunk = std::min<uint32_t>(n - done, (uint32_t)ring_size);
+    LG_HIP(hipMemsetAsync(ring_pos, 0, sizeof(int), stream));
+    for (uint32_t i = 0; i < chunk; ++i) {
+      launch_sweep(dev(cur), cfg, stream, &ev[(size_t)(done + i) * (K + 1)]);
+      cur ^= 1;
+    }
+    LG_HIP(hipStreamSynchronize(stream));
+    for (uint32_t i = 0; i < chunk; ++i)
+      for (int k = 0; k < K; ++k) {
+        float t = 0;
+        LG_HIP(hipEventElapsedTime(&t, ev[(size_t)(done + i) * (K + 1) + k], ev[(size_t)(done + i) * (K + 1) + k + 1]));
+        ms[k] += t;
+      }
+    done += chunk;
+    for (size_t c = 0; c < chains.size(); ++c) host_sweep[c] += chunk;
+  }
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  for (int i = 0; i < K; ++i) ms[i] /= std::max<uint32_t>(n, 1);
+  return check_flags();
 }
 
 int Batch::synchronize() {
@@ -732,6 +767,22 @@ int lg_batch_chain_info(lg_batch* h, int32_t chain, int64_t* n_ops, uint64_t* sw
     *n_ops = n;
   }
   return LG_OK;
+}
+
+int lg_profile_num_kernels(void) { return lg::NUM_SWEEP_KERNELS; }
+
+const char* lg_profile_kernel_name(int i) {
+  return (i >= 0 && i < lg::NUM_SWEEP_KERNELS) ? lg::kKernelNames[i] : "";
+}
+
+int lg_chain_profile(lg_chain* h, uint32_t n, double* avg_ms) {
+  if (!h || !avg_ms) return set_error(LG_ERR_ARG, "null pointer");
+  return h->b.profile(n, avg_ms);
+}
+
+int lg_batch_profile(lg_batch* h, uint32_t n, double* avg_ms) {
+  if (!h || !avg_ms) return set_error(LG_ERR_ARG, "null pointer");
+  return h->b.profile(n, avg_ms);
 }
 
 int lg_batch_destroy(lg_batch* h) {

@@ -177,10 +177,22 @@ enum {
 };
 
 // Per-cluster accumulator, one 64-byte record per root node.
+// Segments are split into four classes k = 2*spin + (gauge < 0): 0 up/+,
+// 1 up/-, 2 down/+, 3 down/-, so each segment adds to one slot only.
 struct ClusterAcc {
-  double len, lmag, slen, smag;   // sum of segment lengths: plain, spin-, gauge-, gauge*spin-weighted
-  int n0, m0, sn0, sm0;           // tau = 0 boundary: sites, sum sigma, sum gauge, sum gauge*sigma
-  int w0, w1, pad0, pad1;         // winding numbers (x, y)
+  double L[4];      // world-line length per class
+  int n[4];         // tau = 0 boundary sites per class
+  long long w;      // winding numbers packed as w_x + 2^32 w_y
+  long long pad;
 };
+
+LG_HD int seg_class(int spin, int g) { return (spin << 1) | (g < 0 ? 1 : 0); }
+
+LG_HD long long pack_winding(int dw0, int dw1) { return (long long)dw0 + ((long long)dw1 << 32); }
+
+LG_HD void unpack_winding(long long w, int& w0, int& w1) {
+  w0 = (int)(uint32_t)(uint64_t)w;      // |w_x| < 2^31: low word sign-extended
+  w1 = (int)((w - (long long)w0) >> 32);
+}
 
 }  // namespace lg

@@ -35,6 +35,7 @@ class Batch {
   int init_for_config(const lg_params* p, long long nops);
   int run(uint32_t n, lg_sweep_record* out);
   int tune(uint32_t sweeps_per_trial);
+  int profile(uint32_t n, double* ms);
   int synchronize();
   int set_config(int c, const int32_t* spins, const int32_t* bond_offsets, const double* times,
                  const int32_t* types, uint64_t sweep);
@@ -89,12 +90,14 @@ class Batch {
   ClusterAcc* acc = nullptr;
   int* scan_part = nullptr;
   int scan_nparts = 0;
+  double* col_part = nullptr;
   int* totals = nullptr;
   int* flags = nullptr;
   uint32_t* d_sweep = nullptr;
   double* rec_acc = nullptr;
   double* rec_ring = nullptr;
   int* ring_pos = nullptr;
+  int* end_count = nullptr;
   int ring_size = 0;
 };
 

@@ -48,12 +48,14 @@ struct DevBatch {
   uint8_t* fbit;               // [nodes] flip bit of the node's cluster
   ClusterAcc* acc;             // [nodes] (roots only)
   int* scan_part;
+  double* col_part;            // [col_chunks(node_cap)][16] collector partial sums
   int* totals;                 // [0] operators, [1] nodes of the new configuration
   int* flags;
   uint32_t* sweep;             // [nchains]
   double* rec_acc;             // [nchains][R_COUNT]
   double* rec_ring;            // [ring][nchains][R_COUNT]
   int* ring_pos;
+  int* end_count;              // blocks of end_sweep finished (reset by the last one)
   int ring_size;
   int op_cap, node_cap;
 };
@@ -65,11 +67,15 @@ struct LaunchCfg {
   int blocks_node = 1024;      // grid of the per-node kernels (grid-stride)
 };
 
-void launch_sweep(const DevBatch& B, const LaunchCfg& L, hipStream_t st);
-void launch_cluster_stage(const DevBatch& B, const LaunchCfg& L, hipStream_t st);
+constexpr int NUM_SWEEP_KERNELS = 11;
+extern const char* const kKernelNames[NUM_SWEEP_KERNELS];
+
+void launch_sweep(const DevBatch& B, const LaunchCfg& L, hipStream_t st, hipEvent_t* ev = nullptr);
+void launch_cluster_stage(const DevBatch& B, const LaunchCfg& L, hipStream_t st, hipEvent_t* ev = nullptr);
 void launch_site_scan(const DevBatch& B, hipStream_t st);
 void launch_canon(const DevBatch& B, int chain, int nelem, int* canon, int* labels, int* roots,
                   hipStream_t st);
 int scan_blocks(const DevBatch& B);
+int col_chunks(int node_cap);
 
 }  // namespace lg

@@ -499,75 +499,183 @@ __global__ void __launch_bounds__(256) k_uf_compress(DevBatch B) {
       while ((q = B.parent[r]) != r) r = q;
       B.parent[i] = r;
     } else {
-      B.acc[i] = ClusterAcc{0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+      B.acc[i] = ClusterAcc{{0, 0, 0, 0}, {0, 0, 0, 0}, 0, 0};
     }
   }
 }
 
 // ---------------------------------------------------------------------------
-// (e) improved estimators + (d) flip bits, per segment node
+// (e) improved estimators + (d) flip bits, per segment node.
+// A segment adds its length to one class slot of its root (ClusterAcc), its
+// tau = 0 boundary (first segment of a site) to one class counter and its
+// winding contributions to one packed 64-bit counter.  Each block handles
+// chunks of EST_CHUNK consecutive nodes and merges the segments that share a
+// root in an LDS hash table (open addressing, HT slots) before touching global
+// memory: the large loops of a cold antiferromagnet (a few clusters holding
+// most segments) then cost one global update per chunk instead of one per
+// segment.  Segments whose root finds no slot update global memory directly.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_estimate(DevBatch B) {
+constexpr int EST_THREADS = 256;
+constexpr int EST_ITEMS = 16;
+constexpr int EST_CHUNK = EST_THREADS * EST_ITEMS;
+constexpr int HT_BITS = 9;
+constexpr int HT = 1 << HT_BITS;
+constexpr int HT_PROBES = 8;
+
+__device__ __forceinline__ double wave_sum_d(double v) {
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d, 64);
+  return v;
+}
+
+__global__ void __launch_bounds__(EST_THREADS) k_estimate(DevBatch B) {
   if (overflowed(B)) return;
+  __shared__ int hkey[HT];
+  __shared__ double hL[4][HT];
+  __shared__ unsigned long long hw[HT];
+  __shared__ unsigned int hn[2][HT];   // class counts: [0] = n0 | n1 << 16, [1] = n2 | n3 << 16
+  const int tid = threadIdx.x;
+  for (int s = tid; s < HT; s += EST_THREADS) {
+    hkey[s] = -1;
+    hL[0][s] = hL[1][s] = hL[2][s] = hL[3][s] = 0;
+    hw[s] = 0;
+    hn[0][s] = hn[1][s] = 0;
+  }
+  __syncthreads();
   const int M = B.totals[1];
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < M; i += gridDim.x * blockDim.x) {
-    const int gs = B.node_site[i];
-    const int c = B.site_chain[gs];
-    const ChainDev& P = B.chains[c];
-    const int s = gs - P.site_base;
-    const int r = B.parent[i];
-    const int chain_node0 = B.soff_new[P.site_base];
-    const uint32_t f = philox(B.sweep[c], STAGE_FLIP, (uint32_t)(r - chain_node0), 0u, P.key0, P.key1).x & 1u;
-    B.fbit[i] = (uint8_t)f;
-    const uint8_t fl = B.evflags[i];
-    const int spin = B.segspin_new[i];
-    const int sig = 1 - 2 * spin;
-    const int gg = gauge(P, s);
-    double len;
-    bool first;
-    int dw0 = 0, dw1 = 0;
-    if (fl & EV_EMPTY) {
-      len = 1.0;
-      first = true;
-    } else {
-      const int base = B.soff_new[gs];
-      const int last = B.soff_new[gs + 1] - 1;
-      first = (i == base);
-      const double thi = B.evt_new[i];
-      len = first ? ((thi + 1.0) - B.evt_new[last]) : (thi - B.evt_new[i - 1]);
-      const uint8_t flo = B.evflags[first ? last : i - 1];
-      // this segment is the lower leg of the event above it and the upper leg
-      // of the event below it (stiffness.h:94-105, source legs only)
-      if (fl & EV_SRC) { if (fl & EV_DIR1) dw1 += sig; else dw0 += sig; }
-      if (flo & EV_SRC) { if (flo & EV_DIR1) dw1 -= sig; else dw0 -= sig; }
+  const int nchunks = (M + EST_CHUNK - 1) / EST_CHUNK;
+  for (int chunk = blockIdx.x; chunk < nchunks; chunk += gridDim.x) {
+    for (int it = 0; it < EST_ITEMS; ++it) {
+      const int i = chunk * EST_CHUNK + it * EST_THREADS + tid;
+      if (i >= M) break;
+      const int gs = B.node_site[i];
+      const int c = B.site_chain[gs];
+      const ChainDev& P = B.chains[c];
+      const int s = gs - P.site_base;
+      const int r = B.parent[i];
+      const int chain_node0 = B.soff_new[P.site_base];
+      const uint32_t f =
+          philox(B.sweep[c], STAGE_FLIP, (uint32_t)(r - chain_node0), 0u, P.key0, P.key1).x & 1u;
+      B.fbit[i] = (uint8_t)f;
+      const uint8_t fl = B.evflags[i];
+      const int spin = B.segspin_new[i];
+      const int sig = 1 - 2 * spin;
+      int dw0 = 0, dw1 = 0;
+      double len;
+      bool first;
+      if (fl & EV_EMPTY) {
+        len = 1.0;
+        first = true;
+      } else {
+        const int base = B.soff_new[gs];
+        const int last = B.soff_new[gs + 1] - 1;
+        first = (i == base);
+        const double thi = B.evt_new[i];
+        len = first ? ((thi + 1.0) - B.evt_new[last]) : (thi - B.evt_new[i - 1]);
+        const uint8_t flo = B.evflags[first ? last : i - 1];
+        // this segment is the lower leg of the event above it and the upper
+        // leg of the event below it (stiffness.h:94-105, source legs only)
+        if (fl & EV_SRC) { if (fl & EV_DIR1) dw1 += sig; else dw0 += sig; }
+        if (flo & EV_SRC) { if (flo & EV_DIR1) dw1 -= sig; else dw0 -= sig; }
+      }
+      if (first) B.spins_new[gs] = (uint8_t)(spin ^ f);
+      B.segspin_new[i] = (uint8_t)(spin ^ f);
+      const int cls = seg_class(spin, gauge(P, s));
+      const long long wp = pack_winding(dw0, dw1);
+      int h = (int)(((uint32_t)r * 2654435761u) >> (32 - HT_BITS));
+      bool placed = false;
+      for (int p = 0; p < HT_PROBES; ++p) {
+        const int k = atomicCAS(&hkey[h], -1, r);
+        if (k == -1 || k == r) {
+          atomicAdd(&hL[cls][h], len);
+          if (first) atomicAdd(&hn[cls >> 1][h], 1u << (16 * (cls & 1)));
+          if (wp) atomicAdd(&hw[h], (unsigned long long)wp);
+          placed = true;
+          break;
+        }
+        h = (h + 1) & (HT - 1);
+      }
+      if (!placed) {
+        ClusterAcc* a = B.acc + r;
+        unsafeAtomicAdd(&a->L[cls], len);
+        if (first) atomicAdd(&a->n[cls], 1);
+        if (wp) atomicAdd((unsigned long long*)&a->w, (unsigned long long)wp);
+      }
     }
-    ClusterAcc* a = B.acc + r;
-    unsafeAtomicAdd(&a->len, len);
-    unsafeAtomicAdd(&a->lmag, sig * len);
-    unsafeAtomicAdd(&a->slen, gg * len);
-    unsafeAtomicAdd(&a->smag, (gg * sig) * len);
-    if (first) {
-      atomicAdd(&a->n0, 1);
-      atomicAdd(&a->m0, sig);
-      atomicAdd(&a->sn0, gg);
-      atomicAdd(&a->sm0, gg * sig);
-      B.spins_new[gs] = (uint8_t)(spin ^ f);
+    __syncthreads();
+    for (int s = tid; s < HT; s += EST_THREADS) {
+      const int r = hkey[s];
+      if (r < 0) continue;
+      ClusterAcc* a = B.acc + r;
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        if (hL[k][s] != 0.0) { unsafeAtomicAdd(&a->L[k], hL[k][s]); hL[k][s] = 0; }
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        const unsigned int v = hn[q][s];
+        if (v) {
+          if (v & 0xffffu) atomicAdd(&a->n[2 * q], (int)(v & 0xffffu));
+          if (v >> 16) atomicAdd(&a->n[2 * q + 1], (int)(v >> 16));
+          hn[q][s] = 0;
+        }
+      }
+      if (hw[s]) { atomicAdd((unsigned long long*)&a->w, hw[s]); hw[s] = 0; }
+      hkey[s] = -1;
     }
-    if (dw0) atomicAdd(&a->w0, dw0);
-    if (dw1) atomicAdd(&a->w1, dw1);
-    B.segspin_new[i] = (uint8_t)(spin ^ f);
+    __syncthreads();
   }
 }
 
 // ---------------------------------------------------------------------------
-// (d) operator flip + per-cluster collection into the chain's record
+// (d) operator flip, then the improved-estimator collector
+// (susceptibility.h:177-198, stiffness.h:117-121) over the roots.
+// Collection is chunked: chunk j covers nodes [j*COL_CHUNK, (j+1)*COL_CHUNK)
+// and belongs to the chain of its first node; its sum over the roots of that
+// chain is written to col_part[j] (no atomics, fixed order), roots of other
+// chains in the chunk (chunks straddling a chain boundary) are added to
+// rec_acc atomically.  end_sweep adds the partials of each chain in order.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_flip_collect(DevBatch B) {
+constexpr int COL_THREADS = 256;
+constexpr int COL_ITEMS = 16;
+constexpr int COL_CHUNK = COL_THREADS * COL_ITEMS;
+constexpr int NCOL = 16;
+
+__device__ __forceinline__ void cluster_terms(const ClusterAcc& a, double* v) {
+  const double L0 = a.L[0], L1 = a.L[1], L2 = a.L[2], L3 = a.L[3];
+  const int n0 = a.n[0], n1 = a.n[1], n2 = a.n[2], n3 = a.n[3];
+  int w0, w1;
+  unpack_winding(a.w, w0, w1);
+  // per-cluster improved estimates (susceptibility.h:96-147 with sigma = 1-2c)
+  const double usize0 = 0.5 * (n0 + n1 + n2 + n3), umag0 = 0.5 * (n0 + n1 - n2 - n3);
+  const double ssize0 = 0.5 * (n0 - n1 + n2 - n3), smag0 = 0.5 * (n0 - n1 - n2 + n3);
+  const double usize = 0.5 * ((L0 + L1) + (L2 + L3)), umag = 0.5 * ((L0 + L1) - (L2 + L3));
+  const double ssize = 0.5 * ((L0 - L1) + (L2 - L3)), smag = 0.5 * ((L0 - L1) - (L2 - L3));
+  v[0] = 1.0;  // clusters
+  v[1] = umag0;
+  v[2] = usize0 * usize0;
+  v[3] = umag0 * umag0;
+  v[4] = v[2] * v[2];
+  v[5] = v[3] * v[3];
+  v[6] = usize * usize;
+  v[7] = umag * umag;
+  v[8] = smag0;
+  v[9] = ssize0 * ssize0;
+  v[10] = smag0 * smag0;
+  v[11] = v[9] * v[9];
+  v[12] = v[10] * v[10];
+  v[13] = ssize * ssize;
+  v[14] = smag * smag;
+  v[15] = 0.25 * ((double)w0 * w0 + (double)w1 * w1);
+}
+
+// record slot of collector term q (R_NC, then R_UMAG0 .. R_W2)
+__device__ __forceinline__ int term_slot(int q) { return q == 0 ? R_NC : R_UMAG0 + (q - 1); }
+
+__global__ void __launch_bounds__(COL_THREADS) k_flip_collect(DevBatch B) {
   if (overflowed(B)) return;
   const int nops = B.totals[0];
   const int M = B.totals[1];
-  const int stride = gridDim.x * blockDim.x;
-  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nops; k += stride) {
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nops; k += gridDim.x * blockDim.x) {
     const int info = B.oi_new[k];
     const int gb = info >> 3;
     const ChainDev& P = B.chains[B.bond_chain[gb]];
@@ -577,91 +685,102 @@ __global__ void __launch_bounds__(256) k_flip_collect(DevBatch B) {
     const int u0 = next_node(B, P.site_base + s0, l0);
     if (B.fbit[l0] ^ B.fbit[u0]) B.oi_new[k] = info ^ 1;
   }
-  // improved-estimator collector (susceptibility.h:177-198, stiffness.h:117-121)
-  for (int i0 = blockIdx.x * blockDim.x; i0 < M; i0 += stride) {
-    const int i = i0 + threadIdx.x;
-    double v[16];
+  __shared__ double red[COL_THREADS / 64][NCOL];
+  const int nchunks = (M + COL_CHUNK - 1) / COL_CHUNK;
+  for (int chunk = blockIdx.x; chunk < nchunks; chunk += gridDim.x) {
+    const int base = chunk * COL_CHUNK;
+    const int c0 = B.site_chain[B.node_site[base]];
+    double v[NCOL];
 #pragma unroll
-    for (int q = 0; q < 16; ++q) v[q] = 0;
-    int c = -1;
-    if (i < M && B.parent[i] == i) {
-      c = B.site_chain[B.node_site[i]];
-      const ClusterAcc a = B.acc[i];
-      const double usize0 = 0.5 * a.n0, umag0 = 0.5 * a.m0;
-      const double ssize0 = 0.5 * a.sn0, smag0 = 0.5 * a.sm0;
-      const double usize = 0.5 * a.len, umag = 0.5 * a.lmag;
-      const double ssize = 0.5 * a.slen, smag = 0.5 * a.smag;
-      v[0] = 1.0;                                  // clusters
-      v[1] = umag0;
-      v[2] = usize0 * usize0;
-      v[3] = umag0 * umag0;
-      v[4] = v[2] * v[2];
-      v[5] = v[3] * v[3];
-      v[6] = usize * usize;
-      v[7] = umag * umag;
-      v[8] = smag0;
-      v[9] = ssize0 * ssize0;
-      v[10] = smag0 * smag0;
-      v[11] = v[9] * v[9];
-      v[12] = v[10] * v[10];
-      v[13] = ssize * ssize;
-      v[14] = smag * smag;
-      v[15] = 0.25 * ((double)a.w0 * a.w0 + (double)a.w1 * a.w1);
-    }
-    // wave-level reduction when the whole wave belongs to one chain
-    int c0 = __shfl(c, 0, 64);
-    int cmax = c, cmin = (c < 0) ? 0x7fffffff : c;
-    for (int d = 32; d > 0; d >>= 1) {
-      cmax = max(cmax, __shfl_xor(cmax, d, 64));
-      cmin = min(cmin, __shfl_xor(cmin, d, 64));
-    }
-    if (cmax < 0) continue;
-    if (cmin == cmax) {
+    for (int q = 0; q < NCOL; ++q) v[q] = 0;
+    for (int it = 0; it < COL_ITEMS; ++it) {
+      const int i = base + it * COL_THREADS + threadIdx.x;
+      if (i >= M) break;
+      if (B.parent[i] != i) continue;
+      const int c = B.site_chain[B.node_site[i]];
+      double w[NCOL];
+      cluster_terms(B.acc[i], w);
+      if (c == c0) {
 #pragma unroll
-      for (int q = 0; q < 16; ++q)
-        for (int d = 32; d > 0; d >>= 1) v[q] += __shfl_xor(v[q], d, 64);
-      if (lane_id() == 0) {
-        double* rec = B.rec_acc + (size_t)cmax * R_COUNT;
-        unsafeAtomicAdd(rec + R_NC, v[0]);
-#pragma unroll
-        for (int q = 1; q < 16; ++q) unsafeAtomicAdd(rec + R_UMAG0 + (q - 1), v[q]);
+        for (int q = 0; q < NCOL; ++q) v[q] += w[q];
+      } else {
+        double* rec = B.rec_acc + (size_t)c * R_COUNT;
+        for (int q = 0; q < NCOL; ++q) unsafeAtomicAdd(rec + term_slot(q), w[q]);
       }
-    } else if (c >= 0) {
-      double* rec = B.rec_acc + (size_t)c * R_COUNT;
-      unsafeAtomicAdd(rec + R_NC, v[0]);
-      for (int q = 1; q < 16; ++q) unsafeAtomicAdd(rec + R_UMAG0 + (q - 1), v[q]);
     }
-    (void)c0;
+#pragma unroll
+    for (int q = 0; q < NCOL; ++q) v[q] = wave_sum_d(v[q]);
+    if (lane_id() == 0)
+      for (int q = 0; q < NCOL; ++q) red[threadIdx.x >> 6][q] = v[q];
+    __syncthreads();
+    if (threadIdx.x < NCOL) {
+      double t = 0;
+      for (int w = 0; w < COL_THREADS / 64; ++w) t += red[w][threadIdx.x];
+      B.col_part[(size_t)chunk * NCOL + threadIdx.x] = t;
+    }
+    __syncthreads();
   }
 }
 
 // ---------------------------------------------------------------------------
-// end of sweep: finish records, advance the sweep counters
+// end of sweep, one block per chain: add the chain's collector partials in a
+// fixed order, finish and store the record, advance the sweep counter.  The
+// last block to finish advances the record ring.
 // ---------------------------------------------------------------------------
-__global__ void k_end_sweep(DevBatch B) {
+constexpr int END_THREADS = 256;
+
+__global__ void __launch_bounds__(END_THREADS) k_end_sweep(DevBatch B) {
   const int slot = *B.ring_pos;
+  const int c = blockIdx.x;
+  const int tid = threadIdx.x;
+  __shared__ double red[END_THREADS / 64][NCOL];
+  __shared__ int is_last;
   if (slot >= B.ring_size) {
-    if (threadIdx.x == 0) atomicOr(B.flags, FLAG_RING_OVERFLOW);
+    if (tid == 0) atomicOr(B.flags, FLAG_RING_OVERFLOW);
     return;
   }
-  for (int c = threadIdx.x; c < B.nchains; c += blockDim.x) {
-    const ChainDev& P = B.chains[c];
-    double* rec = B.rec_acc + (size_t)c * R_COUNT;
-    double* out = B.rec_ring + ((size_t)slot * B.nchains + c) * R_COUNT;
-    const bool bad = overflowed(B);
+  const bool bad = overflowed(B);
+  const ChainDev& P = B.chains[c];
+  double v[NCOL];
+#pragma unroll
+  for (int q = 0; q < NCOL; ++q) v[q] = 0;
+  if (!bad) {
+    const int start = B.soff_new[P.site_base], end = B.soff_new[P.site_base + P.N];
+    const int j0 = (start + COL_CHUNK - 1) / COL_CHUNK, j1 = (end + COL_CHUNK - 1) / COL_CHUNK;
+    for (int j = j0 + tid; j < j1; j += END_THREADS)
+      for (int q = 0; q < NCOL; ++q) v[q] += B.col_part[(size_t)j * NCOL + q];
+  }
+#pragma unroll
+  for (int q = 0; q < NCOL; ++q) v[q] = wave_sum_d(v[q]);
+  if (lane_id() == 0)
+    for (int q = 0; q < NCOL; ++q) red[tid >> 6][q] = v[q];
+  __syncthreads();
+  double* rec = B.rec_acc + (size_t)c * R_COUNT;
+  double* out = B.rec_ring + ((size_t)slot * B.nchains + c) * R_COUNT;
+  if (tid < R_COUNT) {
+    double t = rec[tid];
+    rec[tid] = 0;
+    for (int q = 0; q < NCOL; ++q)
+      if (term_slot(q) == tid)
+        for (int w = 0; w < END_THREADS / 64; ++w) t += red[w][q];
+    out[tid] = t;
+  }
+  __syncthreads();
+  if (tid == 0) {
     const double nop = bad ? 0.0 : (double)(B.bo_new[P.bond_base + P.Nb] - B.bo_new[P.bond_base]);
-    for (int q = 0; q < R_COUNT; ++q) {
-      out[q] = rec[q];
-      rec[q] = 0;
-    }
     out[R_NOP] = nop;
     out[R_ENERGY] = P.offset_total - nop / P.beta;
     out[R_BETA] = P.beta;
     out[R_SWEEP] = (double)B.sweep[c];
     B.sweep[c] += 1;
+    __threadfence();
+    is_last = (atomicAdd(B.end_count, 1) == (int)gridDim.x - 1);
   }
   __syncthreads();
-  if (threadIdx.x == 0) *B.ring_pos = slot + 1;
+  if (is_last && tid == 0) {
+    *B.end_count = 0;
+    *B.ring_pos = slot + 1;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -727,27 +846,50 @@ static void diag_dispatch(const DevBatch& B, const LaunchCfg& L, hipStream_t st,
   }
 }
 
+int col_chunks(int node_cap) { return (node_cap + COL_CHUNK - 1) / COL_CHUNK; }
+
 int scan_blocks(const DevBatch& B) {
   return (B.nbonds + SCAN_CHUNK - 1) / SCAN_CHUNK + (B.nsites + SCAN_CHUNK - 1) / SCAN_CHUNK;
 }
 
-void launch_sweep(const DevBatch& B, const LaunchCfg& L, hipStream_t st) {
-  diag_dispatch(B, L, st, false);
-  const int sb = scan_blocks(B);
-  hipLaunchKernelGGL(k_scan_reduce, dim3(sb), dim3(SCAN_THREADS), 0, st, B);
-  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, st, B);
-  hipLaunchKernelGGL(k_scan_down, dim3(sb), dim3(SCAN_THREADS), 0, st, B);
-  diag_dispatch(B, L, st, true);
-  launch_cluster_stage(B, L, st);
+const char* const kKernelNames[NUM_SWEEP_KERNELS] = {
+    "diag_count", "scan_reduce", "scan_top", "scan_down", "diag_fill", "link",
+    "uf_union", "uf_compress", "estimate", "flip_collect", "end_sweep"};
+
+// ev (optional): NUM_SWEEP_KERNELS + 1 events recorded around the launches
+static inline void mark(hipEvent_t* ev, int i, hipStream_t st) {
+  if (ev) (void)hipEventRecord(ev[i], st);
 }
 
-void launch_cluster_stage(const DevBatch& B, const LaunchCfg& L, hipStream_t st) {
+void launch_sweep(const DevBatch& B, const LaunchCfg& L, hipStream_t st, hipEvent_t* ev) {
+  const int sb = scan_blocks(B);
+  mark(ev, 0, st);
+  diag_dispatch(B, L, st, false);
+  mark(ev, 1, st);
+  hipLaunchKernelGGL(k_scan_reduce, dim3(sb), dim3(SCAN_THREADS), 0, st, B);
+  mark(ev, 2, st);
+  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, st, B);
+  mark(ev, 3, st);
+  hipLaunchKernelGGL(k_scan_down, dim3(sb), dim3(SCAN_THREADS), 0, st, B);
+  mark(ev, 4, st);
+  diag_dispatch(B, L, st, true);
+  mark(ev, 5, st);
+  launch_cluster_stage(B, L, st, ev ? ev + 5 : nullptr);
+}
+
+void launch_cluster_stage(const DevBatch& B, const LaunchCfg& L, hipStream_t st, hipEvent_t* ev) {
   hipLaunchKernelGGL(k_link, dim3(L.blocks_op), dim3(256), 0, st, B);
+  mark(ev, 1, st);
   hipLaunchKernelGGL(k_uf_union, dim3(L.blocks_op), dim3(256), 0, st, B);
+  mark(ev, 2, st);
   hipLaunchKernelGGL(k_uf_compress, dim3(L.blocks_node), dim3(256), 0, st, B);
-  hipLaunchKernelGGL(k_estimate, dim3(L.blocks_node), dim3(256), 0, st, B);
-  hipLaunchKernelGGL(k_flip_collect, dim3(L.blocks_node), dim3(256), 0, st, B);
-  hipLaunchKernelGGL(k_end_sweep, dim3(1), dim3(256), 0, st, B);
+  mark(ev, 3, st);
+  hipLaunchKernelGGL(k_estimate, dim3(L.blocks_node), dim3(EST_THREADS), 0, st, B);
+  mark(ev, 4, st);
+  hipLaunchKernelGGL(k_flip_collect, dim3(L.blocks_node), dim3(COL_THREADS), 0, st, B);
+  mark(ev, 5, st);
+  hipLaunchKernelGGL(k_end_sweep, dim3(B.nchains), dim3(END_THREADS), 0, st, B);
+  mark(ev, 6, st);
 }
 
 void launch_site_scan(const DevBatch& B, hipStream_t st) {

@@ -28,6 +28,7 @@ EXPORTED = [
     "lg_chain_set_config", "lg_chain_destroy", "lg_cluster_config",
     "lg_batch_create", "lg_batch_run", "lg_batch_tune", "lg_batch_set_graph",
     "lg_batch_chain_info", "lg_batch_destroy",
+    "lg_profile_num_kernels", "lg_profile_kernel_name", "lg_chain_profile", "lg_batch_profile",
 ]
 
 
@@ -66,6 +67,11 @@ def _load():
     lib.lg_batch_set_graph.argtypes = [vp, ctypes.c_int]
     lib.lg_batch_chain_info.argtypes = [vp, ctypes.c_int32, P(ctypes.c_int64), P(ctypes.c_uint64)]
     lib.lg_batch_destroy.argtypes = [vp]
+    lib.lg_profile_num_kernels.argtypes = []
+    lib.lg_profile_kernel_name.argtypes = [ctypes.c_int]
+    lib.lg_profile_kernel_name.restype = ctypes.c_char_p
+    lib.lg_chain_profile.argtypes = [vp, ctypes.c_uint32, vp]
+    lib.lg_batch_profile.argtypes = [vp, ctypes.c_uint32, vp]
     return lib
 
 
@@ -83,6 +89,10 @@ def device_count():
     n = ctypes.c_int(0)
     _check(lib.lg_device_count(ctypes.byref(n)))
     return n.value
+
+
+def kernel_names():
+    return [lib.lg_profile_kernel_name(i).decode() for i in range(lib.lg_profile_num_kernels())]
 
 
 def params(lattice, L, beta, jxy=1.0, jz=None, seed=23125, chain_id=0, force_scatter=0.0):
@@ -113,6 +123,13 @@ class Chain:
         rec = np.zeros((n, REC_SIZE)) if records else None
         _check(lib.lg_chain_sweep(self.h, n, _ptr(rec)))
         return rec
+
+    def profile(self, n):
+        """average per-sweep duration (ms) of each sweep kernel over n sweeps"""
+        k = lib.lg_profile_num_kernels()
+        ms = np.zeros(k)
+        _check(lib.lg_chain_profile(self.h, n, _ptr(ms)))
+        return dict(zip(kernel_names(), ms))
 
     def info(self):
         ns, nb, no, sw = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int64(), ctypes.c_uint64()
@@ -166,6 +183,12 @@ class Batch:
 
     def set_graph(self, enable):
         _check(lib.lg_batch_set_graph(self.h, 1 if enable else 0))
+
+    def profile(self, n):
+        k = lib.lg_profile_num_kernels()
+        ms = np.zeros(k)
+        _check(lib.lg_batch_profile(self.h, n, _ptr(ms)))
+        return dict(zip(kernel_names(), ms))
 
     def chain_info(self, c):
         no, sw = ctypes.c_int64(), ctypes.c_uint64()

@@ -122,6 +122,17 @@ int lg_batch_set_graph(lg_batch* h, int enable);
 int lg_batch_chain_info(lg_batch* h, int32_t chain, int64_t* n_ops, uint64_t* sweep);
 int lg_batch_destroy(lg_batch* h);
 
+/* ---- measurement ------------------------------------------------------------
+ * n sweeps with HIP events recorded between the kernel launches on the
+ * handle's stream; avg_ms[i] = mean duration per sweep of kernel i, in the
+ * order of lg_profile_kernel_name(i), i < lg_profile_num_kernels().  The
+ * configuration advances by n sweeps (records are not returned).  These are
+ * the reference's timer regions (path_integral.C:284-299) as device kernels. */
+int lg_profile_num_kernels(void);
+const char* lg_profile_kernel_name(int i);
+int lg_chain_profile(lg_chain* h, uint32_t n, double* avg_ms);
+int lg_batch_profile(lg_batch* h, uint32_t n, double* avg_ms);
+
 #ifdef __cplusplus
 }
 #endif
