@@ -146,6 +146,10 @@ int Batch::init_for_config(const lg_params* p, long long nops) {
 
 int Batch::init(const lg_params* p, int n) {
   if (n <= 0) return set_error(LG_ERR_ARG, "need at least one chain");
+  for (int c = 0; c < n; ++c) {
+    int rc = check_params(p[c]);
+    if (rc) return rc;
+  }
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
     return set_error(LG_ERR_DEVICE, "no HIP device available");

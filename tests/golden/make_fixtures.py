@@ -1,0 +1,153 @@
+"""Extract golden vectors from the reference's own test outputs into small JSON
+fixtures (run once in a container that has the reference checkout; the GPU box
+never reads the reference).
+
+Sources (data files of the reference's tests):
+  test/union_find.op            100 random unions and the resulting forest
+  test/compatibility.op         cut-rule truth table
+  test/weight.op                XXZ graph weights (standard and FORCE_SCATTER=0.1)
+  test/spinmodel_helper.op      chain lattice bond list (ALPS conventions)
+  test/virtual_lattice.op       square lattice L=2 vertices, edges, gauge
+  standalone/loop.op            standalone kernel output (L=8, T=0.2)
+  loop.op                       end-to-end run: ED and path-integral QMC, chain L=4, T=0.25
+"""
+
+import json
+import os
+import re
+import sys
+
+REF = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
+OUT = os.path.dirname(os.path.abspath(__file__))
+
+
+def read(rel):
+    with open(os.path.join(REF, rel)) as f:
+        return f.read().splitlines()
+
+
+def union_find():
+    lines = read("test/union_find.op")
+    pairs = []
+    for l in lines:
+        m = re.match(r"connecting node (\d+) to node (\d+)", l)
+        if m:
+            pairs.append([int(m.group(1)), int(m.group(2))])
+    nodes = {}
+    start = lines.index("[results]") + 1
+    for l in lines[start:start + 100]:
+        m = re.match(r"node (\d+) is root and tree size is (\d+)", l)
+        if m:
+            nodes[int(m.group(1))] = {"parent": -1, "root": int(m.group(1)), "size": int(m.group(2))}
+            continue
+        m = re.match(r"node (\d+)'s parent is (\d+) and its root is (\d+)", l)
+        nodes[int(m.group(1))] = {"parent": int(m.group(2)), "root": int(m.group(3)), "size": 0}
+    return {"n": 100, "pairs": pairs,
+            "parent": [nodes[i]["parent"] for i in range(100)],
+            "root": [nodes[i]["root"] for i in range(100)],
+            "size": [nodes[i]["size"] for i in range(100)]}
+
+
+def compatibility():
+    out = []
+    for l in read("test/compatibility.op"):
+        if l.startswith("[bond graph (general location + general graph)]"):
+            collecting = True
+            continue
+        m = re.match(r"Delta\(g=(\d), c0=([+-]), c1=([+-])\) = (\d)", l)
+        if m and len(out) < 16:
+            out.append([int(m.group(1)), 0 if m.group(2) == "+" else 1, 0 if m.group(3) == "+" else 1,
+                        int(m.group(4))])
+    return out
+
+
+def weights():
+    rows = []
+    for l in read("test/weight.op"):
+        m = re.match(r"bond weight \((standard|ergodic)\): C = (\S+), Jxy = (\S+), Jz = (\S+) : "
+                     r"v\[0\] = (\S+), v\[1\] = (\S+), v\[2\] = (\S+), v\[3\] = (\S+), offset = (\S+), sign = (\S+)", l)
+        if m:
+            g = m.groups()
+            rows.append({"force_scatter": 0.0 if g[0] == "standard" else 0.1,
+                         "C": float(g[1].rstrip(",")), "jxy": float(g[2].rstrip(",")), "jz": float(g[3]),
+                         "v": [float(g[4].rstrip(",")), float(g[5].rstrip(",")), float(g[6].rstrip(",")),
+                               float(g[7].rstrip(","))],
+                         "offset": float(g[8].rstrip(",")), "sign": int(g[9])})
+    return rows
+
+
+def lattices():
+    sm = read("test/spinmodel_helper.op")
+    chain = []
+    for l in sm:
+        m = re.match(r'\s*<EDGE source="(\d+)" target="(\d+)" id="(\d+)"', l)
+        if m:
+            chain.append([int(m.group(1)) - 1, int(m.group(2)) - 1])
+        if "</GRAPH>" in l:
+            break
+    vl = read("test/virtual_lattice.op")
+    verts, edges = [], []
+    gauge = None
+    for l in vl:
+        m = re.match(r'\s*<VERTEX id="(\d+)" type="\d+"><COORDINATE>(\d+) (\d+)</COORDINATE>', l)
+        if m:
+            verts.append([int(m.group(2)), int(m.group(3))])
+        m = re.match(r'\s*<EDGE source="(\d+)" target="(\d+)" id="(\d+)" type="\d+" vector="(\d+) (\d+)"', l)
+        if m:
+            edges.append([int(m.group(1)) - 1, int(m.group(2)) - 1, int(m.group(4)), int(m.group(5))])
+        if "</GRAPH>" in l and gauge is None:
+            idx = vl.index(l)
+            gauge = [int(x) for x in vl[idx + 1].split()]
+            break
+    return {"chain4_bonds": chain, "square2_coords": verts, "square2_edges": edges, "square2_gauge": gauge}
+
+
+def standalone():
+    res = {}
+    for l in read("standalone/loop.op"):
+        m = re.match(r"(.+?)\s+= (\S+)(?: \+- (\S+))?$", l)
+        if m:
+            res[m.group(1).strip()] = [float(m.group(2))] + ([float(m.group(3))] if m.group(3) else [])
+    return res
+
+
+def loop_op():
+    lines = read("loop.op")
+    blocks = []
+    cur = None
+    for l in lines:
+        if l.startswith("[input parameters]"):
+            cur = {"params": {}, "results": {}}
+            blocks.append(cur)
+            continue
+        m = re.match(r"(\S[^=]*?) = (.*);$", l)
+        if cur is not None and m and not cur["results"]:
+            cur["params"][m.group(1).strip()] = m.group(2).strip().strip('"')
+            continue
+        m = re.match(r"(.+?): (\S+) \+/- (\S+)", l)
+        if cur is not None and m:
+            err = m.group(3).rstrip(";")
+            cur["results"][m.group(1)] = [float(m.group(2)), float("inf") if err == "inf" else float(err)]
+    keep = []
+    for b in blocks[:2]:  # L=4 chain, T=0.25: diagonalization and "loop; path integral"
+        keep.append({"algorithm": b["params"].get("ALGORITHM"), "params": b["params"],
+                     "results": {k: v for k, v in b["results"].items() if not k.startswith("Local")}})
+    return keep
+
+
+def main():
+    fx = {
+        "union_find": union_find(),
+        "compatibility": compatibility(),
+        "weights": weights(),
+        "lattices": lattices(),
+        "standalone_loop_op": standalone(),
+        "loop_op_chain4": loop_op(),
+    }
+    with open(os.path.join(OUT, "reference_fixtures.json"), "w") as f:
+        json.dump(fx, f, indent=1, sort_keys=True)
+    print("wrote", os.path.join(OUT, "reference_fixtures.json"))
+
+
+if __name__ == "__main__":
+    main()
