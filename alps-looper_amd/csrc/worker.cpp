@@ -1,0 +1,606 @@
+// looper_amd::loop_worker -- host C++ mirror of the reference worker
+// "loop; path integral" (path_integral.C:57-351) on top of the C-ABI of
+// include/loopgpu.h.
+//
+//  * parameters by the reference's names (alps::Parameters semantics for the
+//    subset this path supports): LATTICE, L, W, T / TEMPERATURE / BETA /
+//    INVERSE_TEMPERATURE (temperature.h:43-79), J / Jxy / Jz, MODEL, local_S,
+//    SWEEPS (default "[65536:]") and THERMALIZATION (default SWEEPS.min/8,
+//    montecarlo.h:41-45), SEED, FORCE_SCATTER (weight_impl.h:169);
+//  * init_observables / run / is_thermalized / progress / save / load with the
+//    worker's observable names (path_integral.C:307-321,831-862, energy.h,
+//    susceptibility.h, stiffness.h) and its evaluator (Specific Heat,
+//    Binder ratios: energy.h:89-102, susceptibility.h:340-374);
+//  * binning analysis of every observable (mean, error, integrated
+//    autocorrelation time), measurements before thermalization discarded.
+//
+// Errors are reported like the reference's (std::invalid_argument for bad
+// parameters) inside C++, and as status codes across the C-ABI.
+#include <algorithm>
+#include <cctype>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/loopgpu.h"
+#include "../../include/loopgpu_worker.h"
+
+namespace looper_amd {
+
+// ---------------------------------------------------------------------------
+// parameters: "KEY = value" separated by ';' or newlines, values are numbers,
+// quoted strings or arithmetic expressions over other parameters (loop.ip)
+// ---------------------------------------------------------------------------
+class Parameters {
+ public:
+  explicit Parameters(const std::string& text) {
+    std::string cur;
+    bool quoted = false;
+    for (char ch : text) {
+      if (ch == '"') quoted = !quoted;
+      if (!quoted && (ch == ';' || ch == '\n')) { add(cur); cur.clear(); }
+      else cur += ch;
+    }
+    add(cur);
+  }
+  bool defined(const std::string& k) const { return kv_.count(k) > 0; }
+  std::string str(const std::string& k, const std::string& def = "") const {
+    auto it = kv_.find(k);
+    if (it == kv_.end()) return def;
+    std::string v = it->second;
+    if (v.size() >= 2 && v.front() == '"' && v.back() == '"') v = v.substr(1, v.size() - 2);
+    return v;
+  }
+  double num(const std::string& k) const {
+    if (!defined(k)) throw std::invalid_argument("parameter " + k + " not defined");
+    return eval(str(k), 0);
+  }
+  double num(const std::string& k, double def) const { return defined(k) ? num(k) : def; }
+
+ private:
+  static std::string trim(const std::string& s) {
+    size_t a = s.find_first_not_of(" \t\r"), b = s.find_last_not_of(" \t\r");
+    return (a == std::string::npos) ? "" : s.substr(a, b - a + 1);
+  }
+  void add(const std::string& stmt) {
+    std::string s = trim(stmt);
+    if (s.empty() || s[0] == '#' || s == "{" || s == "}") return;
+    size_t eq = s.find('=');
+    if (eq == std::string::npos) throw std::invalid_argument("malformed parameter: " + s);
+    kv_[trim(s.substr(0, eq))] = trim(s.substr(eq + 1));
+  }
+  // recursive-descent evaluation: expr = term {(+|-) term}, term = factor
+  // {(*|/) factor}, factor = number | name | (expr) | -factor
+  double eval(const std::string& e, int depth) const {
+    if (depth > 32) throw std::invalid_argument("parameter expression nests too deeply");
+    size_t pos = 0;
+    double v = expr(e, pos, depth);
+    skip(e, pos);
+    if (pos != e.size()) throw std::invalid_argument("cannot evaluate parameter expression '" + e + "'");
+    return v;
+  }
+  static void skip(const std::string& e, size_t& p) { while (p < e.size() && std::isspace((unsigned char)e[p])) ++p; }
+  double expr(const std::string& e, size_t& p, int d) const {
+    double v = term(e, p, d);
+    for (;;) {
+      skip(e, p);
+      if (p < e.size() && (e[p] == '+' || e[p] == '-')) { char o = e[p++]; double w = term(e, p, d); v = (o == '+') ? v + w : v - w; }
+      else return v;
+    }
+  }
+  double term(const std::string& e, size_t& p, int d) const {
+    double v = factor(e, p, d);
+    for (;;) {
+      skip(e, p);
+      if (p < e.size() && (e[p] == '*' || e[p] == '/')) { char o = e[p++]; double w = factor(e, p, d); v = (o == '*') ? v * w : v / w; }
+      else return v;
+    }
+  }
+  double factor(const std::string& e, size_t& p, int d) const {
+    skip(e, p);
+    if (p >= e.size()) throw std::invalid_argument("empty parameter expression '" + e + "'");
+    if (e[p] == '-') { ++p; return -factor(e, p, d); }
+    if (e[p] == '+') { ++p; return factor(e, p, d); }
+    if (e[p] == '(') { ++p; double v = expr(e, p, d); skip(e, p); if (p < e.size() && e[p] == ')') ++p; return v; }
+    if (std::isdigit((unsigned char)e[p]) || e[p] == '.') {
+      size_t q = p;
+      while (q < e.size() && (std::isdigit((unsigned char)e[q]) || e[q] == '.' || e[q] == 'e' || e[q] == 'E' ||
+                              ((e[q] == '-' || e[q] == '+') && q > p && (e[q - 1] == 'e' || e[q - 1] == 'E'))))
+        ++q;
+      double v = std::stod(e.substr(p, q - p));
+      p = q;
+      return v;
+    }
+    size_t q = p;
+    while (q < e.size() && (std::isalnum((unsigned char)e[q]) || e[q] == '_')) ++q;
+    std::string name = e.substr(p, q - p);
+    p = q;
+    if (name.empty() || !defined(name)) throw std::invalid_argument("unknown name in expression '" + e + "'");
+    return eval(str(name), d + 1);
+  }
+  std::map<std::string, std::string> kv_;
+};
+
+// ---------------------------------------------------------------------------
+// binning: level-0 moments plus a bounded array of bin means (bin size
+// doubles whenever 2*NBINS bins are full); error from the bins, tau from the
+// ratio of binned and naive variances, functions of means by jackknife.
+// ---------------------------------------------------------------------------
+class Observable {
+ public:
+  static const size_t NBINS = 512;
+  void operator<<(double x) {
+    ++n_;
+    s1_ += x;
+    s2_ += x * x;
+    cur_ += x;
+    if (++cur_n_ == bin_size_) {
+      bins_.push_back(cur_ / bin_size_);
+      cur_ = 0;
+      cur_n_ = 0;
+      if (bins_.size() == 2 * NBINS) {
+        for (size_t i = 0; i < NBINS; ++i) bins_[i] = 0.5 * (bins_[2 * i] + bins_[2 * i + 1]);
+        bins_.resize(NBINS);
+        bin_size_ *= 2;
+      }
+    }
+  }
+  uint64_t count() const { return n_; }
+  double mean() const { return n_ ? s1_ / n_ : 0.0; }
+  double naive_error() const {
+    if (n_ < 2) return 0;
+    double m = mean(), v = s2_ / n_ - m * m;
+    return v > 0 ? std::sqrt(v / (n_ - 1)) : 0.0;
+  }
+  double error() const {
+    size_t nb = bins_.size();
+    if (nb < 2) return naive_error();
+    double m = 0, v = 0;
+    for (double b : bins_) m += b;
+    m /= nb;
+    for (double b : bins_) v += (b - m) * (b - m);
+    return std::max(naive_error(), std::sqrt(v / (nb - 1) / nb));
+  }
+  double tau() const {
+    double e0 = naive_error(), e = error();
+    return (e0 > 0) ? 0.5 * (e * e / (e0 * e0) - 1) : 0.0;
+  }
+  const std::vector<double>& bins() const { return bins_; }
+  uint64_t bin_size() const { return bin_size_; }
+  void save(std::vector<double>& out) const {
+    out.push_back((double)n_); out.push_back(s1_); out.push_back(s2_); out.push_back(cur_);
+    out.push_back((double)cur_n_); out.push_back((double)bin_size_); out.push_back((double)bins_.size());
+    out.insert(out.end(), bins_.begin(), bins_.end());
+  }
+  size_t load(const double* p) {
+    n_ = (uint64_t)p[0]; s1_ = p[1]; s2_ = p[2]; cur_ = p[3]; cur_n_ = (uint64_t)p[4];
+    bin_size_ = (uint64_t)p[5];
+    size_t nb = (size_t)p[6];
+    bins_.assign(p + 7, p + 7 + nb);
+    return 7 + nb;
+  }
+
+ private:
+  uint64_t n_ = 0, cur_n_ = 0, bin_size_ = 1;
+  double s1_ = 0, s2_ = 0, cur_ = 0;
+  std::vector<double> bins_;
+};
+
+struct Result { double mean, error, tau; bool evaluated; };
+
+// jackknife over the common bins of two observables for f(<a>, <b>)
+template <class F>
+Result jackknife(const Observable& a, const Observable& b, F f) {
+  size_t nb = std::min(a.bins().size(), b.bins().size());
+  Result r{f(a.mean(), b.mean()), 0.0, 0.0, true};
+  if (nb < 2 || a.bin_size() != b.bin_size()) return r;
+  double sa = 0, sb = 0;
+  for (size_t i = 0; i < nb; ++i) { sa += a.bins()[i]; sb += b.bins()[i]; }
+  std::vector<double> jk(nb);
+  double m = 0;
+  for (size_t i = 0; i < nb; ++i) {
+    jk[i] = f((sa - a.bins()[i]) / (nb - 1), (sb - b.bins()[i]) / (nb - 1));
+    m += jk[i];
+  }
+  m /= nb;
+  double v = 0;
+  for (double x : jk) v += (x - m) * (x - m);
+  r.error = std::sqrt(v * (nb - 1) / nb);
+  return r;
+}
+
+// ---------------------------------------------------------------------------
+// the worker
+// ---------------------------------------------------------------------------
+static int parse_sweeps(const std::string& s, uint64_t& mn, uint64_t& mx) {
+  // "[a:b]", "[a:]", or an integer
+  std::string t = s;
+  t.erase(std::remove(t.begin(), t.end(), ' '), t.end());
+  try {
+    if (!t.empty() && t.front() == '[') {
+      size_t c = t.find(':');
+      if (c == std::string::npos || t.back() != ']') return -1;
+      std::string a = t.substr(1, c - 1), b = t.substr(c + 1, t.size() - c - 2);
+      mn = a.empty() ? 0 : std::stoull(a);
+      // "[a:]": no upper bound -- runs until stopped, like parapack's open range
+      mx = b.empty() ? UINT64_MAX / 4 : std::stoull(b);
+      if (mx < mn) return -1;
+      return 0;
+    }
+    mn = mx = (uint64_t)std::llround(std::stod(t));
+  } catch (const std::exception&) {
+    return -1;
+  }
+  return 0;
+}
+
+class loop_worker {
+ public:
+  explicit loop_worker(const Parameters& p) {
+    std::string lat = p.str("LATTICE", "chain lattice");
+    if (lat == "chain lattice" || lat == "chain") lp_.lattice = LG_LATTICE_CHAIN;
+    else if (lat == "square lattice" || lat == "square") lp_.lattice = LG_LATTICE_SQUARE;
+    else throw std::invalid_argument("unsupported LATTICE \"" + lat + "\" (chain lattice, square lattice)");
+    lp_.L = (int)std::llround(p.num("L"));
+    if (lp_.lattice == LG_LATTICE_SQUARE && p.defined("W") && (int)std::llround(p.num("W")) != lp_.L)
+      throw std::invalid_argument("square lattice with W != L is not supported");
+    if (p.str("MODEL", "spin") != "spin") throw std::invalid_argument("only MODEL = spin is supported");
+    if (p.defined("local_S") && std::fabs(p.num("local_S") - 0.5) > 1e-12)
+      throw std::invalid_argument("only local_S = 1/2 is supported");
+    for (const char* k : {"h", "Gamma", "D"})
+      if (p.defined(k) && p.num(k) != 0.0)
+        throw std::invalid_argument(std::string("parameter ") + k + " != 0 (field / single-ion terms) is not supported");
+    if (p.defined("DISABLE_IMPROVED_ESTIMATOR"))
+      throw std::invalid_argument("DISABLE_IMPROVED_ESTIMATOR is not supported (improved estimators only)");
+    // temperature.h:43-55
+    double T;
+    if (p.defined("T")) T = p.num("T");
+    else if (p.defined("TEMPERATURE")) T = p.num("TEMPERATURE");
+    else if (p.defined("BETA")) T = 1 / p.num("BETA");
+    else if (p.defined("INVERSE_TEMPERATURE")) T = 1 / p.num("INVERSE_TEMPERATURE");
+    else throw std::invalid_argument("temperature is not initialized");
+    if (!(T > 0)) throw std::invalid_argument("negative temperature");
+    lp_.beta = 1 / T;
+    double J = p.num("J", 1.0);
+    lp_.jxy = p.num("Jxy", J);
+    lp_.jz = p.num("Jz", J);
+    lp_.force_scatter = p.num("FORCE_SCATTER", 0.0);
+    lp_.seed = (uint32_t)(int64_t)std::llround(p.num("SEED", 0.0));
+    lp_.chain_id = (uint32_t)std::llround(p.num("CHAIN_ID", 0.0));
+    if (parse_sweeps(p.str("SWEEPS", "[65536:]"), sweeps_min_, sweeps_max_))
+      throw std::invalid_argument("malformed SWEEPS");
+    therm_ = p.defined("THERMALIZATION") ? (uint64_t)std::llround(p.num("THERMALIZATION")) : (sweeps_min_ >> 3);
+    n_sites_ = (lp_.lattice == LG_LATTICE_CHAIN) ? lp_.L : lp_.L * lp_.L;
+    dim_ = (lp_.lattice == LG_LATTICE_CHAIN) ? 1 : 2;
+    int rc = lg_chain_create(&lp_, &h_);
+    if (rc != LG_OK) {
+      char buf[512];
+      lg_last_error(buf, sizeof buf);
+      if (rc == LG_ERR_ARG) throw std::invalid_argument(buf);
+      throw std::runtime_error(buf);
+    }
+  }
+  ~loop_worker() { if (h_) lg_chain_destroy(h_); }
+  loop_worker(const loop_worker&) = delete;
+  loop_worker& operator=(const loop_worker&) = delete;
+
+  // path_integral.C:307-321 + estimator init_observables
+  void init_observables(std::map<std::string, Observable>& obs) const {
+    for (const auto& n : names()) obs[n];
+  }
+  bool is_thermalized() const { return mcs_ >= therm_; }
+  double progress() const { return (double)mcs_ / (double)(therm_ + sweeps_min_); }
+  bool can_work() const { return mcs_ < therm_ || mcs_ - therm_ < sweeps_max_; }
+  uint64_t mcs() const { return mcs_; }
+
+  // one Monte Carlo step (path_integral.C:323-351); sweeps are fetched from the
+  // device in chunks, so run() usually only consumes a buffered record
+  void run(std::map<std::string, Observable>& obs) {
+    if (next_ == buf_.size()) {
+      uint64_t left = (mcs_ < therm_) ? therm_ - mcs_ : (therm_ + sweeps_max_ - mcs_);
+      uint32_t k = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(left, chunk_));
+      buf_.resize(k);
+      int rc = lg_chain_sweep(h_, k, buf_.data());
+      if (rc != LG_OK) {
+        char b[512];
+        lg_last_error(b, sizeof b);
+        throw std::runtime_error(b);
+      }
+      next_ = 0;
+    }
+    const lg_sweep_record& r = buf_[next_++];
+    if (is_thermalized()) measure(r, obs);
+    ++mcs_;
+  }
+
+  // checkpoint (path_integral.C:111-124): counters, spins, operators
+  std::vector<uint8_t> save() const {
+    if (next_ != buf_.size()) throw std::logic_error("save() with buffered sweeps; call at a chunk boundary");
+    int32_t ns = 0, nb = 0;
+    int64_t nops = 0;
+    uint64_t sw = 0;
+    lg_chain_info(h_, &ns, &nb, &nops, &sw);
+    std::vector<int32_t> spins(ns), bo(nb + 1), types(nops);
+    std::vector<double> times(nops);
+    lg_chain_get_config(h_, spins.data(), bo.data(), times.data(), types.data());
+    std::vector<uint8_t> out;
+    auto put = [&](const void* p, size_t n) { const uint8_t* b = (const uint8_t*)p; out.insert(out.end(), b, b + n); };
+    const char magic[8] = {'L', 'G', 'C', 'H', 'K', 'P', 'T', '1'};
+    put(magic, 8);
+    put(&mcs_, 8); put(&therm_, 8); put(&sweeps_min_, 8); put(&sweeps_max_, 8); put(&sw, 8);
+    put(&ns, 4); put(&nb, 4); put(&nops, 8);
+    put(spins.data(), 4 * ns); put(bo.data(), 4 * (nb + 1));
+    put(times.data(), 8 * nops); put(types.data(), 4 * nops);
+    return out;
+  }
+  void load(const uint8_t* p, size_t len) {
+    size_t off = 0;
+    auto get = [&](void* d, size_t n) {
+      if (off + n > len) throw std::invalid_argument("truncated checkpoint");
+      std::memcpy(d, p + off, n);
+      off += n;
+    };
+    char magic[8];
+    get(magic, 8);
+    if (std::memcmp(magic, "LGCHKPT1", 8) != 0) throw std::invalid_argument("not a loopgpu checkpoint");
+    uint64_t sw;
+    int32_t ns, nb;
+    int64_t nops;
+    get(&mcs_, 8); get(&therm_, 8); get(&sweeps_min_, 8); get(&sweeps_max_, 8); get(&sw, 8);
+    get(&ns, 4); get(&nb, 4); get(&nops, 8);
+    if (ns != n_sites_) throw std::invalid_argument("checkpoint lattice does not match the parameters");
+    std::vector<int32_t> spins(ns), bo(nb + 1), types(nops);
+    std::vector<double> times(nops);
+    get(spins.data(), 4 * ns); get(bo.data(), 4 * (nb + 1));
+    get(times.data(), 8 * nops); get(types.data(), 4 * nops);
+    int rc = lg_chain_set_config(h_, spins.data(), bo.data(), times.data(), types.data(), sw);
+    if (rc != LG_OK) {
+      char b[512];
+      lg_last_error(b, sizeof b);
+      throw std::invalid_argument(b);
+    }
+    buf_.clear();
+    next_ = 0;
+  }
+  void set_chunk(uint32_t c) { chunk_ = std::max<uint32_t>(1, c); }
+
+  static std::vector<std::string> names() {
+    static const char* base[] = {
+        "Temperature", "Inverse Temperature", "Volume", "Number of Sites", "Number of Clusters",
+        "Energy", "Energy Density", "Energy^2", "Stiffness"};
+    static const char* mag[] = {
+        "Magnetization", "Magnetization Density", "|Magnetization|", "|Magnetization Density|",
+        "Magnetization^2", "Magnetization Density^2", "Magnetization^4", "Magnetization Density^4",
+        "Susceptibility", "Generalized Magnetization^2", "Generalized Magnetization Density^2",
+        "Generalized Magnetization^4", "Generalized Magnetization Density^4", "Generalized Susceptibility"};
+    std::vector<std::string> v(base, base + 9);
+    for (const char* m : mag) v.push_back(m);
+    for (const char* m : mag) {
+      std::string s(m);
+      // insert "Staggered " after a leading "|" or "Generalized "
+      if (s[0] == '|') s = "|Staggered " + s.substr(1);
+      else if (s.rfind("Generalized ", 0) == 0) s = "Generalized Staggered " + s.substr(12);
+      else s = "Staggered " + s;
+      v.push_back(s);
+    }
+    return v;
+  }
+
+  int n_sites() const { return n_sites_; }
+
+ private:
+  // loop_worker::dispatch measurement (path_integral.C:831-862) and the
+  // collector commits (energy.h:70-80, susceptibility.h:200-256,
+  // stiffness.h:123-128), sign = 1
+  void measure(const lg_sweep_record& r, std::map<std::string, Observable>& m) const {
+    const double beta = r.beta, vol = n_sites_, nop = r.n_op;
+    auto p2 = [](double x) { return x * x; };
+    auto p4 = [](double x) { return x * x * x * x; };
+    m["Temperature"] << 1 / beta;
+    m["Inverse Temperature"] << beta;
+    m["Volume"] << vol;
+    m["Number of Sites"] << (double)n_sites_;
+    m["Number of Clusters"] << r.n_clusters;
+    m["Energy"] << r.energy;
+    m["Energy Density"] << r.energy / vol;
+    m["Energy^2"] << p2(r.energy) - nop / p2(beta);
+    m["Stiffness"] << r.w2 / (beta * dim_);
+    struct Set { const char* pre; double mag0, size2, mag2, size4, mag4, sus, gsus; };
+    const Set sets[2] = {
+        {"", r.umag0, r.usize2, r.umag2, r.usize4, r.umag4, r.umag, r.usize},
+        {"Staggered ", r.smag0, r.ssize2, r.smag2, r.ssize4, r.smag4, r.smag, r.ssize}};
+    for (const Set& s : sets) {
+      std::string P(s.pre);
+      std::string bar = "|" + P;
+      std::string gen = "Generalized " + P;
+      m[P + "Magnetization"] << 0.0;
+      m[P + "Magnetization Density"] << 0.0;
+      m[bar + "Magnetization|"] << std::fabs(s.mag0);
+      m[bar + "Magnetization Density|"] << std::fabs(s.mag0) / vol;
+      m[P + "Magnetization^2"] << s.mag2;
+      m[P + "Magnetization Density^2"] << s.mag2 / p2(vol);
+      m[P + "Magnetization^4"] << 3 * p2(s.mag2) - 2 * s.mag4;
+      m[P + "Magnetization Density^4"] << (3 * p2(s.mag2) - 2 * s.mag4) / p4(vol);
+      m[P + "Susceptibility"] << beta * s.sus / vol;
+      m[gen + "Magnetization^2"] << s.size2;
+      m[gen + "Magnetization Density^2"] << s.size2 / p2(vol);
+      m[gen + "Magnetization^4"] << 3 * p2(s.size2) - 2 * s.size4;
+      m[gen + "Magnetization Density^4"] << (3 * p2(s.size2) - 2 * s.size4) / p4(vol);
+      m[gen + "Susceptibility"] << beta * s.gsus / vol;
+    }
+  }
+
+  lg_params lp_{};
+  lg_chain* h_ = nullptr;
+  uint64_t mcs_ = 0, therm_ = 0, sweeps_min_ = 0, sweeps_max_ = 0;
+  int n_sites_ = 0, dim_ = 1;
+  std::vector<lg_sweep_record> buf_;
+  size_t next_ = 0;
+  uint32_t chunk_ = 1024;
+};
+
+// evaluator (energy.h:89-102, susceptibility.h:340-374)
+std::map<std::string, Result> evaluate(const std::map<std::string, Observable>& obs) {
+  std::map<std::string, Result> out;
+  for (const auto& kv : obs)
+    if (kv.second.count()) out[kv.first] = Result{kv.second.mean(), kv.second.error(), kv.second.tau(), false};
+  auto has = [&](const char* k) { return obs.count(k) && obs.at(k).count() > 0; };
+  if (has("Energy") && has("Energy^2") && has("Inverse Temperature") && has("Volume")) {
+    double b = obs.at("Inverse Temperature").mean(), v = obs.at("Volume").mean();
+    out["Specific Heat"] = jackknife(obs.at("Energy^2"), obs.at("Energy"),
+                                     [&](double e2, double e) { return b * b * (e2 - e * e) / v; });
+  }
+  const char* binder[][3] = {
+      {"Magnetization^2", "Magnetization^4", "Binder Ratio of Magnetization"},
+      {"Staggered Magnetization^2", "Staggered Magnetization^4", "Binder Ratio of Staggered Magnetization"},
+      {"Generalized Magnetization^2", "Generalized Magnetization^4", "Binder Ratio of Generalized Magnetization"},
+      {"Generalized Staggered Magnetization^2", "Generalized Staggered Magnetization^4",
+       "Binder Ratio of Generalized Staggered Magnetization"}};
+  for (auto& b : binder)
+    if (has(b[0]) && has(b[1]))
+      out[b[2]] = jackknife(obs.at(b[0]), obs.at(b[1]), [](double m2, double m4) { return m2 * m2 / m4; });
+  return out;
+}
+
+std::string report(const std::map<std::string, Result>& res) {
+  std::ostringstream os;
+  os.precision(6);
+  os << "[results]\n";
+  for (const auto& kv : res) {
+    os << kv.first << ": " << kv.second.mean << " +/- " << kv.second.error;
+    if (!kv.second.evaluated) os << "; tau = " << std::round(kv.second.tau * 100) / 100;
+    os << "\n";
+  }
+  return os.str();
+}
+
+}  // namespace looper_amd
+
+// ===========================================================================
+// C-ABI of the worker (include/loopgpu_worker.h)
+// ===========================================================================
+struct lg_worker {
+  std::unique_ptr<looper_amd::loop_worker> w;
+  std::map<std::string, looper_amd::Observable> obs;
+};
+
+namespace {
+thread_local std::string g_werr;
+int werr(int code, const std::string& m) { g_werr = m; return code; }
+}  // namespace
+
+extern "C" {
+
+int lg_worker_last_error(char* buf, size_t len) {
+  if (!buf || !len) return LG_ERR_ARG;
+  std::snprintf(buf, len, "%s", g_werr.c_str());
+  return LG_OK;
+}
+
+int lg_worker_create(const char* params, lg_worker** out) {
+  if (!params || !out) return werr(LG_ERR_ARG, "null pointer");
+  *out = nullptr;
+  try {
+    looper_amd::Parameters p(params);
+    lg_worker* h = new lg_worker();
+    try {
+      h->w.reset(new looper_amd::loop_worker(p));
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    h->w->init_observables(h->obs);
+    *out = h;
+    return LG_OK;
+  } catch (const std::invalid_argument& e) {
+    return werr(LG_ERR_ARG, e.what());
+  } catch (const std::exception& e) {
+    return werr(LG_ERR_DEVICE, e.what());
+  }
+}
+
+int lg_worker_run(lg_worker* h, uint64_t max_steps, uint64_t* done) {
+  if (!h) return werr(LG_ERR_ARG, "null handle");
+  uint64_t n = 0;
+  try {
+    while (h->w->can_work() && n < max_steps) { h->w->run(h->obs); ++n; }
+  } catch (const std::exception& e) {
+    if (done) *done = n;
+    return werr(LG_ERR_DEVICE, e.what());
+  }
+  if (done) *done = n;
+  return LG_OK;
+}
+
+int lg_worker_status(lg_worker* h, uint64_t* mcs, int* thermalized, double* progress, int* can_work) {
+  if (!h) return werr(LG_ERR_ARG, "null handle");
+  if (mcs) *mcs = h->w->mcs();
+  if (thermalized) *thermalized = h->w->is_thermalized();
+  if (progress) *progress = h->w->progress();
+  if (can_work) *can_work = h->w->can_work();
+  return LG_OK;
+}
+
+int lg_worker_result(lg_worker* h, const char* name, double* mean, double* error, double* tau) {
+  if (!h || !name) return werr(LG_ERR_ARG, "null pointer");
+  auto res = looper_amd::evaluate(h->obs);
+  auto it = res.find(name);
+  if (it == res.end()) return werr(LG_ERR_ARG, std::string("no observable named ") + name);
+  if (mean) *mean = it->second.mean;
+  if (error) *error = it->second.error;
+  if (tau) *tau = it->second.tau;
+  return LG_OK;
+}
+
+int lg_worker_report(lg_worker* h, char* buf, size_t len, size_t* needed) {
+  if (!h) return werr(LG_ERR_ARG, "null handle");
+  std::string s = looper_amd::report(looper_amd::evaluate(h->obs));
+  if (needed) *needed = s.size() + 1;
+  if (buf && len) std::snprintf(buf, len, "%s", s.c_str());
+  return LG_OK;
+}
+
+int lg_worker_save(lg_worker* h, void* buf, size_t* len) {
+  if (!h || !len) return werr(LG_ERR_ARG, "null pointer");
+  try {
+    std::vector<uint8_t> b = h->w->save();
+    if (buf && *len >= b.size()) std::memcpy(buf, b.data(), b.size());
+    bool fits = buf && *len >= b.size();
+    *len = b.size();
+    return (buf && !fits) ? werr(LG_ERR_ARG, "buffer too small") : LG_OK;
+  } catch (const std::exception& e) {
+    return werr(LG_ERR_STATE, e.what());
+  }
+}
+
+int lg_worker_load(lg_worker* h, const void* buf, size_t len) {
+  if (!h || !buf) return werr(LG_ERR_ARG, "null pointer");
+  try {
+    h->w->load((const uint8_t*)buf, len);
+    return LG_OK;
+  } catch (const std::invalid_argument& e) {
+    return werr(LG_ERR_ARG, e.what());
+  } catch (const std::exception& e) {
+    return werr(LG_ERR_DEVICE, e.what());
+  }
+}
+
+int lg_worker_set_chunk(lg_worker* h, uint32_t sweeps) {
+  if (!h) return werr(LG_ERR_ARG, "null handle");
+  h->w->set_chunk(sweeps);
+  return LG_OK;
+}
+
+int lg_worker_destroy(lg_worker* h) {
+  delete h;
+  return LG_OK;
+}
+
+}  // extern "C"

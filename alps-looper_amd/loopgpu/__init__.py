@@ -72,6 +72,16 @@ def _load():
     lib.lg_profile_kernel_name.restype = ctypes.c_char_p
     lib.lg_chain_profile.argtypes = [vp, ctypes.c_uint32, vp]
     lib.lg_batch_profile.argtypes = [vp, ctypes.c_uint32, vp]
+    lib.lg_worker_last_error.argtypes = [ctypes.c_char_p, ctypes.c_size_t]
+    lib.lg_worker_create.argtypes = [ctypes.c_char_p, P(vp)]
+    lib.lg_worker_run.argtypes = [vp, ctypes.c_uint64, P(ctypes.c_uint64)]
+    lib.lg_worker_status.argtypes = [vp, P(ctypes.c_uint64), P(ctypes.c_int), P(ctypes.c_double), P(ctypes.c_int)]
+    lib.lg_worker_result.argtypes = [vp, ctypes.c_char_p, P(ctypes.c_double), P(ctypes.c_double), P(ctypes.c_double)]
+    lib.lg_worker_report.argtypes = [vp, ctypes.c_char_p, ctypes.c_size_t, P(ctypes.c_size_t)]
+    lib.lg_worker_save.argtypes = [vp, vp, P(ctypes.c_size_t)]
+    lib.lg_worker_load.argtypes = [vp, vp, ctypes.c_size_t]
+    lib.lg_worker_set_chunk.argtypes = [vp, ctypes.c_uint32]
+    lib.lg_worker_destroy.argtypes = [vp]
     return lib
 
 
@@ -222,3 +232,67 @@ def cluster_config(p, spins, bo, times, types, sweep):
     _check(lib.lg_cluster_config(ctypes.byref(p), _ptr(spins), _ptr(bo), _ptr(times), _ptr(types),
                                  int(sweep), _ptr(labels), _ptr(spo), _ptr(tyo), ctypes.byref(nc)))
     return labels, spo, tyo[:nops], nc.value
+
+
+def _wcheck(rc):
+    if rc != 0:
+        buf = ctypes.create_string_buffer(1024)
+        lib.lg_worker_last_error(buf, 1024)
+        raise LoopGPUError(rc, buf.value.decode(errors="replace"))
+
+
+class Worker:
+    """The reference worker's parameter / run / measure interface
+    (include/loopgpu_worker.h)."""
+
+    def __init__(self, params):
+        if isinstance(params, dict):
+            params = "\n".join("%s = %s" % (k, ('"%s"' % v) if isinstance(v, str) else v) for k, v in params.items())
+        self.h = ctypes.c_void_p()
+        _wcheck(lib.lg_worker_create(params.encode(), ctypes.byref(self.h)))
+
+    def run(self, max_steps=1 << 62):
+        done = ctypes.c_uint64()
+        _wcheck(lib.lg_worker_run(self.h, max_steps, ctypes.byref(done)))
+        return done.value
+
+    def status(self):
+        mcs, th, pr, cw = ctypes.c_uint64(), ctypes.c_int(), ctypes.c_double(), ctypes.c_int()
+        _wcheck(lib.lg_worker_status(self.h, ctypes.byref(mcs), ctypes.byref(th), ctypes.byref(pr), ctypes.byref(cw)))
+        return {"mcs": mcs.value, "thermalized": bool(th.value), "progress": pr.value, "can_work": bool(cw.value)}
+
+    def result(self, name):
+        m, e, t = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        _wcheck(lib.lg_worker_result(self.h, name.encode(), ctypes.byref(m), ctypes.byref(e), ctypes.byref(t)))
+        return m.value, e.value, t.value
+
+    def report(self):
+        need = ctypes.c_size_t()
+        _wcheck(lib.lg_worker_report(self.h, None, 0, ctypes.byref(need)))
+        buf = ctypes.create_string_buffer(need.value)
+        _wcheck(lib.lg_worker_report(self.h, buf, need.value, None))
+        return buf.value.decode()
+
+    def save(self):
+        n = ctypes.c_size_t(0)
+        _wcheck(lib.lg_worker_save(self.h, None, ctypes.byref(n)))
+        buf = ctypes.create_string_buffer(n.value)
+        _wcheck(lib.lg_worker_save(self.h, buf, ctypes.byref(n)))
+        return buf.raw[:n.value]
+
+    def load(self, blob):
+        _wcheck(lib.lg_worker_load(self.h, blob, len(blob)))
+
+    def set_chunk(self, n):
+        _wcheck(lib.lg_worker_set_chunk(self.h, n))
+
+    def close(self):
+        if self.h:
+            lib.lg_worker_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,61 @@
+/*
+ * loopgpu worker: the reference worker's parameter and update/measure
+ * interface over libloopgpu.so.
+ *
+ * Mirrors alps::parapack::mc_worker as implemented by loop_worker
+ * (path_integral.C:57-351): construction from parameters
+ * (loop_worker(alps::Parameters const&), path_integral.C:202-305),
+ * run() = one Monte Carlo step (path_integral.C:323-351), is_thermalized() /
+ * progress() (montecarlo.h:55-61), save()/load() (path_integral.C:111-124),
+ * and the results of the evaluator (energy.h:89-102,
+ * susceptibility.h:340-374) with binning errors -- the observable names of
+ * init_observables (path_integral.C:307-321, energy.h:49-53,
+ * susceptibility.h:59-94, stiffness.h:78-80).
+ *
+ * Parameters are text in the reference's .ip syntax ("KEY = value;" or one
+ * per line; expressions over other parameters allowed), keys: LATTICE
+ * ("chain lattice" | "square lattice"), L, W, T | TEMPERATURE | BETA |
+ * INVERSE_TEMPERATURE, J, Jxy, Jz, MODEL (= spin), local_S (= 1/2), SWEEPS
+ * (default "[65536:]"), THERMALIZATION (default SWEEPS.min/8), SEED,
+ * FORCE_SCATTER, CHAIN_ID.  Invalid or unsupported parameters give
+ * LG_ERR_ARG (the reference throws std::invalid_argument).
+ */
+#ifndef LOOPGPU_WORKER_H
+#define LOOPGPU_WORKER_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "loopgpu.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct lg_worker lg_worker;
+
+int lg_worker_last_error(char* buf, size_t len);
+/* reference: loop_worker(alps::Parameters const&) + init_observables */
+int lg_worker_create(const char* params, lg_worker** out);
+/* up to max_steps calls of run() while the schedule can_work()
+ * (montecarlo.h:55-57); measurements start once is_thermalized() */
+int lg_worker_run(lg_worker* h, uint64_t max_steps, uint64_t* done);
+/* reference: mcs(), is_thermalized(), progress(), can_work() */
+int lg_worker_status(lg_worker* h, uint64_t* mcs, int* thermalized, double* progress, int* can_work);
+/* mean, binning error and integrated autocorrelation time of an observable
+ * or evaluated quantity ("Specific Heat", "Binder Ratio of ...") */
+int lg_worker_result(lg_worker* h, const char* name, double* mean, double* error, double* tau);
+/* "[results]" block, one "Name: mean +/- error; tau = t" line per observable */
+int lg_worker_report(lg_worker* h, char* buf, size_t len, size_t* needed);
+/* checkpoint blob: MC counters, spins, operators (types and times) */
+int lg_worker_save(lg_worker* h, void* buf, size_t* len);
+int lg_worker_load(lg_worker* h, const void* buf, size_t len);
+/* sweeps fetched from the device per batch of run() calls (default 1024) */
+int lg_worker_set_chunk(lg_worker* h, uint32_t sweeps);
+int lg_worker_destroy(lg_worker* h);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LOOPGPU_WORKER_H */

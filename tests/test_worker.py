@@ -1,0 +1,78 @@
+"""The worker boundary (include/loopgpu_worker.h): the reference worker's
+parameters, observable names and update/measure interface."""
+
+import json
+import os
+
+import numpy as np
+import pytest
+
+import stats
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+FX = json.load(open(os.path.join(HERE, "golden", "reference_fixtures.json")))
+
+
+@pytest.mark.parametrize("txt,frag", [
+    ("L=7; T=1", "even"),
+    ('LATTICE="chain lattice"; T=0.1', "L not defined"),
+    ('LATTICE="triangular lattice"; L=4; T=1', "unsupported LATTICE"),
+    ("L=4; T=1; h=0.3", "not supported"),
+    ("L=4; local_S=1; T=1", "local_S"),
+    ("L=4; T=-1", "negative temperature"),
+    ("L=4", "temperature is not initialized"),
+    ("L=4; T=1; SWEEPS=[100:x]", "SWEEPS"),
+    ("L=4; T=1; DISABLE_IMPROVED_ESTIMATOR=1", "improved"),
+    ("L=4; T=1/Q", "unknown name"),
+])
+def test_worker_parameter_errors(txt, frag):
+    import loopgpu
+    with pytest.raises(loopgpu.LoopGPUError) as e:
+        loopgpu.Worker(txt)
+    assert e.value.code == -1
+    assert frag in str(e.value)
+
+
+@pytest.mark.gpu
+def test_worker_matches_reference_run_and_ed(loopgpu):
+    # loop.ip block: chain L=4, J=1, T=0.25 (loop.op: ED and "loop; path integral")
+    w = loopgpu.Worker('LATTICE = "chain lattice"\nL = 4\nJ = 1\nT = 0.25\nSWEEPS = 65536\nSEED = 23125')
+    print("worker run", flush=True)
+    w.run()
+    st = w.status()
+    assert st["mcs"] == 65536 + 8192 and not st["can_work"]
+    ed, qmc = FX["loop_op_chain4"][0]["results"], FX["loop_op_chain4"][1]["results"]
+    for name in ["Energy", "Energy Density", "Staggered Magnetization^2", "Staggered Susceptibility",
+                 "Susceptibility", "Magnetization^2", "Specific Heat", "Binder Ratio of Staggered Magnetization"]:
+        m, e, _ = w.result(name)
+        assert stats.sigmas(m, e, ed[name][0]) < 3.5, (name, m, e, ed[name][0])
+        if name in qmc:
+            assert stats.sigmas(m, e, qmc[name][0], qmc[name][1]) < 3.5, (name, m, e, qmc[name])
+    rep = w.report()
+    assert rep.startswith("[results]")
+    for name in ["Number of Clusters", "Generalized Staggered Susceptibility", "|Staggered Magnetization|",
+                 "Stiffness", "Temperature", "Volume"]:
+        assert name + ": " in rep
+
+
+@pytest.mark.gpu
+def test_worker_checkpoint_resume_is_exact(loopgpu):
+    p = 'LATTICE = "square lattice"\nL = 6\nT = 0.3\nSWEEPS = 3000\nTHERMALIZATION = 200\nSEED = 9'
+    a = loopgpu.Worker(p)
+    a.set_chunk(100)
+    a.run(1000)
+    blob = a.save()
+    a.run()
+    b = loopgpu.Worker(p)
+    b.load(blob)
+    b.set_chunk(100)
+    # b has not seen the first 800 measurements: compare the raw trajectory
+    # through a third worker that ran straight through
+    c = loopgpu.Worker(p)
+    c.set_chunk(100)
+    c.run()
+    ma, ea, _ = a.result("Energy")
+    mc, ec, _ = c.result("Energy")
+    assert ma == mc and ea == ec
+    b.run()
+    assert b.status()["mcs"] == c.status()["mcs"]
