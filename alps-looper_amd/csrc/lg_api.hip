@@ -773,6 +773,41 @@ int lg_batch_chain_info(lg_batch* h, int32_t chain, int64_t* n_ops, uint64_t* sw
   return LG_OK;
 }
 
+// plain streaming copy used to calibrate HBM counters and measure the
+// achievable bandwidth
+__global__ void __launch_bounds__(256) k_copy_stream(const double4* __restrict__ a, double4* __restrict__ b,
+                                                     size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    b[i] = a[i];
+}
+
+int lg_copy_bandwidth(uint64_t bytes, int iters, double* gbs) {
+  if (!gbs || bytes < 32 || iters < 1) return set_error(LG_ERR_ARG, "invalid copy size");
+  size_t n = bytes / sizeof(double4);
+  double4* a = nullptr;
+  double4* b = nullptr;
+  hipEvent_t e0, e1;
+  LG_HIP(hipMalloc(&a, n * sizeof(double4)));
+  if (hipMalloc(&b, n * sizeof(double4)) != hipSuccess) { (void)hipFree(a); return set_error(LG_ERR_DEVICE, "hipMalloc"); }
+  LG_HIP(hipMemset(a, 1, n * sizeof(double4)));
+  LG_HIP(hipEventCreate(&e0));
+  LG_HIP(hipEventCreate(&e1));
+  int blocks = 256 * 16;
+  hipLaunchKernelGGL(k_copy_stream, dim3(blocks), dim3(256), 0, 0, a, b, n);  // warm
+  LG_HIP(hipEventRecord(e0, 0));
+  for (int i = 0; i < iters; ++i) hipLaunchKernelGGL(k_copy_stream, dim3(blocks), dim3(256), 0, 0, a, b, n);
+  LG_HIP(hipEventRecord(e1, 0));
+  LG_HIP(hipEventSynchronize(e1));
+  float ms = 0;
+  LG_HIP(hipEventElapsedTime(&ms, e0, e1));
+  *gbs = 2.0 * n * sizeof(double4) * iters / (ms * 1e-3) / 1e9;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(a);
+  (void)hipFree(b);
+  return LG_OK;
+}
+
 int lg_profile_num_kernels(void) { return lg::NUM_SWEEP_KERNELS; }
 
 const char* lg_profile_kernel_name(int i) {

@@ -29,6 +29,7 @@ EXPORTED = [
     "lg_batch_create", "lg_batch_run", "lg_batch_tune", "lg_batch_set_graph",
     "lg_batch_chain_info", "lg_batch_destroy",
     "lg_profile_num_kernels", "lg_profile_kernel_name", "lg_chain_profile", "lg_batch_profile",
+    "lg_copy_bandwidth",
 ]
 
 
@@ -72,6 +73,7 @@ def _load():
     lib.lg_profile_kernel_name.restype = ctypes.c_char_p
     lib.lg_chain_profile.argtypes = [vp, ctypes.c_uint32, vp]
     lib.lg_batch_profile.argtypes = [vp, ctypes.c_uint32, vp]
+    lib.lg_copy_bandwidth.argtypes = [ctypes.c_uint64, ctypes.c_int, P(ctypes.c_double)]
     lib.lg_worker_last_error.argtypes = [ctypes.c_char_p, ctypes.c_size_t]
     lib.lg_worker_create.argtypes = [ctypes.c_char_p, P(vp)]
     lib.lg_worker_run.argtypes = [vp, ctypes.c_uint64, P(ctypes.c_uint64)]
@@ -99,6 +101,13 @@ def device_count():
     n = ctypes.c_int(0)
     _check(lib.lg_device_count(ctypes.byref(n)))
     return n.value
+
+
+def copy_bandwidth(nbytes, iters=20):
+    """achieved GB/s (read + write) of a plain device copy of nbytes"""
+    g = ctypes.c_double()
+    _check(lib.lg_copy_bandwidth(int(nbytes), int(iters), ctypes.byref(g)))
+    return g.value
 
 
 def kernel_names():

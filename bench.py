@@ -107,14 +107,27 @@ def cpu_baseline(kind, L, beta, seconds):
     }
 
 
-def roofline(prof, nop_mean):
+def measured_traffic(name, kernel, nop_mean):
+    """HBM bytes per launch of `kernel` from the committed FETCH_SIZE/WRITE_SIZE
+    summary (profiles/<name>.json, copy-kernel calibrated), scaled to nop_mean"""
+    path = os.path.join(ROOT, "profiles", name + ".json")
+    try:
+        with open(path) as f:
+            d = json.load(f)
+        return d["kernels"][kernel]["bytes_per_op"] * nop_mean
+    except (OSError, KeyError, TypeError):
+        return None
+
+
+def roofline(prof, nop_mean, traffic_profile=None):
     dom = max(prof, key=lambda k: prof[k])
     ms = prof[dom]
     bytes_launch = BYTES_PER_OP[dom] * nop_mean
     achieved = bytes_launch / (ms * 1e-3) / 1e9
     return {
         "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-        "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+        "frac": achieved / HBM_PEAK_GBS,
+        "traffic": measured_traffic(traffic_profile, dom, nop_mean) if traffic_profile else None,
         "kernel": dom, "kernel_ms": ms, "bytes_per_op": BYTES_PER_OP[dom],
         "bytes_per_launch": bytes_launch,
     }
@@ -135,7 +148,7 @@ def run_c2(args, lg):
     value = nop_total / dt
     log("profile %d sweeps" % args.profile_sweeps)
     prof = ch.profile(args.profile_sweeps)
-    rl = roofline(prof, nop_mean)
+    rl = roofline(prof, nop_mean, "c2_square128_hbm_traffic")
     out = {
         "metric": METRIC, "value": value, "unit": "spin-operator updates/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -204,6 +217,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", choices=["c2", "scan"], default="c2")
+    ap.add_argument("--copy", type=int, default=0, metavar="MIB",
+                    help="only run the plain copy kernel on MIB MiB (counter calibration) and print GB/s")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on exactly one GPU; --gpus %d is not supported "
@@ -214,6 +229,10 @@ def main():
     if lg.device_count() < 1:
         print("bench.py: no HIP device", file=sys.stderr)
         sys.exit(1)
+    if args.copy:
+        gbs = lg.copy_bandwidth(args.copy << 20, 20)
+        print(json.dumps({"copy_mib": args.copy, "iters": 21, "gbs": gbs}), flush=True)
+        return
     if args.workload == "c2":
         out, cpu_cfg = run_c2(args, lg)
     else:

@@ -132,6 +132,11 @@ int lg_profile_num_kernels(void);
 const char* lg_profile_kernel_name(int i);
 int lg_chain_profile(lg_chain* h, uint32_t n, double* avg_ms);
 int lg_batch_profile(lg_batch* h, uint32_t n, double* avg_ms);
+/* Plain device copy of `bytes` bytes repeated `iters` times (kernel
+ * k_copy_stream); gbs = (read + write bytes) / time.  No counterpart in the
+ * reference: it calibrates the HBM byte counters and measures the achievable
+ * bandwidth against which the kernels are judged. */
+int lg_copy_bandwidth(uint64_t bytes, int iters, double* gbs);
 
 #ifdef __cplusplus
 }
