@@ -25,6 +25,8 @@ int set_error(int code, const std::string& msg) {
   return code;
 }
 
+std::string g_last_error_copy() { return g_last_error; }
+
 #define LG_HIP(call)                                                                   \
   do {                                                                                 \
     hipError_t e_ = (call);                                                            \
@@ -144,64 +146,108 @@ int Batch::init_for_config(const lg_params* p, long long nops) {
   return init(p, 1);
 }
 
+Footprint footprint(const lg_params& p) {
+  ChainDev d = make_chain_dev(p);
+  Footprint f;
+  f.sites = d.N;
+  f.bonds = d.Nb;
+  f.ops = op_capacity(p, d);
+  f.cand = (long long)d.Nb * d.cand_cap;
+  return f;
+}
+
+static int check_device() {
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+    return set_error(LG_ERR_DEVICE, "no HIP device available");
+  return LG_OK;
+}
+
+// Static batch: capacity is exactly what the given chains need.
 int Batch::init(const lg_params* p, int n) {
   if (n <= 0) return set_error(LG_ERR_ARG, "need at least one chain");
   for (int c = 0; c < n; ++c) {
     int rc = check_params(p[c]);
     if (rc) return rc;
   }
-  int ndev = 0;
-  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
-    return set_error(LG_ERR_DEVICE, "no HIP device available");
-  params.assign(p, p + n);
-  chains.resize(n);
-  long long sites = 0, bonds = 0, cand = 0, ops = 0;
+  int rc = check_device();
+  if (rc) return rc;
+  Capacity cap;
+  cap.chains = n;
   for (int c = 0; c < n; ++c) {
-    int rc = check_params(p[c]);
-    if (rc) return rc;
-    ChainDev d = make_chain_dev(p[c]);
-    d.site_base = d.old_site_base = (int)sites;
-    d.bond_base = d.old_bond_base = (int)bonds;
-    d.cand_base = cand;
-    sites += d.N;
-    bonds += d.Nb;
-    cand += (long long)d.Nb * d.cand_cap;
-    ops += op_capacity(p[c], d);
-    chains[c] = d;
+    Footprint f = footprint(p[c]);
+    cap.sites += f.sites;
+    cap.bonds += f.bonds;
+    cap.ops += f.ops;
+    cap.cand += f.cand;
   }
-  if (sites >= (1ll << 30) || bonds >= (1ll << 28) || ops >= (1ll << 30))
-    return set_error(LG_ERR_ARG, "batch too large for 32-bit indexing");
-  nsites = (int)sites;
-  nbonds = (int)bonds;
-  cand_total = cand;
-  ops = std::max(ops, min_op_cap);
-  op_cap = (int)ops;
-  long long ncap = 2 * ops + sites;
-  if (ncap >= (1ll << 31) - 1) return set_error(LG_ERR_ARG, "node capacity exceeds 32-bit indexing");
-  node_cap = (int)ncap;
-  LG_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-  int rc = allocate();
+  cap.ops = std::max(cap.ops, min_op_cap);
+  rc = reserve(cap, false);
+  if (rc) return rc;
+  std::vector<ChainDev> ds(n);
+  for (int c = 0; c < n; ++c) ds[c] = make_chain_dev(p[c]);
+  params.assign(p, p + n);
+  rc = set_layout(ds, std::vector<uint64_t>(n, 0));
   if (rc) return rc;
   return reset_state();
+}
+
+// Server batch: capacity from a budget, no chains yet.  Site/bond arrays get
+// room for the previous layout plus newly appended chains (2x the budget).
+int Batch::init_capacity(const Capacity& cap) {
+  int rc = check_device();
+  if (rc) return rc;
+  rc = reserve(cap, true);
+  if (rc) return rc;
+  chains.clear();
+  params.clear();
+  host_sweep.clear();
+  nsites = nbonds = 0;
+  old_ext_sites = old_ext_bonds = 0;
+  old_ext_nodes = 0;
+  cur = 0;
+  std::vector<int> z(2, 0);
+  LG_HIP(hipMemcpy(st[cur].bo, z.data(), sizeof(int), hipMemcpyHostToDevice));
+  LG_HIP(hipMemcpy(st[cur].soff, z.data(), sizeof(int), hipMemcpyHostToDevice));
+  return LG_OK;
+}
+
+int Batch::reserve(const Capacity& cap, bool dynamic) {
+  if (cap.chains < 1) return set_error(LG_ERR_ARG, "capacity needs at least one chain");
+  const long long sites = dynamic ? 2 * cap.sites : cap.sites;
+  const long long bonds = dynamic ? 2 * cap.bonds : cap.bonds;
+  if (sites >= (1ll << 30) || bonds >= (1ll << 28) || cap.ops >= (1ll << 30))
+    return set_error(LG_ERR_ARG, "batch too large for 32-bit indexing");
+  long long ncap = 2 * cap.ops + sites;
+  if (ncap >= (1ll << 31) - 1) return set_error(LG_ERR_ARG, "node capacity exceeds 32-bit indexing");
+  budget = cap;
+  cap_chains = cap.chains;
+  cap_sites = (int)sites;
+  cap_bonds = (int)bonds;
+  op_cap = (int)cap.ops;
+  node_cap = (int)ncap;
+  cand_total = cap.cand;
+  LG_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  return allocate();
 }
 
 int Batch::allocate() {
   int rc = 0;
   for (int i = 0; i < 2; ++i) {
     State& s = st[i];
-    if ((rc = alloc(s.spins, nsites))) return rc;
-    if ((rc = alloc(s.bo, nbonds + 1))) return rc;
+    if ((rc = alloc(s.spins, cap_sites))) return rc;
+    if ((rc = alloc(s.bo, cap_bonds + 1))) return rc;
     if ((rc = alloc(s.ot, op_cap))) return rc;
     if ((rc = alloc(s.oi, op_cap))) return rc;
-    if ((rc = alloc(s.soff, nsites + 1))) return rc;
+    if ((rc = alloc(s.soff, cap_sites + 1))) return rc;
     if ((rc = alloc(s.evt, node_cap))) return rc;
     if ((rc = alloc(s.segspin, node_cap))) return rc;
   }
-  if ((rc = alloc(d_chains, chains.size()))) return rc;
-  if ((rc = alloc(bond_chain, nbonds))) return rc;
-  if ((rc = alloc(site_chain, nsites))) return rc;
-  if ((rc = alloc(counts, nbonds))) return rc;
-  if ((rc = alloc(acc_count, nbonds))) return rc;
+  if ((rc = alloc(d_chains, cap_chains))) return rc;
+  if ((rc = alloc(bond_chain, cap_bonds))) return rc;
+  if ((rc = alloc(site_chain, cap_sites))) return rc;
+  if ((rc = alloc(counts, cap_bonds))) return rc;
+  if ((rc = alloc(acc_count, cap_bonds))) return rc;
   if ((rc = alloc(kpre, op_cap))) return rc;
   if ((rc = alloc(cand_t, cand_total))) return rc;
   if ((rc = alloc(cand_g, cand_total))) return rc;
@@ -213,51 +259,72 @@ int Batch::allocate() {
   if ((rc = alloc(parent, node_cap))) return rc;
   if ((rc = alloc(fbit, node_cap))) return rc;
   if ((rc = alloc(acc, node_cap))) return rc;
-  scan_nparts = (nbonds + 2047) / 2048 + (nsites + 2047) / 2048;
+  scan_nparts = (cap_bonds + 2047) / 2048 + (cap_sites + 2047) / 2048;
   if ((rc = alloc(scan_part, scan_nparts))) return rc;
   if ((rc = alloc(col_part, (size_t)col_chunks(node_cap) * 16))) return rc;
   if ((rc = alloc(totals, 2))) return rc;
   if ((rc = alloc(flags, 1))) return rc;
-  if ((rc = alloc(d_sweep, chains.size()))) return rc;
-  if ((rc = alloc(rec_acc, chains.size() * R_COUNT))) return rc;
-  ring_size = (int)std::max<long long>(16, std::min<long long>(4096, (64ll << 20) / (chains.size() * R_COUNT * 8)));
-  if ((rc = alloc(rec_ring, (size_t)ring_size * chains.size() * R_COUNT))) return rc;
+  if ((rc = alloc(d_sweep, cap_chains))) return rc;
+  if ((rc = alloc(rec_acc, (size_t)cap_chains * R_COUNT))) return rc;
+  ring_size = (int)std::max<long long>(16, std::min<long long>(4096, (64ll << 20) / ((long long)cap_chains * R_COUNT * 8)));
+  if ((rc = alloc(rec_ring, (size_t)ring_size * cap_chains * R_COUNT))) return rc;
   if ((rc = alloc(ring_pos, 1))) return rc;
   if ((rc = alloc(end_count, 1))) return rc;
   LG_HIP(hipMemset(end_count, 0, sizeof(int)));
-  // static tables
-  std::vector<int> bc(nbonds), sc(nsites);
-  for (size_t c = 0; c < chains.size(); ++c) {
-    for (int b = 0; b < chains[c].Nb; ++b) bc[chains[c].bond_base + b] = (int)c;
-    for (int s = 0; s < chains[c].N; ++s) sc[chains[c].site_base + s] = (int)c;
-  }
-  LG_HIP(hipMemcpy(d_chains, chains.data(), chains.size() * sizeof(ChainDev), hipMemcpyHostToDevice));
-  LG_HIP(hipMemcpy(bond_chain, bc.data(), nbonds * sizeof(int), hipMemcpyHostToDevice));
-  LG_HIP(hipMemcpy(site_chain, sc.data(), nsites * sizeof(int), hipMemcpyHostToDevice));
-  LG_HIP(hipMemset(rec_acc, 0, chains.size() * R_COUNT * sizeof(double)));
+  LG_HIP(hipMemset(rec_acc, 0, (size_t)cap_chains * R_COUNT * sizeof(double)));
   LG_HIP(hipMemset(flags, 0, sizeof(int)));
   LG_HIP(hipMemset(ring_pos, 0, sizeof(int)));
   LG_HIP(hipMemset(totals, 0, 2 * sizeof(int)));
-  // launch geometry
   int dev = 0, cus = 256;
-  hipGetDevice(&dev);
+  (void)hipGetDevice(&dev);
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, dev) == hipSuccess) cus = prop.multiProcessorCount;
   cfg.blocks_op = std::max(64, cus * 8);
   cfg.blocks_node = std::max(64, cus * 8);
-  double lam_max = 0, ops_per_bond = 0;
-  for (size_t c = 0; c < chains.size(); ++c) {
-    double lam = chains[c].has_weight ? std::ldexp(1.0, 52) / chains[c].scale : 0.0;
-    lam_max = std::max(lam_max, lam);
-  }
-  ops_per_bond = lam_max;
-  int G = 4;
-  while (G < 64 && G < ops_per_bond) G *= 2;
-  cfg.group = G;
   return LG_OK;
 }
 
-// all-up spins, empty operator string (path_integral.C:225)
+// New bases (contiguous in the given order), chain tables, sweep counters.
+int Batch::set_layout(std::vector<ChainDev>& ds, const std::vector<uint64_t>& sweeps) {
+  long long sites = 0, bonds = 0, cand = 0;
+  for (auto& d : ds) {
+    d.site_base = (int)sites;
+    d.bond_base = (int)bonds;
+    d.cand_base = cand;
+    sites += d.N;
+    bonds += d.Nb;
+    cand += (long long)d.Nb * d.cand_cap;
+  }
+  if ((int)ds.size() > cap_chains || sites > budget.sites || bonds > budget.bonds || cand > cand_total)
+    return set_error(LG_ERR_CAPACITY, "chains exceed the batch capacity");
+  chains = ds;
+  nsites = (int)sites;
+  nbonds = (int)bonds;
+  host_sweep = sweeps;
+  std::vector<int> bc(std::max(nbonds, 1)), sc(std::max(nsites, 1));
+  std::vector<uint32_t> sw(std::max<size_t>(chains.size(), 1));
+  for (size_t c = 0; c < chains.size(); ++c) {
+    for (int b = 0; b < chains[c].Nb; ++b) bc[chains[c].bond_base + b] = (int)c;
+    for (int s = 0; s < chains[c].N; ++s) sc[chains[c].site_base + s] = (int)c;
+    sw[c] = (uint32_t)sweeps[c];
+  }
+  if (!chains.empty()) {
+    LG_HIP(hipMemcpy(d_chains, chains.data(), chains.size() * sizeof(ChainDev), hipMemcpyHostToDevice));
+    LG_HIP(hipMemcpy(bond_chain, bc.data(), nbonds * sizeof(int), hipMemcpyHostToDevice));
+    LG_HIP(hipMemcpy(site_chain, sc.data(), nsites * sizeof(int), hipMemcpyHostToDevice));
+    LG_HIP(hipMemcpy(d_sweep, sw.data(), chains.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+  }
+  double lam_max = 0;
+  for (auto& d : chains)
+    if (d.has_weight) lam_max = std::max(lam_max, std::ldexp(1.0, 52) / d.scale);
+  int G = 4;
+  while (G < 64 && G < lam_max) G *= 2;
+  if (!tuned) cfg.group = G;
+  invalidate_graphs();
+  return LG_OK;
+}
+
+// all-up spins, empty operator string (path_integral.C:225) for every chain
 int Batch::reset_state() {
   cur = 0;
   State& s = st[cur];
@@ -269,10 +336,80 @@ int Batch::reset_state() {
   LG_HIP(hipMemcpy(s.soff, soff.data(), (nsites + 1) * sizeof(int), hipMemcpyHostToDevice));
   LG_HIP(hipMemcpy(s.evt, evt.data(), nsites * sizeof(double), hipMemcpyHostToDevice));
   LG_HIP(hipMemset(s.segspin, 0, nsites));
-  LG_HIP(hipMemset(d_sweep, 0, chains.size() * sizeof(uint32_t)));
+  for (auto& d : chains) { d.old_site_base = d.site_base; d.old_bond_base = d.bond_base; }
+  LG_HIP(hipMemcpy(d_chains, chains.data(), chains.size() * sizeof(ChainDev), hipMemcpyHostToDevice));
+  std::vector<uint32_t> sw(chains.size(), 0);
+  LG_HIP(hipMemcpy(d_sweep, sw.data(), chains.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
   host_sweep.assign(chains.size(), 0);
+  old_ext_sites = nsites;
+  old_ext_bonds = nbonds;
+  old_ext_nodes = nsites;
   invalidate_graphs();
   return LG_OK;
+}
+
+// Change the set of chains between sweeps: chains keep[i] continue from their
+// current configuration (read at their present bases), chains `add` start
+// fresh (all-up spins, no operators), appended to the previous configuration
+// after its current extent.  Nothing is copied on the device; the next sweep
+// reads each chain at its old bases and writes the new layout.
+int Batch::relayout(const std::vector<int>& keep, const std::vector<lg_params>& add) {
+  std::vector<ChainDev> ds;
+  std::vector<uint64_t> sw;
+  std::vector<lg_params> ps;
+  for (int i : keep) {
+    if (i < 0 || i >= (int)chains.size()) return set_error(LG_ERR_ARG, "relayout: bad chain index");
+    ChainDev d = chains[i];
+    if (swept_since_layout) { d.old_site_base = d.site_base; d.old_bond_base = d.bond_base; }
+    ds.push_back(d);
+    sw.push_back(host_sweep[i]);
+    ps.push_back(params[i]);
+  }
+  if (swept_since_layout) {
+    old_ext_sites = nsites;
+    old_ext_bonds = nbonds;
+    int m = 0;
+    if (nsites > 0) LG_HIP(hipMemcpy(&m, st[cur].soff + nsites, sizeof(int), hipMemcpyDeviceToHost));
+    old_ext_nodes = m;
+  }
+  if (!add.empty()) {
+    State& s = st[cur];
+    int b0 = 0;
+    LG_HIP(hipMemcpy(&b0, s.bo + old_ext_bonds, sizeof(int), hipMemcpyDeviceToHost));
+    long long addN = 0, addNb = 0;
+    for (const auto& p : add) {
+      int rc = check_params(p);
+      if (rc) return rc;
+      ChainDev d = make_chain_dev(p);
+      addN += d.N;
+      addNb += d.Nb;
+    }
+    if (old_ext_sites + addN > cap_sites || old_ext_bonds + addNb > cap_bonds ||
+        old_ext_nodes + addN > node_cap)
+      return set_error(LG_ERR_CAPACITY, "relayout: no room to append chains");
+    std::vector<int> bo(addNb, b0), soff(addN);
+    for (long long k = 0; k < addN; ++k) soff[k] = (int)(old_ext_nodes + k + 1);
+    std::vector<double> evt(addN, EMPTY_TIME);
+    LG_HIP(hipMemcpy(s.bo + old_ext_bonds + 1, bo.data(), addNb * sizeof(int), hipMemcpyHostToDevice));
+    LG_HIP(hipMemcpy(s.soff + old_ext_sites + 1, soff.data(), addN * sizeof(int), hipMemcpyHostToDevice));
+    LG_HIP(hipMemcpy(s.evt + old_ext_nodes, evt.data(), addN * sizeof(double), hipMemcpyHostToDevice));
+    LG_HIP(hipMemset(s.segspin + old_ext_nodes, 0, addN));
+    LG_HIP(hipMemset(s.spins + old_ext_sites, 0, addN));
+    for (const auto& p : add) {
+      ChainDev d = make_chain_dev(p);
+      d.old_site_base = old_ext_sites;
+      d.old_bond_base = old_ext_bonds;
+      old_ext_sites += d.N;
+      old_ext_bonds += d.Nb;
+      old_ext_nodes += d.N;
+      ds.push_back(d);
+      sw.push_back(0);
+      ps.push_back(p);
+    }
+  }
+  params = ps;
+  swept_since_layout = false;
+  return set_layout(ds, sw);
 }
 
 void Batch::invalidate_graphs() {
@@ -342,13 +479,23 @@ int Batch::enqueue_sweep() {
     LG_HIP(hipGetLastError());
   }
   cur ^= 1;
+  note_swept();
   return LG_OK;
+}
+
+// k_end_sweep moves every descriptor's old bases to its new bases; keep the
+// host copy the same.
+void Batch::note_swept() {
+  if (!swept_since_layout)
+    for (auto& d : chains) { d.old_site_base = d.site_base; d.old_bond_base = d.bond_base; }
+  swept_since_layout = true;
 }
 
 int Batch::run(uint32_t n, lg_sweep_record* out) {
   int rc = check_flags();
   if (rc) return rc;
   const size_t nc = chains.size();
+  if (nc == 0) return LG_OK;
   std::vector<double> buf;
   uint32_t done = 0;
   while (done < n) {
@@ -407,6 +554,7 @@ int Batch::tune(uint32_t sweeps_per_trial) {
     }
   }
   cfg = best;
+  tuned = true;
   invalidate_graphs();
   return LG_OK;
 }
@@ -427,6 +575,7 @@ int Batch::profile(uint32_t n, double* ms) {
     for (uint32_t i = 0; i < chunk; ++i) {
       launch_sweep(dev(cur), cfg, stream, &ev[(size_t)(done + i) * (K + 1)]);
       cur ^= 1;
+      note_swept();
     }
     LG_HIP(hipStreamSynchronize(stream));
     for (uint32_t i = 0; i < chunk; ++i)

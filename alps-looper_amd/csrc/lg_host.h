@@ -13,6 +13,7 @@
 namespace lg {
 
 int set_error(int code, const std::string& msg);
+std::string g_last_error_copy();  // message of the calling thread's last failure
 int check_params(const lg_params& p);
 ChainDev make_chain_dev(const lg_params& p);
 
@@ -28,11 +29,25 @@ struct State {
   uint8_t* segspin = nullptr;
 };
 
+// Device memory a chain needs (operators: upper bound of the mean count plus
+// 10 standard deviations; candidate slots per bond).
+struct Footprint {
+  long long sites = 0, bonds = 0, ops = 0, cand = 0;
+};
+Footprint footprint(const lg_params& p);
+
+struct Capacity {
+  int chains = 0;
+  long long sites = 0, bonds = 0, ops = 0, cand = 0;
+};
+
 class Batch {
  public:
   ~Batch();
   int init(const lg_params* p, int n);
   int init_for_config(const lg_params* p, long long nops);
+  int init_capacity(const Capacity& cap);
+  int relayout(const std::vector<int>& keep, const std::vector<lg_params>& add);
   int run(uint32_t n, lg_sweep_record* out);
   int tune(uint32_t sweeps_per_trial);
   int profile(uint32_t n, double* ms);
@@ -53,7 +68,14 @@ class Batch {
   int nsites = 0, nbonds = 0;
   long long cand_total = 0;
   int op_cap = 0, node_cap = 0;
+  int cap_chains = 0, cap_sites = 0, cap_bonds = 0;
+  Capacity budget;
   long long min_op_cap = 0;
+  // extent of the site / bond / node arrays of st[cur] (previous layout plus
+  // chains appended since)
+  long long old_ext_sites = 0, old_ext_bonds = 0, old_ext_nodes = 0;
+  bool swept_since_layout = false;
+  bool tuned = false;
   State st[2];
   int cur = 0;                 // st[cur] holds the latest configuration
   LaunchCfg cfg;
@@ -63,12 +85,15 @@ class Batch {
  private:
   template <class T>
   int alloc(T*& ptr, size_t n);
+  int reserve(const Capacity& cap, bool dynamic);
+  int set_layout(std::vector<ChainDev>& ds, const std::vector<uint64_t>& sweeps);
   int allocate();
   int reset_state();
   void release();
   void invalidate_graphs();
   int capture(int from);
   int check_flags();
+  void note_swept();
 
   std::vector<void*> allocations;
   hipGraphExec_t graph_exec[2] = {nullptr, nullptr};

@@ -14,7 +14,10 @@ enum : int { FLAG_CAND_OVERFLOW = 1, FLAG_OP_OVERFLOW = 2, FLAG_NODE_OVERFLOW = 
 struct DevBatch {
   int nchains;
   int nsites, nbonds;          // totals of the new layout
-  const ChainDev* chains;
+  // descriptors; old_site_base / old_bond_base locate a chain in the previous
+  // configuration and differ from site_base / bond_base only in the first
+  // sweep after a relayout: k_end_sweep resets them to the new bases
+  ChainDev* chains;
   const int* bond_chain;       // [nbonds]
   const int* site_chain;       // [nsites]
   // previous configuration

@@ -685,38 +685,61 @@ __global__ void __launch_bounds__(COL_THREADS) k_flip_collect(DevBatch B) {
     const int u0 = next_node(B, P.site_base + s0, l0);
     if (B.fbit[l0] ^ B.fbit[u0]) B.oi_new[k] = info ^ 1;
   }
-  __shared__ double red[COL_THREADS / 64][NCOL];
+  // per-chunk sums by chain: slot k holds chain c_first + k (chains are
+  // contiguous in node order, so a chunk meets few of them)
+  constexpr int CSLOTS = 8;
+  __shared__ double slot[CSLOTS][NCOL];
   const int nchunks = (M + COL_CHUNK - 1) / COL_CHUNK;
   for (int chunk = blockIdx.x; chunk < nchunks; chunk += gridDim.x) {
     const int base = chunk * COL_CHUNK;
     const int c0 = B.site_chain[B.node_site[base]];
-    double v[NCOL];
-#pragma unroll
-    for (int q = 0; q < NCOL; ++q) v[q] = 0;
+    for (int s = threadIdx.x; s < CSLOTS * NCOL; s += COL_THREADS) slot[s / NCOL][s % NCOL] = 0;
+    __syncthreads();
     for (int it = 0; it < COL_ITEMS; ++it) {
       const int i = base + it * COL_THREADS + threadIdx.x;
-      if (i >= M) break;
-      if (B.parent[i] != i) continue;
-      const int c = B.site_chain[B.node_site[i]];
       double w[NCOL];
-      cluster_terms(B.acc[i], w);
-      if (c == c0) {
 #pragma unroll
-        for (int q = 0; q < NCOL; ++q) v[q] += w[q];
-      } else {
-        double* rec = B.rec_acc + (size_t)c * R_COUNT;
-        for (int q = 0; q < NCOL; ++q) unsafeAtomicAdd(rec + term_slot(q), w[q]);
+      for (int q = 0; q < NCOL; ++q) w[q] = 0;
+      int c = -1;
+      if (i < M && B.parent[i] == i) {
+        c = B.site_chain[B.node_site[i]];
+        cluster_terms(B.acc[i], w);
+      }
+      int cmax = c, cmin = (c < 0) ? 0x7fffffff : c;
+      for (int d = 32; d > 0; d >>= 1) {
+        cmax = max(cmax, __shfl_xor(cmax, d, 64));
+        cmin = min(cmin, __shfl_xor(cmin, d, 64));
+      }
+      if (cmax < 0) continue;  // no root in this wave
+      if (cmin == cmax) {
+#pragma unroll
+        for (int q = 0; q < NCOL; ++q) w[q] = wave_sum_d(w[q]);
+        if (lane_id() == 0) {
+          const int k = cmax - c0;
+          if (k < CSLOTS) {
+            for (int q = 0; q < NCOL; ++q) atomicAdd(&slot[k][q], w[q]);
+          } else {
+            double* rec = B.rec_acc + (size_t)cmax * R_COUNT;
+            for (int q = 0; q < NCOL; ++q) unsafeAtomicAdd(rec + term_slot(q), w[q]);
+          }
+        }
+      } else if (c >= 0) {
+        const int k = c - c0;
+        if (k < CSLOTS) {
+          for (int q = 0; q < NCOL; ++q) atomicAdd(&slot[k][q], w[q]);
+        } else {
+          double* rec = B.rec_acc + (size_t)c * R_COUNT;
+          for (int q = 0; q < NCOL; ++q) unsafeAtomicAdd(rec + term_slot(q), w[q]);
+        }
       }
     }
-#pragma unroll
-    for (int q = 0; q < NCOL; ++q) v[q] = wave_sum_d(v[q]);
-    if (lane_id() == 0)
-      for (int q = 0; q < NCOL; ++q) red[threadIdx.x >> 6][q] = v[q];
     __syncthreads();
-    if (threadIdx.x < NCOL) {
-      double t = 0;
-      for (int w = 0; w < COL_THREADS / 64; ++w) t += red[w][threadIdx.x];
-      B.col_part[(size_t)chunk * NCOL + threadIdx.x] = t;
+    if (threadIdx.x < NCOL) B.col_part[(size_t)chunk * NCOL + threadIdx.x] = slot[0][threadIdx.x];
+    for (int s = NCOL + threadIdx.x; s < CSLOTS * NCOL; s += COL_THREADS) {
+      const int k = s / NCOL, q = s % NCOL;
+      const double val = slot[k][q];
+      if (val != 0.0 && c0 + k < B.nchains)
+        unsafeAtomicAdd(B.rec_acc + (size_t)(c0 + k) * R_COUNT + term_slot(q), val);
     }
     __syncthreads();
   }
@@ -735,6 +758,14 @@ __global__ void __launch_bounds__(END_THREADS) k_end_sweep(DevBatch B) {
   const int tid = threadIdx.x;
   __shared__ double red[END_THREADS / 64][NCOL];
   __shared__ int is_last;
+  // The configuration just built lives at the new bases: the next sweep reads
+  // it there.  No kernel after this one in the sweep reads the old bases, and
+  // this kernel reads neither.
+  if (tid == 0) {
+    ChainDev* W = B.chains + c;
+    W->old_site_base = W->site_base;
+    W->old_bond_base = W->bond_base;
+  }
   if (slot >= B.ring_size) {
     if (tid == 0) atomicOr(B.flags, FLAG_RING_OVERFLOW);
     return;

@@ -305,3 +305,110 @@ class Worker:
             self.close()
         except Exception:
             pass
+
+
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("max_jobs", ctypes.c_int32), ("max_sites", ctypes.c_int64), ("max_bonds", ctypes.c_int64),
+                ("max_ops", ctypes.c_int64), ("max_cand", ctypes.c_int64), ("max_chunk", ctypes.c_uint32)]
+
+
+class Job(ctypes.Structure):
+    _fields_ = [("params", LGParams), ("thermalization", ctypes.c_uint32), ("sweeps", ctypes.c_uint32)]
+
+
+JOB_NOBS = 7
+JOB_OBS = ["Energy Density", "Susceptibility", "Staggered Structure Factor", "Staggered Susceptibility",
+           "Number of Clusters", "Stiffness", "Operators"]
+JOB_QUEUED, JOB_RUNNING, JOB_DONE, JOB_CANCELLED = 0, 1, 2, 3
+
+
+class JobResult(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int32), ("sweeps_done", ctypes.c_uint64), ("latency_s", ctypes.c_double),
+                ("queue_s", ctypes.c_double), ("mean", ctypes.c_double * JOB_NOBS),
+                ("error", ctypes.c_double * JOB_NOBS)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_uint64), ("done", ctypes.c_uint64), ("running", ctypes.c_uint64),
+                ("queued", ctypes.c_uint64), ("p50_latency_s", ctypes.c_double), ("p99_latency_s", ctypes.c_double),
+                ("mean_latency_s", ctypes.c_double), ("updates_per_s", ctypes.c_double), ("busy_s", ctypes.c_double),
+                ("total_updates", ctypes.c_uint64), ("sweep_launches", ctypes.c_uint64)]
+
+
+def _server_bind():
+    P = ctypes.POINTER
+    vp = ctypes.c_void_p
+    lib.lg_server_last_error.argtypes = [ctypes.c_char_p, ctypes.c_size_t]
+    lib.lg_server_start.argtypes = [P(ServerConfig), P(vp)]
+    lib.lg_server_submit.argtypes = [vp, P(Job), P(ctypes.c_uint64)]
+    lib.lg_server_poll.argtypes = [vp, ctypes.c_uint64, P(JobResult)]
+    lib.lg_server_wait.argtypes = [vp, ctypes.c_uint64, ctypes.c_int64, P(JobResult)]
+    lib.lg_server_cancel.argtypes = [vp, ctypes.c_uint64]
+    lib.lg_server_stats_get.argtypes = [vp, P(ServerStats)]
+    lib.lg_server_stop.argtypes = [vp]
+
+
+_server_bind()
+EXPORTED += ["lg_server_last_error", "lg_server_start", "lg_server_submit", "lg_server_poll", "lg_server_wait",
+             "lg_server_cancel", "lg_server_stats_get", "lg_server_stop",
+             "lg_worker_last_error", "lg_worker_create", "lg_worker_run", "lg_worker_status", "lg_worker_result",
+             "lg_worker_report", "lg_worker_save", "lg_worker_load", "lg_worker_set_chunk", "lg_worker_destroy"]
+
+
+def _scheck(rc):
+    if rc != 0:
+        buf = ctypes.create_string_buffer(1024)
+        lib.lg_server_last_error(buf, 1024)
+        raise LoopGPUError(rc, buf.value.decode(errors="replace"))
+
+
+class Server:
+    """Continuous-batching job server (include/loopgpu_server.h)."""
+
+    def __init__(self, max_jobs=64, max_sites=1 << 20, max_bonds=1 << 21, max_ops=64 << 20, max_cand=64 << 20,
+                 max_chunk=64):
+        cfg = ServerConfig(max_jobs, max_sites, max_bonds, max_ops, max_cand, max_chunk)
+        self.h = ctypes.c_void_p()
+        _scheck(lib.lg_server_start(ctypes.byref(cfg), ctypes.byref(self.h)))
+
+    def submit(self, p, sweeps, thermalization=None):
+        if thermalization is None:
+            thermalization = sweeps // 8
+        j = Job(p, thermalization, sweeps)
+        jid = ctypes.c_uint64()
+        _scheck(lib.lg_server_submit(self.h, ctypes.byref(j), ctypes.byref(jid)))
+        return jid.value
+
+    @staticmethod
+    def _res(r):
+        return {"status": r.status, "sweeps_done": r.sweeps_done, "latency_s": r.latency_s, "queue_s": r.queue_s,
+                "mean": dict(zip(JOB_OBS, list(r.mean))), "error": dict(zip(JOB_OBS, list(r.error)))}
+
+    def poll(self, jid):
+        r = JobResult()
+        _scheck(lib.lg_server_poll(self.h, jid, ctypes.byref(r)))
+        return self._res(r)
+
+    def wait(self, jid, timeout_s=600.0):
+        r = JobResult()
+        _scheck(lib.lg_server_wait(self.h, jid, int(timeout_s * 1000), ctypes.byref(r)))
+        return self._res(r)
+
+    def cancel(self, jid):
+        _scheck(lib.lg_server_cancel(self.h, jid))
+
+    def stats(self):
+        s = ServerStats()
+        _scheck(lib.lg_server_stats_get(self.h, ctypes.byref(s)))
+        return {k: getattr(s, k) for k, _ in ServerStats._fields_}
+
+    def stop(self):
+        if self.h:
+            lib.lg_server_stop(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.stop()
+        except Exception:
+            pass

@@ -175,17 +175,23 @@ def scan_params(lg, n=512, L=32):
 def run_scan(args, lg):
     plist = scan_params(lg)
     b = lg.Batch(plist)
-    b.run(args.warmup, records=False)
+    log("batched warmup %d sweeps" % args.warmup)
+    run_chunked(lambda k: b.run(k), args.warmup)
+    log("tune")
     b.tune(8)
     b.set_graph(True)
+    log("batched timed %d sweeps" % args.steps)
     t0 = time.perf_counter()
-    rec = b.run(args.steps)
+    rec = run_chunked(lambda k: b.run(k), args.steps)
     dt = time.perf_counter() - t0
     nop_total = float(rec[:, :, 0].sum())
     batched = nop_total / dt
-    # plain path: the same chains, one chain per launch, a few sweeps each
+    # plain path: the same chains, one chain (and one launch per stage) at a
+    # time; every 8th chain of the scan, a few sweeps each
     plain_sweeps = max(2, min(20, args.steps // 10))
-    chains = [lg.Chain(p) for p in plist[:: max(1, len(plist) // 64)]]
+    sub = plist[:: max(1, len(plist) // 64)]
+    log("plain path: %d chains" % len(sub))
+    chains = [lg.Chain(p) for p in sub]
     for c in chains:
         c.sweep(args.warmup, records=False)
     t0 = time.perf_counter()
@@ -193,6 +199,7 @@ def run_scan(args, lg):
     for c in chains:
         nops += float(c.sweep(plain_sweeps)[:, 0].sum())
     plain = nops / (time.perf_counter() - t0)
+    log("profile")
     prof = b.profile(args.profile_sweeps)
     rl = roofline(prof, nop_total / args.steps)
     out = {
