@@ -13,6 +13,15 @@ HBM.  Prints one JSON line.
 --workload scan runs the batched temperature scan instead (512 chains of the
 32x32 square lattice, T_i = 0.05 * 40^(i/511)), plain (one chain per launch)
 and batched (all chains per launch, HIP graph), and reports both.
+
+--workload serve runs the serving mix through the continuous-batching job
+server: a burst of jobs measures the saturated rate, then Poisson arrivals at
+--load times that rate for --serve-seconds; reports the delivered updates/s
+and p50/p99 job latency (submit -> result).  Job sweep counts are multiplied
+by --sweep-scale (0.1 by default, so that the run takes minutes).
+
+CPU baselines run the oracle's looper-mode restatement: one core for the
+one-chain workload, a thread pool over the host cores for scan and serve.
 """
 
 import argparse
@@ -212,7 +221,172 @@ def run_scan(args, lg):
         "plain_value": plain, "batched_over_plain": batched / plain,
         "kernel_ms": prof, "roofline": rl,
     }
-    return out, None
+    return out, ("scan", [(1, 32, 1.0 / T, 1.0, 23125) for T in (0.05 * 40.0 ** (i / 511) for i in range(512))])
+
+
+SERVE_KINDS = [("chain", 64), ("chain", 256), ("chain", 1024),
+               ("square", 16), ("square", 32), ("square", 48), ("square", 64), ("square", 96)]
+
+
+def serve_trace(n, sweep_scale, seed=23125):
+    """Job mix of the serving workload: kind uniform over SERVE_KINDS,
+    beta ~ U[1, 4L], Jz ~ U[0.5, 1] with Jxy = 1, sweeps ~ U{1000..20000}
+    (times sweep_scale), thermalization = sweeps / 8; job i is keyed
+    (23125, i)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    jobs = []
+    for i in range(n):
+        kind, L = SERVE_KINDS[int(rng.integers(len(SERVE_KINDS)))]
+        beta = float(rng.uniform(1.0, 4.0 * L))
+        jz = float(rng.uniform(0.5, 1.0))
+        sweeps = max(8, int(round(int(rng.integers(1000, 20001)) * sweep_scale)))
+        jobs.append({"kind": kind, "L": L, "beta": beta, "jz": jz, "sweeps": sweeps,
+                     "thermalization": sweeps // 8, "chain_id": i})
+    return jobs
+
+
+def _job_params(lg, j):
+    return lg.params(lg.CHAIN if j["kind"] == "chain" else lg.SQUARE, j["L"], j["beta"], 1.0, j["jz"],
+                     seed=23125, chain_id=j["chain_id"])
+
+
+def _percentile(xs, q):
+    import numpy as np
+    return float(np.percentile(np.asarray(xs), q)) if xs else None
+
+
+def run_serve(args, lg):
+    """Continuous batching of a Poisson job stream.  A burst of jobs measures
+    the saturated rate; the arrival rate is then set for ~80 % of it and the
+    stream runs for --serve-seconds of arrivals, after which it drains."""
+    import numpy as np
+    burst_n = args.serve_burst
+    trace = serve_trace(burst_n + 4096, args.sweep_scale)
+    cfg = dict(max_jobs=256, max_ops=96 << 20, max_cand=96 << 20, max_sites=1 << 21, max_bonds=1 << 22,
+               max_chunk=args.serve_chunk)
+    # saturation: every burst job submitted at once
+    srv = lg.Server(**cfg)
+    t0 = time.perf_counter()
+    ids = [srv.submit(_job_params(lg, j), j["sweeps"], j["thermalization"]) for j in trace[:burst_n]]
+    for k, i in enumerate(ids):
+        srv.wait(i, 3600)
+        if (k + 1) % 8 == 0:
+            log("burst: %d/%d jobs done" % (k + 1, burst_n))
+    burst_wall = time.perf_counter() - t0
+    st = srv.stats()
+    srv.stop()
+    upd_per_job = st["total_updates"] / burst_n
+    sat_rate = st["total_updates"] / burst_wall
+    lam = args.load * sat_rate / upd_per_job
+    log("saturated %.3g updates/s, %.3g updates/job -> arrival rate %.3g jobs/s" % (sat_rate, upd_per_job, lam))
+    # Poisson stream
+    rng = np.random.default_rng(4242)
+    srv = lg.Server(**cfg)
+    ids = []
+    t_start = time.perf_counter()
+    t_next = t_start + rng.exponential(1.0 / lam)
+    k = burst_n
+    last_log = t_start
+    while True:
+        now = time.perf_counter()
+        if t_next - t_start > args.serve_seconds:
+            break
+        if now < t_next:
+            time.sleep(min(t_next - now, 0.05))
+            continue
+        j = trace[k]
+        k += 1
+        ids.append(srv.submit(_job_params(lg, j), j["sweeps"], j["thermalization"]))
+        t_next += rng.exponential(1.0 / lam)
+        if now - last_log > 10:
+            s = srv.stats()
+            log("stream %.0f s: %d submitted, %d done, %d running, %d queued" %
+                (now - t_start, s["submitted"], s["done"], s["running"], s["queued"]))
+            last_log = now
+    log("arrivals closed after %d jobs; draining" % len(ids))
+    res = []
+    for n, i in enumerate(ids):
+        res.append(srv.wait(i, 3600))
+        if (n + 1) % 32 == 0:
+            log("drain: %d/%d" % (n + 1, len(ids)))
+    wall = time.perf_counter() - t_start
+    st = srv.stats()
+    srv.stop()
+    lat = [r["latency_s"] for r in res if r["status"] == lg.JOB_DONE]
+    qs = [r["queue_s"] for r in res if r["status"] == lg.JOB_DONE]
+    value = st["total_updates"] / wall
+    out = {
+        "metric": METRIC, "value": value, "unit": "spin-operator updates/s",
+        "n_gpus": 1, "steps": int(st["sweep_launches"]), "warmup": 0,
+        "ms_per_step": 1e3 * st["busy_s"] / max(1, st["sweep_launches"]), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "serving mix: Poisson job stream, chain L in {64,256,1024} / square L in "
+                               "{16,32,48,64,96}, beta~U[1,4L], Jz~U[0.5,1], Jxy=1, sweeps~U{1000..20000}"
+                               "x%g, thermalization sweeps/8, continuous batching" % args.sweep_scale,
+                   "arrival_rate_jobs_per_s": lam, "target_load": args.load, "arrival_seconds": args.serve_seconds,
+                   "sweep_scale": args.sweep_scale, "max_chunk": args.serve_chunk, "burst_jobs": burst_n},
+        "jobs": len(ids), "jobs_done": len(lat),
+        "p50_latency_s": _percentile(lat, 50), "p99_latency_s": _percentile(lat, 99),
+        "mean_latency_s": float(np.mean(lat)) if lat else None,
+        "p50_queue_s": _percentile(qs, 50), "p99_queue_s": _percentile(qs, 99),
+        "busy_fraction": st["busy_s"] / wall, "updates_per_busy_s": st["updates_per_s"],
+        "saturated_value": sat_rate, "wall_s": wall, "roofline": None,
+    }
+    return out, ("serve", trace[burst_n:k] or trace[:burst_n])
+
+
+def _pool_size():
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def cpu_pool_baseline(chains, seconds, what):
+    """Oracle looper-mode chains run by a thread pool on the host cores (the
+    oracle releases the GIL inside each sweep); each worker takes the next
+    chain of the list and sweeps it until the shared deadline.  Returns the
+    aggregate operator updates/s."""
+    import threading
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    cores = _pool_size()
+    lock = threading.Lock()
+    state = {"next": 0, "ops": 0.0, "sweeps": 0, "chains": 0}
+    t_end = time.perf_counter() + seconds
+
+    def worker():
+        while time.perf_counter() < t_end:
+            with lock:
+                i = state["next"]
+                state["next"] += 1
+            kind, L, beta, jz, seed = chains[i % len(chains)]
+            ch = O.LooperChain(kind, L, beta, 1.0, jz, seed=seed)
+            n_ops = 0.0
+            n = 0
+            while time.perf_counter() < t_end and n < 64:
+                n_ops += float(ch.sweep(1)[0, 0])
+                n += 1
+            with lock:
+                state["ops"] += n_ops
+                state["sweeps"] += n
+                state["chains"] += 1
+
+    t0 = time.perf_counter()
+    th = [threading.Thread(target=worker) for _ in range(cores)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    wall = time.perf_counter() - t0
+    return {
+        "value": state["ops"] / wall, "unit": "spin-operator updates/s", "cores": cores, "kind": "port",
+        "sample": ("oracle looper-mode restatement of path_integral.C (g++ -O2) on a %d-thread pool for %.1f s: "
+                   "%s; %d chains started from the empty operator string, up to 64 sweeps each, %d sweeps in "
+                   "total; CPU: %s" % (cores, wall, what, state["chains"], state["sweeps"], cpu_model())),
+    }
 
 
 def main():
@@ -223,7 +397,12 @@ def main():
     ap.add_argument("--profile-sweeps", type=int, default=100)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["c2", "scan"], default="c2")
+    ap.add_argument("--workload", choices=["c2", "scan", "serve"], default="c2")
+    ap.add_argument("--serve-seconds", type=float, default=60.0, help="serve: seconds of Poisson arrivals")
+    ap.add_argument("--serve-burst", type=int, default=48, help="serve: jobs of the saturation burst")
+    ap.add_argument("--sweep-scale", type=float, default=0.1, help="serve: factor on every job's sweep count")
+    ap.add_argument("--load", type=float, default=0.8, help="serve: offered load / saturated rate")
+    ap.add_argument("--serve-chunk", type=int, default=64, help="serve: sweeps between admission points")
     ap.add_argument("--copy", type=int, default=0, metavar="MIB",
                     help="only run the plain copy kernel on MIB MiB (counter calibration) and print GB/s")
     args = ap.parse_args()
@@ -242,11 +421,21 @@ def main():
         return
     if args.workload == "c2":
         out, cpu_cfg = run_c2(args, lg)
-    else:
+    elif args.workload == "scan":
         out, cpu_cfg = run_scan(args, lg)
+    else:
+        out, cpu_cfg = run_serve(args, lg)
     if cpu_cfg and not args.no_cpu:
         log("cpu baseline %.0f s" % args.cpu_seconds)
-        out["cpu_baseline"] = cpu_baseline(*cpu_cfg, args.cpu_seconds)
+        if cpu_cfg[0] == "scan":
+            out["cpu_baseline"] = cpu_pool_baseline(cpu_cfg[1], args.cpu_seconds,
+                                                    "the 512 chains of the scan in order")
+        elif cpu_cfg[0] == "serve":
+            jobs = [(0 if j["kind"] == "chain" else 1, j["L"], j["beta"], j["jz"], 23125) for j in cpu_cfg[1]]
+            out["cpu_baseline"] = cpu_pool_baseline(jobs, args.cpu_seconds, "the jobs of the same arrival "
+                                                    "trace in arrival order")
+        else:
+            out["cpu_baseline"] = cpu_baseline(*cpu_cfg, args.cpu_seconds)
         out["gpu_over_cpu"] = out["value"] / out["cpu_baseline"]["value"]
     else:
         out["cpu_baseline"] = None
