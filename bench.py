@@ -82,7 +82,8 @@ def cpu_model():
 
 def cpu_baseline(kind, L, beta, seconds):
     """Oracle looper-mode port (serial restatement of path_integral.C),
-    1 core, bounded sample of the same workload."""
+    1 core, bounded sample of the same workload: the first sweep (from the
+    empty operator string) and the first quarter of the time are untimed."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
     ch = O.LooperChain(kind, L, beta, 1.0, 1.0, seed=23125)
@@ -97,7 +98,9 @@ def cpu_baseline(kind, L, beta, seconds):
         rec = ch.sweep(1)
         dt = time.perf_counter() - t0
         el = time.perf_counter() - t_start
-        if el <= warm:
+        if dt > 5:
+            log("cpu sweep %d: %.1f s, n_op %.0f" % (n_warm + n_timed + 1, dt, rec[0, 0]))
+        if el <= warm or n_warm == 0:
             n_warm += 1
         else:
             nops += rec[0, 0]
@@ -142,14 +145,31 @@ def roofline(prof, nop_mean, traffic_profile=None):
     }
 
 
-def run_c2(args, lg):
-    kind, L, beta = lg.SQUARE, 128, 128.0
+STAGES = {"diagonal update": ("diag_count", "scan_reduce", "scan_top", "scan_down", "diag_fill"),
+          "linking": ("link",), "union-find": ("uf_union", "uf_compress"),
+          "estimators": ("estimate",), "flip + collect": ("flip_collect", "end_sweep")}
+
+
+def stage_gbs(prof, nop_mean):
+    """algorithmic GB/s of each stage (per-stage bytes per operator-sweep)"""
+    out = {}
+    for st, ks in STAGES.items():
+        ms = sum(prof[k] for k in ks)
+        b = sum(BYTES_PER_OP[k] for k in ks) * nop_mean
+        out[st] = {"ms": ms, "bytes_per_op": sum(BYTES_PER_OP[k] for k in ks), "gbs": b / (ms * 1e-3) / 1e9}
+    return out
+
+
+def run_c2(args, lg, L=128, beta=128.0, traffic_profile="c2_square128_hbm_traffic"):
+    kind = lg.SQUARE
+    log("allocating square %dx%d beta=%g" % (L, L, beta))
     ch = lg.Chain(lg.params(kind, L, beta, 1.0, 1.0, seed=23125))
+    chunk = 250 if L <= 128 else 25  # progress lines every few seconds
     log("warmup %d sweeps" % args.warmup)
-    run_chunked(lambda k: ch.sweep(k), args.warmup)
+    run_chunked(lambda k: ch.sweep(k), args.warmup, chunk)
     log("timed %d sweeps" % args.steps)
     t0 = time.perf_counter()
-    rec = run_chunked(lambda k: ch.sweep(k), args.steps)
+    rec = run_chunked(lambda k: ch.sweep(k), args.steps, chunk)
     t1 = time.perf_counter()
     dt = t1 - t0
     nop_total = float(rec[:, 0].sum())
@@ -157,20 +177,21 @@ def run_c2(args, lg):
     value = nop_total / dt
     log("profile %d sweeps" % args.profile_sweeps)
     prof = ch.profile(args.profile_sweeps)
-    rl = roofline(prof, nop_mean, "c2_square128_hbm_traffic")
+    rl = roofline(prof, nop_mean, traffic_profile)
     out = {
         "metric": METRIC, "value": value, "unit": "spin-operator updates/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": 1e3 * dt / args.steps, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": "square 128x128 periodic S=1/2 Heisenberg AF, J=1, beta=128, "
-                               "one chain, plain GPU path", "lattice": "square lattice", "L": L,
+        "config": {"workload": "square %dx%d periodic S=1/2 Heisenberg AF, J=1, beta=%g, "
+                               "one chain, plain GPU path" % (L, L, beta), "lattice": "square lattice", "L": L,
                    "beta": beta, "J": 1.0, "chains": 1, "step": "1 sweep", "seed": 23125},
         "sweeps_per_s": args.steps / dt,
         "n_op_mean": nop_mean,
         "energy_density": float(rec[:, 2].mean() / (L * L)),
         "sweep_hbm_frac": value * SWEEP_BYTES_PER_OP / (HBM_PEAK_GBS * 1e9),
         "kernel_ms": prof,
+        "stages": stage_gbs(prof, nop_mean),
         "roofline": rl,
     }
     return out, (kind, L, beta)
@@ -397,7 +418,7 @@ def main():
     ap.add_argument("--profile-sweeps", type=int, default=100)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["c2", "scan", "serve"], default="c2")
+    ap.add_argument("--workload", choices=["c2", "c5", "scan", "serve"], default="c2")
     ap.add_argument("--serve-seconds", type=float, default=60.0, help="serve: seconds of Poisson arrivals")
     ap.add_argument("--serve-burst", type=int, default=48, help="serve: jobs of the saturation burst")
     ap.add_argument("--sweep-scale", type=float, default=0.1, help="serve: factor on every job's sweep count")
@@ -421,6 +442,8 @@ def main():
         return
     if args.workload == "c2":
         out, cpu_cfg = run_c2(args, lg)
+    elif args.workload == "c5":
+        out, cpu_cfg = run_c2(args, lg, 512, 512.0, None)
     elif args.workload == "scan":
         out, cpu_cfg = run_scan(args, lg)
     else:
