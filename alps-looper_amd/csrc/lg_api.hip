@@ -254,8 +254,10 @@ int Batch::allocate() {
   if ((rc = alloc(kc_new, op_cap))) return rc;
   if ((rc = alloc(l0, op_cap))) return rc;
   if ((rc = alloc(l1, op_cap))) return rc;
+  if ((rc = alloc(u0, op_cap))) return rc;
+  if ((rc = alloc(u1, op_cap))) return rc;
   if ((rc = alloc(evflags, node_cap))) return rc;
-  if ((rc = alloc(node_site, node_cap))) return rc;
+  if ((rc = alloc(chain_node0, cap_chains))) return rc;
   if ((rc = alloc(parent, node_cap))) return rc;
   if ((rc = alloc(fbit, node_cap))) return rc;
   if ((rc = alloc(acc, node_cap))) return rc;
@@ -320,6 +322,15 @@ int Batch::set_layout(std::vector<ChainDev>& ds, const std::vector<uint64_t>& sw
   int G = 4;
   while (G < 64 && G < lam_max) G *= 2;
   if (!tuned) cfg.group = G;
+  // expected events per site (two legs per operator; operators per bond about
+  // beta * (offset + half the largest diagonal energy)) picks the staging size
+  double ev = 0;
+  for (size_t c = 0; c < chains.size(); ++c) {
+    const double per_bond = chains[c].offset_total / chains[c].Nb +
+                            0.5 * (std::fabs(params[c].jz) / 4 + std::fabs(params[c].jxy) / 2);
+    ev += 2.0 * chains[c].beta * chains[c].Nb * per_bond;
+  }
+  cfg.link_cap = (nsites > 0 && ev / nsites > 400) ? 1536 : 512;
   invalidate_graphs();
   return LG_OK;
 }
@@ -433,7 +444,7 @@ DevBatch Batch::dev(int from) const {
   B.soff_new = w.soff; B.evt_new = w.evt; B.segspin_new = w.segspin;
   B.counts = counts; B.acc_count = acc_count; B.kpre = kpre;
   B.cand_t = cand_t; B.cand_g = cand_g; B.kc_new = kc_new;
-  B.l0 = l0; B.l1 = l1; B.evflags = evflags; B.node_site = node_site;
+  B.l0 = l0; B.l1 = l1; B.u0 = u0; B.u1 = u1; B.evflags = evflags; B.chain_node0 = chain_node0;
   B.parent = parent; B.fbit = fbit; B.acc = acc;
   B.scan_part = scan_part; B.col_part = col_part; B.totals = totals; B.flags = flags;
   B.sweep = d_sweep; B.rec_acc = rec_acc; B.rec_ring = rec_ring;

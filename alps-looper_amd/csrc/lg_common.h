@@ -35,7 +35,9 @@ constexpr double EMPTY_TIME = 2.0;  // ev_time of the single node of an event-fr
 enum : uint32_t { STAGE_DIAG = 1, STAGE_REGRAPH = 2, STAGE_FLIP = 3 };
 
 // ev_flags bits
-enum : uint8_t { EV_OFFDIAG = 1, EV_SRC = 2, EV_DIR1 = 4, EV_EMPTY = 8 };
+// EV_FIRST: lowest segment of its site (wraps through tau = 1); EV_GAUGE: the
+// site's gauge factor is -1
+enum : uint8_t { EV_OFFDIAG = 1, EV_SRC = 2, EV_DIR1 = 4, EV_EMPTY = 8, EV_FIRST = 16, EV_GAUGE = 32 };
 
 // Per-chain descriptor (device resident, rebuilt by the host when the set of
 // chains in a batch changes).

@@ -108,8 +108,10 @@ class Batch {
   int* kc_new = nullptr;
   int* l0 = nullptr;
   int* l1 = nullptr;
+  int* u0 = nullptr;
+  int* u1 = nullptr;
   uint8_t* evflags = nullptr;
-  int* node_site = nullptr;
+  int* chain_node0 = nullptr;
   int* parent = nullptr;
   uint8_t* fbit = nullptr;
   ClusterAcc* acc = nullptr;

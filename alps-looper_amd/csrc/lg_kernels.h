@@ -45,8 +45,10 @@ struct DevBatch {
   int* kc_new;                 // [new ops] off-diagonal operators before, within the bond
   int* l0;                     // [new ops] lower-leg node on the source site
   int* l1;                     // [new ops] lower-leg node on the target site
+  int* u0;                     // [new ops] upper-leg node on the source site
+  int* u1;                     // [new ops] upper-leg node on the target site
   uint8_t* evflags;            // [nodes]
-  int* node_site;              // [nodes]
+  int* chain_node0;            // [nchains] first segment node of each chain (new configuration)
   int* parent;                 // [nodes]
   uint8_t* fbit;               // [nodes] flip bit of the node's cluster
   ClusterAcc* acc;             // [nodes] (roots only)
@@ -68,6 +70,7 @@ struct LaunchCfg {
   int max_blocks_bond = 1 << 20;
   int blocks_op = 1024;        // grid of the per-operator kernels (grid-stride)
   int blocks_node = 1024;      // grid of the per-node kernels (grid-stride)
+  int link_cap = 512;          // events per site staged in LDS by the linking kernel (512 or 1536)
 };
 
 constexpr int NUM_SWEEP_KERNELS = 11;

@@ -10,12 +10,15 @@
 //                the new world lines (max(events,1) per site)
 //   diag_fill    per bond: kept operators (regraphed) merged with accepted
 //                candidates in time order
-//   link         per operator: rank of its two legs on the site world lines
-//                (merge by binary search), segment spins by kink parity
+//   link         per site: rank of each event on the site's world line
+//                (binary search in the other bond lists, staged in LDS), the
+//                segments below and above each leg, segment spins by kink
+//                parity
 //   uf_union     per operator: lock-free union of segments by the cut rules
-//   uf_compress  per node: full compression, zero the root accumulators
-//   estimate     per node: improved-estimator terms into the root, flip bit of
-//                the root, flipped segment spins and tau=0 spins
+//   uf_compress  per node: full compression; per root: clear the accumulator,
+//                draw the cluster's flip bit
+//   estimate     per node: improved-estimator terms into the root, flip bit
+//                copied from the root, flipped segment spins and tau=0 spins
 //   flip_collect per operator: type flip; per root: cluster terms -> record
 //   end_sweep    per chain: finish and store the sweep record
 #include <hip/hip_runtime.h>
@@ -87,8 +90,46 @@ __device__ __forceinline__ int upper_bound_d(const double* a, int n, double x) {
   return lo;
 }
 
+// Items [lo, hi) of this block.  [0, n) is cut into one contiguous slice per
+// XCD (the dispatcher sends workgroup i to XCD i % 8), in proportion to the
+// XCD's block count, and each slice into near-equal shares of its blocks, so
+// each XCD's L2 serves one contiguous part of the node and operator arrays
+// and every XCD gets work also when n is below the block count.
+constexpr int NXCD = 8;
+__device__ __forceinline__ void xcd_share(int n, int& lo, int& hi) {
+  const int nb = gridDim.x;
+  const int x = blockIdx.x % NXCD, j = blockIdx.x / NXCD;
+  const int q = nb / NXCD, r = nb % NXCD;
+  const int nbx = q + (x < r ? 1 : 0);
+  const long long s0 = (long long)n * (x * q + min(x, r)) / nb;
+  const long long s1 = (long long)n * ((x + 1) * q + min(x + 1, r)) / nb;
+  lo = (int)(s0 + (s1 - s0) * j / nbx);
+  hi = (int)(s0 + (s1 - s0) * (j + 1) / nbx);
+}
+
 __device__ __forceinline__ bool overflowed(const DevBatch& B) {
   return __hip_atomic_load(B.flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
+}
+
+// chain of a segment node of the new configuration (last chain whose first
+// node is <= node; chain_node0 is written by k_scan_down)
+__device__ __forceinline__ int node_chain(const DevBatch& B, int node) {
+  int lo = 0, n = B.nchains;
+  while (n > 1) {
+    const int h = n >> 1;
+    if (B.chain_node0[lo + h] <= node) { lo += h; n -= h; } else { n = h; }
+  }
+  return lo;
+}
+
+// site (global index) whose first segment is `node`
+__device__ __forceinline__ int first_node_site(const DevBatch& B, const ChainDev& P, int node) {
+  int lo = P.site_base, n = P.N;
+  while (n > 1) {
+    const int h = n >> 1;
+    if (B.soff_new[lo + h] <= node) { lo += h; n -= h; } else { n = h; }
+  }
+  return lo;
 }
 
 // spin at tau on (chain-local) site s in the previous configuration;
@@ -112,8 +153,9 @@ __global__ void __launch_bounds__(256) k_diag_count(DevBatch B) {
   const int groups_per_block = blockDim.x / G;
   const int gl = glane<G>();
   const uint64_t below = (gl == 0) ? 0ull : ((1ull << gl) - 1);
-  for (int gb = blockIdx.x * groups_per_block + threadIdx.x / G; gb < B.nbonds;
-       gb += gridDim.x * groups_per_block) {
+  int glo, ghi;
+  xcd_share(B.nbonds, glo, ghi);
+  for (int gb = glo + threadIdx.x / G; gb < ghi; gb += groups_per_block) {
     const int c = B.bond_chain[gb];
     const ChainDev P = B.chains[c];
     const int b = gb - P.bond_base;
@@ -290,6 +332,7 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan_down(DevBatch B) {
   int vals[SCAN_ITEMS];
   bool empty[SCAN_ITEMS];
   int sum = 0;
+#pragma unroll
   for (int k = 0; k < SCAN_ITEMS; ++k) {
     int v = 0;
     empty[k] = false;
@@ -308,19 +351,23 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan_down(DevBatch B) {
   int tot;
   int off = block_excl_scan(sum, &tot) + B.scan_part[blockIdx.x];
   int* out = which ? B.soff_new : B.bo_new;
+#pragma unroll
   for (int k = 0; k < SCAN_ITEMS; ++k) {
     int i = base + k;
     if (i < n) {
       out[i] = off;
-      if (empty[k]) {
-        // event-free site: one node spanning [0,1)
-        const ChainDev& P = B.chains[B.site_chain[i]];
-        int node = off;
-        B.evt_new[node] = EMPTY_TIME;
-        B.segspin_new[node] = B.spins_old[P.old_site_base + (i - P.site_base)];
-        B.evflags[node] = EV_EMPTY;
-        B.node_site[node] = i;
-        B.parent[node] = node;
+      if (which) {
+        const int c = B.site_chain[i];
+        const ChainDev& P = B.chains[c];
+        if (i == P.site_base) B.chain_node0[c] = off;
+        if (empty[k]) {
+          // event-free site: one node spanning [0,1)
+          int node = off;
+          B.evt_new[node] = EMPTY_TIME;
+          B.segspin_new[node] = B.spins_old[P.old_site_base + (i - P.site_base)];
+          B.evflags[node] = (uint8_t)(EV_EMPTY | EV_FIRST | (gauge(P, i - P.site_base) < 0 ? EV_GAUGE : 0));
+          B.parent[node] = node;
+        }
       }
     }
     off += vals[k];
@@ -336,8 +383,9 @@ __global__ void __launch_bounds__(256) k_diag_fill(DevBatch B) {
   if (overflowed(B)) return;
   const int groups_per_block = blockDim.x / G;
   const int gl = glane<G>();
-  for (int gb = blockIdx.x * groups_per_block + threadIdx.x / G; gb < B.nbonds;
-       gb += gridDim.x * groups_per_block) {
+  int glo, ghi;
+  xcd_share(B.nbonds, glo, ghi);
+  for (int gb = glo + threadIdx.x / G; gb < ghi; gb += groups_per_block) {
     const int c = B.bond_chain[gb];
     const ChainDev P = B.chains[c];
     const int b = gb - P.bond_base;
@@ -377,72 +425,177 @@ __global__ void __launch_bounds__(256) k_diag_fill(DevBatch B) {
 }
 
 // ---------------------------------------------------------------------------
-// (b) world-line linking
+// (b) world-line linking, one wave per site.  The events of a site are the
+// operators of its bonds (each bond list sorted by tau); an event's rank on
+// the site is its index in its own list plus, in each other list, the number
+// of operators before it (ties ordered by bond index), and the spin of the
+// segment below it flips with the off-diagonal operators counted the same
+// way.  The lists (times and off-diagonal prefix counts) are staged in LDS,
+// ranked there, and the site's segment arrays are written in node order
+// through the inverse permutation (whole cache lines).  A site with more
+// than CAP events searches the lists in global memory and writes directly.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_link(DevBatch B) {
-  if (overflowed(B)) return;
-  const int nops = B.totals[0];
-  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nops; k += gridDim.x * blockDim.x) {
-    const int info = B.oi_new[k];
-    const int gb = info >> 3;
-    const int c = B.bond_chain[gb];
-    const ChainDev& P = B.chains[c];
-    const int b = gb - P.bond_base;
-    const double tau = B.ot_new[k];
-    const int r = k - B.bo_new[gb];
-    const int dir = bond_dir(P, b);
-    int s0, s1;
-    bond_sites(P, b, s0, s1);
+struct SiteLists {
+  int nbd;
+  int bond[4];   // chain-local bond index
+  int lo[4];     // first operator of the list
+  int len[4];
+  int off[4];    // staged offset of the list
+  int side[4];   // 1 when the site is the bond's target site
+};
+
+// off-diagonal operators among the first q of list j (q <= len)
+__device__ __forceinline__ int list_pre_global(const DevBatch& B, const SiteLists& S, int j, int q) {
+  if (q < S.len[j]) return B.kc_new[S.lo[j] + q];
+  if (S.len[j] == 0) return 0;
+  const int last = S.lo[j] + S.len[j] - 1;
+  return B.kc_new[last] + (B.oi_new[last] & 1);
+}
+
+// rank and kink parity of an event of list i (bond bi) at position p; the
+// loops over the lists are unrolled so that SiteLists stays in registers
+template <bool kLds>
+__device__ __forceinline__ void rank_event(const DevBatch& B, const SiteLists& S, int i, int bi, int p,
+                                           double tau, const double* lt, const uint16_t* lpre, int& rank,
+                                           int& par) {
+  rank = p;
+  par = 0;
 #pragma unroll
-    for (int side = 0; side < 2; ++side) {
-      const int s = side ? s1 : s0;
-      int bl[4];
-      const int nbd = site_bonds(P, s, bl);
-      int pos = r;
-      int par = B.kc_new[k];
-      for (int i = 0; i < nbd; ++i) {
-        const int bp = bl[i];
-        if (bp == b) continue;
-        const int gbp = P.bond_base + bp;
-        const int lo = B.bo_new[gbp], hi = B.bo_new[gbp + 1];
-        const int q = (bp < b) ? upper_bound_d(B.ot_new + lo, hi - lo, tau)
-                               : lower_bound_d(B.ot_new + lo, hi - lo, tau);
-        pos += q;
-        if (q < hi - lo) par += B.kc_new[lo + q];
-        else if (hi > lo) par += B.kc_new[hi - 1] + (B.oi_new[hi - 1] & 1);
-      }
-      const int gs = P.site_base + s;
-      const int node = B.soff_new[gs] + pos;
-      B.evt_new[node] = tau;
-      B.segspin_new[node] = B.spins_old[P.old_site_base + s] ^ (uint8_t)(par & 1);
-      B.evflags[node] = (uint8_t)((info & 1) | (side ? 0 : EV_SRC) | (dir ? EV_DIR1 : 0));
-      B.node_site[node] = gs;
-      B.parent[node] = node;
-      if (side) B.l1[k] = node; else B.l0[k] = node;
+  for (int j = 0; j < 4; ++j) {
+    if (j >= S.nbd) break;
+    if (j == i) {
+      par += kLds ? lpre[S.off[j] + j + p] : list_pre_global(B, S, j, p);
+      continue;
     }
+    const double* t = kLds ? lt + S.off[j] : B.ot_new + S.lo[j];
+    const int q = (S.bond[j] < bi) ? upper_bound_d(t, S.len[j], tau) : lower_bound_d(t, S.len[j], tau);
+    rank += q;
+    par += kLds ? lpre[S.off[j] + j + q] : list_pre_global(B, S, j, q);
+  }
+}
+
+template <int CAP, int WAVES>
+__global__ void __launch_bounds__(64 * WAVES) k_link(DevBatch B) {
+  if (overflowed(B)) return;
+  __shared__ double lt_all[WAVES][CAP];           // staged times, list order
+  __shared__ uint16_t lpre_all[WAVES][CAP + 4];   // off-diagonal prefix counts, len+1 per list
+  __shared__ uint16_t inv_all[WAVES][CAP];        // rank -> staged element
+  __shared__ uint8_t lsp_all[WAVES][CAP];         // staged element -> spin of the segment below
+  __shared__ uint8_t lfl_all[WAVES][CAP];         // staged element -> event flags
+  const int w = threadIdx.x >> 6;
+  double* lt = lt_all[w];
+  uint16_t* lpre = lpre_all[w];
+  uint16_t* inv = inv_all[w];
+  uint8_t* lsp = lsp_all[w];
+  uint8_t* lfl = lfl_all[w];
+  int slo, shi;
+  xcd_share(B.nsites, slo, shi);
+  for (int g = slo; g < shi; g += WAVES) {  // block-uniform steps
+    const int gs = g + w;
+    const bool have = gs < shi;
+    SiteLists S;
+    S.nbd = 0;
+    int tot = 0, s = 0, n0 = 0, n1 = 0;
+    uint8_t spin0 = 0, gflag = 0;
+    const ChainDev* Pp = nullptr;
+    if (have) {
+      Pp = &B.chains[B.site_chain[gs]];
+      const ChainDev& P = *Pp;
+      s = gs - P.site_base;
+      int bl[4];
+      S.nbd = site_bonds(P, s, bl);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        if (j >= S.nbd) break;
+        const int gb = P.bond_base + bl[j];
+        int s0, s1;
+        bond_sites(P, bl[j], s0, s1);
+        S.bond[j] = bl[j];
+        S.lo[j] = B.bo_new[gb];
+        S.len[j] = B.bo_new[gb + 1] - S.lo[j];
+        S.off[j] = tot;
+        S.side[j] = (s0 == s) ? 0 : 1;
+        tot += S.len[j];
+      }
+      n0 = B.soff_new[gs];
+      n1 = B.soff_new[gs + 1];
+      spin0 = B.spins_old[P.old_site_base + s];
+      gflag = gauge(P, s) < 0 ? EV_GAUGE : 0;
+      if (tot <= CAP) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          if (j >= S.nbd) break;
+          for (int q = lane_id(); q < S.len[j]; q += 64) {
+            lt[S.off[j] + q] = B.ot_new[S.lo[j] + q];
+            lpre[S.off[j] + j + q] = (uint16_t)B.kc_new[S.lo[j] + q];
+          }
+          if (lane_id() == 0) lpre[S.off[j] + j + S.len[j]] = (uint16_t)list_pre_global(B, S, j, S.len[j]);
+        }
+      }
+    }
+    __syncthreads();
+    if (have && tot > 0) {
+      const bool lds = tot <= CAP;
+      for (int e = lane_id(); e < tot; e += 64) {
+        // the element's list, selected with constant indices
+        int i = 0, offi = S.off[0], loi = S.lo[0], bi = S.bond[0], side = S.side[0];
+#pragma unroll
+        for (int j = 1; j < 4; ++j)
+          if (j < S.nbd && e >= S.off[j]) { i = j; offi = S.off[j]; loi = S.lo[j]; bi = S.bond[j]; side = S.side[j]; }
+        const int p = e - offi;
+        const int k = loi + p;
+        const double tau = lds ? lt[e] : B.ot_new[k];
+        int rank, par;
+        if (lds) rank_event<true>(B, S, i, bi, p, tau, lt, lpre, rank, par);
+        else rank_event<false>(B, S, i, bi, p, tau, nullptr, nullptr, rank, par);
+        const uint8_t fl = (uint8_t)((B.oi_new[k] & 1) | (side ? 0 : EV_SRC) |
+                                     (bond_dir(*Pp, bi) ? EV_DIR1 : 0) | (rank == 0 ? EV_FIRST : 0) | gflag);
+        const uint8_t sp = spin0 ^ (uint8_t)(par & 1);
+        const int node = n0 + rank;
+        const int up = (node + 1 == n1) ? n0 : node + 1;  // segment above the event
+        if (side) { B.l1[k] = node; B.u1[k] = up; } else { B.l0[k] = node; B.u0[k] = up; }
+        if (lds) {
+          inv[rank] = (uint16_t)e;
+          lsp[e] = sp;
+          lfl[e] = fl;
+        } else {
+          B.evt_new[node] = tau;
+          B.segspin_new[node] = sp;
+          B.evflags[node] = fl;
+          B.parent[node] = node;
+        }
+      }
+    }
+    __syncthreads();
+    if (have && tot > 0 && tot <= CAP) {
+      for (int r = lane_id(); r < tot; r += 64) {
+        const int e = inv[r];
+        B.evt_new[n0 + r] = lt[e];
+        B.segspin_new[n0 + r] = lsp[e];
+        B.evflags[n0 + r] = lfl[e];
+        B.parent[n0 + r] = n0 + r;
+      }
+    }
+    __syncthreads();
   }
 }
 
 // ---------------------------------------------------------------------------
 // (c) lock-free union-find on segments: hook the larger root under the
-// smaller with atomicCAS (so every root is its cluster's minimum node) and
-// shorten paths while walking (the ECL-CC scheme of Jaiganesh & Burtscher,
-// HPDC'18).  parent[x] <= x always holds.
+// smaller with atomicCAS, so every root is its cluster's minimum node (the
+// hooking rule of ECL-CC, Jaiganesh & Burtscher, HPDC'18).  parent[x] <= x
+// always holds.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ int ld_parent(int* P, int x) {
   return __hip_atomic_load(P + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Plain walk to the root: shortening the path with stores while walking cost
+// more write traffic than the longer walks it saves (k_uf_compress flattens
+// every tree once all unions are done).
 __device__ __forceinline__ int uf_find(int* P, int v) {
-  int cur = ld_parent(P, v);
-  if (cur != v) {
-    int prev = v, next;
-    while (cur > (next = ld_parent(P, cur))) {
-      __hip_atomic_store(P + prev, next, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      prev = cur;
-      cur = next;
-    }
-  }
+  int cur = v, next;
+  while (cur != (next = ld_parent(P, cur))) cur = next;
   return cur;
 }
 
@@ -458,22 +611,14 @@ __device__ __forceinline__ void uf_unite(int* P, int a, int b) {
   }
 }
 
-__device__ __forceinline__ int next_node(const DevBatch& B, int gs, int node) {
-  return (node + 1 == B.soff_new[gs + 1]) ? B.soff_new[gs] : node + 1;
-}
-
 __global__ void __launch_bounds__(256) k_uf_union(DevBatch B) {
   if (overflowed(B)) return;
   const int nops = B.totals[0];
-  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nops; k += gridDim.x * blockDim.x) {
+  int klo, khi;
+  xcd_share(nops, klo, khi);
+  for (int k = klo + threadIdx.x; k < khi; k += blockDim.x) {
     const int info = B.oi_new[k];
-    const int gb = info >> 3;
-    const ChainDev& P = B.chains[B.bond_chain[gb]];
-    int s0, s1;
-    bond_sites(P, gb - P.bond_base, s0, s1);
-    const int l0 = B.l0[k], l1 = B.l1[k];
-    const int u0 = next_node(B, P.site_base + s0, l0);
-    const int u1 = next_node(B, P.site_base + s1, l1);
+    const int l0 = B.l0[k], l1 = B.l1[k], u0 = B.u0[k], u1 = B.u1[k];
     const int g = (info >> 1) & 3;
     if (g & 2) {            // frozen: all four legs (graph_impl.h:282-283)
       uf_unite(B.parent, l0, l1);
@@ -492,14 +637,21 @@ __global__ void __launch_bounds__(256) k_uf_union(DevBatch B) {
 __global__ void __launch_bounds__(256) k_uf_compress(DevBatch B) {
   if (overflowed(B)) return;
   const int M = B.totals[1];
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < M; i += gridDim.x * blockDim.x) {
+  int ilo, ihi;
+  xcd_share(M, ilo, ihi);
+  for (int i = ilo + threadIdx.x; i < ihi; i += blockDim.x) {
     int p = B.parent[i];
     if (p != i) {
       int r = p, q;
       while ((q = B.parent[r]) != r) r = q;
       B.parent[i] = r;
     } else {
+      // root: clear its accumulator and draw the cluster's flip bit, keyed by
+      // the chain-local index of the root (the cluster's minimum segment)
       B.acc[i] = ClusterAcc{{0, 0, 0, 0}, {0, 0, 0, 0}, 0, 0};
+      const int c = node_chain(B, i);
+      const ChainDev& P = B.chains[c];
+      B.fbit[i] = (uint8_t)(philox(B.sweep[c], STAGE_FLIP, (uint32_t)(i - B.chain_node0[c]), 0u, P.key0, P.key1).x & 1u);
     }
   }
 }
@@ -528,6 +680,63 @@ __device__ __forceinline__ double wave_sum_d(double v) {
   return v;
 }
 
+constexpr int EST_LEADERS = 2;     // wave-aggregated roots per item
+constexpr int EST_MIN_GROUP = 8;   // lanes sharing a root that make aggregation pay
+
+__device__ __forceinline__ int wave_sum_i(int v) {
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d, 64);
+  return v;
+}
+
+__device__ __forceinline__ long long wave_sum_ll(long long v) {
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) {
+    const uint32_t lo = __shfl_xor((unsigned int)(uint64_t)v, d, 64);
+    const uint32_t hi = __shfl_xor((unsigned int)((uint64_t)v >> 32), d, 64);
+    v += (long long)(((uint64_t)hi << 32) | lo);
+  }
+  return v;
+}
+
+// Add one contribution (class lengths, tau = 0 boundary counts packed 8 bits
+// per class, packed winding) for root r to the block's LDS table, or to the
+// root's global accumulator when its probe sequence is full.  The key is
+// inserted with a CAS only when a plain read does not already find it, so a
+// hot key costs one broadcast read.
+__device__ __forceinline__ void ht_add(int* hkey, double (*hL)[HT], unsigned int (*hn)[HT],
+                                       unsigned long long* hw, const DevBatch& B, int r,
+                                       const double* sL, int n4, long long w) {
+  int h = (int)(((uint32_t)r * 2654435761u) >> (32 - HT_BITS));
+  int slot = -1;
+  for (int p = 0; p < HT_PROBES; ++p) {
+    int k = __hip_atomic_load(&hkey[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (k == -1) k = atomicCAS(&hkey[h], -1, r);
+    if (k == -1 || k == r) { slot = h; break; }
+    h = (h + 1) & (HT - 1);
+  }
+  if (slot >= 0) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      if (sL[q] != 0.0) atomicAdd(&hL[q][slot], sL[q]);
+    if (n4) {
+      const unsigned int c0 = n4 & 0xff, c1 = (n4 >> 8) & 0xff, c2 = (n4 >> 16) & 0xff, c3 = (n4 >> 24) & 0xff;
+      if (c0 | c1) atomicAdd(&hn[0][slot], c0 | (c1 << 16));
+      if (c2 | c3) atomicAdd(&hn[1][slot], c2 | (c3 << 16));
+    }
+    if (w) atomicAdd(&hw[slot], (unsigned long long)w);
+  } else {
+    ClusterAcc* a = B.acc + r;
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      if (sL[q] != 0.0) unsafeAtomicAdd(&a->L[q], sL[q]);
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      if ((n4 >> (8 * q)) & 0xff) atomicAdd(&a->n[q], (n4 >> (8 * q)) & 0xff);
+    if (w) atomicAdd((unsigned long long*)&a->w, (unsigned long long)w);
+  }
+}
+
 __global__ void __launch_bounds__(EST_THREADS) k_estimate(DevBatch B) {
   if (overflowed(B)) return;
   __shared__ int hkey[HT];
@@ -544,62 +753,69 @@ __global__ void __launch_bounds__(EST_THREADS) k_estimate(DevBatch B) {
   __syncthreads();
   const int M = B.totals[1];
   const int nchunks = (M + EST_CHUNK - 1) / EST_CHUNK;
-  for (int chunk = blockIdx.x; chunk < nchunks; chunk += gridDim.x) {
+  int clo, chi;
+  xcd_share(nchunks, clo, chi);
+  for (int chunk = clo; chunk < chi; ++chunk) {
     for (int it = 0; it < EST_ITEMS; ++it) {
       const int i = chunk * EST_CHUNK + it * EST_THREADS + tid;
-      if (i >= M) break;
-      const int gs = B.node_site[i];
-      const int c = B.site_chain[gs];
-      const ChainDev& P = B.chains[c];
-      const int s = gs - P.site_base;
-      const int r = B.parent[i];
-      const int chain_node0 = B.soff_new[P.site_base];
-      const uint32_t f =
-          philox(B.sweep[c], STAGE_FLIP, (uint32_t)(r - chain_node0), 0u, P.key0, P.key1).x & 1u;
-      B.fbit[i] = (uint8_t)f;
-      const uint8_t fl = B.evflags[i];
-      const int spin = B.segspin_new[i];
-      const int sig = 1 - 2 * spin;
-      int dw0 = 0, dw1 = 0;
-      double len;
-      bool first;
-      if (fl & EV_EMPTY) {
-        len = 1.0;
-        first = true;
-      } else {
-        const int base = B.soff_new[gs];
-        const int last = B.soff_new[gs + 1] - 1;
-        first = (i == base);
-        const double thi = B.evt_new[i];
-        len = first ? ((thi + 1.0) - B.evt_new[last]) : (thi - B.evt_new[i - 1]);
-        const uint8_t flo = B.evflags[first ? last : i - 1];
-        // this segment is the lower leg of the event above it and the upper
-        // leg of the event below it (stiffness.h:94-105, source legs only)
-        if (fl & EV_SRC) { if (fl & EV_DIR1) dw1 += sig; else dw0 += sig; }
-        if (flo & EV_SRC) { if (flo & EV_DIR1) dw1 -= sig; else dw0 -= sig; }
-      }
-      if (first) B.spins_new[gs] = (uint8_t)(spin ^ f);
-      B.segspin_new[i] = (uint8_t)(spin ^ f);
-      const int cls = seg_class(spin, gauge(P, s));
-      const long long wp = pack_winding(dw0, dw1);
-      int h = (int)(((uint32_t)r * 2654435761u) >> (32 - HT_BITS));
-      bool placed = false;
-      for (int p = 0; p < HT_PROBES; ++p) {
-        const int k = atomicCAS(&hkey[h], -1, r);
-        if (k == -1 || k == r) {
-          atomicAdd(&hL[cls][h], len);
-          if (first) atomicAdd(&hn[cls >> 1][h], 1u << (16 * (cls & 1)));
-          if (wp) atomicAdd(&hw[h], (unsigned long long)wp);
-          placed = true;
-          break;
+      if (__builtin_amdgcn_readfirstlane(i) >= M) break;  // lane 0 holds the wave's lowest node
+      const bool valid = i < M;
+      int r = -1, cls = 0;
+      bool first = false;
+      double len = 0;
+      long long wp = 0;
+      if (valid) {
+        r = B.parent[i];  // compressed: the root
+        const uint8_t fl = B.evflags[i];
+        const int spin = B.segspin_new[i];
+        const uint32_t f = B.fbit[r];  // drawn for the root by k_uf_compress
+        if (i != r) B.fbit[i] = (uint8_t)f;
+        const int sig = 1 - 2 * spin;
+        first = (fl & EV_FIRST) != 0;
+        int dw0 = 0, dw1 = 0;
+        int gs = 0;
+        if (first) gs = first_node_site(B, B.chains[node_chain(B, i)], i);
+        if (fl & EV_EMPTY) {
+          len = 1.0;
+        } else {
+          // the segment below the site's first event wraps through tau = 1
+          const int prev = first ? B.soff_new[gs + 1] - 1 : i - 1;
+          const double thi = B.evt_new[i];
+          len = first ? ((thi + 1.0) - B.evt_new[prev]) : (thi - B.evt_new[prev]);
+          const uint8_t flo = B.evflags[prev];
+          // this segment is the lower leg of the event above it and the upper
+          // leg of the event below it (stiffness.h:94-105, source legs only)
+          if (fl & EV_SRC) { if (fl & EV_DIR1) dw1 += sig; else dw0 += sig; }
+          if (flo & EV_SRC) { if (flo & EV_DIR1) dw1 -= sig; else dw0 -= sig; }
         }
-        h = (h + 1) & (HT - 1);
+        if (first) B.spins_new[gs] = (uint8_t)(spin ^ f);
+        B.segspin_new[i] = (uint8_t)(spin ^ f);
+        cls = (spin << 1) | ((fl & EV_GAUGE) ? 1 : 0);  // seg_class
+        wp = pack_winding(dw0, dw1);
       }
-      if (!placed) {
-        ClusterAcc* a = B.acc + r;
-        unsafeAtomicAdd(&a->L[cls], len);
-        if (first) atomicAdd(&a->n[cls], 1);
-        if (wp) atomicAdd((unsigned long long*)&a->w, (unsigned long long)wp);
+      // Lanes that share the wave's leading root (the big loops of a cold
+      // antiferromagnet) are summed across the wave first, so one lane
+      // updates the table for all of them.
+      uint64_t pend = __ballot(valid);
+      for (int a = 0; a < EST_LEADERS && pend; ++a) {
+        const int leader = __builtin_ctzll(pend);
+        const int rl = __shfl(r, leader);
+        const bool mine = ((pend >> lane_id()) & 1) && r == rl;
+        const uint64_t m = __ballot(mine);
+        if (__popcll(m) < EST_MIN_GROUP) break;
+        double sL[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) sL[q] = wave_sum_d((mine && cls == q) ? len : 0.0);
+        const int n4 = wave_sum_i((mine && first) ? (1 << (8 * cls)) : 0);  // < 64 per class
+        const long long sw = wave_sum_ll(mine ? wp : 0);
+        if (lane_id() == leader)
+          ht_add(hkey, hL, hn, hw, B, rl, sL, n4, sw);
+        pend &= ~m;
+      }
+      if ((pend >> lane_id()) & 1) {
+        double sL[4] = {0, 0, 0, 0};
+        sL[cls] = len;
+        ht_add(hkey, hL, hn, hw, B, r, sL, first ? (1 << (8 * cls)) : 0, wp);
       }
     }
     __syncthreads();
@@ -675,62 +891,68 @@ __global__ void __launch_bounds__(COL_THREADS) k_flip_collect(DevBatch B) {
   if (overflowed(B)) return;
   const int nops = B.totals[0];
   const int M = B.totals[1];
-  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nops; k += gridDim.x * blockDim.x) {
-    const int info = B.oi_new[k];
-    const int gb = info >> 3;
-    const ChainDev& P = B.chains[B.bond_chain[gb]];
-    int s0, s1;
-    bond_sites(P, gb - P.bond_base, s0, s1);
-    const int l0 = B.l0[k];
-    const int u0 = next_node(B, P.site_base + s0, l0);
-    if (B.fbit[l0] ^ B.fbit[u0]) B.oi_new[k] = info ^ 1;
+  int klo, khi;
+  xcd_share(nops, klo, khi);
+  for (int k = klo + threadIdx.x; k < khi; k += blockDim.x) {
+    const int l0 = B.l0[k], u0 = B.u0[k];
+    if (B.fbit[l0] ^ B.fbit[u0]) B.oi_new[k] ^= 1;
   }
   // per-chunk sums by chain: slot k holds chain c_first + k (chains are
   // contiguous in node order, so a chunk meets few of them)
   constexpr int CSLOTS = 8;
   __shared__ double slot[CSLOTS][NCOL];
   const int nchunks = (M + COL_CHUNK - 1) / COL_CHUNK;
-  for (int chunk = blockIdx.x; chunk < nchunks; chunk += gridDim.x) {
+  int clo, chi;
+  xcd_share(nchunks, clo, chi);
+  for (int chunk = clo; chunk < chi; ++chunk) {
     const int base = chunk * COL_CHUNK;
-    const int c0 = B.site_chain[B.node_site[base]];
+    const int c0 = node_chain(B, base);
+    const int c0_end = (c0 + 1 < B.nchains) ? B.chain_node0[c0 + 1] : M;  // nodes of chain c0 end here
     for (int s = threadIdx.x; s < CSLOTS * NCOL; s += COL_THREADS) slot[s / NCOL][s % NCOL] = 0;
     __syncthreads();
+    // each lane sums the terms of its roots while they belong to one chain
+    // (cl) and hands the sum on when the chain changes and at the chunk's end
+    double w[NCOL];
+#pragma unroll
+    for (int q = 0; q < NCOL; ++q) w[q] = 0;
+    int cl = -1;
+    auto flush = [&](int c) {
+      const int k = c - c0;
+      if (k < CSLOTS) {
+        for (int q = 0; q < NCOL; ++q) atomicAdd(&slot[k][q], w[q]);
+      } else {
+        double* rec = B.rec_acc + (size_t)c * R_COUNT;
+        for (int q = 0; q < NCOL; ++q) unsafeAtomicAdd(rec + term_slot(q), w[q]);
+      }
+    };
     for (int it = 0; it < COL_ITEMS; ++it) {
       const int i = base + it * COL_THREADS + threadIdx.x;
-      double w[NCOL];
-#pragma unroll
-      for (int q = 0; q < NCOL; ++q) w[q] = 0;
-      int c = -1;
       if (i < M && B.parent[i] == i) {
-        c = B.site_chain[B.node_site[i]];
-        cluster_terms(B.acc[i], w);
+        const int c = (i < c0_end) ? c0 : node_chain(B, i);
+        if (c != cl) {
+          if (cl >= 0) flush(cl);
+#pragma unroll
+          for (int q = 0; q < NCOL; ++q) w[q] = 0;
+          cl = c;
+        }
+        double v[NCOL];
+        cluster_terms(B.acc[i], v);
+#pragma unroll
+        for (int q = 0; q < NCOL; ++q) w[q] += v[q];
       }
-      int cmax = c, cmin = (c < 0) ? 0x7fffffff : c;
-      for (int d = 32; d > 0; d >>= 1) {
-        cmax = max(cmax, __shfl_xor(cmax, d, 64));
-        cmin = min(cmin, __shfl_xor(cmin, d, 64));
-      }
-      if (cmax < 0) continue;  // no root in this wave
+    }
+    int cmax = cl, cmin = (cl < 0) ? 0x7fffffff : cl;
+    for (int d = 32; d > 0; d >>= 1) {
+      cmax = max(cmax, __shfl_xor(cmax, d, 64));
+      cmin = min(cmin, __shfl_xor(cmin, d, 64));
+    }
+    if (cmax >= 0) {  // some lane of the wave holds a sum
       if (cmin == cmax) {
 #pragma unroll
         for (int q = 0; q < NCOL; ++q) w[q] = wave_sum_d(w[q]);
-        if (lane_id() == 0) {
-          const int k = cmax - c0;
-          if (k < CSLOTS) {
-            for (int q = 0; q < NCOL; ++q) atomicAdd(&slot[k][q], w[q]);
-          } else {
-            double* rec = B.rec_acc + (size_t)cmax * R_COUNT;
-            for (int q = 0; q < NCOL; ++q) unsafeAtomicAdd(rec + term_slot(q), w[q]);
-          }
-        }
-      } else if (c >= 0) {
-        const int k = c - c0;
-        if (k < CSLOTS) {
-          for (int q = 0; q < NCOL; ++q) atomicAdd(&slot[k][q], w[q]);
-        } else {
-          double* rec = B.rec_acc + (size_t)c * R_COUNT;
-          for (int q = 0; q < NCOL; ++q) unsafeAtomicAdd(rec + term_slot(q), w[q]);
-        }
+        if (lane_id() == 0) flush(cmax);
+      } else if (cl >= 0) {
+        flush(cl);
       }
     }
     __syncthreads();
@@ -824,13 +1046,10 @@ __device__ __forceinline__ int element_node(const DevBatch& B, const ChainDev& P
   if (e < P.N) return B.soff_new[P.site_base + e];
   int k = (e - P.N) >> 2, leg = (e - P.N) & 3;
   int kg = op0 + k;
-  int gb = B.oi_new[kg] >> 3;
-  int s0, s1;
-  bond_sites(P, gb - P.bond_base, s0, s1);
   if (leg == 0) return B.l0[kg];
   if (leg == 1) return B.l1[kg];
-  if (leg == 2) return next_node(B, P.site_base + s0, B.l0[kg]);
-  return next_node(B, P.site_base + s1, B.l1[kg]);
+  if (leg == 2) return B.u0[kg];
+  return B.u1[kg];
 }
 
 __global__ void k_canon_min(DevBatch B, int chain, int nelem, int* canon) {
@@ -909,7 +1128,15 @@ void launch_sweep(const DevBatch& B, const LaunchCfg& L, hipStream_t st, hipEven
 }
 
 void launch_cluster_stage(const DevBatch& B, const LaunchCfg& L, hipStream_t st, hipEvent_t* ev) {
-  hipLaunchKernelGGL(k_link, dim3(L.blocks_op), dim3(256), 0, st, B);
+  // staging capacity by the expected events per site: 512 events with 4 sites
+  // per block (29 KB of LDS), else 1536 with 2 (43 KB)
+  if (L.link_cap <= 512) {
+    int blocks = min(L.blocks_node, max(1, (B.nsites + 3) / 4));
+    hipLaunchKernelGGL((k_link<512, 4>), dim3(blocks), dim3(256), 0, st, B);
+  } else {
+    int blocks = min(L.blocks_node, max(1, (B.nsites + 1) / 2));
+    hipLaunchKernelGGL((k_link<1536, 2>), dim3(blocks), dim3(128), 0, st, B);
+  }
   mark(ev, 1, st);
   hipLaunchKernelGGL(k_uf_union, dim3(L.blocks_op), dim3(256), 0, st, B);
   mark(ev, 2, st);
