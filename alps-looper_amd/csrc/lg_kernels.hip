@@ -132,15 +132,45 @@ __device__ __forceinline__ int first_node_site(const DevBatch& B, const ChainDev
   return lo;
 }
 
-// spin at tau on (chain-local) site s in the previous configuration;
-// -1 when tau coincides with an event time there (such candidates are rejected)
-__device__ __forceinline__ int spin_prev(const DevBatch& B, const ChainDev& P, int s, double tau) {
-  int sg = P.old_site_base + s;
-  int base = B.soff_old[sg];
-  int m = B.soff_old[sg + 1] - base;
-  const double* t = B.evt_old + base;
-  int j = lower_bound_d(t, m, tau);
-  if (j < m && t[j] == tau) return -1;
+// Lower bound of each lane's time in a site's event times t[0..m) (ascending),
+// for a lane group whose valid times ascend with the lane and are not below
+// the group-uniform cursor `cur` (events before cur are below every time).
+// The group reads the line in windows of G events from the cursor, one load
+// per lane, and each lane searches the window by shuffles; the cursor moves
+// to the last lane's result.  `equal`: the event at the result has the time.
+template <int G>
+__device__ __forceinline__ int group_lower_bound(const double* t, int m, double tau, int& cur, bool& equal) {
+  const int gl = glane<G>();
+  const int lane0 = lane_id() & ~(G - 1);
+  int res = -1;
+  equal = false;
+  for (int pos = cur;; pos += G) {
+    const double w = (pos + gl < m) ? t[pos + gl] : 3.0;  // past the end: above every time
+    int k = 0;
+#pragma unroll
+    for (int step = G / 2; step >= 1; step >>= 1) {
+      const double v = __shfl(w, lane0 + k + step - 1, 64);
+      if (v < tau) k += step;
+    }
+    const double wk = __shfl(w, lane0 + k, 64);
+    if (wk < tau) k += 1;  // k in [0, G]: window events below tau
+    if (res < 0 && (k < G || pos + G >= m)) {
+      res = pos + k;
+      equal = (k < G) && (wk == tau);
+    }
+    if (gballot<G>(res < 0) == 0) break;
+  }
+  cur = __shfl(res, lane0 + G - 1, 64);
+  return res;
+}
+
+// spin of the previous world line of a site at each lane's time; -1 where an
+// event of the line has exactly that time (such candidates are rejected)
+template <int G>
+__device__ __forceinline__ int group_spin_prev(const DevBatch& B, int base, int m, double tau, int& cur) {
+  bool eq;
+  const int j = group_lower_bound<G>(B.evt_old + base, m, tau, cur, eq);
+  if (eq) return -1;
   return B.segspin_old[base + (j == m ? 0 : j)];
 }
 
@@ -177,6 +207,9 @@ __global__ void __launch_bounds__(256) k_diag_count(DevBatch B) {
     int acc = 0;
     if (P.has_weight) {
       const long long cb = P.cand_base + (long long)b * P.cand_cap;
+      const int base0 = B.soff_old[P.old_site_base + s0], m0 = B.soff_old[P.old_site_base + s0 + 1] - base0;
+      const int base1 = B.soff_old[P.old_site_base + s1], m1 = B.soff_old[P.old_site_base + s1 + 1] - base1;
+      int cur0 = 0, cur1 = 0;  // world-line cursors (candidate times ascend)
       uint64_t Tb = 0;
       uint32_t kb = 0;
       while (true) {
@@ -184,16 +217,13 @@ __global__ void __launch_bounds__(256) k_diag_count(DevBatch B) {
         uint64_t gap = exp_gap(x, P.scale);
         uint64_t incl = gscan_incl_u64<G>(gap);
         uint64_t T = Tb + incl;
-        bool accept = false;
-        double tau = 0;
-        int g = 0;
-        if (T < TGRID) {
-          tau = (double)T * TGRID_INV;
-          g = choose_graph(x.z, P);
-          int c0 = spin_prev(B, P, s0, tau);
-          int c1 = spin_prev(B, P, s1, tau);
-          accept = (c0 >= 0) && (c1 >= 0) && (((g & 1) ^ c0 ^ c1) != 0);
-        }
+        const bool valid = T < TGRID;
+        // lanes past the end of [0,1) search for -1 (found at once)
+        const double tau = valid ? (double)T * TGRID_INV : -1.0;
+        const int g = choose_graph(x.z, P);
+        const int c0 = group_spin_prev<G>(B, base0, m0, tau, cur0);
+        const int c1 = group_spin_prev<G>(B, base1, m1, tau, cur1);
+        const bool accept = valid && (c0 >= 0) && (c1 >= 0) && (((g & 1) ^ c0 ^ c1) != 0);
         uint64_t m = gballot<G>(accept);
         if (accept) {
           int slot = acc + __popcll(m & below);
@@ -680,25 +710,6 @@ __device__ __forceinline__ double wave_sum_d(double v) {
   return v;
 }
 
-constexpr int EST_LEADERS = 2;     // wave-aggregated roots per item
-constexpr int EST_MIN_GROUP = 8;   // lanes sharing a root that make aggregation pay
-
-__device__ __forceinline__ int wave_sum_i(int v) {
-#pragma unroll
-  for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d, 64);
-  return v;
-}
-
-__device__ __forceinline__ long long wave_sum_ll(long long v) {
-#pragma unroll
-  for (int d = 32; d > 0; d >>= 1) {
-    const uint32_t lo = __shfl_xor((unsigned int)(uint64_t)v, d, 64);
-    const uint32_t hi = __shfl_xor((unsigned int)((uint64_t)v >> 32), d, 64);
-    v += (long long)(((uint64_t)hi << 32) | lo);
-  }
-  return v;
-}
-
 // Add one contribution (class lengths, tau = 0 boundary counts packed 8 bits
 // per class, packed winding) for root r to the block's LDS table, or to the
 // root's global accumulator when its probe sequence is full.  The key is
@@ -793,26 +804,7 @@ __global__ void __launch_bounds__(EST_THREADS) k_estimate(DevBatch B) {
         cls = (spin << 1) | ((fl & EV_GAUGE) ? 1 : 0);  // seg_class
         wp = pack_winding(dw0, dw1);
       }
-      // Lanes that share the wave's leading root (the big loops of a cold
-      // antiferromagnet) are summed across the wave first, so one lane
-      // updates the table for all of them.
-      uint64_t pend = __ballot(valid);
-      for (int a = 0; a < EST_LEADERS && pend; ++a) {
-        const int leader = __builtin_ctzll(pend);
-        const int rl = __shfl(r, leader);
-        const bool mine = ((pend >> lane_id()) & 1) && r == rl;
-        const uint64_t m = __ballot(mine);
-        if (__popcll(m) < EST_MIN_GROUP) break;
-        double sL[4];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) sL[q] = wave_sum_d((mine && cls == q) ? len : 0.0);
-        const int n4 = wave_sum_i((mine && first) ? (1 << (8 * cls)) : 0);  // < 64 per class
-        const long long sw = wave_sum_ll(mine ? wp : 0);
-        if (lane_id() == leader)
-          ht_add(hkey, hL, hn, hw, B, rl, sL, n4, sw);
-        pend &= ~m;
-      }
-      if ((pend >> lane_id()) & 1) {
+      if (valid) {
         double sL[4] = {0, 0, 0, 0};
         sL[cls] = len;
         ht_add(hkey, hL, hn, hw, B, r, sL, first ? (1 << (8 * cls)) : 0, wp);
