@@ -258,6 +258,8 @@ int Batch::allocate() {
   if ((rc = alloc(u1, op_cap))) return rc;
   if ((rc = alloc(evflags, node_cap))) return rc;
   if ((rc = alloc(chain_node0, cap_chains))) return rc;
+  // node_site only serves batches of more than one chain
+  if ((rc = alloc(node_site, cap_chains > 1 ? node_cap : 1))) return rc;
   if ((rc = alloc(parent, node_cap))) return rc;
   if ((rc = alloc(fbit, node_cap))) return rc;
   if ((rc = alloc(acc, node_cap))) return rc;
@@ -330,7 +332,10 @@ int Batch::set_layout(std::vector<ChainDev>& ds, const std::vector<uint64_t>& sw
                             0.5 * (std::fabs(params[c].jz) / 4 + std::fabs(params[c].jxy) / 2);
     ev += 2.0 * chains[c].beta * chains[c].Nb * per_bond;
   }
-  cfg.link_cap = (nsites > 0 && ev / nsites > 400) ? 1536 : 512;
+  if (!tuned) {
+    const double per_site = nsites > 0 ? ev / nsites : 0;
+    cfg.link_cap = per_site < 96 ? 0 : (per_site <= 400 ? 512 : 1536);
+  }
   invalidate_graphs();
   return LG_OK;
 }
@@ -444,7 +449,7 @@ DevBatch Batch::dev(int from) const {
   B.soff_new = w.soff; B.evt_new = w.evt; B.segspin_new = w.segspin;
   B.counts = counts; B.acc_count = acc_count; B.kpre = kpre;
   B.cand_t = cand_t; B.cand_g = cand_g; B.kc_new = kc_new;
-  B.l0 = l0; B.l1 = l1; B.u0 = u0; B.u1 = u1; B.evflags = evflags; B.chain_node0 = chain_node0;
+  B.l0 = l0; B.l1 = l1; B.u0 = u0; B.u1 = u1; B.evflags = evflags; B.chain_node0 = chain_node0; B.node_site = node_site;
   B.parent = parent; B.fbit = fbit; B.acc = acc;
   B.scan_part = scan_part; B.col_part = col_part; B.totals = totals; B.flags = flags;
   B.sweep = d_sweep; B.rec_acc = rec_acc; B.rec_ring = rec_ring;
@@ -533,8 +538,9 @@ int Batch::run(uint32_t n, lg_sweep_record* out) {
   return LG_OK;
 }
 
-// Autotuner: time candidate lane-group sizes of the diagonal update and grid
-// sizes of the per-operator / per-node kernels on the live configuration.
+// Autotuner: time candidate lane-group sizes of the diagonal update, grid
+// sizes of the per-operator / per-node kernels and the linking kernels on the
+// live configuration.
 int Batch::tune(uint32_t sweeps_per_trial) {
   if (sweeps_per_trial == 0) sweeps_per_trial = 8;
   int rc = run(sweeps_per_trial, nullptr);  // warm state and code
@@ -547,12 +553,15 @@ int Batch::tune(uint32_t sweeps_per_trial) {
   double best_t = 1e300;
   const int groups[] = {4, 8, 16, 32, 64};
   const int mults[] = {2, 4, 8, 16, 32};
-  // group size first with the current grids, then the grids with the best group
-  for (int pass = 0; pass < 2; ++pass) {
-    for (int i = 0; i < 5; ++i) {
+  const int links[] = {0, 512, 1536};
+  // group size first with the current grids, then the grids with the best
+  // group, then the linking kernel
+  for (int pass = 0; pass < 3; ++pass) {
+    for (int i = 0; i < (pass == 2 ? 3 : 5); ++i) {
       LaunchCfg trial = best;
       if (pass == 0) trial.group = groups[i];
-      else trial.blocks_op = trial.blocks_node = cus * mults[i];
+      else if (pass == 1) trial.blocks_op = trial.blocks_node = cus * mults[i];
+      else trial.link_cap = links[i];
       cfg = trial;
       invalidate_graphs();
       rc = synchronize();
@@ -565,6 +574,33 @@ int Batch::tune(uint32_t sweeps_per_trial) {
     }
   }
   cfg = best;
+  tuned = true;
+  invalidate_graphs();
+  return LG_OK;
+}
+
+void Batch::get_launch(lg_launch* out) const {
+  out->group = cfg.group;
+  out->blocks = cfg.blocks_op;
+  out->link = cfg.link_cap;
+}
+
+int Batch::set_launch(const lg_launch& l) {
+  LaunchCfg c = cfg;
+  if (l.group >= 0) {
+    if (l.group != 4 && l.group != 8 && l.group != 16 && l.group != 32 && l.group != 64)
+      return set_error(LG_ERR_ARG, "launch group must be 4, 8, 16, 32 or 64");
+    c.group = l.group;
+  }
+  if (l.blocks >= 0) {
+    if (l.blocks < 1 || l.blocks > (1 << 20)) return set_error(LG_ERR_ARG, "launch blocks must be in [1, 2^20]");
+    c.blocks_op = c.blocks_node = l.blocks;
+  }
+  if (l.link >= 0) {
+    if (l.link != 0 && l.link != 512 && l.link != 1536) return set_error(LG_ERR_ARG, "launch link must be 0, 512 or 1536");
+    c.link_cap = l.link;
+  }
+  cfg = c;
   tuned = true;
   invalidate_graphs();
   return LG_OK;
@@ -913,6 +949,28 @@ int lg_batch_set_graph(lg_batch* h, int enable) {
   if (!h) return set_error(LG_ERR_ARG, "null handle");
   h->b.use_graph = enable != 0;
   return LG_OK;
+}
+
+int lg_chain_get_launch(lg_chain* h, lg_launch* out) {
+  if (!h || !out) return set_error(LG_ERR_ARG, "null pointer");
+  h->b.get_launch(out);
+  return LG_OK;
+}
+
+int lg_chain_set_launch(lg_chain* h, const lg_launch* l) {
+  if (!h || !l) return set_error(LG_ERR_ARG, "null pointer");
+  return h->b.set_launch(*l);
+}
+
+int lg_batch_get_launch(lg_batch* h, lg_launch* out) {
+  if (!h || !out) return set_error(LG_ERR_ARG, "null pointer");
+  h->b.get_launch(out);
+  return LG_OK;
+}
+
+int lg_batch_set_launch(lg_batch* h, const lg_launch* l) {
+  if (!h || !l) return set_error(LG_ERR_ARG, "null pointer");
+  return h->b.set_launch(*l);
 }
 
 int lg_batch_tune(lg_batch* h, uint32_t sweeps_per_trial) {

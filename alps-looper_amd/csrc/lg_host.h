@@ -50,6 +50,8 @@ class Batch {
   int relayout(const std::vector<int>& keep, const std::vector<lg_params>& add);
   int run(uint32_t n, lg_sweep_record* out);
   int tune(uint32_t sweeps_per_trial);
+  void get_launch(lg_launch* out) const;
+  int set_launch(const lg_launch& l);
   int profile(uint32_t n, double* ms);
   int synchronize();
   int set_config(int c, const int32_t* spins, const int32_t* bond_offsets, const double* times,
@@ -112,6 +114,7 @@ class Batch {
   int* u1 = nullptr;
   uint8_t* evflags = nullptr;
   int* chain_node0 = nullptr;
+  int* node_site = nullptr;
   int* parent = nullptr;
   uint8_t* fbit = nullptr;
   ClusterAcc* acc = nullptr;

@@ -49,6 +49,7 @@ struct DevBatch {
   int* u1;                     // [new ops] upper-leg node on the target site
   uint8_t* evflags;            // [nodes]
   int* chain_node0;            // [nchains] first segment node of each chain (new configuration)
+  int* node_site;              // [nodes] site of each segment (written for batches of more than one chain)
   int* parent;                 // [nodes]
   uint8_t* fbit;               // [nodes] flip bit of the node's cluster
   ClusterAcc* acc;             // [nodes] (roots only)
@@ -70,7 +71,7 @@ struct LaunchCfg {
   int max_blocks_bond = 1 << 20;
   int blocks_op = 1024;        // grid of the per-operator kernels (grid-stride)
   int blocks_node = 1024;      // grid of the per-node kernels (grid-stride)
-  int link_cap = 512;          // events per site staged in LDS by the linking kernel (512 or 1536)
+  int link_cap = 512;          // linking: 0 one thread per operator, else a wave per site staging 512 or 1536 events
 };
 
 constexpr int NUM_SWEEP_KERNELS = 11;

@@ -111,20 +111,17 @@ __device__ __forceinline__ bool overflowed(const DevBatch& B) {
   return __hip_atomic_load(B.flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
 }
 
-// chain of a segment node of the new configuration (last chain whose first
-// node is <= node; chain_node0 is written by k_scan_down)
+// Site of each segment node (node_site) is stored only for batches of more
+// than one chain; a single chain finds its sites by binary search over the
+// segment offsets, which only the first segment of each site needs.
 __device__ __forceinline__ int node_chain(const DevBatch& B, int node) {
-  int lo = 0, n = B.nchains;
-  while (n > 1) {
-    const int h = n >> 1;
-    if (B.chain_node0[lo + h] <= node) { lo += h; n -= h; } else { n = h; }
-  }
-  return lo;
+  return B.nchains == 1 ? 0 : B.site_chain[B.node_site[node]];
 }
 
 // site (global index) whose first segment is `node`
-__device__ __forceinline__ int first_node_site(const DevBatch& B, const ChainDev& P, int node) {
-  int lo = P.site_base, n = P.N;
+__device__ __forceinline__ int first_node_site(const DevBatch& B, int node) {
+  if (B.nchains > 1) return B.node_site[node];
+  int lo = 0, n = B.nsites;
   while (n > 1) {
     const int h = n >> 1;
     if (B.soff_new[lo + h] <= node) { lo += h; n -= h; } else { n = h; }
@@ -166,10 +163,19 @@ __device__ __forceinline__ int group_lower_bound(const double* t, int m, double 
 
 // spin of the previous world line of a site at each lane's time; -1 where an
 // event of the line has exactly that time (such candidates are rejected)
+// (groups of fewer than 16 lanes search each lane's time on its own: their
+// windows would cover too few events)
 template <int G>
 __device__ __forceinline__ int group_spin_prev(const DevBatch& B, int base, int m, double tau, int& cur) {
+  const double* t = B.evt_old + base;
+  int j;
   bool eq;
-  const int j = group_lower_bound<G>(B.evt_old + base, m, tau, cur, eq);
+  if (G >= 16) {
+    j = group_lower_bound<G>(t, m, tau, cur, eq);
+  } else {
+    j = tau < 0 ? 0 : lower_bound_d(t, m, tau);
+    eq = j < m && t[j] == tau;
+  }
   if (eq) return -1;
   return B.segspin_old[base + (j == m ? 0 : j)];
 }
@@ -183,9 +189,10 @@ __global__ void __launch_bounds__(256) k_diag_count(DevBatch B) {
   const int groups_per_block = blockDim.x / G;
   const int gl = glane<G>();
   const uint64_t below = (gl == 0) ? 0ull : ((1ull << gl) - 1);
-  int glo, ghi;
-  xcd_share(B.nbonds, glo, ghi);
-  for (int gb = glo + threadIdx.x / G; gb < ghi; gb += groups_per_block) {
+  // bonds interleaved over the grid: the work per bond varies with the
+  // chain's temperature in a batch
+  for (int gb = blockIdx.x * groups_per_block + threadIdx.x / G; gb < B.nbonds;
+       gb += gridDim.x * groups_per_block) {
     const int c = B.bond_chain[gb];
     const ChainDev P = B.chains[c];
     const int b = gb - P.bond_base;
@@ -397,6 +404,7 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan_down(DevBatch B) {
           B.segspin_new[node] = B.spins_old[P.old_site_base + (i - P.site_base)];
           B.evflags[node] = (uint8_t)(EV_EMPTY | EV_FIRST | (gauge(P, i - P.site_base) < 0 ? EV_GAUGE : 0));
           B.parent[node] = node;
+          if (B.nchains > 1) B.node_site[node] = i;
         }
       }
     }
@@ -413,9 +421,8 @@ __global__ void __launch_bounds__(256) k_diag_fill(DevBatch B) {
   if (overflowed(B)) return;
   const int groups_per_block = blockDim.x / G;
   const int gl = glane<G>();
-  int glo, ghi;
-  xcd_share(B.nbonds, glo, ghi);
-  for (int gb = glo + threadIdx.x / G; gb < ghi; gb += groups_per_block) {
+  for (int gb = blockIdx.x * groups_per_block + threadIdx.x / G; gb < B.nbonds;
+       gb += gridDim.x * groups_per_block) {
     const int c = B.bond_chain[gb];
     const ChainDev P = B.chains[c];
     const int b = gb - P.bond_base;
@@ -593,6 +600,7 @@ __global__ void __launch_bounds__(64 * WAVES) k_link(DevBatch B) {
           B.segspin_new[node] = sp;
           B.evflags[node] = fl;
           B.parent[node] = node;
+          if (B.nchains > 1) B.node_site[node] = gs;
         }
       }
     }
@@ -604,9 +612,62 @@ __global__ void __launch_bounds__(64 * WAVES) k_link(DevBatch B) {
         B.segspin_new[n0 + r] = lsp[e];
         B.evflags[n0 + r] = lfl[e];
         B.parent[n0 + r] = n0 + r;
+        if (B.nchains > 1) B.node_site[n0 + r] = gs;
       }
     }
     __syncthreads();
+  }
+}
+
+// Linking, one thread per operator: for each leg, the rank of the event on
+// its site by binary search in the site's other bond lists (global memory).
+// Used when sites hold few events (a wave per site would idle most lanes).
+__global__ void __launch_bounds__(256) k_link_ops(DevBatch B) {
+  if (overflowed(B)) return;
+  const int nops = B.totals[0];
+  int klo, khi;
+  xcd_share(nops, klo, khi);
+  for (int k = klo + threadIdx.x; k < khi; k += blockDim.x) {
+    const int info = B.oi_new[k];
+    const int gb = info >> 3;
+    const ChainDev& P = B.chains[B.bond_chain[gb]];
+    const int b = gb - P.bond_base;
+    const double tau = B.ot_new[k];
+    const int r = k - B.bo_new[gb];
+    const uint8_t dflag = bond_dir(P, b) ? EV_DIR1 : 0;
+    int s0, s1;
+    bond_sites(P, b, s0, s1);
+#pragma unroll
+    for (int side = 0; side < 2; ++side) {
+      const int s = side ? s1 : s0;
+      int bl[4];
+      const int nbd = site_bonds(P, s, bl);
+      int rank = r;
+      int par = B.kc_new[k];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        if (i >= nbd) break;
+        const int bp = bl[i];
+        if (bp == b) continue;
+        const int gbp = P.bond_base + bp;
+        const int lo = B.bo_new[gbp], hi = B.bo_new[gbp + 1];
+        const int q = (bp < b) ? upper_bound_d(B.ot_new + lo, hi - lo, tau) : lower_bound_d(B.ot_new + lo, hi - lo, tau);
+        rank += q;
+        if (q < hi - lo) par += B.kc_new[lo + q];
+        else if (hi > lo) par += B.kc_new[hi - 1] + (B.oi_new[hi - 1] & 1);
+      }
+      const int gs = P.site_base + s;
+      const int n0 = B.soff_new[gs], n1 = B.soff_new[gs + 1];
+      const int node = n0 + rank;
+      const int up = (node + 1 == n1) ? n0 : node + 1;  // segment above the event
+      B.evt_new[node] = tau;
+      B.segspin_new[node] = B.spins_old[P.old_site_base + s] ^ (uint8_t)(par & 1);
+      B.evflags[node] = (uint8_t)((info & 1) | (side ? 0 : EV_SRC) | dflag | (rank == 0 ? EV_FIRST : 0) |
+                                  (gauge(P, s) < 0 ? EV_GAUGE : 0));
+      B.parent[node] = node;
+      if (B.nchains > 1) B.node_site[node] = gs;
+      if (side) { B.l1[k] = node; B.u1[k] = up; } else { B.l0[k] = node; B.u0[k] = up; }
+    }
   }
 }
 
@@ -785,7 +846,7 @@ __global__ void __launch_bounds__(EST_THREADS) k_estimate(DevBatch B) {
         first = (fl & EV_FIRST) != 0;
         int dw0 = 0, dw1 = 0;
         int gs = 0;
-        if (first) gs = first_node_site(B, B.chains[node_chain(B, i)], i);
+        if (first) gs = first_node_site(B, i);
         if (fl & EV_EMPTY) {
           len = 1.0;
         } else {
@@ -1120,9 +1181,12 @@ void launch_sweep(const DevBatch& B, const LaunchCfg& L, hipStream_t st, hipEven
 }
 
 void launch_cluster_stage(const DevBatch& B, const LaunchCfg& L, hipStream_t st, hipEvent_t* ev) {
-  // staging capacity by the expected events per site: 512 events with 4 sites
-  // per block (29 KB of LDS), else 1536 with 2 (43 KB)
-  if (L.link_cap <= 512) {
+  // by the expected events per site: one thread per operator (few events),
+  // a wave per site staging 512 events with 4 sites per block (29 KB of
+  // LDS), or 1536 with 2 (43 KB)
+  if (L.link_cap == 0) {
+    hipLaunchKernelGGL(k_link_ops, dim3(L.blocks_op), dim3(256), 0, st, B);
+  } else if (L.link_cap <= 512) {
     int blocks = min(L.blocks_node, max(1, (B.nsites + 3) / 4));
     hipLaunchKernelGGL((k_link<512, 4>), dim3(blocks), dim3(256), 0, st, B);
   } else {

@@ -28,9 +28,25 @@ EXPORTED = [
     "lg_chain_set_config", "lg_chain_destroy", "lg_cluster_config",
     "lg_batch_create", "lg_batch_run", "lg_batch_tune", "lg_batch_set_graph",
     "lg_batch_chain_info", "lg_batch_destroy",
+    "lg_chain_get_launch", "lg_chain_set_launch", "lg_batch_get_launch", "lg_batch_set_launch",
     "lg_profile_num_kernels", "lg_profile_kernel_name", "lg_chain_profile", "lg_batch_profile",
     "lg_copy_bandwidth",
 ]
+
+
+class LGLaunch(ctypes.Structure):
+    _fields_ = [("group", ctypes.c_int32), ("blocks", ctypes.c_int32), ("link", ctypes.c_int32)]
+
+
+def _launch_get(fn, h):
+    out = LGLaunch()
+    _check(fn(h, ctypes.byref(out)))
+    return {"group": out.group, "blocks": out.blocks, "link": out.link}
+
+
+def _launch_set(fn, h, group, blocks, link):
+    _check(fn(h, ctypes.byref(LGLaunch(-1 if group is None else group, -1 if blocks is None else blocks,
+                                       -1 if link is None else link))))
 
 
 class LGParams(ctypes.Structure):
@@ -68,6 +84,8 @@ def _load():
     lib.lg_batch_set_graph.argtypes = [vp, ctypes.c_int]
     lib.lg_batch_chain_info.argtypes = [vp, ctypes.c_int32, P(ctypes.c_int64), P(ctypes.c_uint64)]
     lib.lg_batch_destroy.argtypes = [vp]
+    for fn in (lib.lg_chain_get_launch, lib.lg_chain_set_launch, lib.lg_batch_get_launch, lib.lg_batch_set_launch):
+        fn.argtypes = [vp, vp]
     lib.lg_profile_num_kernels.argtypes = []
     lib.lg_profile_kernel_name.argtypes = [ctypes.c_int]
     lib.lg_profile_kernel_name.restype = ctypes.c_char_p
@@ -150,6 +168,13 @@ class Chain:
         _check(lib.lg_chain_profile(self.h, n, _ptr(ms)))
         return dict(zip(kernel_names(), ms))
 
+    def launch(self):
+        """current launch settings (include/loopgpu.h lg_launch)"""
+        return _launch_get(lib.lg_chain_get_launch, self.h)
+
+    def set_launch(self, group=None, blocks=None, link=None):
+        _launch_set(lib.lg_chain_set_launch, self.h, group, blocks, link)
+
     def info(self):
         ns, nb, no, sw = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int64(), ctypes.c_uint64()
         _check(lib.lg_chain_info(self.h, ctypes.byref(ns), ctypes.byref(nb), ctypes.byref(no), ctypes.byref(sw)))
@@ -199,6 +224,12 @@ class Batch:
 
     def tune(self, sweeps_per_trial=8):
         _check(lib.lg_batch_tune(self.h, sweeps_per_trial))
+
+    def launch(self):
+        return _launch_get(lib.lg_batch_get_launch, self.h)
+
+    def set_launch(self, group=None, blocks=None, link=None):
+        _launch_set(lib.lg_batch_set_launch, self.h, group, blocks, link)
 
     def set_graph(self, enable):
         _check(lib.lg_batch_set_graph(self.h, 1 if enable else 0))

@@ -122,6 +122,25 @@ int lg_batch_set_graph(lg_batch* h, int enable);
 int lg_batch_chain_info(lg_batch* h, int32_t chain, int64_t* n_ops, uint64_t* sweep);
 int lg_batch_destroy(lg_batch* h);
 
+/* ---- launch settings ----------------------------------------------------------
+ * group: lanes per bond in the diagonal update (4, 8, 16, 32 or 64); blocks:
+ * grid of the per-operator and per-node kernels; link: linking kernel (0 one
+ * thread per operator, 512 or 1536 one wave per site staging that many
+ * events in LDS).  By default they follow the expected operator counts, and
+ * lg_batch_tune times the candidates.  set: fields < 0 keep their value; the
+ * settings then stay as given (no automatic choice) until the next tune.
+ * Results do not depend on them.  No counterpart in the reference (launch
+ * configuration of the device kernels). */
+typedef struct lg_launch {
+  int32_t group;
+  int32_t blocks;
+  int32_t link;
+} lg_launch;
+int lg_chain_get_launch(lg_chain* h, lg_launch* out);
+int lg_chain_set_launch(lg_chain* h, const lg_launch* l);
+int lg_batch_get_launch(lg_batch* h, lg_launch* out);
+int lg_batch_set_launch(lg_batch* h, const lg_launch* l);
+
 /* ---- measurement ------------------------------------------------------------
  * n sweeps with HIP events recorded between the kernel launches on the
  * handle's stream; avg_ms[i] = mean duration per sweep of kernel i, in the

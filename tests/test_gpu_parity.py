@@ -89,6 +89,48 @@ def test_sweeps_match_oracle_bit_exact(loopgpu, cfg):
     assert np.all(rg[:, 19] == np.arange(n))
 
 
+LAUNCH_TRAJ = [
+    # kind, L, beta, jxy, jz, sweeps: a few events per site, tens, and more
+    # than the 512-event staging (the linking kernel's global-memory path)
+    (O.SQUARE, 8, 6.0, 1.0, 0.8, 20),
+    (O.CHAIN, 32, 60.0, 1.0, 1.0, 20),
+    (O.CHAIN, 8, 600.0, 1.0, 1.0, 12),
+]
+LAUNCHES = [dict(link=0), dict(link=512), dict(link=1536), dict(link=512, group=4, blocks=7),
+            dict(link=0, group=16, blocks=64), dict(link=1536, group=32)]
+
+
+@pytest.mark.parametrize("cfg", LAUNCH_TRAJ, ids=lambda c: "k%d_L%d_b%g" % c[:3])
+def test_launch_settings_leave_trajectories_bit_exact(loopgpu, cfg):
+    """every linking kernel, lane-group size and grid gives the oracle's trajectory"""
+    kind, L, beta, jxy, jz, n = cfg
+    c = O.PhiloxChain(kind, L, beta, jxy, jz, seed=31, chain_id=2)
+    rc = c.sweep(n)
+    cs, cbo, ct, cty = c.state()
+    for lc in LAUNCHES:
+        print("launch", cfg, lc, flush=True)
+        g = loopgpu.Chain(loopgpu.params(kind, L, beta, jxy, jz, seed=31, chain_id=2))
+        g.set_launch(**lc)
+        got = g.launch()
+        for k, v in lc.items():
+            assert got[k] == v
+        rg = g.sweep(n)
+        _cmp_records(rg[:, :18], rc, "%s %s" % (cfg, lc))
+        gs, gbo, gt, gty = g.get_config()
+        np.testing.assert_array_equal(gbo, cbo)
+        np.testing.assert_array_equal(gt, ct)
+        np.testing.assert_array_equal(gty, cty)
+        np.testing.assert_array_equal(gs, cs)
+
+
+def test_launch_setting_errors(loopgpu):
+    g = loopgpu.Chain(loopgpu.params(O.CHAIN, 8, 1.0))
+    for bad in (dict(group=3), dict(link=100), dict(blocks=0)):
+        with pytest.raises(loopgpu.LoopGPUError) as e:
+            g.set_launch(**bad)
+        assert e.value.code == -1
+
+
 def test_batch_equals_single_chains(loopgpu):
     plist = [
         loopgpu.params(O.CHAIN, 8, 2.0, 1.0, 1.0, seed=11, chain_id=0),
