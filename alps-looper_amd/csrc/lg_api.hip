@@ -192,6 +192,76 @@ int Batch::init(const lg_params* p, int n) {
   return reset_state();
 }
 
+// Replica-exchange batch: n replicas of model p at inverse temperatures
+// betas[t] (replica t keyed by chain_id p.chain_id + t).  Every replica gets
+// the candidate slots of the largest beta, since temperatures move between
+// chains; the operator capacity is the sum over the temperatures, which the
+// exchange permutes but does not change.
+int Batch::init_exchange(const lg_params& p, const double* betas, int n, uint32_t seed) {
+  if (n < 2) return set_error(LG_ERR_ARG, "exchange needs at least two temperatures");
+  if (!betas) return set_error(LG_ERR_ARG, "null pointer");
+  std::vector<lg_params> ps(n, p);
+  for (int t = 0; t < n; ++t) {
+    ps[t].beta = betas[t];
+    ps[t].chain_id = p.chain_id + (uint32_t)t;
+    int rc = check_params(ps[t]);
+    if (rc) return rc;
+  }
+  int rc = check_device();
+  if (rc) return rc;
+  std::vector<ChainDev> ds(n);
+  int cand_cap = 0;
+  for (int t = 0; t < n; ++t) {
+    ds[t] = make_chain_dev(ps[t]);
+    cand_cap = std::max(cand_cap, ds[t].cand_cap);
+  }
+  Capacity cap;
+  cap.chains = n;
+  for (int t = 0; t < n; ++t) {
+    ds[t].cand_cap = cand_cap;
+    Footprint f = footprint(ps[t]);
+    cap.sites += f.sites;
+    cap.bonds += f.bonds;
+    cap.ops += f.ops;
+    cap.cand += (long long)ds[t].Nb * cand_cap;
+  }
+  if ((rc = reserve(cap, false))) return rc;
+  params = ps;
+  if ((rc = set_layout(ds, std::vector<uint64_t>(n, 0)))) return rc;
+  if ((rc = reset_state())) return rc;
+  if ((rc = alloc(xslot, n)) || (rc = alloc(xbeta, n)) || (rc = alloc(xscale, n)) ||
+      (rc = alloc(xlnr, n - 1)) || (rc = alloc(xacc, n - 1)) || (rc = alloc(xstep, 1)))
+    return rc;
+  std::vector<int> slot(n);
+  std::vector<double> sc(n), lnr(n - 1);
+  for (int t = 0; t < n; ++t) { slot[t] = t; sc[t] = ds[t].scale; }
+  for (int t = 0; t + 1 < n; ++t) lnr[t] = det_log(betas[t + 1] / betas[t]);
+  LG_HIP(hipMemcpy(xslot, slot.data(), n * sizeof(int), hipMemcpyHostToDevice));
+  LG_HIP(hipMemcpy(xbeta, betas, n * sizeof(double), hipMemcpyHostToDevice));
+  LG_HIP(hipMemcpy(xscale, sc.data(), n * sizeof(double), hipMemcpyHostToDevice));
+  LG_HIP(hipMemcpy(xlnr, lnr.data(), (n - 1) * sizeof(double), hipMemcpyHostToDevice));
+  LG_HIP(hipMemset(xacc, 0, (n - 1) * sizeof(unsigned long long)));
+  LG_HIP(hipMemset(xstep, 0, sizeof(uint32_t)));
+  nslots = n;
+  xkey0 = seed;
+  xkey1 = 0xffffffffu;
+  invalidate_graphs();
+  return LG_OK;
+}
+
+int Batch::exchange_state(int32_t* chain_at, uint64_t* accepted, uint64_t* steps) {
+  if (nslots < 2) return set_error(LG_ERR_STATE, "not an exchange batch");
+  LG_HIP(hipStreamSynchronize(stream));
+  if (chain_at) LG_HIP(hipMemcpy(chain_at, xslot, nslots * sizeof(int), hipMemcpyDeviceToHost));
+  if (accepted) LG_HIP(hipMemcpy(accepted, xacc, (nslots - 1) * sizeof(uint64_t), hipMemcpyDeviceToHost));
+  if (steps) {
+    uint32_t s32 = 0;
+    LG_HIP(hipMemcpy(&s32, xstep, sizeof(uint32_t), hipMemcpyDeviceToHost));
+    *steps = s32;
+  }
+  return LG_OK;
+}
+
 // Server batch: capacity from a budget, no chains yet.  Site/bond arrays get
 // room for the previous layout plus newly appended chains (2x the budget).
 int Batch::init_capacity(const Capacity& cap) {
@@ -454,6 +524,8 @@ DevBatch Batch::dev(int from) const {
   B.scan_part = scan_part; B.col_part = col_part; B.totals = totals; B.flags = flags;
   B.sweep = d_sweep; B.rec_acc = rec_acc; B.rec_ring = rec_ring;
   B.ring_pos = ring_pos; B.end_count = end_count; B.ring_size = ring_size;
+  B.nslots = nslots; B.xslot = xslot; B.xbeta = xbeta; B.xscale = xscale; B.xlnr = xlnr;
+  B.xacc = xacc; B.xstep = xstep; B.xkey0 = xkey0; B.xkey1 = xkey1;
   B.op_cap = op_cap; B.node_cap = node_cap;
   return B;
 }
@@ -971,6 +1043,24 @@ int lg_batch_get_launch(lg_batch* h, lg_launch* out) {
 int lg_batch_set_launch(lg_batch* h, const lg_launch* l) {
   if (!h || !l) return set_error(LG_ERR_ARG, "null pointer");
   return h->b.set_launch(*l);
+}
+
+int lg_exchange_create(const lg_params* p, const double* betas, int32_t n, uint32_t seed, lg_batch** out) {
+  if (!p || !out) return set_error(LG_ERR_ARG, "null pointer");
+  *out = nullptr;
+  lg_batch* h = new lg_batch();
+  int rc = h->b.init_exchange(*p, betas, n, seed);
+  if (rc) {
+    delete h;
+    return rc;
+  }
+  *out = h;
+  return LG_OK;
+}
+
+int lg_exchange_state(lg_batch* h, int32_t* chain_at, uint64_t* accepted, uint64_t* steps) {
+  if (!h) return set_error(LG_ERR_ARG, "null handle");
+  return h->b.exchange_state(chain_at, accepted, steps);
 }
 
 int lg_batch_tune(lg_batch* h, uint32_t sweeps_per_trial) {

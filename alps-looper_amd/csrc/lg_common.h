@@ -32,7 +32,7 @@ constexpr uint64_t TGRID = 1ull << 52;
 constexpr double TGRID_INV = 1.0 / 4503599627370496.0;
 constexpr double EMPTY_TIME = 2.0;  // ev_time of the single node of an event-free site
 
-enum : uint32_t { STAGE_DIAG = 1, STAGE_REGRAPH = 2, STAGE_FLIP = 3 };
+enum : uint32_t { STAGE_DIAG = 1, STAGE_REGRAPH = 2, STAGE_FLIP = 3, STAGE_EXCHANGE = 4 };
 
 // ev_flags bits
 // EV_FIRST: lowest segment of its site (wraps through tau = 1); EV_GAUGE: the
@@ -108,15 +108,23 @@ LG_HD double det_log(double x) {
   return (double)e * 0.69314718055994530942 + logm;
 }
 
+// log u for u uniform on (0,1] from the first 53 bits of a Philox output
+LG_HD double log_uniform(const U4& r) {
+#ifdef __HIPCC__
+#pragma clang fp contract(off)
+#endif
+  uint64_t u53 = ((uint64_t)r.x << 21) | (uint64_t)(r.y >> 11);
+  double u = (double)(u53 + 1) * (1.0 / 9007199254740992.0);
+  return det_log(u);
+}
+
 // Exponential gap on the 2^-52 grid for rate lambda (scale = 2^52/lambda),
 // from the first 53 bits of a Philox output; saturates at 2^52.
 LG_HD uint64_t exp_gap(const U4& r, double scale) {
 #ifdef __HIPCC__
 #pragma clang fp contract(off)
 #endif
-  uint64_t u53 = ((uint64_t)r.x << 21) | (uint64_t)(r.y >> 11);
-  double u = (double)(u53 + 1) * (1.0 / 9007199254740992.0);
-  double g = -det_log(u) * scale;
+  double g = -log_uniform(r) * scale;
   if (!(g < 4503599627370496.0)) return TGRID;
   return (uint64_t)g;
 }

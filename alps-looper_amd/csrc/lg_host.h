@@ -47,6 +47,8 @@ class Batch {
   int init(const lg_params* p, int n);
   int init_for_config(const lg_params* p, long long nops);
   int init_capacity(const Capacity& cap);
+  int init_exchange(const lg_params& p, const double* betas, int n, uint32_t seed);
+  int exchange_state(int32_t* chain_at, uint64_t* accepted, uint64_t* steps);
   int relayout(const std::vector<int>& keep, const std::vector<lg_params>& add);
   int run(uint32_t n, lg_sweep_record* out);
   int tune(uint32_t sweeps_per_trial);
@@ -129,6 +131,15 @@ class Batch {
   int* ring_pos = nullptr;
   int* end_count = nullptr;
   int ring_size = 0;
+  // replica exchange (nslots 0: off)
+  int nslots = 0;
+  int* xslot = nullptr;
+  double* xbeta = nullptr;
+  double* xscale = nullptr;
+  double* xlnr = nullptr;
+  unsigned long long* xacc = nullptr;
+  uint32_t* xstep = nullptr;
+  uint32_t xkey0 = 0, xkey1 = 0;
 };
 
 }  // namespace lg

@@ -63,6 +63,15 @@ struct DevBatch {
   int* ring_pos;
   int* end_count;              // blocks of end_sweep finished (reset by the last one)
   int ring_size;
+  // replica exchange between neighbouring temperature slots (nslots 0: off)
+  int nslots;
+  int* xslot;                  // [nslots] chain holding each temperature
+  const double* xbeta;         // [nslots] inverse temperatures
+  const double* xscale;        // [nslots] candidate scale 2^52 / (beta W) at each temperature
+  const double* xlnr;          // [nslots-1] log(beta[t+1] / beta[t])
+  unsigned long long* xacc;    // [nslots-1] accepted swaps per pair
+  uint32_t* xstep;             // exchange steps done
+  uint32_t xkey0, xkey1;
   int op_cap, node_cap;
 };
 

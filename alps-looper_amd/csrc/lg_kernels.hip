@@ -1090,6 +1090,41 @@ __global__ void __launch_bounds__(END_THREADS) k_end_sweep(DevBatch B) {
 }
 
 // ---------------------------------------------------------------------------
+// replica exchange ("loop; path integral; exchange", parapack's exchange
+// worker over loop_worker's hooks set_beta / weight_parameter / log_weight,
+// path_integral.C:97-109): after a sweep, the chains holding neighbouring
+// temperatures t, t+1 (t even on even steps, odd on odd steps) swap
+// temperatures with probability min(1, W(a, b_t+1) W(b, b_t) /
+// (W(a, b_t) W(b, b_t+1))), W(c, beta) = beta^(operators of c), i.e. when
+// log u < (n_a - n_b) log(beta_t+1 / beta_t).  u is Philox(step, 4, t, 0) on
+// the exchange key; log is det_log, so the host reproduces every decision.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_exchange(DevBatch B) {
+  if (overflowed(B)) return;
+  const uint32_t step = *B.xstep;
+  for (int t = (int)(step & 1) + 2 * (int)threadIdx.x; t + 1 < B.nslots; t += 2 * (int)blockDim.x) {
+    const int a = B.xslot[t], b = B.xslot[t + 1];
+    const ChainDev& Pa = B.chains[a];
+    const ChainDev& Pb = B.chains[b];
+    const int na = B.bo_new[Pa.bond_base + Pa.Nb] - B.bo_new[Pa.bond_base];
+    const int nb = B.bo_new[Pb.bond_base + Pb.Nb] - B.bo_new[Pb.bond_base];
+    const double x = (double)(na - nb) * B.xlnr[t];
+    const bool accept = (x >= 0.0) || (log_uniform(philox(step, STAGE_EXCHANGE, (uint32_t)t, 0u, B.xkey0, B.xkey1)) < x);
+    if (accept) {
+      B.xslot[t] = b;
+      B.xslot[t + 1] = a;
+      B.chains[a].beta = B.xbeta[t + 1];
+      B.chains[a].scale = B.xscale[t + 1];
+      B.chains[b].beta = B.xbeta[t];
+      B.chains[b].scale = B.xscale[t];
+      B.xacc[t] += 1;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) *B.xstep = step + 1;
+}
+
+// ---------------------------------------------------------------------------
 // canonical labels of the leg elements of one chain (debug / parity export):
 // element s -> boundary of site s, N + 4k + {0,1,2,3} -> legs l0,l1,u0,u1 of
 // operator k (chain-local bond-major order); label = min element index of
@@ -1204,6 +1239,7 @@ void launch_cluster_stage(const DevBatch& B, const LaunchCfg& L, hipStream_t st,
   mark(ev, 5, st);
   hipLaunchKernelGGL(k_end_sweep, dim3(B.nchains), dim3(END_THREADS), 0, st, B);
   mark(ev, 6, st);
+  if (B.nslots > 1) hipLaunchKernelGGL(k_exchange, dim3(1), dim3(256), 0, st, B);
 }
 
 void launch_site_scan(const DevBatch& B, hipStream_t st) {

@@ -29,6 +29,7 @@ EXPORTED = [
     "lg_batch_create", "lg_batch_run", "lg_batch_tune", "lg_batch_set_graph",
     "lg_batch_chain_info", "lg_batch_destroy",
     "lg_chain_get_launch", "lg_chain_set_launch", "lg_batch_get_launch", "lg_batch_set_launch",
+    "lg_exchange_create", "lg_exchange_state",
     "lg_profile_num_kernels", "lg_profile_kernel_name", "lg_chain_profile", "lg_batch_profile",
     "lg_copy_bandwidth",
 ]
@@ -86,6 +87,8 @@ def _load():
     lib.lg_batch_destroy.argtypes = [vp]
     for fn in (lib.lg_chain_get_launch, lib.lg_chain_set_launch, lib.lg_batch_get_launch, lib.lg_batch_set_launch):
         fn.argtypes = [vp, vp]
+    lib.lg_exchange_create.argtypes = [P(LGParams), vp, ctypes.c_int32, ctypes.c_uint32, P(vp)]
+    lib.lg_exchange_state.argtypes = [vp, vp, vp, vp]
     lib.lg_profile_num_kernels.argtypes = []
     lib.lg_profile_kernel_name.argtypes = [ctypes.c_int]
     lib.lg_profile_kernel_name.restype = ctypes.c_char_p
@@ -256,6 +259,25 @@ class Batch:
         except Exception:
             pass
 
+
+
+class Exchange(Batch):
+    """Replica exchange over inverse temperatures `betas` (include/loopgpu.h
+    lg_exchange_create): records of run() are per chain; their `beta` field
+    is the temperature the chain had in that sweep."""
+
+    def __init__(self, p, betas, seed=0):
+        self.n = len(betas)
+        self.betas = np.asarray(betas, dtype=np.float64)
+        self.h = ctypes.c_void_p()
+        _check(lib.lg_exchange_create(ctypes.byref(p), _ptr(self.betas), self.n, seed, ctypes.byref(self.h)))
+
+    def state(self):
+        at = np.zeros(self.n, np.int32)
+        acc = np.zeros(self.n - 1, np.uint64)
+        steps = ctypes.c_uint64()
+        _check(lib.lg_exchange_state(self.h, _ptr(at), _ptr(acc), ctypes.byref(steps)))
+        return at, acc, steps.value
 
 def cluster_config(p, spins, bo, times, types, sweep):
     """Cluster stage on a given configuration -> (labels, spins_out, types_out, n_clusters)."""

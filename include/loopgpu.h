@@ -122,6 +122,23 @@ int lg_batch_set_graph(lg_batch* h, int enable);
 int lg_batch_chain_info(lg_batch* h, int32_t chain, int64_t* n_ops, uint64_t* sweep);
 int lg_batch_destroy(lg_batch* h);
 
+/* ---- replica exchange ------------------------------------------------------
+ * "loop; path integral; exchange" (path_integral.C:874-875: parapack's
+ * exchange worker over loop_worker's set_beta / weight_parameter /
+ * log_weight, path_integral.C:97-109).  n replicas of model p at inverse
+ * temperatures betas[0..n) (neighbours in this order exchange; replica t is
+ * chain t with chain_id p->chain_id + t).  After every sweep the chains at
+ * temperatures t and t+1 (t even on even steps, odd on odd steps) swap
+ * temperatures with probability min(1, (beta[t+1]/beta[t])^(n_a - n_b)),
+ * n = operator count (weight_parameter), decided on the device from Philox
+ * (key seed, 0xffffffff; counter step, 4, t, 0).  The handle is a batch:
+ * lg_batch_run returns each chain's records, whose `beta` field is the
+ * temperature the chain had during that sweep. */
+int lg_exchange_create(const lg_params* p, const double* betas, int32_t n, uint32_t seed, lg_batch** out);
+/* chain_at[n]: chain holding each temperature; accepted[n-1]: swaps accepted
+ * between temperatures t and t+1; steps: exchange steps done (any may be NULL) */
+int lg_exchange_state(lg_batch* h, int32_t* chain_at, uint64_t* accepted, uint64_t* steps);
+
 /* ---- launch settings ----------------------------------------------------------
  * group: lanes per bond in the diagonal update (4, 8, 16, 32 or 64); blocks:
  * grid of the per-operator and per-node kernels; link: linking kernel (0 one

@@ -1132,6 +1132,13 @@ void or_philox_sweep(void* h, int n, double* rec) {
   PhiloxChain* c = (PhiloxChain*)h;
   for (int i = 0; i < n; ++i) c->sweep_once(rec + (size_t)i * R_COUNT);
 }
+// new inverse temperature for the following sweeps (replica exchange): the
+// candidate scale of a chain created at that beta
+void or_philox_set_beta(void* h, double beta) {
+  PhiloxChain* c = (PhiloxChain*)h;
+  c->beta = beta;
+  c->scale = (c->W > 0) ? std::ldexp(1.0, 52) / (beta * c->W) : 0;
+}
 int or_philox_num_ops(void* h) { return (int)((PhiloxChain*)h)->otime.size(); }
 int or_philox_num_nodes(void* h) { return ((PhiloxChain*)h)->last_nodes; }
 // state after the last sweep: spins[N], bond offsets[Nb+1], times[nops], types[nops]

@@ -44,6 +44,7 @@ lib.or_segment_of_elements.restype = I
 lib.or_philox_create.restype = vp
 lib.or_philox_create.argtypes = [I, I, D, D, D, D, U, U]
 lib.or_philox_sweep.argtypes = [vp, I, vp]
+lib.or_philox_set_beta.argtypes = [vp, D]
 lib.or_philox_num_ops.argtypes = [vp]
 lib.or_philox_num_nodes.argtypes = [vp]
 lib.or_philox_state.argtypes = [vp, vp, vp, vp, vp]
@@ -118,6 +119,9 @@ class PhiloxChain:
         rec = np.zeros((n, R))
         lib.or_philox_sweep(self.h, n, _p(rec))
         return rec
+
+    def set_beta(self, beta):
+        lib.or_philox_set_beta(self.h, ctypes.c_double(beta))
 
     def state(self):
         n = lib.or_philox_num_ops(self.h)
