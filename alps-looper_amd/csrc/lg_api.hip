@@ -839,6 +839,104 @@ int Batch::num_ops(int c, long long* n) {
   return LG_OK;
 }
 
+int Batch::chain_config(int c, std::vector<int>& spins, std::vector<int>& bo, std::vector<double>& times,
+                        std::vector<int>& types) {
+  if (c < 0 || c >= (int)chains.size()) return set_error(LG_ERR_ARG, "chain index out of range");
+  if (!swept_since_layout && (chains[c].old_site_base != chains[c].site_base ||
+                              chains[c].old_bond_base != chains[c].bond_base))
+    return set_error(LG_ERR_STATE, "chain moved since its last sweep");
+  LG_HIP(hipStreamSynchronize(stream));
+  const ChainDev& d = chains[c];
+  const State& s = st[cur];
+  std::vector<uint8_t> sp(d.N);
+  bo.assign(d.Nb + 1, 0);
+  LG_HIP(hipMemcpy(sp.data(), s.spins + d.site_base, d.N, hipMemcpyDeviceToHost));
+  LG_HIP(hipMemcpy(bo.data(), s.bo + d.bond_base, (d.Nb + 1) * sizeof(int), hipMemcpyDeviceToHost));
+  const int op0 = bo[0], nops = bo[d.Nb] - bo[0];
+  for (auto& x : bo) x -= op0;
+  spins.assign(sp.begin(), sp.end());
+  times.assign(nops, 0.0);
+  types.assign(nops, 0);
+  std::vector<int> oi(nops);
+  if (nops) {
+    LG_HIP(hipMemcpy(times.data(), s.ot + op0, nops * sizeof(double), hipMemcpyDeviceToHost));
+    LG_HIP(hipMemcpy(oi.data(), s.oi + op0, nops * sizeof(int), hipMemcpyDeviceToHost));
+  }
+  for (int k = 0; k < nops; ++k) types[k] = (oi[k] & 1) | (((oi[k] >> 1) & 3) << 2);
+  return LG_OK;
+}
+
+int Batch::load_configs(const std::vector<HostConfig>& cfg) {
+  if (cfg.size() != chains.size()) return set_error(LG_ERR_ARG, "one configuration per chain");
+  std::vector<uint8_t> sp(std::max(nsites, 1));
+  std::vector<int> bo(nbonds + 1), soff(nsites + 1);
+  std::vector<double> ot, evt;
+  std::vector<int> oi;
+  std::vector<uint8_t> segspin;
+  long long op_base = 0, node_base = 0;
+  for (size_t c = 0; c < chains.size(); ++c) {
+    const ChainDev& d = chains[c];
+    const HostConfig& h = cfg[c];
+    long long nops = 0;
+    if ((int)h.spins.size() != d.N || (int)h.bo.size() != d.Nb + 1)
+      return set_error(LG_ERR_ARG, "configuration does not match the chain's lattice");
+    int rc = validate_config(d, h.bo.data(), h.times.data(), h.types.data(), nops);
+    if (rc) return rc;
+    if (op_base + nops > op_cap) return set_error(LG_ERR_CAPACITY, "configurations exceed operator capacity");
+    std::vector<int> offd(nops);
+    for (int b = 0; b < d.Nb; ++b) {
+      bo[d.bond_base + b] = (int)(op_base + h.bo[b]);
+      for (int k = h.bo[b]; k < h.bo[b + 1]; ++k) {
+        offd[k] = h.types[k] & 1;
+        oi.push_back(((d.bond_base + b) << 3) | ((h.types[k] >> 2) << 1) | (h.types[k] & 1));
+        ot.push_back(h.times[k]);
+      }
+    }
+    std::vector<double> lt(h.times.begin(), h.times.begin() + nops);
+    std::vector<int> ls;
+    std::vector<double> le;
+    std::vector<uint8_t> lsg;
+    host_site_arrays(d, h.spins, h.bo, lt, offd, ls, le, lsg);
+    for (int i = 0; i < d.N; ++i) {
+      soff[d.site_base + i] = (int)(node_base + ls[i]);
+      sp[d.site_base + i] = (uint8_t)(h.spins[i] & 1);
+    }
+    evt.insert(evt.end(), le.begin(), le.end());
+    segspin.insert(segspin.end(), lsg.begin(), lsg.end());
+    op_base += nops;
+    node_base += ls[d.N];
+    if (node_base > node_cap) return set_error(LG_ERR_CAPACITY, "configurations exceed segment capacity");
+    host_sweep[c] = h.sweep;
+  }
+  bo[nbonds] = (int)op_base;
+  soff[nsites] = (int)node_base;
+  State& s = st[cur];
+  LG_HIP(hipMemcpy(s.spins, sp.data(), nsites, hipMemcpyHostToDevice));
+  LG_HIP(hipMemcpy(s.bo, bo.data(), (nbonds + 1) * sizeof(int), hipMemcpyHostToDevice));
+  if (op_base) {
+    LG_HIP(hipMemcpy(s.ot, ot.data(), op_base * sizeof(double), hipMemcpyHostToDevice));
+    LG_HIP(hipMemcpy(s.oi, oi.data(), op_base * sizeof(int), hipMemcpyHostToDevice));
+  }
+  LG_HIP(hipMemcpy(s.soff, soff.data(), (nsites + 1) * sizeof(int), hipMemcpyHostToDevice));
+  LG_HIP(hipMemcpy(s.evt, evt.data(), node_base * sizeof(double), hipMemcpyHostToDevice));
+  LG_HIP(hipMemcpy(s.segspin, segspin.data(), node_base, hipMemcpyHostToDevice));
+  std::vector<uint32_t> sw(chains.size());
+  for (size_t c = 0; c < chains.size(); ++c) sw[c] = (uint32_t)host_sweep[c];
+  LG_HIP(hipMemcpy(d_sweep, sw.data(), sw.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+  old_ext_sites = nsites;
+  old_ext_bonds = nbonds;
+  old_ext_nodes = node_base;
+  return LG_OK;
+}
+
+void Batch::adopt(Batch& o) {
+  release();
+  *this = o;  // member-wise: device pointers, host tables, settings
+  o.allocations.clear();
+  o.graph_exec[0] = o.graph_exec[1] = nullptr;
+  o.stream = nullptr;
+}
+
 // Cluster stage on a given configuration, written into the "new" buffers as
 // if the diagonal update had produced it.
 int Batch::cluster_only(const int32_t* spins_in, const int32_t* bond_offsets, const double* times,
@@ -1061,6 +1159,106 @@ int lg_exchange_create(const lg_params* p, const double* betas, int32_t n, uint3
 int lg_exchange_state(lg_batch* h, int32_t* chain_at, uint64_t* accepted, uint64_t* steps) {
   if (!h) return set_error(LG_ERR_ARG, "null handle");
   return h->b.exchange_state(chain_at, accepted, steps);
+}
+
+int lg_batch_add(lg_batch* h, const lg_params* p, int32_t n) {
+  if (!h || !p) return set_error(LG_ERR_ARG, "null pointer");
+  if (n <= 0) return set_error(LG_ERR_ARG, "need at least one chain");
+  lg::Batch& b = h->b;
+  if (b.is_exchange()) return set_error(LG_ERR_UNSUPPORTED, "chains cannot join an exchange batch");
+  for (int i = 0; i < n; ++i) {
+    int rc = lg::check_params(p[i]);
+    if (rc) return rc;
+  }
+  const size_t n0 = b.chains.size();
+  std::vector<lg::Batch::HostConfig> cfg(n0 + n);
+  for (size_t c = 0; c < n0; ++c) {
+    int rc = b.chain_config((int)c, cfg[c].spins, cfg[c].bo, cfg[c].times, cfg[c].types);
+    if (rc) return rc;
+    cfg[c].sweep = b.host_sweep[c];
+  }
+  std::vector<lg_params> all(b.params);
+  for (int i = 0; i < n; ++i) {
+    const lg::ChainDev d = lg::make_chain_dev(p[i]);
+    cfg[n0 + i].spins.assign(d.N, 0);  // all up, empty operator string (path_integral.C:225)
+    cfg[n0 + i].bo.assign(d.Nb + 1, 0);
+    all.push_back(p[i]);
+  }
+  lg::Batch nb;
+  nb.use_graph = b.use_graph;
+  int rc = nb.init(all.data(), (int)all.size());
+  if (rc) return rc;
+  if ((rc = nb.load_configs(cfg))) return rc;
+  b.adopt(nb);
+  return LG_OK;
+}
+
+static const char kChainMagic[8] = {'L', 'G', 'C', 'H', 'A', 'I', 'N', '1'};
+
+static size_t chain_blob_size(const lg::ChainDev& d, long long nops) {
+  return 8 + sizeof(lg_params) + 8 + (size_t)d.N * 4 + (size_t)(d.Nb + 1) * 4 + (size_t)nops * 12;
+}
+
+int lg_chain_state_size(lg_chain* h, size_t* bytes) {
+  if (!h || !bytes) return set_error(LG_ERR_ARG, "null pointer");
+  long long n = 0;
+  int rc = h->b.num_ops(0, &n);
+  if (rc) return rc;
+  *bytes = chain_blob_size(h->b.chains[0], n);
+  return LG_OK;
+}
+
+int lg_chain_save(lg_chain* h, void* buf, size_t len) {
+  if (!h || !buf) return set_error(LG_ERR_ARG, "null pointer");
+  std::vector<int> spins, bo, types;
+  std::vector<double> times;
+  int rc = h->b.chain_config(0, spins, bo, times, types);
+  if (rc) return rc;
+  const lg::ChainDev& d = h->b.chains[0];
+  const long long nops = (long long)times.size();
+  if (len < chain_blob_size(d, nops)) return set_error(LG_ERR_ARG, "checkpoint buffer too small");
+  char* o = (char*)buf;
+  std::memcpy(o, kChainMagic, 8); o += 8;
+  std::memcpy(o, &h->b.params[0], sizeof(lg_params)); o += sizeof(lg_params);
+  const uint64_t sw = h->b.host_sweep[0];
+  std::memcpy(o, &sw, 8); o += 8;
+  std::memcpy(o, spins.data(), (size_t)d.N * 4); o += (size_t)d.N * 4;
+  std::memcpy(o, bo.data(), (size_t)(d.Nb + 1) * 4); o += (size_t)(d.Nb + 1) * 4;
+  if (nops) {
+    std::memcpy(o, times.data(), (size_t)nops * 8); o += (size_t)nops * 8;
+    std::memcpy(o, types.data(), (size_t)nops * 4);
+  }
+  return LG_OK;
+}
+
+int lg_chain_load(lg_chain* h, const void* buf, size_t len) {
+  if (!h || !buf) return set_error(LG_ERR_ARG, "null pointer");
+  const lg::ChainDev& d = h->b.chains[0];
+  const size_t head = 8 + sizeof(lg_params) + 8 + (size_t)d.N * 4 + (size_t)(d.Nb + 1) * 4;
+  if (len < head) return set_error(LG_ERR_ARG, "checkpoint too short");
+  const char* in = (const char*)buf;
+  if (std::memcmp(in, kChainMagic, 8) != 0) return set_error(LG_ERR_ARG, "not a chain checkpoint");
+  lg_params q;
+  std::memcpy(&q, in + 8, sizeof(lg_params));
+  const lg_params& p = h->b.params[0];
+  if (q.lattice != p.lattice || q.L != p.L || q.beta != p.beta || q.jxy != p.jxy || q.jz != p.jz ||
+      q.force_scatter != p.force_scatter || q.seed != p.seed || q.chain_id != p.chain_id)
+    return set_error(LG_ERR_ARG, "checkpoint of a different chain (parameters, seed or chain id)");
+  const char* o = in + 8 + sizeof(lg_params);
+  uint64_t sw = 0;
+  std::memcpy(&sw, o, 8); o += 8;
+  std::vector<int32_t> spins(d.N), bo(d.Nb + 1);
+  std::memcpy(spins.data(), o, (size_t)d.N * 4); o += (size_t)d.N * 4;
+  std::memcpy(bo.data(), o, (size_t)(d.Nb + 1) * 4); o += (size_t)(d.Nb + 1) * 4;
+  const long long nops = bo[d.Nb];
+  if (nops < 0 || len < head + (size_t)nops * 12) return set_error(LG_ERR_ARG, "checkpoint truncated");
+  std::vector<double> times(nops);
+  std::vector<int32_t> types(nops);
+  if (nops) {
+    std::memcpy(times.data(), o, (size_t)nops * 8); o += (size_t)nops * 8;
+    std::memcpy(types.data(), o, (size_t)nops * 4);
+  }
+  return h->b.set_config(0, spins.data(), bo.data(), times.data(), types.data(), sw);
 }
 
 int lg_batch_tune(lg_batch* h, uint32_t sweeps_per_trial) {

@@ -60,6 +60,19 @@ class Batch {
                  const int32_t* types, uint64_t sweep);
   int get_config(int c, int32_t* spins, int32_t* bond_offsets, double* times, int32_t* types);
   int num_ops(int c, long long* n);
+  // configuration of chain c of any batch (bond offsets relative to the chain)
+  int chain_config(int c, std::vector<int>& spins, std::vector<int>& bo, std::vector<double>& times,
+                   std::vector<int>& types);
+  // whole-batch state from per-chain configurations (a freshly built batch)
+  struct HostConfig {
+    std::vector<int> spins, bo, types;
+    std::vector<double> times;
+    uint64_t sweep = 0;
+  };
+  int load_configs(const std::vector<HostConfig>& cfg);
+  // take over o's device resources and settings (o is left empty)
+  void adopt(Batch& o);
+  bool is_exchange() const { return nslots > 0; }
   int cluster_only(const int32_t* spins, const int32_t* bond_offsets, const double* times,
                    const int32_t* types, uint64_t sweep, int32_t* labels, int32_t* spins_out,
                    int32_t* types_out, int64_t* n_clusters);

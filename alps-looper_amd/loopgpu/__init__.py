@@ -30,6 +30,7 @@ EXPORTED = [
     "lg_batch_chain_info", "lg_batch_destroy",
     "lg_chain_get_launch", "lg_chain_set_launch", "lg_batch_get_launch", "lg_batch_set_launch",
     "lg_exchange_create", "lg_exchange_state",
+    "lg_chain_state_size", "lg_chain_save", "lg_chain_load", "lg_batch_add",
     "lg_profile_num_kernels", "lg_profile_kernel_name", "lg_chain_profile", "lg_batch_profile",
     "lg_copy_bandwidth",
 ]
@@ -88,6 +89,10 @@ def _load():
     for fn in (lib.lg_chain_get_launch, lib.lg_chain_set_launch, lib.lg_batch_get_launch, lib.lg_batch_set_launch):
         fn.argtypes = [vp, vp]
     lib.lg_exchange_create.argtypes = [P(LGParams), vp, ctypes.c_int32, ctypes.c_uint32, P(vp)]
+    lib.lg_chain_state_size.argtypes = [vp, P(ctypes.c_size_t)]
+    lib.lg_chain_save.argtypes = [vp, vp, ctypes.c_size_t]
+    lib.lg_chain_load.argtypes = [vp, vp, ctypes.c_size_t]
+    lib.lg_batch_add.argtypes = [vp, P(LGParams), ctypes.c_int32]
     lib.lg_exchange_state.argtypes = [vp, vp, vp, vp]
     lib.lg_profile_num_kernels.argtypes = []
     lib.lg_profile_kernel_name.argtypes = [ctypes.c_int]
@@ -171,6 +176,17 @@ class Chain:
         _check(lib.lg_chain_profile(self.h, n, _ptr(ms)))
         return dict(zip(kernel_names(), ms))
 
+    def save(self):
+        """checkpoint blob (include/loopgpu.h lg_chain_save)"""
+        n = ctypes.c_size_t()
+        _check(lib.lg_chain_state_size(self.h, ctypes.byref(n)))
+        buf = ctypes.create_string_buffer(n.value)
+        _check(lib.lg_chain_save(self.h, buf, n.value))
+        return buf.raw
+
+    def load(self, blob):
+        _check(lib.lg_chain_load(self.h, blob, len(blob)))
+
     def launch(self):
         """current launch settings (include/loopgpu.h lg_launch)"""
         return _launch_get(lib.lg_chain_get_launch, self.h)
@@ -227,6 +243,12 @@ class Batch:
 
     def tune(self, sweeps_per_trial=8):
         _check(lib.lg_batch_tune(self.h, sweeps_per_trial))
+
+    def add(self, plist):
+        """append chains (starting from all-up spins, no operators)"""
+        arr = (LGParams * len(plist))(*plist)
+        _check(lib.lg_batch_add(self.h, arr, len(plist)))
+        self.n += len(plist)
 
     def launch(self):
         return _launch_get(lib.lg_batch_get_launch, self.h)

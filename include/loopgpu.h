@@ -98,6 +98,13 @@ int lg_chain_get_config(lg_chain* h, int32_t* spins, int32_t* bond_offsets, doub
 int lg_chain_set_config(lg_chain* h, const int32_t* spins, const int32_t* bond_offsets,
                         const double* times, const int32_t* types, uint64_t sweep);
 int lg_chain_destroy(lg_chain* h);
+/* Checkpoint blob of a chain (the worker's save/load, path_integral.C:111-124:
+ * mcs, spins, operators; here also the parameters, whose seed and chain id
+ * with the sweep count are the Philox key and counter, so a loaded chain
+ * continues the same trajectory).  load rejects a blob of another chain. */
+int lg_chain_state_size(lg_chain* h, size_t* bytes);
+int lg_chain_save(lg_chain* h, void* buf, size_t len);
+int lg_chain_load(lg_chain* h, const void* buf, size_t len);
 
 /* Cluster stage only, on a given configuration (stages 7-15 of
  * path_integral.C without the diagonal update): links world lines, identifies
@@ -120,6 +127,10 @@ int lg_batch_tune(lg_batch* h, uint32_t sweeps_per_trial);
 /* use a captured HIP graph for the sweep (1) or plain launches (0) */
 int lg_batch_set_graph(lg_batch* h, int enable);
 int lg_batch_chain_info(lg_batch* h, int32_t chain, int64_t* n_ops, uint64_t* sweep);
+/* append n chains (all spins up, no operators, path_integral.C:225); the
+ * batch's chains keep their configurations and sweep counts.  Rebuilds the
+ * batch's device memory (launch settings return to the automatic choice). */
+int lg_batch_add(lg_batch* h, const lg_params* p, int32_t n);
 int lg_batch_destroy(lg_batch* h);
 
 /* ---- replica exchange ------------------------------------------------------

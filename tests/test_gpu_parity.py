@@ -171,3 +171,44 @@ def test_set_config_roundtrip_and_resume(loopgpu):
     rb = b.sweep(15)
     np.testing.assert_array_equal(ra[:, EXACT], rb[:, EXACT])
     np.testing.assert_allclose(ra[:, FLOAT], rb[:, FLOAT], rtol=RTOL, atol=1e-9)
+
+
+def test_chain_checkpoint_blob_continues_the_trajectory(loopgpu):
+    p = loopgpu.params(O.SQUARE, 8, 4.0, 1.0, 0.9, seed=21, chain_id=4)
+    a = loopgpu.Chain(p)
+    a.sweep(20)
+    blob = a.save()
+    ra = a.sweep(15)
+    b = loopgpu.Chain(p)
+    b.load(blob)
+    rb = b.sweep(15)
+    np.testing.assert_array_equal(ra[:, EXACT], rb[:, EXACT])
+    np.testing.assert_allclose(ra[:, FLOAT], rb[:, FLOAT], rtol=RTOL, atol=1e-9)
+    np.testing.assert_array_equal(ra[:, 19], rb[:, 19])
+    other = loopgpu.Chain(loopgpu.params(O.SQUARE, 8, 4.0, 1.0, 0.9, seed=22, chain_id=4))
+    for bad in (blob[:40], b"X" + blob[1:]):
+        with pytest.raises(loopgpu.LoopGPUError) as e:
+            b.load(bad)
+        assert e.value.code == -1
+    with pytest.raises(loopgpu.LoopGPUError) as e:
+        other.load(blob)
+    assert e.value.code == -1
+
+
+def test_batch_add_keeps_every_trajectory(loopgpu):
+    first = [loopgpu.params(O.CHAIN, 16, 3.0, 1.0, 1.0, seed=8, chain_id=0),
+             loopgpu.params(O.SQUARE, 6, 2.0, 1.0, 0.7, seed=8, chain_id=1)]
+    later = [loopgpu.params(O.SQUARE, 4, 5.0, 1.0, 1.0, seed=8, chain_id=2),
+             loopgpu.params(O.CHAIN, 8, 1.0, 1.0, 1.3, seed=8, chain_id=3)]
+    b = loopgpu.Batch(first)
+    r1 = b.run(10)
+    b.add(later)
+    r2 = b.run(20)
+    for i, p in enumerate(first):
+        rs = loopgpu.Chain(p).sweep(30)
+        np.testing.assert_array_equal(np.concatenate([r1[:, i, EXACT], r2[:, i, EXACT]]), rs[:, EXACT])
+        np.testing.assert_allclose(r2[:, i, FLOAT], rs[10:, FLOAT], rtol=RTOL, atol=1e-9)
+    for j, p in enumerate(later):
+        rs = loopgpu.Chain(p).sweep(20)
+        np.testing.assert_array_equal(r2[:, 2 + j, EXACT], rs[:, EXACT])
+        np.testing.assert_allclose(r2[:, 2 + j, FLOAT], rs[:, FLOAT], rtol=RTOL, atol=1e-9)
