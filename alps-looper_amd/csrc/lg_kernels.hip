@@ -511,20 +511,48 @@ __device__ __forceinline__ void rank_event(const DevBatch& B, const SiteLists& S
   }
 }
 
+// union-find on block-local segment indices in LDS (same hooking rule as
+// the global one: the larger root goes under the smaller)
+__device__ __forceinline__ int lds_find(int* P, int x) {
+  int p;
+  while ((p = __hip_atomic_load(P + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) != x) x = p;
+  return x;
+}
+__device__ __forceinline__ void lds_unite(int* P, int a, int b) {
+  a = lds_find(P, a);
+  b = lds_find(P, b);
+  while (a != b) {
+    if (a < b) { int t = a; a = b; b = t; }
+    const int old = atomicCAS(P + a, a, b);
+    if (old == a) return;
+    a = lds_find(P, old);
+    b = lds_find(P, b);
+  }
+}
+
+// One block step handles WAVES consecutive sites.  Operators of a bond whose
+// two sites are both in the step ("inner" operators: neighbours along the
+// fast lattice direction) are united here in LDS, and each segment's parent
+// is written as its local root; k_uf_union skips them (u1 is stored
+// complemented).  Roots stay cluster minima, so the partition and every
+// result are those of the plain union-find.
 template <int CAP, int WAVES>
 __global__ void __launch_bounds__(64 * WAVES) k_link(DevBatch B) {
   if (overflowed(B)) return;
-  __shared__ double lt_all[WAVES][CAP];           // staged times, list order
-  __shared__ uint16_t lpre_all[WAVES][CAP + 4];   // off-diagonal prefix counts, len+1 per list
+  __shared__ double lt_all[WAVES][CAP];           // staged times, list order; then local parents
+  __shared__ uint16_t lpre_all[WAVES][CAP + 4];   // off-diagonal prefix counts; then element ranks
   __shared__ uint16_t inv_all[WAVES][CAP];        // rank -> staged element
   __shared__ uint8_t lsp_all[WAVES][CAP];         // staged element -> spin of the segment below
   __shared__ uint8_t lfl_all[WAVES][CAP];         // staged element -> event flags
+  __shared__ int wn0[WAVES], wn1[WAVES], wtot[WAVES], wnbd[WAVES];
+  __shared__ int wbond[WAVES][4], woff[WAVES][4];
   const int w = threadIdx.x >> 6;
   double* lt = lt_all[w];
   uint16_t* lpre = lpre_all[w];
   uint16_t* inv = inv_all[w];
   uint8_t* lsp = lsp_all[w];
   uint8_t* lfl = lfl_all[w];
+  int* lpar = reinterpret_cast<int*>(&lt_all[0][0]);  // WAVES * CAP * 2 ints
   int slo, shi;
   xcd_share(B.nsites, slo, shi);
   for (int g = slo; g < shi; g += WAVES) {  // block-uniform steps
@@ -532,7 +560,10 @@ __global__ void __launch_bounds__(64 * WAVES) k_link(DevBatch B) {
     const bool have = gs < shi;
     SiteLists S;
     S.nbd = 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) { S.bond[j] = -1; S.off[j] = S.lo[j] = S.len[j] = S.side[j] = 0; }
     int tot = 0, s = 0, n0 = 0, n1 = 0;
+    int other[4] = {-1, -1, -1, -1};   // the bond's other site (global index)
     uint8_t spin0 = 0, gflag = 0;
     const ChainDev* Pp = nullptr;
     if (have) {
@@ -552,6 +583,7 @@ __global__ void __launch_bounds__(64 * WAVES) k_link(DevBatch B) {
         S.len[j] = B.bo_new[gb + 1] - S.lo[j];
         S.off[j] = tot;
         S.side[j] = (s0 == s) ? 0 : 1;
+        other[j] = P.site_base + ((s0 == s) ? s1 : s0);
         tot += S.len[j];
       }
       n0 = B.soff_new[gs];
@@ -569,16 +601,33 @@ __global__ void __launch_bounds__(64 * WAVES) k_link(DevBatch B) {
           if (lane_id() == 0) lpre[S.off[j] + j + S.len[j]] = (uint16_t)list_pre_global(B, S, j, S.len[j]);
         }
       }
+      if (lane_id() == 0) {
+        wn0[w] = n0;
+        wn1[w] = n1;
+        wtot[w] = tot;
+        wnbd[w] = S.nbd;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) { wbond[w][j] = S.bond[j]; woff[w][j] = S.off[j]; }
+      }
+    } else if (lane_id() == 0) {
+      wtot[w] = -1;
     }
     __syncthreads();
+    // local unions need every site of the step staged and in one chain
+    bool inner_ok = true;
+    const int c0 = B.site_chain[g];
+    for (int v = 0; v < WAVES; ++v)
+      if (g + v < shi && (wtot[v] > CAP || B.site_chain[g + v] != c0)) inner_ok = false;
     if (have && tot > 0) {
       const bool lds = tot <= CAP;
       for (int e = lane_id(); e < tot; e += 64) {
         // the element's list, selected with constant indices
-        int i = 0, offi = S.off[0], loi = S.lo[0], bi = S.bond[0], side = S.side[0];
+        int i = 0, offi = S.off[0], loi = S.lo[0], bi = S.bond[0], side = S.side[0], oth = other[0];
 #pragma unroll
         for (int j = 1; j < 4; ++j)
-          if (j < S.nbd && e >= S.off[j]) { i = j; offi = S.off[j]; loi = S.lo[j]; bi = S.bond[j]; side = S.side[j]; }
+          if (j < S.nbd && e >= S.off[j]) {
+            i = j; offi = S.off[j]; loi = S.lo[j]; bi = S.bond[j]; side = S.side[j]; oth = other[j];
+          }
         const int p = e - offi;
         const int k = loi + p;
         const double tau = lds ? lt[e] : B.ot_new[k];
@@ -590,7 +639,8 @@ __global__ void __launch_bounds__(64 * WAVES) k_link(DevBatch B) {
         const uint8_t sp = spin0 ^ (uint8_t)(par & 1);
         const int node = n0 + rank;
         const int up = (node + 1 == n1) ? n0 : node + 1;  // segment above the event
-        if (side) { B.l1[k] = node; B.u1[k] = up; } else { B.l0[k] = node; B.u0[k] = up; }
+        const bool inner = inner_ok && oth >= g && oth < g + WAVES && oth < shi;
+        if (side) { B.l1[k] = node; B.u1[k] = inner ? ~up : up; } else { B.l0[k] = node; B.u0[k] = up; }
         if (lds) {
           inv[rank] = (uint16_t)e;
           lsp[e] = sp;
@@ -605,15 +655,58 @@ __global__ void __launch_bounds__(64 * WAVES) k_link(DevBatch B) {
       }
     }
     __syncthreads();
+    // segment arrays in node order; element ranks for the local unions
     if (have && tot > 0 && tot <= CAP) {
       for (int r = lane_id(); r < tot; r += 64) {
         const int e = inv[r];
         B.evt_new[n0 + r] = lt[e];
         B.segspin_new[n0 + r] = lsp[e];
         B.evflags[n0 + r] = lfl[e];
-        B.parent[n0 + r] = n0 + r;
         if (B.nchains > 1) B.node_site[n0 + r] = gs;
+        lpre[e] = (uint16_t)r;
       }
+    }
+    __syncthreads();
+    // step's segments [N0, N1): local unions of the inner operators
+    const int last = min(g + WAVES, shi) - 1 - g;
+    const int N0 = wn0[0], N1 = wn1[last];
+    if (inner_ok) {
+      for (int x = threadIdx.x; x < N1 - N0; x += blockDim.x) lpar[x] = x;
+      __syncthreads();
+      if (have && tot > 0) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          if (j >= S.nbd || S.side[j] != 0) continue;
+          const int wt = other[j] - g;
+          if (other[j] < g || wt >= WAVES || other[j] >= shi) continue;
+          int jt = 0;
+          while (jt < 3 && wbond[wt][jt] != S.bond[j]) ++jt;
+          const uint16_t* rk_t = lpre_all[wt];
+          const int t0 = wn0[wt] - N0, t1 = wn1[wt] - N0, m0 = n0 - N0, m1 = n1 - N0;
+          for (int p = lane_id(); p < S.len[j]; p += 64) {
+            const int a0 = m0 + lpre[S.off[j] + p];
+            const int b0 = (a0 + 1 == m1) ? m0 : a0 + 1;
+            const int a1 = t0 + rk_t[woff[wt][jt] + p];
+            const int b1 = (a1 + 1 == t1) ? t0 : a1 + 1;
+            const int gr = (B.oi_new[S.lo[j] + p] >> 1) & 3;
+            if (gr & 2) {            // frozen: all four legs
+              lds_unite(lpar, a0, a1);
+              lds_unite(lpar, b0, b1);
+              lds_unite(lpar, a0, b0);
+            } else if (gr == 0) {    // horizontal cut
+              lds_unite(lpar, a0, a1);
+              lds_unite(lpar, b0, b1);
+            } else {                 // cross
+              lds_unite(lpar, a0, b1);
+              lds_unite(lpar, a1, b0);
+            }
+          }
+        }
+      }
+      __syncthreads();
+      for (int x = threadIdx.x; x < N1 - N0; x += blockDim.x) B.parent[N0 + x] = N0 + lds_find(lpar, x);
+    } else if (have && tot > 0 && tot <= CAP) {
+      for (int r = lane_id(); r < tot; r += 64) B.parent[n0 + r] = n0 + r;
     }
     __syncthreads();
   }
@@ -709,7 +802,9 @@ __global__ void __launch_bounds__(256) k_uf_union(DevBatch B) {
   xcd_share(nops, klo, khi);
   for (int k = klo + threadIdx.x; k < khi; k += blockDim.x) {
     const int info = B.oi_new[k];
-    const int l0 = B.l0[k], l1 = B.l1[k], u0 = B.u0[k], u1 = B.u1[k];
+    const int u1 = B.u1[k];
+    if (u1 < 0) continue;  // united in LDS by the linking kernel
+    const int l0 = B.l0[k], l1 = B.l1[k], u0 = B.u0[k];
     const int g = (info >> 1) & 3;
     if (g & 2) {            // frozen: all four legs (graph_impl.h:282-283)
       uf_unite(B.parent, l0, l1);
@@ -1137,7 +1232,8 @@ __device__ __forceinline__ int element_node(const DevBatch& B, const ChainDev& P
   if (leg == 0) return B.l0[kg];
   if (leg == 1) return B.l1[kg];
   if (leg == 2) return B.u0[kg];
-  return B.u1[kg];
+  const int u1 = B.u1[kg];
+  return u1 < 0 ? ~u1 : u1;  // complemented: united in LDS
 }
 
 __global__ void k_canon_min(DevBatch B, int chain, int nelem, int* canon) {
