@@ -68,8 +68,8 @@ int check_params(const lg_params& p) {
     return set_error(LG_ERR_ARG, "square lattice too large");
   if (!(p.beta > 0) || !std::isfinite(p.beta))
     return set_error(LG_ERR_ARG, "beta must be positive (negative temperature)");
-  if (!std::isfinite(p.jxy) || !std::isfinite(p.jz))
-    return set_error(LG_ERR_ARG, "couplings must be finite");
+  if (!std::isfinite(p.jxy) || !std::isfinite(p.jz) || !std::isfinite(p.h))
+    return set_error(LG_ERR_ARG, "couplings and field must be finite");
   return LG_OK;
 }
 
@@ -100,6 +100,7 @@ ChainDev make_chain_dev(const lg_params& p) {
   d.toff = (s01 > 0) ? (uint64_t)std::floor(std::ldexp(v[0] / s01, 32)) : (1ull << 32);
   d.key0 = p.seed;
   d.key1 = p.chain_id;
+  d.h = p.h;
   double lam = p.beta * W;
   d.cand_cap = (int)std::ceil(lam + 12.0 * std::sqrt(lam) + 16.0);
   return d;
@@ -335,7 +336,7 @@ int Batch::allocate() {
   if ((rc = alloc(acc, node_cap))) return rc;
   scan_nparts = (cap_bonds + 2047) / 2048 + (cap_sites + 2047) / 2048;
   if ((rc = alloc(scan_part, scan_nparts))) return rc;
-  if ((rc = alloc(col_part, (size_t)col_chunks(node_cap) * 16))) return rc;
+  if ((rc = alloc(col_part, (size_t)col_chunks(node_cap) * NUM_COLLECTOR_TERMS))) return rc;
   if ((rc = alloc(totals, 2))) return rc;
   if ((rc = alloc(flags, 1))) return rc;
   if ((rc = alloc(d_sweep, cap_chains))) return rc;
@@ -524,6 +525,8 @@ DevBatch Batch::dev(int from) const {
   B.scan_part = scan_part; B.col_part = col_part; B.totals = totals; B.flags = flags;
   B.sweep = d_sweep; B.rec_acc = rec_acc; B.rec_ring = rec_ring;
   B.ring_pos = ring_pos; B.end_count = end_count; B.ring_size = ring_size;
+  B.any_field = 0;
+  for (const auto& d : chains) if (d.h != 0.0) B.any_field = 1;
   B.nslots = nslots; B.xslot = xslot; B.xbeta = xbeta; B.xscale = xscale; B.xlnr = xlnr;
   B.xacc = xacc; B.xstep = xstep; B.xkey0 = xkey0; B.xkey1 = xkey1;
   B.op_cap = op_cap; B.node_cap = node_cap;

@@ -54,6 +54,7 @@ struct ChainDev {
   uint64_t toff;                     // off-diagonal regraph threshold
   uint32_t key0, key1;               // Philox key = (seed, chain id)
   int has_weight;
+  double h;                          // longitudinal field (site term -h S^z)
 };
 
 // Philox4x32-10 (Salmon et al., SC'11): 10 rounds, multipliers D2511F53 /
@@ -106,6 +107,53 @@ LG_HD double det_log(double x) {
   p = p * z + 1.0 / 3.0;
   double logm = 2.0 * s + 2.0 * s * (z * p);
   return (double)e * 0.69314718055994530942 + logm;
+}
+
+// exp(x) from IEEE +,-,*,/ and exact scalings only, identical on host and
+// device: x = k ln2 + r with the Cody-Waite split of ln2 (high part 32
+// significant bits, so k*ln2_hi is exact), e^r by its Taylor series to r^14
+// (|r| <= ln2/2), times 2^k.
+LG_HD double det_exp(double x) {
+#ifdef __HIPCC__
+#pragma clang fp contract(off)
+#endif
+  if (!(x > -745.0)) return 0.0;
+  if (x > 709.0) x = 709.0;
+  const double k = floor(x * 1.4426950408889634 + 0.5);
+  const double r = (x - k * 0.6931471803691238) - k * 1.9082149292705877e-10;
+  double p = 1.0 / 87178291200.0;  // 1/14!
+  p = p * r + 1.0 / 6227020800.0;
+  p = p * r + 1.0 / 479001600.0;
+  p = p * r + 1.0 / 39916800.0;
+  p = p * r + 1.0 / 3628800.0;
+  p = p * r + 1.0 / 362880.0;
+  p = p * r + 1.0 / 40320.0;
+  p = p * r + 1.0 / 5040.0;
+  p = p * r + 1.0 / 720.0;
+  p = p * r + 1.0 / 120.0;
+  p = p * r + 1.0 / 24.0;
+  p = p * r + 1.0 / 6.0;
+  p = p * r + 0.5;
+  p = p * r + 1.0;
+  p = p * r + 1.0;
+  return ldexp(p, (int)k);
+}
+
+// tanh x = sign(x) (1 - e^-2|x|) / (1 + e^-2|x|), from det_exp
+LG_HD double det_tanh(double x) {
+#ifdef __HIPCC__
+#pragma clang fp contract(off)
+#endif
+  const double a = x < 0 ? -x : x;
+  const double e = det_exp(-2.0 * a);
+  const double t = (1.0 - e) / (1.0 + e);
+  return x < 0 ? -t : t;
+}
+
+// uniform on (0,1] from the first 53 bits of a Philox output
+LG_HD double uniform53(const U4& r) {
+  uint64_t u53 = ((uint64_t)r.x << 21) | (uint64_t)(r.y >> 11);
+  return (double)(u53 + 1) * (1.0 / 9007199254740992.0);
 }
 
 // log u for u uniform on (0,1] from the first 53 bits of a Philox output
@@ -183,7 +231,9 @@ enum {
   R_NOP = 0, R_NC, R_ENERGY,
   R_UMAG0, R_USIZE2, R_UMAG2, R_USIZE4, R_UMAG4, R_USIZE, R_UMAG,
   R_SMAG0, R_SSIZE2, R_SMAG2, R_SSIZE4, R_SMAG4, R_SSIZE, R_SMAG,
-  R_W2, R_BETA, R_SWEEP, R_COUNT
+  R_W2, R_BETA, R_SWEEP,
+  R_UMAGA, R_SMAGA, R_EFIELD,  // sum_c umag_c, sum_c smag_c (normal estimator), Zeeman energy
+  R_COUNT
 };
 
 // Per-cluster accumulator, one 64-byte record per root node.

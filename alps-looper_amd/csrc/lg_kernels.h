@@ -64,6 +64,7 @@ struct DevBatch {
   int* end_count;              // blocks of end_sweep finished (reset by the last one)
   int ring_size;
   // replica exchange between neighbouring temperature slots (nslots 0: off)
+  int any_field;               // some chain has a longitudinal field
   int nslots;
   int* xslot;                  // [nslots] chain holding each temperature
   const double* xbeta;         // [nslots] inverse temperatures
@@ -84,6 +85,7 @@ struct LaunchCfg {
 };
 
 constexpr int NUM_SWEEP_KERNELS = 11;
+constexpr int NUM_COLLECTOR_TERMS = 19;   // per-cluster terms summed into the sweep record
 extern const char* const kKernelNames[NUM_SWEEP_KERNELS];
 
 void launch_sweep(const DevBatch& B, const LaunchCfg& L, hipStream_t st, hipEvent_t* ev = nullptr);

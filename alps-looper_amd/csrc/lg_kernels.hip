@@ -837,7 +837,9 @@ __global__ void __launch_bounds__(256) k_uf_compress(DevBatch B) {
       B.acc[i] = ClusterAcc{{0, 0, 0, 0}, {0, 0, 0, 0}, 0, 0};
       const int c = node_chain(B, i);
       const ChainDev& P = B.chains[c];
-      B.fbit[i] = (uint8_t)(philox(B.sweep[c], STAGE_FLIP, (uint32_t)(i - B.chain_node0[c]), 0u, P.key0, P.key1).x & 1u);
+      // with a field the flip waits for the cluster's magnetization (k_field_flip)
+      B.fbit[i] = P.h != 0.0 ? (uint8_t)0
+                             : (uint8_t)(philox(B.sweep[c], STAGE_FLIP, (uint32_t)(i - B.chain_node0[c]), 0u, P.key0, P.key1).x & 1u);
     }
   }
 }
@@ -991,6 +993,48 @@ __global__ void __launch_bounds__(EST_THREADS) k_estimate(DevBatch B) {
 }
 
 // ---------------------------------------------------------------------------
+// Longitudinal field (the FIELD branch of path_integral.C:798): the cluster
+// flips iff 2u - 1 < tanh(beta w_c), w_c = -h umag_c, with u from the same
+// Philox counter as the field-free flip; det_tanh keeps host and device
+// decisions identical.  Then every segment of a field chain takes its
+// root's bit: flipped segment and tau = 0 spins (k_estimate left them
+// unflipped, since compression gave field roots bit 0).
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_field_flip(DevBatch B) {
+  if (overflowed(B)) return;
+  const int M = B.totals[1];
+  int ilo, ihi;
+  xcd_share(M, ilo, ihi);
+  for (int i = ilo + threadIdx.x; i < ihi; i += blockDim.x) {
+    if (B.parent[i] != i) continue;
+    const int c = node_chain(B, i);
+    const ChainDev& P = B.chains[c];
+    if (P.h == 0.0) continue;
+    const ClusterAcc a = B.acc[i];
+    const double umag = 0.5 * ((a.L[0] + a.L[1]) - (a.L[2] + a.L[3]));
+    const double x = P.beta * (-P.h * umag);
+    const double u = uniform53(philox(B.sweep[c], STAGE_FLIP, (uint32_t)(i - B.chain_node0[c]), 0u, P.key0, P.key1));
+    B.fbit[i] = (uint8_t)((2.0 * u - 1.0) < det_tanh(x) ? 1 : 0);
+  }
+}
+
+__global__ void __launch_bounds__(256) k_field_apply(DevBatch B) {
+  if (overflowed(B)) return;
+  const int M = B.totals[1];
+  int ilo, ihi;
+  xcd_share(M, ilo, ihi);
+  for (int i = ilo + threadIdx.x; i < ihi; i += blockDim.x) {
+    const int c = node_chain(B, i);
+    if (B.chains[c].h == 0.0) continue;
+    const uint8_t f = B.fbit[B.parent[i]];
+    if (!f) continue;
+    B.fbit[i] = 1;
+    B.segspin_new[i] ^= 1;
+    if (B.evflags[i] & EV_FIRST) B.spins_new[first_node_site(B, i)] ^= 1;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // (d) operator flip, then the improved-estimator collector
 // (susceptibility.h:177-198, stiffness.h:117-121) over the roots.
 // Collection is chunked: chunk j covers nodes [j*COL_CHUNK, (j+1)*COL_CHUNK)
@@ -1002,9 +1046,13 @@ __global__ void __launch_bounds__(EST_THREADS) k_estimate(DevBatch B) {
 constexpr int COL_THREADS = 256;
 constexpr int COL_ITEMS = 16;
 constexpr int COL_CHUNK = COL_THREADS * COL_ITEMS;
-constexpr int NCOL = 16;
+constexpr int NCOL = NUM_COLLECTOR_TERMS;
 
-__device__ __forceinline__ void cluster_terms(const ClusterAcc& a, double* v) {
+// Collector terms of one cluster.  h, flip: the chain's field and the
+// cluster's flip; the field weight of the cluster is w = -h umag
+// (cluster.h:50-63) and its Zeeman energy after the flip flip ? -w : w
+// (path_integral.C:852-858).
+__device__ __forceinline__ void cluster_terms(const ClusterAcc& a, double h, int flip, double* v) {
   const double L0 = a.L[0], L1 = a.L[1], L2 = a.L[2], L3 = a.L[3];
   const int n0 = a.n[0], n1 = a.n[1], n2 = a.n[2], n3 = a.n[3];
   int w0, w1;
@@ -1030,10 +1078,16 @@ __device__ __forceinline__ void cluster_terms(const ClusterAcc& a, double* v) {
   v[13] = ssize * ssize;
   v[14] = smag * smag;
   v[15] = 0.25 * ((double)w0 * w0 + (double)w1 * w1);
+  v[16] = umag;   // normal estimator: tau-integrated magnetization
+  v[17] = smag;
+  const double wf = -h * umag;
+  v[18] = flip ? -wf : wf;
 }
 
-// record slot of collector term q (R_NC, then R_UMAG0 .. R_W2)
-__device__ __forceinline__ int term_slot(int q) { return q == 0 ? R_NC : R_UMAG0 + (q - 1); }
+// record slot of collector term q (R_NC, R_UMAG0 .. R_W2, R_UMAGA, R_SMAGA, R_EFIELD)
+__device__ __forceinline__ int term_slot(int q) {
+  return q == 0 ? R_NC : (q <= 15 ? R_UMAG0 + (q - 1) : R_UMAGA + (q - 16));
+}
 
 __global__ void __launch_bounds__(COL_THREADS) k_flip_collect(DevBatch B) {
   if (overflowed(B)) return;
@@ -1084,7 +1138,7 @@ __global__ void __launch_bounds__(COL_THREADS) k_flip_collect(DevBatch B) {
           cl = c;
         }
         double v[NCOL];
-        cluster_terms(B.acc[i], v);
+        cluster_terms(B.acc[i], B.chains[c].h, B.fbit[i], v);
 #pragma unroll
         for (int q = 0; q < NCOL; ++q) w[q] += v[q];
       }
@@ -1170,7 +1224,7 @@ __global__ void __launch_bounds__(END_THREADS) k_end_sweep(DevBatch B) {
   if (tid == 0) {
     const double nop = bad ? 0.0 : (double)(B.bo_new[P.bond_base + P.Nb] - B.bo_new[P.bond_base]);
     out[R_NOP] = nop;
-    out[R_ENERGY] = P.offset_total - nop / P.beta;
+    out[R_ENERGY] = P.offset_total - nop / P.beta + out[R_EFIELD];
     out[R_BETA] = P.beta;
     out[R_SWEEP] = (double)B.sweep[c];
     B.sweep[c] += 1;
@@ -1331,6 +1385,10 @@ void launch_cluster_stage(const DevBatch& B, const LaunchCfg& L, hipStream_t st,
   mark(ev, 3, st);
   hipLaunchKernelGGL(k_estimate, dim3(L.blocks_node), dim3(EST_THREADS), 0, st, B);
   mark(ev, 4, st);
+  if (B.any_field) {
+    hipLaunchKernelGGL(k_field_flip, dim3(L.blocks_node), dim3(256), 0, st, B);
+    hipLaunchKernelGGL(k_field_apply, dim3(L.blocks_node), dim3(256), 0, st, B);
+  }
   hipLaunchKernelGGL(k_flip_collect, dim3(L.blocks_node), dim3(COL_THREADS), 0, st, B);
   mark(ev, 5, st);
   hipLaunchKernelGGL(k_end_sweep, dim3(B.nchains), dim3(END_THREADS), 0, st, B);

@@ -76,9 +76,15 @@ struct lg_server {
     const double V = (p.lattice == LG_LATTICE_CHAIN) ? p.L : (double)p.L * p.L;
     const double dim = (p.lattice == LG_LATTICE_CHAIN) ? 1 : 2;
     j.obs[0] << r.energy / V;
-    j.obs[1] << r.beta * r.umag / V;
-    j.obs[2] << r.smag2 / V;
-    j.obs[3] << r.beta * r.smag / V;
+    if (p.h == 0.0) {  // improved estimators
+      j.obs[1] << r.beta * r.umag / V;
+      j.obs[2] << r.smag2 / V;
+      j.obs[3] << r.beta * r.smag / V;
+    } else {           // field: normal estimators (susceptibility.h:304-337)
+      j.obs[1] << r.beta * r.umag_a * r.umag_a / V;
+      j.obs[2] << r.smag0 * r.smag0 / V;
+      j.obs[3] << r.beta * r.smag_a * r.smag_a / V;
+    }
     j.obs[4] << r.n_clusters;
     j.obs[5] << r.w2 / (r.beta * dim);
     j.obs[6] << r.n_op;

@@ -190,11 +190,12 @@ class loop_worker {
     if (p.str("MODEL", "spin") != "spin") throw std::invalid_argument("only MODEL = spin is supported");
     if (p.defined("local_S") && std::fabs(p.num("local_S") - 0.5) > 1e-12)
       throw std::invalid_argument("only local_S = 1/2 is supported");
-    for (const char* k : {"h", "Gamma", "D"})
+    for (const char* k : {"Gamma", "D"})
       if (p.defined(k) && p.num(k) != 0.0)
-        throw std::invalid_argument(std::string("parameter ") + k + " != 0 (field / single-ion terms) is not supported");
-    if (p.defined("DISABLE_IMPROVED_ESTIMATOR"))
-      throw std::invalid_argument("DISABLE_IMPROVED_ESTIMATOR is not supported (improved estimators only)");
+        throw std::invalid_argument(std::string("parameter ") + k + " != 0 (transverse field / single-ion terms) is not supported");
+    lp_.h = p.num("h", 0.0);
+    // improved estimators unless a field or DISABLE_IMPROVED_ESTIMATOR (path_integral.C:217)
+    improved_ = lp_.h == 0.0 && !p.defined("DISABLE_IMPROVED_ESTIMATOR");
     // temperature.h:43-55
     double T;
     if (p.defined("T")) T = p.num("T");
@@ -229,7 +230,7 @@ class loop_worker {
 
   // path_integral.C:307-321 + estimator init_observables
   void init_observables(std::map<std::string, Observable>& obs) const {
-    for (const auto& n : names()) obs[n];
+    for (const auto& n : names(improved_)) obs[n];
   }
   bool is_thermalized() const { return mcs_ >= therm_; }
   double progress() const { return (double)mcs_ / (double)(therm_ + sweeps_min_); }
@@ -311,7 +312,9 @@ class loop_worker {
   }
   void set_chunk(uint32_t c) { chunk_ = std::max<uint32_t>(1, c); }
 
-  static std::vector<std::string> names() {
+  bool improved() const { return improved_; }
+
+  static std::vector<std::string> names(bool improved = true) {
     static const char* base[] = {
         "Temperature", "Inverse Temperature", "Volume", "Number of Sites", "Number of Clusters",
         "Energy", "Energy Density", "Energy^2", "Stiffness"};
@@ -320,10 +323,12 @@ class loop_worker {
         "Magnetization^2", "Magnetization Density^2", "Magnetization^4", "Magnetization Density^4",
         "Susceptibility", "Generalized Magnetization^2", "Generalized Magnetization Density^2",
         "Generalized Magnetization^4", "Generalized Magnetization Density^4", "Generalized Susceptibility"};
-    std::vector<std::string> v(base, base + 9);
-    for (const char* m : mag) v.push_back(m);
+    std::vector<std::string> v(base, base + (improved ? 9 : 8));  // Stiffness: improved only
+    for (const char* m : mag)
+      if (improved || std::string(m).rfind("Generalized ", 0) != 0) v.push_back(m);
     for (const char* m : mag) {
       std::string s(m);
+      if (!improved && s.rfind("Generalized ", 0) == 0) continue;
       // insert "Staggered " after a leading "|" or "Generalized "
       if (s[0] == '|') s = "|Staggered " + s.substr(1);
       else if (s.rfind("Generalized ", 0) == 0) s = "Generalized Staggered " + s.substr(12);
@@ -351,6 +356,24 @@ class loop_worker {
     m["Energy"] << r.energy;
     m["Energy Density"] << r.energy / vol;
     m["Energy^2"] << p2(r.energy) - nop / p2(beta);
+    if (!improved_) {
+      // normal estimator (susceptibility.h:304-337), the configuration before the flip
+      const double mags[2][2] = {{r.umag0, r.umag_a}, {r.smag0, r.smag_a}};
+      for (int k = 0; k < 2; ++k) {
+        const std::string P = k ? "Staggered " : "", bar = "|" + P;
+        const double mg = mags[k][0], ma = mags[k][1];
+        m[P + "Magnetization"] << mg;
+        m[P + "Magnetization Density"] << mg / vol;
+        m[bar + "Magnetization|"] << std::fabs(mg);
+        m[bar + "Magnetization Density|"] << std::fabs(mg) / vol;
+        m[P + "Magnetization^2"] << p2(mg);
+        m[P + "Magnetization Density^2"] << p2(mg / vol);
+        m[P + "Magnetization^4"] << p4(mg);
+        m[P + "Magnetization Density^4"] << p4(mg / vol);
+        m[P + "Susceptibility"] << beta * p2(ma) / vol;
+      }
+      return;
+    }
     m["Stiffness"] << r.w2 / (beta * dim_);
     struct Set { const char* pre; double mag0, size2, mag2, size4, mag4, sus, gsus; };
     const Set sets[2] = {
@@ -378,6 +401,7 @@ class loop_worker {
   }
 
   lg_params lp_{};
+  bool improved_ = true;
   lg_chain* h_ = nullptr;
   uint64_t mcs_ = 0, therm_ = 0, sweeps_min_ = 0, sweeps_max_ = 0;
   int n_sites_ = 0, dim_ = 1;

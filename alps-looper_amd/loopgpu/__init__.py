@@ -19,6 +19,7 @@ SQUARE = 1
 REC_FIELDS = [
     "n_op", "n_clusters", "energy", "umag0", "usize2", "umag2", "usize4", "umag4", "usize", "umag",
     "smag0", "ssize2", "smag2", "ssize4", "smag4", "ssize", "smag", "w2", "beta", "sweep",
+    "umag_a", "smag_a", "e_field",
 ]
 REC_SIZE = len(REC_FIELDS)
 
@@ -55,7 +56,7 @@ class LGParams(ctypes.Structure):
     _fields_ = [
         ("lattice", ctypes.c_int32), ("L", ctypes.c_int32), ("beta", ctypes.c_double),
         ("jxy", ctypes.c_double), ("jz", ctypes.c_double), ("force_scatter", ctypes.c_double),
-        ("seed", ctypes.c_uint32), ("chain_id", ctypes.c_uint32),
+        ("seed", ctypes.c_uint32), ("chain_id", ctypes.c_uint32), ("h", ctypes.c_double),
     ]
 
 
@@ -140,11 +141,11 @@ def kernel_names():
     return [lib.lg_profile_kernel_name(i).decode() for i in range(lib.lg_profile_num_kernels())]
 
 
-def params(lattice, L, beta, jxy=1.0, jz=None, seed=23125, chain_id=0, force_scatter=0.0):
+def params(lattice, L, beta, jxy=1.0, jz=None, seed=23125, chain_id=0, force_scatter=0.0, h=0.0):
     if jz is None:
         jz = jxy
     return LGParams(int(lattice), int(L), float(beta), float(jxy), float(jz), float(force_scatter),
-                    int(seed), int(chain_id))
+                    int(seed), int(chain_id), float(h))
 
 
 def lattice_size(lattice, L):

@@ -54,6 +54,7 @@ typedef struct lg_params {
   double force_scatter;  /* FORCE_SCATTER in [0,1] (0 = standard weights) */
   uint32_t seed;         /* SEED */
   uint32_t chain_id;     /* independent stream index */
+  double h;              /* longitudinal field h (site term -h S^z, looper's FIELD path) */
 } lg_params;
 
 /* Per-sweep collector sums (all improved estimators) -- see the collector of
@@ -71,6 +72,11 @@ typedef struct lg_sweep_record {
   double w2;            /* sum_c sum_d (w_{c,d}/2)^2 */
   double beta;
   double sweep;         /* sweep index (mcs) of this record */
+  /* normal estimator (susceptibility.h:258-339), on the configuration before
+   * the cluster flip: tau-integrated magnetization sum_c umag_c and its
+   * staggered form (umag0 / smag0 above are the tau = 0 ones) */
+  double umag_a, smag_a;
+  double e_field;       /* Zeeman energy after the flip, sum_c (flip ? -w_c : w_c), included in energy */
 } lg_sweep_record;
 
 typedef struct lg_chain lg_chain;   /* one chain, plain one-chain-per-launch path */

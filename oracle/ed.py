@@ -63,8 +63,9 @@ def _sector_hamiltonian(states, bonds, jxy, jz):
     return H
 
 
-def diagonalize(kind, L, T, jxy=1.0, jz=1.0):
-    """Thermal averages keyed by the reference's observable names."""
+def diagonalize(kind, L, T, jxy=1.0, jz=1.0, h=0.0):
+    """Thermal averages keyed by the reference's observable names; h is a
+    longitudinal field (site term -h S^z), diagonal in every S^z sector."""
     beta = 1.0 / T
     N, bonds, gauge = lattice_bonds(kind, L)
     V = float(N)
@@ -73,6 +74,7 @@ def diagonalize(kind, L, T, jxy=1.0, jz=1.0):
         states = _sector_states(N, nup)
         H = _sector_hamiltonian(states, bonds, jxy, jz)
         E, U = np.linalg.eigh(H)
+        E = E - h * (nup - 0.5 * N)
         sz = np.array([[0.5 if (int(s) >> i) & 1 else -0.5 for i in range(N)] for s in states])
         mu = sz.sum(axis=1)
         ms = sz @ gauge
@@ -112,6 +114,7 @@ def diagonalize(kind, L, T, jxy=1.0, jz=1.0):
             tot += (A * A * wij).sum()
         return tot / Z / V
 
+    res["Magnetization"] = sum((np.exp(-beta * (E - e0)) * mu[0]).sum() for (E, _, mu, _) in sectors) / Z
     m2, m4, ma = static(lambda mu, ms: mu)
     res["Magnetization^2"] = m2
     res["Magnetization^4"] = m4

@@ -108,6 +108,30 @@ inline double det_log(double x) {
   return (double)e * 0.69314718055994530942 + logm;
 }
 
+// exp(x) for the field flip, from +,-,*,/ and exact scalings (floor, ldexp):
+// Cody-Waite reduction x = k ln2 + r (ln2 split into a 32-bit high part
+// and the rest), e^r by its Taylor series to r^14, times 2^k.
+inline double det_exp(double x) {
+  if (!(x > -745.0)) return 0.0;
+  if (x > 709.0) x = 709.0;
+  const double k = std::floor(x * 1.4426950408889634 + 0.5);
+  const double r = (x - k * 0.6931471803691238) - k * 1.9082149292705877e-10;
+  static const double inv_fact[15] = {1.0, 1.0, 0.5, 1.0 / 6.0, 1.0 / 24.0, 1.0 / 120.0, 1.0 / 720.0,
+                                      1.0 / 5040.0, 1.0 / 40320.0, 1.0 / 362880.0, 1.0 / 3628800.0,
+                                      1.0 / 39916800.0, 1.0 / 479001600.0, 1.0 / 6227020800.0,
+                                      1.0 / 87178291200.0};
+  double p = inv_fact[14];
+  for (int n = 13; n >= 0; --n) p = p * r + inv_fact[n];
+  return std::ldexp(p, (int)k);
+}
+
+inline double det_tanh(double x) {
+  const double a = x < 0 ? -x : x;
+  const double e = det_exp(-2.0 * a);
+  const double t = (1.0 - e) / (1.0 + e);
+  return x < 0 ? -t : t;
+}
+
 const uint64_t TGRID = 1ull << 52;  // imaginary time tau = T * 2^-52, T in [0, 2^52)
 
 inline uint64_t exp_gap(const u32x4& x, double scale) {
@@ -313,7 +337,9 @@ enum {
   R_NOP = 0, R_NC, R_ENERGY,
   R_UMAG0, R_USIZE2, R_UMAG2, R_USIZE4, R_UMAG4, R_USIZE, R_UMAG,
   R_SMAG0, R_SSIZE2, R_SMAG2, R_SSIZE4, R_SMAG4, R_SSIZE, R_SMAG,
-  R_W2, R_COUNT
+  R_W2, R_BETA, R_SWEEP,
+  R_UMAGA, R_SMAGA, R_EFIELD,  // normal estimator sums, Zeeman energy (FIELD path)
+  R_COUNT
 };
 
 struct Estimate {   // susceptibility + stiffness improved estimate (per cluster)
@@ -341,6 +367,8 @@ void collect(const Estimate& e, double* rec) {
   rec[R_SSIZE] += p2(e.ssize);
   rec[R_SMAG] += p2(e.smag);
   for (int d = 0; d < 3; ++d) rec[R_W2] += p2(0.5 * e.winding[d]);
+  rec[R_UMAGA] += e.umag;   // normal estimator (susceptibility.h:258-339)
+  rec[R_SMAGA] += e.smag;
 }
 
 // end_s / begin_s / start_bottom of susceptibility.h:113-147
@@ -423,10 +451,13 @@ struct LooperChain {
   std::vector<char> to_flip;
   uint64_t mcs = 0;
 
-  void init(int kind, int L, double b, double jxy, double jz, double fs, uint32_t seed) {
+  double h = 0;   // longitudinal field, site term -h S^z
+
+  void init(int kind, int L, double b, double jxy, double jz, double fs, uint32_t seed, double field = 0) {
     lat.init(kind, L);
     w = xxz_weights(jxy, jz, fs);
     beta = b;
+    h = field;
     eng.seed(seed);
     std::vector<double> wl;
     for (int bd = 0; bd < lat.Nb; ++bd)
@@ -520,9 +551,18 @@ struct LooperChain {
     rec[R_NOP] = (double)ops.size();
     rec[R_NC] = nc;
     rec[R_ENERGY] = offset_total - ops.size() / beta;
-    // determine flip (781-803), FIELD=false: (2r-1) < 0
+    rec[R_BETA] = beta;
+    rec[R_SWEEP] = (double)mcs;
+    // determine flip (781-803): (2r-1) < (FIELD ? tanh(beta w_c) : 0), with
+    // the field weight w_c = -h umag_c (cluster.h:50-63); Zeeman energy of
+    // the flipped cluster added to the energy (852-858)
     to_flip.assign(nc, 0);
-    for (int c = 0; c < nc; ++c) to_flip[c] = (2 * rnd() - 1) < 0;
+    for (int c = 0; c < nc; ++c) {
+      const double wc = -h * est[c].umag;
+      to_flip[c] = (2 * rnd() - 1) < (h != 0 ? std::tanh(beta * wc) : 0.0);
+      rec[R_EFIELD] += to_flip[c] ? -wc : wc;
+    }
+    rec[R_ENERGY] += rec[R_EFIELD];
     // flip operators & spins (806-825)
     for (auto& op : ops)
       if ((to_flip[frag[op.loop0].id_] ^ to_flip[frag[op.loop1].id_]) & 1) op.type ^= 1;
@@ -794,6 +834,7 @@ struct PhiloxChain {
   Weights w;
   Thresholds th;
   double beta = 1, scale = 0, offset_total = 0, W = 0;
+  double h = 0;   // longitudinal field
   uint32_t key0 = 0, key1 = 0;
   uint32_t sweep = 0;
   std::vector<int> spins;                 // tau = 0 spins
@@ -811,7 +852,9 @@ struct PhiloxChain {
   std::vector<int> last_l0, last_l1;      // per op
   int last_nodes = 0;
 
-  void init(int kind, int L, double b, double jxy, double jz, double fs, uint32_t seed, uint32_t chain) {
+  void init(int kind, int L, double b, double jxy, double jz, double fs, uint32_t seed, uint32_t chain,
+            double field = 0) {
+    h = field;
     lat.init(kind, L);
     w = xxz_weights(jxy, jz, fs);
     th = make_thresholds(w);
@@ -993,10 +1036,25 @@ struct PhiloxChain {
     rec[R_NOP] = nops;
     rec[R_NC] = nc;
     rec[R_ENERGY] = offset_total - nops / beta;
-    // (d) flip: one bit per root
+    rec[R_BETA] = beta;
+    rec[R_SWEEP] = (double)sweep;
+    // (d) flip: one bit per root; with a field the cluster flips iff
+    // 2u - 1 < tanh(beta w), w = -h umag, u from the first 53 bits
     std::vector<char> fb(M, 0);
-    for (int i = 0; i < M; ++i)
-      if (parent[i] == i) fb[i] = philox4x32_10(sweep, STAGE_FLIP, (uint32_t)i, 0, key0, key1).v[0] & 1;
+    for (int i = 0; i < M; ++i) {
+      if (parent[i] != i) continue;
+      u32x4 r = philox4x32_10(sweep, STAGE_FLIP, (uint32_t)i, 0, key0, key1);
+      if (h == 0) {
+        fb[i] = r.v[0] & 1;
+      } else {
+        const uint64_t u53 = ((uint64_t)r.v[0] << 21) | (uint64_t)(r.v[1] >> 11);
+        const double u = (double)(u53 + 1) * (1.0 / 9007199254740992.0);
+        const double wc = -h * acc[i].umag;
+        fb[i] = (2.0 * u - 1.0) < det_tanh(beta * wc);
+        rec[R_EFIELD] += fb[i] ? -wc : wc;
+      }
+    }
+    rec[R_ENERGY] += rec[R_EFIELD];
     for (int k = 0; k < nops; ++k) {
       int b = bond[k], s0 = lat.src[b];
       int l0 = last_l0[k], u0 = next(s0, l0);
@@ -1025,6 +1083,8 @@ void or_philox4x32_10(const uint32_t* ctr, const uint32_t* key, uint32_t* out) {
 }
 
 void or_det_log(const double* x, double* y, int n) { for (int i = 0; i < n; ++i) y[i] = det_log(x[i]); }
+void or_det_tanh(const double* x, double* y, int n) { for (int i = 0; i < n; ++i) y[i] = det_tanh(x[i]); }
+void or_det_exp(const double* x, double* y, int n) { for (int i = 0; i < n; ++i) y[i] = det_exp(x[i]); }
 
 void or_exp_gap(const uint32_t* x4, double scale, uint64_t* gap) {
   u32x4 x = {{x4[0], x4[1], x4[2], x4[3]}};
@@ -1073,9 +1133,9 @@ void or_uf_pairs(int n, int npairs, const int* pairs, int variant, int* parent, 
 }
 
 // ---- looper mode ---------------------------------------------------------
-void* or_looper_create(int kind, int L, double beta, double jxy, double jz, double fs, uint32_t seed) {
+void* or_looper_create(int kind, int L, double beta, double jxy, double jz, double fs, uint32_t seed, double h) {
   LooperChain* c = new LooperChain();
-  c->init(kind, L, beta, jxy, jz, fs, seed);
+  c->init(kind, L, beta, jxy, jz, fs, seed, h);
   return c;
 }
 void or_looper_sweep(void* h, int n, double* rec) {
@@ -1122,10 +1182,10 @@ int or_segment_of_elements(int kind, int L, int nops, const int* bond, const dou
 }
 
 // ---- philox mode -------------------------------------------------------------
-void* or_philox_create(int kind, int L, double beta, double jxy, double jz, double fs, uint32_t seed,
+void* or_philox_create(int kind, int L, double beta, double jxy, double jz, double fs, double h, uint32_t seed,
                        uint32_t chain) {
   PhiloxChain* c = new PhiloxChain();
-  c->init(kind, L, beta, jxy, jz, fs, seed, chain);
+  c->init(kind, L, beta, jxy, jz, fs, seed, chain, h);
   return c;
 }
 void or_philox_sweep(void* h, int n, double* rec) {

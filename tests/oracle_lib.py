@@ -26,6 +26,8 @@ U = ctypes.c_uint32
 lib.or_record_size.restype = I
 lib.or_philox4x32_10.argtypes = [vp, vp, vp]
 lib.or_det_log.argtypes = [vp, vp, I]
+lib.or_det_tanh.argtypes = [vp, vp, I]
+lib.or_det_exp.argtypes = [vp, vp, I]
 lib.or_exp_gap.argtypes = [vp, D, vp]
 lib.or_xxz_weights.argtypes = [D, D, D, vp, vp, vp]
 lib.or_thresholds.argtypes = [D, D, D, vp, vp]
@@ -33,7 +35,7 @@ lib.or_lattice_info.argtypes = [I, I, vp, vp, vp]
 lib.or_lattice_bonds.argtypes = [I, I, vp, vp, vp]
 lib.or_uf_pairs.argtypes = [I, I, vp, I, vp, vp, vp]
 lib.or_looper_create.restype = vp
-lib.or_looper_create.argtypes = [I, I, D, D, D, D, U]
+lib.or_looper_create.argtypes = [I, I, D, D, D, D, U, D]
 lib.or_looper_sweep.argtypes = [vp, I, vp]
 lib.or_looper_destroy.argtypes = [vp]
 lib.or_standalone_run.argtypes = [U, D, U, U, vp, vp]
@@ -42,7 +44,7 @@ lib.or_looper_labels.argtypes = [I, I, I, vp, vp, vp, I, vp]
 lib.or_segment_of_elements.argtypes = [I, I, I, vp, vp, vp]
 lib.or_segment_of_elements.restype = I
 lib.or_philox_create.restype = vp
-lib.or_philox_create.argtypes = [I, I, D, D, D, D, U, U]
+lib.or_philox_create.argtypes = [I, I, D, D, D, D, D, U, U]
 lib.or_philox_sweep.argtypes = [vp, I, vp]
 lib.or_philox_set_beta.argtypes = [vp, D]
 lib.or_philox_num_ops.argtypes = [vp]
@@ -53,6 +55,9 @@ lib.or_philox_load.argtypes = [vp, vp, vp, vp, vp, U]
 lib.or_philox_destroy.argtypes = [vp]
 
 R = lib.or_record_size()
+# record fields (the device library's lg_sweep_record layout)
+R_NOP, R_NC, R_ENERGY, R_UMAG0 = 0, 1, 2, 3
+R_BETA, R_SWEEP, R_UMAGA, R_SMAGA, R_EFIELD = 18, 19, 20, 21, 22
 CHAIN, SQUARE = 0, 1
 STAGE_DIAG, STAGE_REGRAPH, STAGE_FLIP = 1, 2, 3
 
@@ -67,6 +72,20 @@ def philox(ctr, key):
     out = np.zeros(4, np.uint32)
     lib.or_philox4x32_10(_p(c), _p(k), _p(out))
     return out
+
+
+def det_tanh(x):
+    x = np.ascontiguousarray(x, np.float64)
+    y = np.zeros_like(x)
+    lib.or_det_tanh(_p(x), _p(y), len(x))
+    return y
+
+
+def det_exp(x):
+    x = np.ascontiguousarray(x, np.float64)
+    y = np.zeros_like(x)
+    lib.or_det_exp(_p(x), _p(y), len(x))
+    return y
 
 
 def det_log(x):
@@ -95,9 +114,9 @@ def xxz_weights(jxy, jz, fs=0.0):
 
 
 class LooperChain:
-    def __init__(self, kind, L, beta, jxy=1.0, jz=None, seed=23125, fs=0.0):
+    def __init__(self, kind, L, beta, jxy=1.0, jz=None, seed=23125, fs=0.0, field=0.0):
         jz = jxy if jz is None else jz
-        self.h = lib.or_looper_create(kind, L, beta, jxy, jz, fs, seed)
+        self.h = lib.or_looper_create(kind, L, beta, jxy, jz, fs, seed, field)
 
     def sweep(self, n):
         rec = np.zeros((n, R))
@@ -109,11 +128,11 @@ class LooperChain:
 
 
 class PhiloxChain:
-    def __init__(self, kind, L, beta, jxy=1.0, jz=None, seed=23125, chain_id=0, fs=0.0):
+    def __init__(self, kind, L, beta, jxy=1.0, jz=None, seed=23125, chain_id=0, fs=0.0, field=0.0):
         jz = jxy if jz is None else jz
         self.kind, self.L = kind, L
         self.N, self.Nb, _, _, _ = lattice(kind, L)
-        self.h = lib.or_philox_create(kind, L, beta, jxy, jz, fs, seed, chain_id)
+        self.h = lib.or_philox_create(kind, L, beta, jxy, jz, fs, field, seed, chain_id)
 
     def sweep(self, n):
         rec = np.zeros((n, R))

@@ -11,7 +11,7 @@ import stats
 
 R_NOP, R_ENERGY, R_UMAG, R_BETA = 0, 2, 9, 18  # lg_sweep_record fields
 EXACT = [0, 1]
-FLOAT = list(range(2, 18))
+FLOAT = list(range(2, 18)) + [20, 21, 22]
 
 
 def test_exchange_argument_errors():
@@ -42,7 +42,7 @@ def oracle_exchange(kind, L, jxy, jz, betas, seed, chain0, xseed, n):
     slot = list(range(R))
     beta_of = list(betas)
     acc = [0] * (R - 1)
-    recs = np.zeros((n, R, 18))
+    recs = np.zeros((n, R, O.R))
     temps = np.zeros((n, R))
     for step in range(n):
         for c in range(R):

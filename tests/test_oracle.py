@@ -309,3 +309,65 @@ def test_planted_field_mix_up_is_caught():
     right = 10.0 * rec[:, 16] / 12
     m, e = stats.mean_err(right)
     assert stats.sigmas(m, e, ref) < 4.0
+
+
+# ---------------------------------------------------------------------------
+# longitudinal field (looper's FIELD path, path_integral.C:798, 852-858)
+# ---------------------------------------------------------------------------
+
+def test_det_exp_and_tanh_against_libm():
+    x = np.concatenate([np.linspace(-40, 40, 4001), [-700.0, -745.5, 0.0, 1e-300, -1e-12, 709.0]])
+    e = O.det_exp(x)
+    ref = np.exp(np.minimum(x, 709.0))
+    ok = ref > 0
+    np.testing.assert_allclose(e[ok], ref[ok], rtol=4e-16 * 4)
+    t = O.det_tanh(np.linspace(-25, 25, 5001))
+    np.testing.assert_allclose(t, np.tanh(np.linspace(-25, 25, 5001)), rtol=0, atol=4e-16)
+
+
+def test_ed_field_two_site_closed_form():
+    import ed
+    # chain L=2: two bonds between the sites, singlet -3J/2, triplet J/2 - h m
+    b, J, h = 1.3, 1.0, 0.45
+    lev = [(-1.5 * J, 0), (0.5 * J - h, 1), (0.5 * J, 0), (0.5 * J + h, -1)]
+    w = np.array([np.exp(-b * e) for e, _ in lev])
+    m = np.array([mm for _, mm in lev], float)
+    en = np.array([e for e, _ in lev])
+    r = ed.diagonalize("chain", 2, 1 / b, 1.0, 1.0, h)
+    assert r["Magnetization"] == pytest.approx((w * m).sum() / w.sum(), rel=1e-12)
+    assert r["Energy"] == pytest.approx((w * en).sum() / w.sum(), rel=1e-12)
+    assert r["Susceptibility"] == pytest.approx(b * (w * m * m).sum() / w.sum() / 2, rel=1e-12)
+
+
+FIELD_CASES = [
+    # kind, L, T, jxy, jz, h
+    (O.CHAIN, 8, 0.5, 1.0, 1.0, 0.3),
+    (O.CHAIN, 6, 0.3, 1.0, 0.5, 0.8),
+    (O.SQUARE, 2, 0.7, 1.0, 1.0, 0.6),
+]
+
+
+def _field_observables(rec, beta, V):
+    return {"Energy Density": rec[:, O.R_ENERGY] / V, "Magnetization": rec[:, O.R_UMAG0],
+            "Susceptibility": beta * rec[:, O.R_UMAGA] ** 2 / V}
+
+
+@pytest.mark.parametrize("mode", ["looper", "philox"])
+@pytest.mark.parametrize("case", FIELD_CASES, ids=lambda c: "k%d_L%d_T%g_jz%g_h%g" % (c[0], c[1], c[2], c[4], c[5]))
+def test_field_modes_match_exact_diagonalisation(case, mode):
+    import ed
+    kind, L, T, jxy, jz, h = case
+    beta = 1 / T
+    if mode == "looper":
+        c = O.LooperChain(kind, L, beta, jxy, jz, seed=1234, field=h)
+    else:
+        c = O.PhiloxChain(kind, L, beta, jxy, jz, seed=1234, chain_id=3, field=h)
+    c.sweep(2000)
+    rec = c.sweep(150000)  # the field lengthens autocorrelations: long runs for honest binning errors
+    V = L if kind == O.CHAIN else L * L
+    ref = ed.diagonalize("chain" if kind == O.CHAIN else "square", L, T, jxy, jz, h)
+    ref_obs = {"Energy Density": ref["Energy"] / V, "Magnetization": ref["Magnetization"],
+               "Susceptibility": ref["Susceptibility"]}
+    for k, x in _field_observables(rec, beta, V).items():
+        m, err = x.mean(), stats.binning_error(x)
+        assert stats.sigmas(m, err, ref_obs[k]) < 4.0, (mode, case, k, m, err, ref_obs[k])

@@ -17,12 +17,11 @@ FX = json.load(open(os.path.join(HERE, "golden", "reference_fixtures.json")))
     ("L=7; T=1", "even"),
     ('LATTICE="chain lattice"; T=0.1', "L not defined"),
     ('LATTICE="triangular lattice"; L=4; T=1', "unsupported LATTICE"),
-    ("L=4; T=1; h=0.3", "not supported"),
+    ("L=4; T=1; Gamma=0.3", "not supported"),
     ("L=4; local_S=1; T=1", "local_S"),
     ("L=4; T=-1", "negative temperature"),
     ("L=4", "temperature is not initialized"),
     ("L=4; T=1; SWEEPS=[100:x]", "SWEEPS"),
-    ("L=4; T=1; DISABLE_IMPROVED_ESTIMATOR=1", "improved"),
     ("L=4; T=1/Q", "unknown name"),
 ])
 def test_worker_parameter_errors(txt, frag):
