@@ -334,6 +334,9 @@ int Batch::allocate() {
   if ((rc = alloc(parent, node_cap))) return rc;
   if ((rc = alloc(fbit, node_cap))) return rc;
   if ((rc = alloc(acc, node_cap))) return rc;
+  // the estimator accumulates into zeroed accumulators; the collector
+  // clears each root's after reading it
+  LG_HIP(hipMemset(acc, 0, (size_t)node_cap * sizeof(ClusterAcc)));
   scan_nparts = (cap_bonds + 2047) / 2048 + (cap_sites + 2047) / 2048;
   if ((rc = alloc(scan_part, scan_nparts))) return rc;
   if ((rc = alloc(col_part, (size_t)col_chunks(node_cap) * NUM_COLLECTOR_TERMS))) return rc;
@@ -403,10 +406,25 @@ int Batch::set_layout(std::vector<ChainDev>& ds, const std::vector<uint64_t>& sw
                             0.5 * (std::fabs(params[c].jz) / 4 + std::fabs(params[c].jxy) / 2);
     ev += 2.0 * chains[c].beta * chains[c].Nb * per_bond;
   }
+  LaunchCfg c = cfg;
   if (!tuned) {
     const double per_site = nsites > 0 ? ev / nsites : 0;
-    cfg.link_cap = per_site < 96 ? 0 : (per_site <= 400 ? 512 : 1536);
+    c.link_cap = per_site < 96 ? 0 : (per_site <= 400 ? 512 : 1536);
+    // one chain: the estimator finds the roots itself (a pass less); many
+    // chains, mostly small and hot with many small clusters: clearing every
+    // root's accumulator costs more than the compress pass
+    c.compress = chains.size() > 1 ? 1 : 0;
   }
+  return apply_cfg(c);
+}
+
+int Batch::apply_cfg(const LaunchCfg& c) {
+  // the compress pass clears the roots' accumulators before the estimator
+  // adds into them; without it they must be zero between sweeps, which the
+  // collector keeps once they are
+  if (cfg.compress && !c.compress && acc)
+    LG_HIP(hipMemsetAsync(acc, 0, (size_t)node_cap * sizeof(ClusterAcc), stream));
+  cfg = c;
   invalidate_graphs();
   return LG_OK;
 }
@@ -630,15 +648,17 @@ int Batch::tune(uint32_t sweeps_per_trial) {
   const int mults[] = {2, 4, 8, 16, 32};
   const int links[] = {0, 512, 1536};
   // group size first with the current grids, then the grids with the best
-  // group, then the linking kernel
-  for (int pass = 0; pass < 3; ++pass) {
-    for (int i = 0; i < (pass == 2 ? 3 : 5); ++i) {
+  // group, then the linking kernel, then the compress pass
+  const int trials[] = {5, 5, 3, 2};
+  for (int pass = 0; pass < 4; ++pass) {
+    for (int i = 0; i < trials[pass]; ++i) {
       LaunchCfg trial = best;
       if (pass == 0) trial.group = groups[i];
       else if (pass == 1) trial.blocks_op = trial.blocks_node = cus * mults[i];
-      else trial.link_cap = links[i];
-      cfg = trial;
-      invalidate_graphs();
+      else if (pass == 2) trial.link_cap = links[i];
+      else trial.compress = i;
+      rc = apply_cfg(trial);
+      if (rc) return rc;
       rc = synchronize();
       if (rc) return rc;
       auto t0 = std::chrono::steady_clock::now();
@@ -648,16 +668,15 @@ int Batch::tune(uint32_t sweeps_per_trial) {
       if (t < best_t) { best_t = t; best = trial; }
     }
   }
-  cfg = best;
   tuned = true;
-  invalidate_graphs();
-  return LG_OK;
+  return apply_cfg(best);
 }
 
 void Batch::get_launch(lg_launch* out) const {
   out->group = cfg.group;
   out->blocks = cfg.blocks_op;
   out->link = cfg.link_cap;
+  out->compress = cfg.compress;
 }
 
 int Batch::set_launch(const lg_launch& l) {
@@ -675,10 +694,12 @@ int Batch::set_launch(const lg_launch& l) {
     if (l.link != 0 && l.link != 512 && l.link != 1536) return set_error(LG_ERR_ARG, "launch link must be 0, 512 or 1536");
     c.link_cap = l.link;
   }
-  cfg = c;
+  if (l.compress >= 0) {
+    if (l.compress > 1) return set_error(LG_ERR_ARG, "launch compress must be 0 or 1");
+    c.compress = l.compress;
+  }
   tuned = true;
-  invalidate_graphs();
-  return LG_OK;
+  return apply_cfg(c);
 }
 
 // n sweeps with HIP events between the kernel launches on the batch's stream;

@@ -108,6 +108,8 @@ class Batch {
   int reset_state();
   void release();
   void invalidate_graphs();
+  // switch launch settings (clears the accumulators when leaving compress mode)
+  int apply_cfg(const LaunchCfg& c);
   int capture(int from);
   int check_flags();
   void note_swept();

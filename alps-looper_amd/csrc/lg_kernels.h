@@ -82,6 +82,10 @@ struct LaunchCfg {
   int blocks_op = 1024;        // grid of the per-operator kernels (grid-stride)
   int blocks_node = 1024;      // grid of the per-node kernels (grid-stride)
   int link_cap = 512;          // linking: 0 one thread per operator, else a wave per site staging 512 or 1536 events
+  // roots: 1 a compression pass before the estimator (which then reads roots
+  // and flip bits in one load), 0 the estimator walks to the roots itself and
+  // the collector clears the accumulators it has read
+  int compress = 1;
 };
 
 constexpr int NUM_SWEEP_KERNELS = 11;

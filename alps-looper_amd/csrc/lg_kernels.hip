@@ -15,11 +15,15 @@
 //                segments below and above each leg, segment spins by kink
 //                parity
 //   uf_union     per operator: lock-free union of segments by the cut rules
-//   uf_compress  per node: full compression; per root: clear the accumulator,
-//                draw the cluster's flip bit
-//   estimate     per node: improved-estimator terms into the root, flip bit
-//                copied from the root, flipped segment spins and tau=0 spins
+//   uf_compress  (compress mode) per node: full compression; per root: clear
+//                the accumulator, draw the cluster's flip bit
+//   estimate     per node: the root (compress mode: one load; else a walk that
+//                also points the node at its root) and the cluster's flip bit
+//                (compress mode: the root's; else drawn from the root's Philox
+//                counter), improved-estimator terms into the root, flipped
+//                segment spins and tau=0 spins
 //   flip_collect per operator: type flip; per root: cluster terms -> record
+//                (without compress mode: accumulator cleared for the next sweep)
 //   end_sweep    per chain: finish and store the sweep record
 #include <hip/hip_runtime.h>
 
@@ -775,8 +779,8 @@ __device__ __forceinline__ int ld_parent(int* P, int x) {
 }
 
 // Plain walk to the root: shortening the path with stores while walking cost
-// more write traffic than the longer walks it saves (k_uf_compress flattens
-// every tree once all unions are done).
+// more write traffic than the longer walks it saves (k_estimate points every
+// node at its root once all unions are done).
 __device__ __forceinline__ int uf_find(int* P, int v) {
   int cur = v, next;
   while (cur != (next = ld_parent(P, cur))) cur = next;
@@ -820,6 +824,9 @@ __global__ void __launch_bounds__(256) k_uf_union(DevBatch B) {
   }
 }
 
+// Compress mode: flatten every tree once all unions are done, clear the
+// roots' accumulators and draw each cluster's flip bit at its root, keyed by
+// the chain-local index of the root (the cluster's minimum segment).
 __global__ void __launch_bounds__(256) k_uf_compress(DevBatch B) {
   if (overflowed(B)) return;
   const int M = B.totals[1];
@@ -832,8 +839,6 @@ __global__ void __launch_bounds__(256) k_uf_compress(DevBatch B) {
       while ((q = B.parent[r]) != r) r = q;
       B.parent[i] = r;
     } else {
-      // root: clear its accumulator and draw the cluster's flip bit, keyed by
-      // the chain-local index of the root (the cluster's minimum segment)
       B.acc[i] = ClusterAcc{{0, 0, 0, 0}, {0, 0, 0, 0}, 0, 0};
       const int c = node_chain(B, i);
       const ChainDev& P = B.chains[c];
@@ -844,8 +849,30 @@ __global__ void __launch_bounds__(256) k_uf_compress(DevBatch B) {
   }
 }
 
+// chain of a segment node by binary search over the chains' first nodes
+// (a few cached loads; no per-node table read)
+__device__ __forceinline__ int chain_of_node(const DevBatch& B, int node) {
+  int lo = 0, n = B.nchains;
+  while (n > 1) {
+    const int h = n >> 1;
+    if (B.chain_node0[lo + h] <= node) { lo += h; n -= h; } else { n = h; }
+  }
+  return lo;
+}
+
 // ---------------------------------------------------------------------------
 // (e) improved estimators + (d) flip bits, per segment node.
+// WALK (no compress pass): each node walks to its root (the cluster's minimum
+// segment) and, when the walk took more than one step, points itself at the
+// root, so later kernels find roots in one load.  The cluster's flip bit is
+// drawn from the Philox counter of the root's chain-local index by every
+// segment of the cluster (the counter k_uf_compress uses), so no kernel has
+// to publish it first; with a field the flip waits for the cluster's
+// magnetization (k_field_flip) and the bit is 0 here.  Accumulators then
+// start at zero because k_flip_collect clears each root's after reading it
+// and fresh allocations are cleared.  This saves a pass over the nodes; the
+// clearing costs a store per root, so batches of many small clusters keep
+// the compress pass.
 // A segment adds its length to one class slot of its root (ClusterAcc), its
 // tau = 0 boundary (first segment of a site) to one class counter and its
 // winding contributions to one packed 64-bit counter.  Each block handles
@@ -906,6 +933,7 @@ __device__ __forceinline__ void ht_add(int* hkey, double (*hL)[HT], unsigned int
   }
 }
 
+template <bool WALK>
 __global__ void __launch_bounds__(EST_THREADS) k_estimate(DevBatch B) {
   if (overflowed(B)) return;
   __shared__ int hkey[HT];
@@ -934,11 +962,27 @@ __global__ void __launch_bounds__(EST_THREADS) k_estimate(DevBatch B) {
       double len = 0;
       long long wp = 0;
       if (valid) {
-        r = B.parent[i];  // compressed: the root
+        uint32_t f;
+        if (WALK) {
+          const int p = B.parent[i];
+          r = p;
+          if (p != i) {
+            int q;
+            while ((q = B.parent[r]) != r) r = q;
+            if (r != p) B.parent[i] = r;
+          }
+          const int c = chain_of_node(B, i);
+          const ChainDev& P = B.chains[c];
+          f = P.h != 0.0 ? 0u
+              : (philox(B.sweep[c], STAGE_FLIP, (uint32_t)(r - B.chain_node0[c]), 0u, P.key0, P.key1).x & 1u);
+          B.fbit[i] = (uint8_t)f;
+        } else {
+          r = B.parent[i];  // compressed: the root
+          f = B.fbit[r];    // drawn for the root by k_uf_compress
+          if (i != r) B.fbit[i] = (uint8_t)f;
+        }
         const uint8_t fl = B.evflags[i];
         const int spin = B.segspin_new[i];
-        const uint32_t f = B.fbit[r];  // drawn for the root by k_uf_compress
-        if (i != r) B.fbit[i] = (uint8_t)f;
         const int sig = 1 - 2 * spin;
         first = (fl & EV_FIRST) != 0;
         int dw0 = 0, dw1 = 0;
@@ -998,7 +1042,7 @@ __global__ void __launch_bounds__(EST_THREADS) k_estimate(DevBatch B) {
 // Philox counter as the field-free flip; det_tanh keeps host and device
 // decisions identical.  Then every segment of a field chain takes its
 // root's bit: flipped segment and tau = 0 spins (k_estimate left them
-// unflipped, since compression gave field roots bit 0).
+// unflipped: it gives field clusters bit 0).
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_field_flip(DevBatch B) {
   if (overflowed(B)) return;
@@ -1089,7 +1133,8 @@ __device__ __forceinline__ int term_slot(int q) {
   return q == 0 ? R_NC : (q <= 15 ? R_UMAG0 + (q - 1) : R_UMAGA + (q - 16));
 }
 
-__global__ void __launch_bounds__(COL_THREADS) k_flip_collect(DevBatch B) {
+// clear: zero each root's accumulator after reading it (no compress pass)
+__global__ void __launch_bounds__(COL_THREADS) k_flip_collect(DevBatch B, int clear) {
   if (overflowed(B)) return;
   const int nops = B.totals[0];
   const int M = B.totals[1];
@@ -1139,6 +1184,7 @@ __global__ void __launch_bounds__(COL_THREADS) k_flip_collect(DevBatch B) {
         }
         double v[NCOL];
         cluster_terms(B.acc[i], B.chains[c].h, B.fbit[i], v);
+        if (clear) B.acc[i] = ClusterAcc{{0, 0, 0, 0}, {0, 0, 0, 0}, 0, 0};
 #pragma unroll
         for (int q = 0; q < NCOL; ++q) w[q] += v[q];
       }
@@ -1381,15 +1427,20 @@ void launch_cluster_stage(const DevBatch& B, const LaunchCfg& L, hipStream_t st,
   mark(ev, 1, st);
   hipLaunchKernelGGL(k_uf_union, dim3(L.blocks_op), dim3(256), 0, st, B);
   mark(ev, 2, st);
-  hipLaunchKernelGGL(k_uf_compress, dim3(L.blocks_node), dim3(256), 0, st, B);
-  mark(ev, 3, st);
-  hipLaunchKernelGGL(k_estimate, dim3(L.blocks_node), dim3(EST_THREADS), 0, st, B);
+  if (L.compress) {
+    hipLaunchKernelGGL(k_uf_compress, dim3(L.blocks_node), dim3(256), 0, st, B);
+    mark(ev, 3, st);
+    hipLaunchKernelGGL(k_estimate<false>, dim3(L.blocks_node), dim3(EST_THREADS), 0, st, B);
+  } else {
+    mark(ev, 3, st);
+    hipLaunchKernelGGL(k_estimate<true>, dim3(L.blocks_node), dim3(EST_THREADS), 0, st, B);
+  }
   mark(ev, 4, st);
   if (B.any_field) {
     hipLaunchKernelGGL(k_field_flip, dim3(L.blocks_node), dim3(256), 0, st, B);
     hipLaunchKernelGGL(k_field_apply, dim3(L.blocks_node), dim3(256), 0, st, B);
   }
-  hipLaunchKernelGGL(k_flip_collect, dim3(L.blocks_node), dim3(COL_THREADS), 0, st, B);
+  hipLaunchKernelGGL(k_flip_collect, dim3(L.blocks_node), dim3(COL_THREADS), 0, st, B, L.compress ? 0 : 1);
   mark(ev, 5, st);
   hipLaunchKernelGGL(k_end_sweep, dim3(B.nchains), dim3(END_THREADS), 0, st, B);
   mark(ev, 6, st);

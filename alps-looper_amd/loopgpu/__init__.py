@@ -38,18 +38,19 @@ EXPORTED = [
 
 
 class LGLaunch(ctypes.Structure):
-    _fields_ = [("group", ctypes.c_int32), ("blocks", ctypes.c_int32), ("link", ctypes.c_int32)]
+    _fields_ = [("group", ctypes.c_int32), ("blocks", ctypes.c_int32), ("link", ctypes.c_int32),
+                ("compress", ctypes.c_int32)]
 
 
 def _launch_get(fn, h):
     out = LGLaunch()
     _check(fn(h, ctypes.byref(out)))
-    return {"group": out.group, "blocks": out.blocks, "link": out.link}
+    return {"group": out.group, "blocks": out.blocks, "link": out.link, "compress": out.compress}
 
 
-def _launch_set(fn, h, group, blocks, link):
-    _check(fn(h, ctypes.byref(LGLaunch(-1 if group is None else group, -1 if blocks is None else blocks,
-                                       -1 if link is None else link))))
+def _launch_set(fn, h, group, blocks, link, compress):
+    v = [-1 if x is None else x for x in (group, blocks, link, compress)]
+    _check(fn(h, ctypes.byref(LGLaunch(*v))))
 
 
 class LGParams(ctypes.Structure):
@@ -192,8 +193,8 @@ class Chain:
         """current launch settings (include/loopgpu.h lg_launch)"""
         return _launch_get(lib.lg_chain_get_launch, self.h)
 
-    def set_launch(self, group=None, blocks=None, link=None):
-        _launch_set(lib.lg_chain_set_launch, self.h, group, blocks, link)
+    def set_launch(self, group=None, blocks=None, link=None, compress=None):
+        _launch_set(lib.lg_chain_set_launch, self.h, group, blocks, link, compress)
 
     def info(self):
         ns, nb, no, sw = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int64(), ctypes.c_uint64()
@@ -254,8 +255,8 @@ class Batch:
     def launch(self):
         return _launch_get(lib.lg_batch_get_launch, self.h)
 
-    def set_launch(self, group=None, blocks=None, link=None):
-        _launch_set(lib.lg_batch_set_launch, self.h, group, blocks, link)
+    def set_launch(self, group=None, blocks=None, link=None, compress=None):
+        _launch_set(lib.lg_batch_set_launch, self.h, group, blocks, link, compress)
 
     def set_graph(self, enable):
         _check(lib.lg_batch_set_graph(self.h, 1 if enable else 0))

@@ -39,17 +39,17 @@ HBM_PEAK_GBS = 8000.0
 
 # Algorithmic HBM bytes per operator and sweep of each kernel (compulsory
 # traffic of the SoA layout: 12-byte operators, two segment nodes per
-# operator).  Stage totals: diagonal update 32, linking 64, union-find 96,
-# flip 24, estimators 64 = 280 B per operator-sweep.
+# operator).  Stage totals: diagonal update 32, linking 64, union-find 64,
+# flip 24, estimators 64 = 248 B per operator-sweep.
 BYTES_PER_OP = {
     "diag_count": 12.0, "diag_fill": 20.0,       # diagonal update      32
     "link": 64.0,                                 # world-line linking   64
-    "uf_union": 64.0, "uf_compress": 32.0,        # union-find           96
+    "uf_union": 64.0,                             # union-find           64
     "estimate": 64.0,                             # estimators           64
     "flip_collect": 24.0,                         # flip                 24
     "scan_reduce": 0.0, "scan_top": 0.0, "scan_down": 0.0, "end_sweep": 0.0,
 }
-SWEEP_BYTES_PER_OP = 280.0
+SWEEP_BYTES_PER_OP = 248.0
 
 
 def log(msg):
@@ -146,7 +146,7 @@ def roofline(prof, nop_mean, traffic_profile=None):
 
 
 STAGES = {"diagonal update": ("diag_count", "scan_reduce", "scan_top", "scan_down", "diag_fill"),
-          "linking": ("link",), "union-find": ("uf_union", "uf_compress"),
+          "linking": ("link",), "union-find": ("uf_union",),
           "estimators": ("estimate",), "flip + collect": ("flip_collect", "end_sweep")}
 
 

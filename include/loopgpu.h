@@ -160,15 +160,18 @@ int lg_exchange_state(lg_batch* h, int32_t* chain_at, uint64_t* accepted, uint64
  * group: lanes per bond in the diagonal update (4, 8, 16, 32 or 64); blocks:
  * grid of the per-operator and per-node kernels; link: linking kernel (0 one
  * thread per operator, 512 or 1536 one wave per site staging that many
- * events in LDS).  By default they follow the expected operator counts, and
- * lg_batch_tune times the candidates.  set: fields < 0 keep their value; the
- * settings then stay as given (no automatic choice) until the next tune.
- * Results do not depend on them.  No counterpart in the reference (launch
- * configuration of the device kernels). */
+ * events in LDS); compress: 1 a union-find compression pass before the
+ * estimator, 0 the estimator finds the roots itself.  By default they follow
+ * the expected operator counts and the number of chains, and lg_batch_tune
+ * times the candidates.  set: fields < 0 keep their value; the settings then
+ * stay as given (no automatic choice) until the next tune.  Results do not
+ * depend on them.  No counterpart in the reference (launch configuration of
+ * the device kernels). */
 typedef struct lg_launch {
   int32_t group;
   int32_t blocks;
   int32_t link;
+  int32_t compress;
 } lg_launch;
 int lg_chain_get_launch(lg_chain* h, lg_launch* out);
 int lg_chain_set_launch(lg_chain* h, const lg_launch* l);

@@ -97,7 +97,8 @@ LAUNCH_TRAJ = [
     (O.CHAIN, 8, 600.0, 1.0, 1.0, 12),
 ]
 LAUNCHES = [dict(link=0), dict(link=512), dict(link=1536), dict(link=512, group=4, blocks=7),
-            dict(link=0, group=16, blocks=64), dict(link=1536, group=32)]
+            dict(link=0, group=16, blocks=64), dict(link=1536, group=32),
+            dict(link=512, compress=1), dict(link=0, compress=0, blocks=9), dict(link=1536, compress=1, group=8)]
 
 
 @pytest.mark.parametrize("cfg", LAUNCH_TRAJ, ids=lambda c: "k%d_L%d_b%g" % c[:3])
@@ -123,9 +124,39 @@ def test_launch_settings_leave_trajectories_bit_exact(loopgpu, cfg):
         np.testing.assert_array_equal(gs, cs)
 
 
+def test_compress_mode_switches_keep_trajectory(loopgpu):
+    """switching between the compress pass and the estimator's own root walk
+    between sweeps (accumulators cleared on the switch) keeps the oracle's
+    trajectory, for a chain and for a batch with a field chain"""
+    n, modes = 24, [1, 0, 0, 1, 0, 1]
+    c = O.PhiloxChain(O.SQUARE, 8, 4.0, 1.0, 1.0, seed=8, chain_id=5)
+    rc = c.sweep(n)
+    g = loopgpu.Chain(loopgpu.params(O.SQUARE, 8, 4.0, 1.0, 1.0, seed=8, chain_id=5))
+    assert g.launch()["compress"] == 0          # one chain: no compress pass by default
+    parts = []
+    for m in modes:
+        g.set_launch(compress=m)
+        parts.append(g.sweep(n // len(modes)))
+    _cmp_records(np.concatenate(parts), rc, "chain mode switches")
+    plist = [loopgpu.params(O.CHAIN, 16, 3.0, 1.0, 1.0, seed=4, chain_id=0, h=0.5),
+             loopgpu.params(O.SQUARE, 6, 2.0, 1.0, 0.8, seed=4, chain_id=1),
+             loopgpu.params(O.CHAIN, 10, 1.0, 1.0, 1.2, seed=4, chain_id=3)]
+    b = loopgpu.Batch(plist)
+    assert b.launch()["compress"] == 1          # many chains: compress pass by default
+    parts = []
+    for m in modes:
+        b.set_launch(compress=m)
+        parts.append(b.run(n // len(modes)))
+    rb = np.concatenate(parts)
+    for i, p in enumerate(plist):
+        rs = loopgpu.Chain(p).sweep(n)
+        np.testing.assert_array_equal(rb[:, i, [0, 1]], rs[:, [0, 1]])
+        np.testing.assert_allclose(rb[:, i, FLOAT], rs[:, FLOAT], rtol=1e-9, atol=1e-9)
+
+
 def test_launch_setting_errors(loopgpu):
     g = loopgpu.Chain(loopgpu.params(O.CHAIN, 8, 1.0))
-    for bad in (dict(group=3), dict(link=100), dict(blocks=0)):
+    for bad in (dict(group=3), dict(link=100), dict(blocks=0), dict(compress=2)):
         with pytest.raises(loopgpu.LoopGPUError) as e:
             g.set_launch(**bad)
         assert e.value.code == -1
