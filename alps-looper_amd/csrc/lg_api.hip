@@ -338,7 +338,10 @@ int Batch::allocate() {
   // clears each root's after reading it
   LG_HIP(hipMemset(acc, 0, (size_t)node_cap * sizeof(ClusterAcc)));
   scan_nparts = (cap_bonds + 2047) / 2048 + (cap_sites + 2047) / 2048;
-  if ((rc = alloc(scan_part, scan_nparts))) return rc;
+  if ((rc = alloc(scan_status, scan_nparts))) return rc;
+  if ((rc = alloc(scan_ctl, 2))) return rc;
+  LG_HIP(hipMemset(scan_status, 0, scan_nparts * sizeof(unsigned long long)));
+  LG_HIP(hipMemset(scan_ctl, 0, 2 * sizeof(int)));
   if ((rc = alloc(col_part, (size_t)col_chunks(node_cap) * NUM_COLLECTOR_TERMS))) return rc;
   if ((rc = alloc(totals, 2))) return rc;
   if ((rc = alloc(flags, 1))) return rc;
@@ -540,7 +543,7 @@ DevBatch Batch::dev(int from) const {
   B.cand_t = cand_t; B.cand_g = cand_g; B.kc_new = kc_new;
   B.l0 = l0; B.l1 = l1; B.u0 = u0; B.u1 = u1; B.evflags = evflags; B.chain_node0 = chain_node0; B.node_site = node_site;
   B.parent = parent; B.fbit = fbit; B.acc = acc;
-  B.scan_part = scan_part; B.col_part = col_part; B.totals = totals; B.flags = flags;
+  B.scan_status = scan_status; B.scan_ctl = scan_ctl; B.col_part = col_part; B.totals = totals; B.flags = flags;
   B.sweep = d_sweep; B.rec_acc = rec_acc; B.rec_ring = rec_ring;
   B.ring_pos = ring_pos; B.end_count = end_count; B.ring_size = ring_size;
   B.any_field = 0;

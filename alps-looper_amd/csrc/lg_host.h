@@ -135,7 +135,8 @@ class Batch {
   int* parent = nullptr;
   uint8_t* fbit = nullptr;
   ClusterAcc* acc = nullptr;
-  int* scan_part = nullptr;
+  unsigned long long* scan_status = nullptr;
+  int* scan_ctl = nullptr;
   int scan_nparts = 0;
   double* col_part = nullptr;
   int* totals = nullptr;

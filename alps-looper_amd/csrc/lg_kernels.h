@@ -53,7 +53,8 @@ struct DevBatch {
   int* parent;                 // [nodes]
   uint8_t* fbit;               // [nodes] flip bit of the node's cluster
   ClusterAcc* acc;             // [nodes] (roots only)
-  int* scan_part;
+  unsigned long long* scan_status;  // per scan chunk: epoch | state | value
+  int* scan_ctl;                    // [0] chunk ticket, [1] launch epoch
   double* col_part;            // [col_chunks(node_cap)][16] collector partial sums
   int* totals;                 // [0] operators, [1] nodes of the new configuration
   int* flags;
@@ -88,7 +89,7 @@ struct LaunchCfg {
   int compress = 1;
 };
 
-constexpr int NUM_SWEEP_KERNELS = 11;
+constexpr int NUM_SWEEP_KERNELS = 9;
 constexpr int NUM_COLLECTOR_TERMS = 19;   // per-cluster terms summed into the sweep record
 extern const char* const kKernelNames[NUM_SWEEP_KERNELS];
 

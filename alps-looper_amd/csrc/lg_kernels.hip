@@ -184,11 +184,26 @@ __device__ __forceinline__ int group_spin_prev(const DevBatch& B, int base, int 
   return B.segspin_old[base + (j == m ? 0 : j)];
 }
 
+// Status word of a scan chunk: launch epoch (30 bits), state (1 sum, 2
+// inclusive prefix), value.
+__device__ __forceinline__ unsigned long long scan_word(uint32_t epoch, int state, int v) {
+  return ((unsigned long long)epoch << 34) | ((unsigned long long)state << 32) | (uint32_t)v;
+}
+
+// one thread, in the kernel before a scan: clear the ticket, next epoch
+// (never 0, the value of freshly cleared status words)
+__device__ __forceinline__ void scan_advance_epoch(const DevBatch& B) {
+  B.scan_ctl[0] = 0;
+  uint32_t e = ((uint32_t)B.scan_ctl[1] + 1u) & 0x3fffffffu;
+  B.scan_ctl[1] = (int)(e == 0 ? 1u : e);
+}
+
 // ---------------------------------------------------------------------------
 // (a) diagonal update, pass 1: count and store accepted candidates
 // ---------------------------------------------------------------------------
 template <int G>
 __global__ void __launch_bounds__(256) k_diag_count(DevBatch B) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) scan_advance_epoch(B);  // for k_scan, which follows
   if (overflowed(B)) return;
   const int groups_per_block = blockDim.x / G;
   const int gl = glane<G>();
@@ -310,66 +325,27 @@ __device__ int block_excl_scan(int v, int* total) {
   return off + x - v;
 }
 
-__global__ void __launch_bounds__(SCAN_THREADS) k_scan_reduce(DevBatch B) {
-  if (overflowed(B)) return;
-  const int nbB = (B.nbonds + SCAN_CHUNK - 1) / SCAN_CHUNK;
-  int blk = blockIdx.x, which = 0, n = B.nbonds;
-  if (blk >= nbB) { blk -= nbB; which = 1; n = B.nsites; }
-  int base = blk * SCAN_CHUNK + threadIdx.x * SCAN_ITEMS;
-  int sum = 0;
-  for (int k = 0; k < SCAN_ITEMS; ++k)
-    if (base + k < n) sum += scan_value(B, which, base + k);
-  int tot;
-  block_excl_scan(sum, &tot);
-  if (threadIdx.x == 0) B.scan_part[blockIdx.x] = tot;
-}
+// Single-pass scan with decoupled look-back (Merrill & Garland, "Single-pass
+// parallel prefix scan with decoupled look-back", NVIDIA TR 2016).  Blocks
+// take chunks in the order they start (a ticket), so a block only waits for
+// blocks that are already running.  Chunks [0, nbB) scan the bonds' operator
+// counts, the rest the sites' segment counts; each chunk publishes its sum,
+// then its inclusive prefix, in one 64-bit status word tagged with the
+// launch's epoch (k_diag_count or k_scan_reset advances the epoch and clears
+// the ticket before the scan), so the status words never need clearing.
+__global__ void k_scan_reset(DevBatch B) { scan_advance_epoch(B); }
 
-__global__ void __launch_bounds__(1024) k_scan_top(DevBatch B) {
+__global__ void __launch_bounds__(SCAN_THREADS) k_scan(DevBatch B) {
   if (overflowed(B)) return;
-  __shared__ int carry[2];
+  __shared__ int s_ticket, s_excl;
+  if (threadIdx.x == 0) s_ticket = atomicAdd(B.scan_ctl, 1);
+  __syncthreads();
+  const int t = s_ticket;
+  const uint32_t epoch = (uint32_t)B.scan_ctl[1];
   const int nbB = (B.nbonds + SCAN_CHUNK - 1) / SCAN_CHUNK;
-  const int nbS = (B.nsites + SCAN_CHUNK - 1) / SCAN_CHUNK;
-  for (int which = 0; which < 2; ++which) {
-    int start = which ? nbB : 0, n = which ? nbS : nbB;
-    if (threadIdx.x == 0) carry[0] = 0;
-    __syncthreads();
-    for (int c0 = 0; c0 < n; c0 += 1024) {
-      int i = c0 + threadIdx.x;
-      int v = (i < n) ? B.scan_part[start + i] : 0;
-      // block inclusive scan over 1024 threads
-      __shared__ int ws[16];
-      int lane = lane_id(), w = threadIdx.x >> 6;
-      int x = v;
-      for (int d = 1; d < 64; d <<= 1) {
-        int o = __shfl_up(x, d, 64);
-        if (lane >= d) x += o;
-      }
-      if (lane == 63) ws[w] = x;
-      __syncthreads();
-      int off = 0, tot = 0;
-      for (int k = 0; k < 16; ++k) { if (k < w) off += ws[k]; tot += ws[k]; }
-      int cbase = carry[0];
-      if (i < n) B.scan_part[start + i] = cbase + off + x - v;
-      __syncthreads();
-      if (threadIdx.x == 0) carry[0] = cbase + tot;
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-      int total = carry[0];
-      B.totals[which] = total;
-      int cap = which ? B.node_cap : B.op_cap;
-      if (total > cap) atomicOr(B.flags, which ? FLAG_NODE_OVERFLOW : FLAG_OP_OVERFLOW);
-    }
-    __syncthreads();
-  }
-}
-
-__global__ void __launch_bounds__(SCAN_THREADS) k_scan_down(DevBatch B) {
-  if (overflowed(B)) return;
-  const int nbB = (B.nbonds + SCAN_CHUNK - 1) / SCAN_CHUNK;
-  int blk = blockIdx.x, which = 0, n = B.nbonds;
-  if (blk >= nbB) { blk -= nbB; which = 1; n = B.nsites; }
-  int base = blk * SCAN_CHUNK + threadIdx.x * SCAN_ITEMS;
+  int blk = t, which = 0, n = B.nbonds, first = 0;
+  if (blk >= nbB) { blk -= nbB; which = 1; n = B.nsites; first = nbB; }
+  const int base = blk * SCAN_CHUNK + threadIdx.x * SCAN_ITEMS;
   int vals[SCAN_ITEMS];
   bool empty[SCAN_ITEMS];
   int sum = 0;
@@ -390,7 +366,42 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan_down(DevBatch B) {
     sum += v;
   }
   int tot;
-  int off = block_excl_scan(sum, &tot) + B.scan_part[blockIdx.x];
+  const int loc = block_excl_scan(sum, &tot);
+  if (threadIdx.x < 64) {  // the first wave looks back, 64 predecessors at a time
+    const int lane = threadIdx.x;
+    int excl = 0;
+    unsigned long long* st = B.scan_status;
+    if (t == first) {
+      if (lane == 0) __hip_atomic_store(st + t, scan_word(epoch, 2, tot), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      if (lane == 0) __hip_atomic_store(st + t, scan_word(epoch, 1, tot), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      // every predecessor down to `first` holds a lower ticket, so it is
+      // running and publishes its sum without waiting; `first` publishes its
+      // inclusive prefix, so the window holding it ends the walk
+      for (int top = t - 1;; top -= 64) {
+        const int p = top - lane;
+        const bool valid = p >= first;
+        unsigned long long w = 0;
+        if (valid) {
+          while ((uint32_t)((w = __hip_atomic_load(st + p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) >> 34) != epoch)
+            __builtin_amdgcn_s_sleep(1);
+        }
+        const unsigned long long incl = __ballot(valid && ((w >> 32) & 3) == 2);
+        // nearest inclusive prefix: lanes up to it contribute, else the window
+        const int k = incl ? __ffsll((long long)incl) - 1 : 63;
+        int v = (valid && lane <= k) ? (int)(uint32_t)w : 0;
+#pragma unroll
+        for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d, 64);
+        excl += v;
+        if (incl) break;
+      }
+      if (lane == 0) __hip_atomic_store(st + t, scan_word(epoch, 2, excl + tot), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if (lane == 0) s_excl = excl;
+  }
+  __syncthreads();
+  const int cap = which ? B.node_cap : B.op_cap;
+  int off = loc + s_excl;
   int* out = which ? B.soff_new : B.bo_new;
 #pragma unroll
   for (int k = 0; k < SCAN_ITEMS; ++k) {
@@ -401,7 +412,7 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan_down(DevBatch B) {
         const int c = B.site_chain[i];
         const ChainDev& P = B.chains[c];
         if (i == P.site_base) B.chain_node0[c] = off;
-        if (empty[k]) {
+        if (empty[k] && off < cap) {
           // event-free site: one node spanning [0,1)
           int node = off;
           B.evt_new[node] = EMPTY_TIME;
@@ -414,7 +425,12 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan_down(DevBatch B) {
     }
     off += vals[k];
   }
-  if (blk == (n - 1) / SCAN_CHUNK && threadIdx.x == 0) out[n] = B.totals[which];
+  if (blk == (n - 1) / SCAN_CHUNK && threadIdx.x == 0) {
+    const int total = s_excl + tot;
+    B.totals[which] = total;
+    out[n] = total;
+    if (total > cap) atomicOr(B.flags, which ? FLAG_NODE_OVERFLOW : FLAG_OP_OVERFLOW);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1387,7 +1403,7 @@ int scan_blocks(const DevBatch& B) {
 }
 
 const char* const kKernelNames[NUM_SWEEP_KERNELS] = {
-    "diag_count", "scan_reduce", "scan_top", "scan_down", "diag_fill", "link",
+    "diag_count", "scan", "diag_fill", "link",
     "uf_union", "uf_compress", "estimate", "flip_collect", "end_sweep"};
 
 // ev (optional): NUM_SWEEP_KERNELS + 1 events recorded around the launches
@@ -1398,17 +1414,13 @@ static inline void mark(hipEvent_t* ev, int i, hipStream_t st) {
 void launch_sweep(const DevBatch& B, const LaunchCfg& L, hipStream_t st, hipEvent_t* ev) {
   const int sb = scan_blocks(B);
   mark(ev, 0, st);
-  diag_dispatch(B, L, st, false);
+  diag_dispatch(B, L, st, false);  // also clears the scan's ticket, next epoch
   mark(ev, 1, st);
-  hipLaunchKernelGGL(k_scan_reduce, dim3(sb), dim3(SCAN_THREADS), 0, st, B);
+  hipLaunchKernelGGL(k_scan, dim3(sb), dim3(SCAN_THREADS), 0, st, B);
   mark(ev, 2, st);
-  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, st, B);
-  mark(ev, 3, st);
-  hipLaunchKernelGGL(k_scan_down, dim3(sb), dim3(SCAN_THREADS), 0, st, B);
-  mark(ev, 4, st);
   diag_dispatch(B, L, st, true);
-  mark(ev, 5, st);
-  launch_cluster_stage(B, L, st, ev ? ev + 5 : nullptr);
+  mark(ev, 3, st);
+  launch_cluster_stage(B, L, st, ev ? ev + 3 : nullptr);
 }
 
 void launch_cluster_stage(const DevBatch& B, const LaunchCfg& L, hipStream_t st, hipEvent_t* ev) {
@@ -1449,9 +1461,8 @@ void launch_cluster_stage(const DevBatch& B, const LaunchCfg& L, hipStream_t st,
 
 void launch_site_scan(const DevBatch& B, hipStream_t st) {
   const int sb = scan_blocks(B);
-  hipLaunchKernelGGL(k_scan_reduce, dim3(sb), dim3(SCAN_THREADS), 0, st, B);
-  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, st, B);
-  hipLaunchKernelGGL(k_scan_down, dim3(sb), dim3(SCAN_THREADS), 0, st, B);
+  hipLaunchKernelGGL(k_scan_reset, dim3(1), dim3(1), 0, st, B);
+  hipLaunchKernelGGL(k_scan, dim3(sb), dim3(SCAN_THREADS), 0, st, B);
 }
 
 void launch_canon(const DevBatch& B, int chain, int nelem, int* canon, int* labels, int* roots,

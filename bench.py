@@ -47,7 +47,7 @@ BYTES_PER_OP = {
     "uf_union": 64.0,                             # union-find           64
     "estimate": 64.0,                             # estimators           64
     "flip_collect": 24.0,                         # flip                 24
-    "scan_reduce": 0.0, "scan_top": 0.0, "scan_down": 0.0, "end_sweep": 0.0,
+    "scan": 0.0, "end_sweep": 0.0,
 }
 SWEEP_BYTES_PER_OP = 248.0
 
@@ -145,7 +145,7 @@ def roofline(prof, nop_mean, traffic_profile=None):
     }
 
 
-STAGES = {"diagonal update": ("diag_count", "scan_reduce", "scan_top", "scan_down", "diag_fill"),
+STAGES = {"diagonal update": ("diag_count", "scan", "diag_fill"),
           "linking": ("link",), "union-find": ("uf_union",),
           "estimators": ("estimate",), "flip + collect": ("flip_collect", "end_sweep")}
 
