@@ -6,7 +6,9 @@
 //    subset this path supports): LATTICE, L, W, T / TEMPERATURE / BETA /
 //    INVERSE_TEMPERATURE (temperature.h:43-79), J / Jxy / Jz, MODEL, local_S,
 //    SWEEPS (default "[65536:]") and THERMALIZATION (default SWEEPS.min/8,
-//    montecarlo.h:41-45), SEED, FORCE_SCATTER (weight_impl.h:169);
+//    montecarlo.h:41-45), SEED, FORCE_SCATTER (weight_impl.h:169), h,
+//    DISABLE_IMPROVED_ESTIMATOR, MEASURE[Correlations] / MEASURE[Structure
+//    Factor] / INITIAL_SITE (correlation.h:74-108);
 //  * init_observables / run / is_thermalized / progress / save / load with the
 //    worker's observable names (path_integral.C:307-321,831-862, energy.h,
 //    susceptibility.h, stiffness.h) and its evaluator (Specific Heat,
@@ -31,6 +33,7 @@
 
 #include "../../include/loopgpu.h"
 #include "../../include/loopgpu_worker.h"
+#include "correlation.h"
 #include "lg_binning.h"
 
 namespace looper_amd {
@@ -216,6 +219,24 @@ class loop_worker {
     therm_ = p.defined("THERMALIZATION") ? (uint64_t)std::llround(p.num("THERMALIZATION")) : (sweeps_min_ >> 3);
     n_sites_ = (lp_.lattice == LG_LATTICE_CHAIN) ? lp_.L : lp_.L * lp_.L;
     dim_ = (lp_.lattice == LG_LATTICE_CHAIN) ? 1 : 2;
+    // correlation.h:74-108; the Green function is disabled there as well
+    auto flag = [&](const char* k) {
+      if (!p.defined(k)) return false;
+      const std::string v = p.str(k);
+      return v == "true" || v == "True" || v == "1" || (v != "false" && v != "False" && p.num(k) != 0.0);
+    };
+    if (flag("MEASURE[Green Function]"))
+      std::fprintf(stderr, "WARNING: Green funciton measurement is not implemented yet\n");
+    const bool mc = flag("MEASURE[Correlations]"), ms = flag("MEASURE[Structure Factor]");
+    if (mc || ms) {
+      if (lp_.L < 2 || lp_.L % 2) throw std::invalid_argument("L must be even");
+      const int origin = p.defined("INITIAL_SITE") ? (int)std::llround(p.num("INITIAL_SITE")) : -1;
+      if (p.defined("INITIAL_SITE") && (origin < 0 || origin >= n_sites_))
+        throw std::invalid_argument("INITIAL_SITE out of range");
+      corr_.reset(new Correlation(lp_.lattice, lp_.L, mc, ms, origin));
+      spins_.resize(n_sites_);
+      spins8_.resize(n_sites_);
+    }
     int rc = lg_chain_create(&lp_, &h_);
     if (rc != LG_OK) {
       char buf[512];
@@ -231,6 +252,8 @@ class loop_worker {
   // path_integral.C:307-321 + estimator init_observables
   void init_observables(std::map<std::string, Observable>& obs) const {
     for (const auto& n : names(improved_)) obs[n];
+    if (corr_)
+      for (const auto& n : corr_->names()) obs[n];
   }
   bool is_thermalized() const { return mcs_ >= therm_; }
   double progress() const { return (double)mcs_ / (double)(therm_ + sweeps_min_); }
@@ -246,6 +269,8 @@ class loop_worker {
   void run(std::map<std::string, Observable>& obs) {
     if (next_ == buf_.size()) {
       uint64_t left = (mcs_ < therm_) ? therm_ - mcs_ : (therm_ + sweeps_max_ - mcs_);
+      // correlations are measured on every sweep's spins: one sweep at a time
+      if (corr_ && is_thermalized()) left = 1;
       uint32_t k = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(left, chunk_));
       buf_.resize(k);
       int rc = lg_chain_sweep(h_, k, buf_.data());
@@ -257,7 +282,20 @@ class loop_worker {
       next_ = 0;
     }
     const lg_sweep_record& r = buf_[next_++];
-    if (is_thermalized()) measure(r, obs);
+    if (is_thermalized()) {
+      measure(r, obs);
+      if (corr_) {
+        // tau = 0 spins after this sweep's flip (the chunk held this sweep only)
+        int rc = lg_chain_get_config(h_, spins_.data(), nullptr, nullptr, nullptr);
+        if (rc != LG_OK) {
+          char b[512];
+          lg_last_error(b, sizeof b);
+          throw std::runtime_error(b);
+        }
+        for (int i = 0; i < n_sites_; ++i) spins8_[i] = (uint8_t)spins_[i];
+        corr_->measure(spins8_.data(), obs);
+      }
+    }
     ++mcs_;
   }
 
@@ -408,6 +446,9 @@ class loop_worker {
   std::vector<lg_sweep_record> buf_;
   size_t next_ = 0;
   uint32_t chunk_ = 1024;
+  std::unique_ptr<Correlation> corr_;
+  std::vector<int32_t> spins_;
+  std::vector<uint8_t> spins8_;
 };
 
 // evaluator (energy.h:89-102, susceptibility.h:340-374)

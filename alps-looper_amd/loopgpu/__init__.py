@@ -112,6 +112,7 @@ def _load():
     lib.lg_worker_load.argtypes = [vp, vp, ctypes.c_size_t]
     lib.lg_worker_set_chunk.argtypes = [vp, ctypes.c_uint32]
     lib.lg_worker_destroy.argtypes = [vp]
+    lib.lg_spin_correlations.argtypes = [ctypes.c_int32, ctypes.c_int32, vp, ctypes.c_int32, vp, vp, vp]
     return lib
 
 
@@ -429,7 +430,23 @@ _server_bind()
 EXPORTED += ["lg_server_last_error", "lg_server_start", "lg_server_submit", "lg_server_poll", "lg_server_wait",
              "lg_server_cancel", "lg_server_stats_get", "lg_server_stop",
              "lg_worker_last_error", "lg_worker_create", "lg_worker_run", "lg_worker_status", "lg_worker_result",
-             "lg_worker_report", "lg_worker_save", "lg_worker_load", "lg_worker_set_chunk", "lg_worker_destroy"]
+             "lg_worker_report", "lg_worker_save", "lg_worker_load", "lg_worker_set_chunk", "lg_worker_destroy",
+             "lg_spin_correlations"]
+
+
+def spin_correlations(kind, L, spins, origin=-1):
+    """Correlation estimators of one tau = 0 configuration (include/loopgpu_worker.h
+    lg_spin_correlations): (Spin Correlations, Staggered Spin Correlations,
+    Spin Structure Factor), N values each."""
+    sp = np.ascontiguousarray(spins, dtype=np.uint8)
+    n = L if kind == CHAIN else L * L
+    if sp.size != n:
+        raise LoopGPUError(-1, "spins must hold %d sites" % n)
+    u, s, f = np.zeros(n), np.zeros(n), np.zeros(n)
+    rc = lib.lg_spin_correlations(kind, L, sp.ctypes.data, origin, u.ctypes.data, s.ctypes.data, f.ctypes.data)
+    if rc != 0:
+        raise LoopGPUError(rc, "lg_spin_correlations: invalid arguments")
+    return u, s, f
 
 
 def _scheck(rc):

@@ -17,7 +17,9 @@
  * ("chain lattice" | "square lattice"), L, W, T | TEMPERATURE | BETA |
  * INVERSE_TEMPERATURE, J, Jxy, Jz, MODEL (= spin), local_S (= 1/2), SWEEPS
  * (default "[65536:]"), THERMALIZATION (default SWEEPS.min/8), SEED,
- * FORCE_SCATTER, CHAIN_ID.  Invalid or unsupported parameters give
+ * FORCE_SCATTER, CHAIN_ID, h, DISABLE_IMPROVED_ESTIMATOR,
+ * MEASURE[Correlations], MEASURE[Structure Factor], INITIAL_SITE
+ * (correlation.h:59-109).  Invalid or unsupported parameters give
  * LG_ERR_ARG (the reference throws std::invalid_argument).
  */
 #ifndef LOOPGPU_WORKER_H
@@ -50,9 +52,23 @@ int lg_worker_report(lg_worker* h, char* buf, size_t len, size_t* needed);
 /* checkpoint blob: MC counters, spins, operators (types and times) */
 int lg_worker_save(lg_worker* h, void* buf, size_t* len);
 int lg_worker_load(lg_worker* h, const void* buf, size_t len);
-/* sweeps fetched from the device per batch of run() calls (default 1024) */
+/* sweeps fetched from the device per batch of run() calls (default 1024;
+ * 1 while correlations are measured, which need every sweep's spins) */
 int lg_worker_set_chunk(lg_worker* h, uint32_t sweeps);
 int lg_worker_destroy(lg_worker* h);
+
+/* The correlation estimators of one tau = 0 configuration (reference:
+ * looper::correlation::normal_estimator::collector::commit,
+ * correlation.h:230-296), as the worker measures them with
+ * MEASURE[Correlations] / MEASURE[Structure Factor] (and INITIAL_SITE):
+ * spins[N] (0 = up) of an even-L chain (N = L) or square (N = L^2, site
+ * x*L + y) lattice; origin < 0 for all pairs.  Outputs N values each:
+ * ucorr / scorr ("Spin Correlations" / "Staggered Spin Correlations") by
+ * displacement index (dx*L + dy), or by site with an origin; sfac ("Spin
+ * Structure Factor") by momentum index (n1*L + n2, k = 2 pi n / L folded into
+ * (-pi, pi]).  Host computation, no device needed. */
+int lg_spin_correlations(int32_t lattice, int32_t L, const uint8_t* spins, int32_t origin, double* ucorr,
+                         double* scorr, double* sfac);
 
 #ifdef __cplusplus
 }

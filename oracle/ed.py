@@ -126,6 +126,43 @@ def diagonalize(kind, L, T, jxy=1.0, jz=1.0, h=0.0):
     res["|Staggered Magnetization|"] = sa
     res["Staggered Susceptibility"] = kubo(lambda mu, ms: ms)
     res["Staggered Structure Factor"] = s2 / V
+    # equal-time correlations <Sz_i Sz_j> (diagonal in the basis): basis-state
+    # probabilities of each sector times products of the site spins
+    C = np.zeros((N, N))
+    for (E, U, mu, ms), nup in zip(sectors, range(N + 1)):
+        w = np.exp(-beta * (E - e0))
+        pb = (U * U) @ w
+        states = _sector_states(N, nup)
+        sz = np.array([[0.5 if (int(st) >> i) & 1 else -0.5 for i in range(N)] for st in states]).reshape(len(states), N)
+        C += sz.T @ (pb[:, None] * sz)
+    C /= Z
+    nx = 1 if kind == "chain" else L
+    ny = L
+    corr = np.zeros(N)
+    scorr = np.zeros(N)
+    for i in range(N):
+        xi, yi = divmod(i, ny)
+        for j in range(N):
+            xj, yj = divmod(j, ny)
+            d = ((xj - xi) % nx) * ny + (yj - yi) % ny
+            corr[d] += C[i, j] / N
+            scorr[d] += gauge[i] * gauge[j] * C[i, j] / N
+    res["Spin Correlations"] = corr
+    res["Staggered Spin Correlations"] = scorr
+    res["Spin Correlations Matrix"] = C
+    sfac = np.zeros(N)
+    for a in range(nx):
+        for b in range(ny):
+            k1 = 2 * math.pi * a / nx
+            k2 = 2 * math.pi * b / ny
+            val = 0.0
+            for i in range(N):
+                xi, yi = divmod(i, ny)
+                for j in range(N):
+                    xj, yj = divmod(j, ny)
+                    val += C[i, j] * math.cos(k1 * (xi - xj) + k2 * (yi - yj))
+            sfac[a * ny + b] = val / N
+    res["Spin Structure Factor"] = sfac
     return res
 
 

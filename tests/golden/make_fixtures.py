@@ -9,7 +9,10 @@ Sources (data files of the reference's tests):
   test/spinmodel_helper.op      chain lattice bond list (ALPS conventions)
   test/virtual_lattice.op       square lattice L=2 vertices, edges, gauge
   standalone/loop.op            standalone kernel output (L=8, T=0.2)
-  loop.op                       end-to-end run: ED and path-integral QMC, chain L=4, T=0.25
+  loop.op                       end-to-end run: ED and path-integral QMC, chain L=4, T=0.25;
+                                and the path-integral run with MEASURE[Correlations] and
+                                MEASURE[Structure Factor] (chain L=4, S=1/2, T=0.5): its
+                                scalar results and vector entries ("Name[label]")
 """
 
 import json
@@ -135,6 +138,45 @@ def loop_op():
     return keep
 
 
+def loop_op_correlations():
+    """the "loop; path integral" block of loop.op with MEASURE[Correlations]
+    and local_S = 1/2: scalar results and vector entries keyed Name[label]"""
+    lines = read("loop.op")
+    blocks = []
+    cur = None
+    vec = None
+    for l in lines:
+        if l.startswith("[input parameters]"):
+            cur = {"params": {}, "results": {}}
+            blocks.append(cur)
+            vec = None
+            continue
+        if cur is None:
+            continue
+        m = re.match(r"(\S[^=]*?) = (.*);$", l)
+        if m and not cur["results"] and vec is None:
+            cur["params"][m.group(1).strip()] = m.group(2).strip().strip('"')
+            continue
+        m = re.match(r"Entry\[(.+)\]: (\S+) \+/- (\S+?);", l)
+        if m and vec is not None:
+            cur["results"]["%s[%s]" % (vec, m.group(1))] = [float(m.group(2)), float(m.group(3))]
+            continue
+        if l and not l.startswith(" ") and ":" not in l and "=" not in l:
+            vec = l.strip()  # name line of a vector observable
+            continue
+        m = re.match(r"(.+?): (\S+) \+/- (\S+)", l)
+        if m and not l.startswith(" "):
+            vec = None
+            err = m.group(3).rstrip(";")
+            cur["results"][m.group(1)] = [float(m.group(2)), float("inf") if err == "inf" else float(err)]
+    for b in blocks:
+        p = b["params"]
+        if (p.get("ALGORITHM") == "loop; path integral" and p.get("MEASURE[Correlations]") == "true"
+                and p.get("local_S") == "1/2"):
+            return {"params": p, "results": b["results"]}
+    raise RuntimeError("correlation block not found in loop.op")
+
+
 def main():
     fx = {
         "union_find": union_find(),
@@ -143,6 +185,7 @@ def main():
         "lattices": lattices(),
         "standalone_loop_op": standalone(),
         "loop_op_chain4": loop_op(),
+        "loop_op_chain4_correlations": loop_op_correlations(),
     }
     with open(os.path.join(OUT, "reference_fixtures.json"), "w") as f:
         json.dump(fx, f, indent=1, sort_keys=True)
