@@ -900,6 +900,7 @@ __device__ __forceinline__ int chain_of_node(const DevBatch& B, int node) {
 // ---------------------------------------------------------------------------
 constexpr int EST_THREADS = 256;
 constexpr int EST_ITEMS = 16;
+constexpr int EST_U = 4;         // items per group (EST_ITEMS / EST_U groups)
 constexpr int EST_CHUNK = EST_THREADS * EST_ITEMS;
 constexpr int HT_BITS = 9;
 constexpr int HT = 1 << HT_BITS;
@@ -969,63 +970,98 @@ __global__ void __launch_bounds__(EST_THREADS) k_estimate(DevBatch B) {
   int clo, chi;
   xcd_share(nchunks, clo, chi);
   for (int chunk = clo; chunk < chi; ++chunk) {
-    for (int it = 0; it < EST_ITEMS; ++it) {
-      const int i = chunk * EST_CHUNK + it * EST_THREADS + tid;
-      if (__builtin_amdgcn_readfirstlane(i) >= M) break;  // lane 0 holds the wave's lowest node
-      const bool valid = i < M;
-      int r = -1, cls = 0;
-      bool first = false;
-      double len = 0;
-      long long wp = 0;
-      if (valid) {
-        uint32_t f;
-        if (WALK) {
-          const int p = B.parent[i];
-          r = p;
-          if (p != i) {
-            int q;
-            while ((q = B.parent[r]) != r) r = q;
-            if (r != p) B.parent[i] = r;
-          }
+    // the thread's items in groups of EST_U: every memory step of a group
+    // (parents, the walks to the roots, the segments' own data, the events
+    // below them) is issued for all its items before any is waited for
+    for (int g0 = 0; g0 < EST_ITEMS; g0 += EST_U) {
+      const int ib = chunk * EST_CHUNK + g0 * EST_THREADS + tid;
+      if (__builtin_amdgcn_readfirstlane(ib) >= M) break;  // lane 0 holds the wave's lowest node
+      int r[EST_U];
+      uint32_t f[EST_U];
+#pragma unroll
+      for (int u = 0; u < EST_U; ++u) {
+        const int i = ib + u * EST_THREADS;
+        r[u] = i < M ? B.parent[i] : -1;
+      }
+      if (WALK) {
+        // walks to the roots, one hop of every item at a time
+        int p0[EST_U];
+#pragma unroll
+        for (int u = 0; u < EST_U; ++u) p0[u] = r[u];
+        for (bool more = true; more;) {
+          int q[EST_U];
+#pragma unroll
+          for (int u = 0; u < EST_U; ++u) q[u] = r[u] >= 0 ? B.parent[r[u]] : r[u];
+          more = false;
+#pragma unroll
+          for (int u = 0; u < EST_U; ++u)
+            if (q[u] != r[u]) { r[u] = q[u]; more = true; }
+        }
+#pragma unroll
+        for (int u = 0; u < EST_U; ++u) {
+          const int i = ib + u * EST_THREADS;
+          if (i >= M) continue;
+          if (r[u] != p0[u]) B.parent[i] = r[u];
           const int c = chain_of_node(B, i);
           const ChainDev& P = B.chains[c];
-          f = P.h != 0.0 ? 0u
-              : (philox(B.sweep[c], STAGE_FLIP, (uint32_t)(r - B.chain_node0[c]), 0u, P.key0, P.key1).x & 1u);
-          B.fbit[i] = (uint8_t)f;
-        } else {
-          r = B.parent[i];  // compressed: the root
-          f = B.fbit[r];    // drawn for the root by k_uf_compress
-          if (i != r) B.fbit[i] = (uint8_t)f;
+          f[u] = P.h != 0.0 ? 0u
+               : (philox(B.sweep[c], STAGE_FLIP, (uint32_t)(r[u] - B.chain_node0[c]), 0u, P.key0, P.key1).x & 1u);
+          B.fbit[i] = (uint8_t)f[u];
         }
-        const uint8_t fl = B.evflags[i];
-        const int spin = B.segspin_new[i];
+      } else {
+#pragma unroll
+        for (int u = 0; u < EST_U; ++u) f[u] = r[u] >= 0 ? B.fbit[r[u]] : 0u;  // drawn for the root by k_uf_compress
+#pragma unroll
+        for (int u = 0; u < EST_U; ++u) {
+          const int i = ib + u * EST_THREADS;
+          if (i < M && i != r[u]) B.fbit[i] = (uint8_t)f[u];
+        }
+      }
+      uint8_t fl[EST_U], sp[EST_U], flo[EST_U];
+      double thi[EST_U], tlo[EST_U];
+      int gs[EST_U], prev[EST_U];
+#pragma unroll
+      for (int u = 0; u < EST_U; ++u) {
+        const int i = ib + u * EST_THREADS;
+        fl[u] = i < M ? B.evflags[i] : (uint8_t)EV_EMPTY;
+        sp[u] = i < M ? B.segspin_new[i] : 0;
+        thi[u] = i < M ? B.evt_new[i] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < EST_U; ++u) {
+        const int i = ib + u * EST_THREADS;
+        gs[u] = (i < M && (fl[u] & EV_FIRST)) ? first_node_site(B, i) : 0;
+        // the segment below the site's first event wraps through tau = 1
+        prev[u] = (fl[u] & EV_FIRST) ? B.soff_new[gs[u] + 1] - 1 : i - 1;
+      }
+#pragma unroll
+      for (int u = 0; u < EST_U; ++u) {
+        const bool need = !(fl[u] & EV_EMPTY);
+        tlo[u] = need ? B.evt_new[prev[u]] : 0.0;
+        flo[u] = need ? B.evflags[prev[u]] : (uint8_t)0;
+      }
+#pragma unroll
+      for (int u = 0; u < EST_U; ++u) {
+        const int i = ib + u * EST_THREADS;
+        if (i >= M) continue;
+        const int spin = sp[u];
         const int sig = 1 - 2 * spin;
-        first = (fl & EV_FIRST) != 0;
+        const bool first = (fl[u] & EV_FIRST) != 0;
         int dw0 = 0, dw1 = 0;
-        int gs = 0;
-        if (first) gs = first_node_site(B, i);
-        if (fl & EV_EMPTY) {
-          len = 1.0;
-        } else {
-          // the segment below the site's first event wraps through tau = 1
-          const int prev = first ? B.soff_new[gs + 1] - 1 : i - 1;
-          const double thi = B.evt_new[i];
-          len = first ? ((thi + 1.0) - B.evt_new[prev]) : (thi - B.evt_new[prev]);
-          const uint8_t flo = B.evflags[prev];
+        double len = 1.0;
+        if (!(fl[u] & EV_EMPTY)) {
+          len = first ? ((thi[u] + 1.0) - tlo[u]) : (thi[u] - tlo[u]);
           // this segment is the lower leg of the event above it and the upper
           // leg of the event below it (stiffness.h:94-105, source legs only)
-          if (fl & EV_SRC) { if (fl & EV_DIR1) dw1 += sig; else dw0 += sig; }
-          if (flo & EV_SRC) { if (flo & EV_DIR1) dw1 -= sig; else dw0 -= sig; }
+          if (fl[u] & EV_SRC) { if (fl[u] & EV_DIR1) dw1 += sig; else dw0 += sig; }
+          if (flo[u] & EV_SRC) { if (flo[u] & EV_DIR1) dw1 -= sig; else dw0 -= sig; }
         }
-        if (first) B.spins_new[gs] = (uint8_t)(spin ^ f);
-        B.segspin_new[i] = (uint8_t)(spin ^ f);
-        cls = (spin << 1) | ((fl & EV_GAUGE) ? 1 : 0);  // seg_class
-        wp = pack_winding(dw0, dw1);
-      }
-      if (valid) {
+        if (first) B.spins_new[gs[u]] = (uint8_t)(spin ^ f[u]);
+        B.segspin_new[i] = (uint8_t)(spin ^ f[u]);
+        const int cls = (spin << 1) | ((fl[u] & EV_GAUGE) ? 1 : 0);  // seg_class
         double sL[4] = {0, 0, 0, 0};
         sL[cls] = len;
-        ht_add(hkey, hL, hn, hw, B, r, sL, first ? (1 << (8 * cls)) : 0, wp);
+        ht_add(hkey, hL, hn, hw, B, r[u], sL, first ? (1 << (8 * cls)) : 0, pack_winding(dw0, dw1));
       }
     }
     __syncthreads();
