@@ -413,10 +413,11 @@ int Batch::set_layout(std::vector<ChainDev>& ds, const std::vector<uint64_t>& sw
   if (!tuned) {
     const double per_site = nsites > 0 ? ev / nsites : 0;
     c.link_cap = per_site < 96 ? 0 : (per_site <= 400 ? 512 : 1536);
-    // one chain: the estimator finds the roots itself (a pass less); many
-    // chains, mostly small and hot with many small clusters: clearing every
-    // root's accumulator costs more than the compress pass
-    c.compress = chains.size() > 1 ? 1 : 0;
+    // the estimator finds the roots itself (a pass over the segments less,
+    // but the collector then clears every root's accumulator) for one large
+    // chain; the compress pass for many chains (mostly small and hot, many
+    // small clusters) and for small configurations, where it measured faster
+    c.compress = (chains.size() > 1 || ev < 7.0e6) ? 1 : 0;
   }
   return apply_cfg(c);
 }

@@ -132,7 +132,7 @@ def test_compress_mode_switches_keep_trajectory(loopgpu):
     c = O.PhiloxChain(O.SQUARE, 8, 4.0, 1.0, 1.0, seed=8, chain_id=5)
     rc = c.sweep(n)
     g = loopgpu.Chain(loopgpu.params(O.SQUARE, 8, 4.0, 1.0, 1.0, seed=8, chain_id=5))
-    assert g.launch()["compress"] == 0          # one chain: no compress pass by default
+    assert g.launch()["compress"] == 1          # a small configuration: compress pass by default
     parts = []
     for m in modes:
         g.set_launch(compress=m)
