@@ -37,19 +37,20 @@ sys.path.insert(0, os.path.join(ROOT, "alps-looper_amd"))
 METRIC = "spin-operator updates/s per GPU, batched vs plain; p50/p99 scan-job latency"
 HBM_PEAK_GBS = 8000.0
 
-# Algorithmic HBM bytes per operator and sweep of each kernel (compulsory
-# traffic of the SoA layout: 12-byte operators, two segment nodes per
-# operator).  Stage totals: diagonal update 32, linking 64, union-find 64,
-# flip 24, estimators 64 = 248 B per operator-sweep.
+# Algorithmic HBM bytes per operator and sweep of each kernel (the stage
+# budget of the SoA layout: 12-byte operators, two segment nodes per
+# operator).  Stage totals: diagonal update 32, linking 64, union-find 96
+# (hooking 64, compression 32, done by uf_compress or inside estimate),
+# flip 24, estimators 64 = 280 B per operator-sweep.
 BYTES_PER_OP = {
     "diag_count": 12.0, "diag_fill": 20.0,       # diagonal update      32
     "link": 64.0,                                 # world-line linking   64
-    "uf_union": 64.0,                             # union-find           64
+    "uf_union": 64.0, "uf_compress": 32.0,        # union-find           96
     "estimate": 64.0,                             # estimators           64
     "flip_collect": 24.0,                         # flip                 24
     "scan": 0.0, "end_sweep": 0.0,
 }
-SWEEP_BYTES_PER_OP = 248.0
+SWEEP_BYTES_PER_OP = 280.0
 
 
 def log(msg):
@@ -146,7 +147,7 @@ def roofline(prof, nop_mean, traffic_profile=None):
 
 
 STAGES = {"diagonal update": ("diag_count", "scan", "diag_fill"),
-          "linking": ("link",), "union-find": ("uf_union",),
+          "linking": ("link",), "union-find": ("uf_union", "uf_compress"),
           "estimators": ("estimate",), "flip + collect": ("flip_collect", "end_sweep")}
 
 
