@@ -550,32 +550,37 @@ __device__ __forceinline__ void lds_unite(int* P, int a, int b) {
   }
 }
 
-// One block step handles WAVES consecutive sites.  Operators of a bond whose
-// two sites are both in the step ("inner" operators: neighbours along the
-// fast lattice direction) are united here in LDS, and each segment's parent
-// is written as its local root; k_uf_union skips them (u1 is stored
-// complemented).  Roots stay cluster minima, so the partition and every
-// result are those of the plain union-find.
-template <int CAP, int WAVES>
+// One block step handles SITES = WAVES / SUBW consecutive sites, SUBW waves
+// per site (more waves per site for long world lines, where one wave would
+// rank each list element by element).  Operators of a bond whose two sites
+// are both in the step ("inner" operators: neighbours along the fast lattice
+// direction) are united here in LDS, and each segment's parent is written as
+// its local root; k_uf_union skips them (u1 is stored complemented).  Roots
+// stay cluster minima, so the partition and every result are those of the
+// plain union-find.
+template <int CAP, int WAVES, int SUBW>
 __global__ void __launch_bounds__(64 * WAVES) k_link(DevBatch B) {
   if (overflowed(B)) return;
-  __shared__ double lt_all[WAVES][CAP];           // staged times, list order; then local parents
-  __shared__ uint16_t lpre_all[WAVES][CAP + 4];   // off-diagonal prefix counts; then element ranks
-  __shared__ uint16_t inv_all[WAVES][CAP];        // rank -> staged element
-  __shared__ uint8_t lsp_all[WAVES][CAP];         // staged element -> spin of the segment below
-  __shared__ uint8_t lfl_all[WAVES][CAP];         // staged element -> event flags
-  __shared__ int wn0[WAVES], wn1[WAVES], wtot[WAVES], wnbd[WAVES];
-  __shared__ int wbond[WAVES][4], woff[WAVES][4];
-  const int w = threadIdx.x >> 6;
+  constexpr int SITES = WAVES / SUBW;
+  constexpr int TPS = 64 * SUBW;                  // threads per site
+  __shared__ double lt_all[SITES][CAP];           // staged times, list order; then local parents
+  __shared__ uint16_t lpre_all[SITES][CAP + 4];   // off-diagonal prefix counts; then element ranks
+  __shared__ uint16_t inv_all[SITES][CAP];        // rank -> staged element
+  __shared__ uint8_t lsp_all[SITES][CAP];         // staged element -> spin of the segment below
+  __shared__ uint8_t lfl_all[SITES][CAP];         // staged element -> event flags
+  __shared__ int wn0[SITES], wn1[SITES], wtot[SITES], wnbd[SITES];
+  __shared__ int wbond[SITES][4], woff[SITES][4];
+  const int w = threadIdx.x / TPS;                // site slot of the step
+  const int sl = threadIdx.x % TPS;               // thread within the site's group
   double* lt = lt_all[w];
   uint16_t* lpre = lpre_all[w];
   uint16_t* inv = inv_all[w];
   uint8_t* lsp = lsp_all[w];
   uint8_t* lfl = lfl_all[w];
-  int* lpar = reinterpret_cast<int*>(&lt_all[0][0]);  // WAVES * CAP * 2 ints
+  int* lpar = reinterpret_cast<int*>(&lt_all[0][0]);  // SITES * CAP * 2 ints
   int slo, shi;
   xcd_share(B.nsites, slo, shi);
-  for (int g = slo; g < shi; g += WAVES) {  // block-uniform steps
+  for (int g = slo; g < shi; g += SITES) {  // block-uniform steps
     const int gs = g + w;
     const bool have = gs < shi;
     SiteLists S;
@@ -614,14 +619,14 @@ __global__ void __launch_bounds__(64 * WAVES) k_link(DevBatch B) {
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
           if (j >= S.nbd) break;
-          for (int q = lane_id(); q < S.len[j]; q += 64) {
+          for (int q = sl; q < S.len[j]; q += TPS) {
             lt[S.off[j] + q] = B.ot_new[S.lo[j] + q];
             lpre[S.off[j] + j + q] = (uint16_t)B.kc_new[S.lo[j] + q];
           }
-          if (lane_id() == 0) lpre[S.off[j] + j + S.len[j]] = (uint16_t)list_pre_global(B, S, j, S.len[j]);
+          if (sl == 0) lpre[S.off[j] + j + S.len[j]] = (uint16_t)list_pre_global(B, S, j, S.len[j]);
         }
       }
-      if (lane_id() == 0) {
+      if (sl == 0) {
         wn0[w] = n0;
         wn1[w] = n1;
         wtot[w] = tot;
@@ -629,18 +634,18 @@ __global__ void __launch_bounds__(64 * WAVES) k_link(DevBatch B) {
 #pragma unroll
         for (int j = 0; j < 4; ++j) { wbond[w][j] = S.bond[j]; woff[w][j] = S.off[j]; }
       }
-    } else if (lane_id() == 0) {
+    } else if (sl == 0) {
       wtot[w] = -1;
     }
     __syncthreads();
     // local unions need every site of the step staged and in one chain
     bool inner_ok = true;
     const int c0 = B.site_chain[g];
-    for (int v = 0; v < WAVES; ++v)
+    for (int v = 0; v < SITES; ++v)
       if (g + v < shi && (wtot[v] > CAP || B.site_chain[g + v] != c0)) inner_ok = false;
     if (have && tot > 0) {
       const bool lds = tot <= CAP;
-      for (int e = lane_id(); e < tot; e += 64) {
+      for (int e = sl; e < tot; e += TPS) {
         // the element's list, selected with constant indices
         int i = 0, offi = S.off[0], loi = S.lo[0], bi = S.bond[0], side = S.side[0], oth = other[0];
 #pragma unroll
@@ -659,7 +664,7 @@ __global__ void __launch_bounds__(64 * WAVES) k_link(DevBatch B) {
         const uint8_t sp = spin0 ^ (uint8_t)(par & 1);
         const int node = n0 + rank;
         const int up = (node + 1 == n1) ? n0 : node + 1;  // segment above the event
-        const bool inner = inner_ok && oth >= g && oth < g + WAVES && oth < shi;
+        const bool inner = inner_ok && oth >= g && oth < g + SITES && oth < shi;
         if (side) { B.l1[k] = node; B.u1[k] = inner ? ~up : up; } else { B.l0[k] = node; B.u0[k] = up; }
         if (lds) {
           inv[rank] = (uint16_t)e;
@@ -677,7 +682,7 @@ __global__ void __launch_bounds__(64 * WAVES) k_link(DevBatch B) {
     __syncthreads();
     // segment arrays in node order; element ranks for the local unions
     if (have && tot > 0 && tot <= CAP) {
-      for (int r = lane_id(); r < tot; r += 64) {
+      for (int r = sl; r < tot; r += TPS) {
         const int e = inv[r];
         B.evt_new[n0 + r] = lt[e];
         B.segspin_new[n0 + r] = lsp[e];
@@ -688,7 +693,7 @@ __global__ void __launch_bounds__(64 * WAVES) k_link(DevBatch B) {
     }
     __syncthreads();
     // step's segments [N0, N1): local unions of the inner operators
-    const int last = min(g + WAVES, shi) - 1 - g;
+    const int last = min(g + SITES, shi) - 1 - g;
     const int N0 = wn0[0], N1 = wn1[last];
     if (inner_ok) {
       for (int x = threadIdx.x; x < N1 - N0; x += blockDim.x) lpar[x] = x;
@@ -698,12 +703,12 @@ __global__ void __launch_bounds__(64 * WAVES) k_link(DevBatch B) {
         for (int j = 0; j < 4; ++j) {
           if (j >= S.nbd || S.side[j] != 0) continue;
           const int wt = other[j] - g;
-          if (other[j] < g || wt >= WAVES || other[j] >= shi) continue;
+          if (other[j] < g || wt >= SITES || other[j] >= shi) continue;
           int jt = 0;
           while (jt < 3 && wbond[wt][jt] != S.bond[j]) ++jt;
           const uint16_t* rk_t = lpre_all[wt];
           const int t0 = wn0[wt] - N0, t1 = wn1[wt] - N0, m0 = n0 - N0, m1 = n1 - N0;
-          for (int p = lane_id(); p < S.len[j]; p += 64) {
+          for (int p = sl; p < S.len[j]; p += TPS) {
             const int a0 = m0 + lpre[S.off[j] + p];
             const int b0 = (a0 + 1 == m1) ? m0 : a0 + 1;
             const int a1 = t0 + rk_t[woff[wt][jt] + p];
@@ -726,7 +731,7 @@ __global__ void __launch_bounds__(64 * WAVES) k_link(DevBatch B) {
       __syncthreads();
       for (int x = threadIdx.x; x < N1 - N0; x += blockDim.x) B.parent[N0 + x] = N0 + lds_find(lpar, x);
     } else if (have && tot > 0 && tot <= CAP) {
-      for (int r = lane_id(); r < tot; r += 64) B.parent[n0 + r] = n0 + r;
+      for (int r = sl; r < tot; r += TPS) B.parent[n0 + r] = n0 + r;
     }
     __syncthreads();
   }
@@ -1462,15 +1467,15 @@ void launch_sweep(const DevBatch& B, const LaunchCfg& L, hipStream_t st, hipEven
 void launch_cluster_stage(const DevBatch& B, const LaunchCfg& L, hipStream_t st, hipEvent_t* ev) {
   // by the expected events per site: one thread per operator (few events),
   // a wave per site staging 512 events with 4 sites per block (29 KB of
-  // LDS), or 1536 with 2 (43 KB)
+  // LDS), or 1536 events with 2 sites per block and 4 waves per site (43 KB)
   if (L.link_cap == 0) {
     hipLaunchKernelGGL(k_link_ops, dim3(L.blocks_op), dim3(256), 0, st, B);
   } else if (L.link_cap <= 512) {
     int blocks = min(L.blocks_node, max(1, (B.nsites + 3) / 4));
-    hipLaunchKernelGGL((k_link<512, 4>), dim3(blocks), dim3(256), 0, st, B);
+    hipLaunchKernelGGL((k_link<512, 4, 1>), dim3(blocks), dim3(256), 0, st, B);
   } else {
     int blocks = min(L.blocks_node, max(1, (B.nsites + 1) / 2));
-    hipLaunchKernelGGL((k_link<1536, 2>), dim3(blocks), dim3(128), 0, st, B);
+    hipLaunchKernelGGL((k_link<1536, 8, 4>), dim3(blocks), dim3(512), 0, st, B);
   }
   mark(ev, 1, st);
   hipLaunchKernelGGL(k_uf_union, dim3(L.blocks_op), dim3(256), 0, st, B);

@@ -95,6 +95,7 @@ LAUNCH_TRAJ = [
     (O.SQUARE, 8, 6.0, 1.0, 0.8, 20),
     (O.CHAIN, 32, 60.0, 1.0, 1.0, 20),
     (O.CHAIN, 8, 600.0, 1.0, 1.0, 12),
+    (O.CHAIN, 4, 2000.0, 1.0, 0.9, 6),     # about 2500 events per site: beyond the 1536-event staging
 ]
 LAUNCHES = [dict(link=0), dict(link=512), dict(link=1536), dict(link=512, group=4, blocks=7),
             dict(link=0, group=16, blocks=64), dict(link=1536, group=32),
