@@ -138,6 +138,7 @@ class Batch {
   unsigned long long* scan_status = nullptr;
   int* scan_ctl = nullptr;
   int scan_nparts = 0;
+  int cus = 256;                  // compute units of the device
   double* col_part = nullptr;
   int* totals = nullptr;
   int* flags = nullptr;
