@@ -212,7 +212,9 @@ __global__ void __launch_bounds__(256, 6) k_diag_count(DevBatch B) {
   const uint64_t below = (gl == 0) ? 0ull : ((1ull << gl) - 1);
   // bonds interleaved over the grid: the work per bond varies with the
   // chain's temperature in a batch
-  for (int gb = blockIdx.x * groups_per_block + threadIdx.x / G; gb < B.nbonds;
+  int gb_first = blockIdx.x * groups_per_block + threadIdx.x / G;
+  if (G == 64) gb_first = __builtin_amdgcn_readfirstlane(gb_first);  // one bond per wave: uniform
+  for (int gb = gb_first; gb < B.nbonds;
        gb += gridDim.x * groups_per_block) {
     const int c = B.bond_chain[gb];
     const ChainDev P = B.chains[c];
@@ -443,7 +445,9 @@ __global__ void __launch_bounds__(256) k_diag_fill(DevBatch B) {
   if (overflowed(B)) return;
   const int groups_per_block = blockDim.x / G;
   const int gl = glane<G>();
-  for (int gb = blockIdx.x * groups_per_block + threadIdx.x / G; gb < B.nbonds;
+  int gb_first = blockIdx.x * groups_per_block + threadIdx.x / G;
+  if (G == 64) gb_first = __builtin_amdgcn_readfirstlane(gb_first);  // one bond per wave: uniform
+  for (int gb = gb_first; gb < B.nbonds;
        gb += gridDim.x * groups_per_block) {
     const int c = B.bond_chain[gb];
     const ChainDev P = B.chains[c];
@@ -572,7 +576,9 @@ __global__ void __launch_bounds__(64 * WAVES) k_link(DevBatch B) {
   __shared__ uint8_t lfl_all[SITES][CAP];         // staged element -> event flags
   __shared__ int wn0[SITES], wn1[SITES], wtot[SITES], wnbd[SITES];
   __shared__ int wbond[SITES][4], woff[SITES][4];
-  const int w = threadIdx.x / TPS;                // site slot of the step
+  // site slot of the step: one per wave group, so wave-uniform (read as such,
+  // which keeps the site's list descriptors in scalar registers)
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / TPS);
   const int sl = threadIdx.x % TPS;               // thread within the site's group
   double* lt = lt_all[w];
   uint16_t* lpre = lpre_all[w];
