@@ -412,7 +412,7 @@ int Batch::set_layout(std::vector<ChainDev>& ds, const std::vector<uint64_t>& sw
   LaunchCfg c = cfg;
   if (!tuned) {
     const double per_site = nsites > 0 ? ev / nsites : 0;
-    c.link_cap = per_site < 96 ? 0 : (per_site <= 400 ? 512 : 1536);
+    c.link_cap = per_site < 48 ? 0 : (per_site < 96 ? 128 : (per_site <= 400 ? 512 : 1536));
     // the estimator finds the roots itself (a pass over the segments less,
     // but the collector then clears every root's accumulator) for one large
     // chain; the compress pass for many chains (mostly small and hot, many
@@ -651,10 +651,10 @@ int Batch::tune(uint32_t sweeps_per_trial) {
   double best_t = 1e300;
   const int groups[] = {4, 8, 16, 32, 64};
   const int mults[] = {2, 4, 8, 16, 32, 64};
-  const int links[] = {0, 512, 1536};
+  const int links[] = {0, 128, 512, 1536};
   // group size first with the current grids, then the grids with the best
   // group, then the linking kernel, then the compress pass
-  const int trials[] = {5, 6, 3, 2};
+  const int trials[] = {5, 6, 4, 2};
   for (int pass = 0; pass < 4; ++pass) {
     for (int i = 0; i < trials[pass]; ++i) {
       LaunchCfg trial = best;
@@ -696,7 +696,8 @@ int Batch::set_launch(const lg_launch& l) {
     c.blocks_op = c.blocks_node = l.blocks;
   }
   if (l.link >= 0) {
-    if (l.link != 0 && l.link != 512 && l.link != 1536) return set_error(LG_ERR_ARG, "launch link must be 0, 512 or 1536");
+    if (l.link != 0 && l.link != 128 && l.link != 512 && l.link != 1536)
+      return set_error(LG_ERR_ARG, "launch link must be 0, 128, 512 or 1536");
     c.link_cap = l.link;
   }
   if (l.compress >= 0) {

@@ -556,19 +556,19 @@ __device__ __forceinline__ void lds_unite(int* P, int a, int b) {
   }
 }
 
-// One block step handles SITES = WAVES / SUBW consecutive sites, SUBW waves
-// per site (more waves per site for long world lines, where one wave would
-// rank each list element by element).  Operators of a bond whose two sites
+// One block step handles SITES = 64 * WAVES / TPS consecutive sites, TPS
+// threads per site (several waves per site for long world lines, where one
+// wave would rank each list element by element; a quarter wave for short
+// ones, where a wave would idle most lanes).  Operators of a bond whose two sites
 // are both in the step ("inner" operators: neighbours along the fast lattice
 // direction) are united here in LDS, and each segment's parent is written as
 // its local root; k_uf_union skips them (u1 is stored complemented).  Roots
 // stay cluster minima, so the partition and every result are those of the
 // plain union-find.
-template <int CAP, int WAVES, int SUBW>
+template <int CAP, int WAVES, int TPS>
 __global__ void __launch_bounds__(64 * WAVES) k_link(DevBatch B) {
   if (overflowed(B)) return;
-  constexpr int SITES = WAVES / SUBW;
-  constexpr int TPS = 64 * SUBW;                  // threads per site
+  constexpr int SITES = 64 * WAVES / TPS;
   __shared__ double lt_all[SITES][CAP];           // staged times, list order; then local parents
   __shared__ uint16_t lpre_all[SITES][CAP + 4];   // off-diagonal prefix counts; then element ranks
   __shared__ uint16_t inv_all[SITES][CAP];        // rank -> staged element
@@ -576,9 +576,9 @@ __global__ void __launch_bounds__(64 * WAVES) k_link(DevBatch B) {
   __shared__ uint8_t lfl_all[SITES][CAP];         // staged element -> event flags
   __shared__ int wn0[SITES], wn1[SITES], wtot[SITES], wnbd[SITES];
   __shared__ int wbond[SITES][4], woff[SITES][4];
-  // site slot of the step: one per wave group, so wave-uniform (read as such,
-  // which keeps the site's list descriptors in scalar registers)
-  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / TPS);
+  // site slot of the step; with whole waves per site it is wave-uniform (read
+  // as such, which keeps the site's list descriptors in scalar registers)
+  const int w = TPS >= 64 ? __builtin_amdgcn_readfirstlane(threadIdx.x / TPS) : (int)(threadIdx.x / TPS);
   const int sl = threadIdx.x % TPS;               // thread within the site's group
   double* lt = lt_all[w];
   uint16_t* lpre = lpre_all[w];
@@ -1474,16 +1474,20 @@ void launch_sweep(const DevBatch& B, const LaunchCfg& L, hipStream_t st, hipEven
 
 void launch_cluster_stage(const DevBatch& B, const LaunchCfg& L, hipStream_t st, hipEvent_t* ev) {
   // by the expected events per site: one thread per operator (few events),
-  // a wave per site staging 512 events with 4 sites per block (29 KB of
-  // LDS), or 1536 events with 2 sites per block and 4 waves per site (43 KB)
+  // 16 threads per site staging 128 events with 16 sites per block (29 KB of
+  // LDS), a wave per site staging 512 events with 4 sites per block (29 KB),
+  // or 1536 events with 2 sites per block and 4 waves per site (43 KB)
   if (L.link_cap == 0) {
     hipLaunchKernelGGL(k_link_ops, dim3(L.blocks_op), dim3(256), 0, st, B);
+  } else if (L.link_cap <= 128) {
+    int blocks = min(L.blocks_node, max(1, (B.nsites + 15) / 16));
+    hipLaunchKernelGGL((k_link<128, 4, 16>), dim3(blocks), dim3(256), 0, st, B);
   } else if (L.link_cap <= 512) {
     int blocks = min(L.blocks_node, max(1, (B.nsites + 3) / 4));
-    hipLaunchKernelGGL((k_link<512, 4, 1>), dim3(blocks), dim3(256), 0, st, B);
+    hipLaunchKernelGGL((k_link<512, 4, 64>), dim3(blocks), dim3(256), 0, st, B);
   } else {
     int blocks = min(L.blocks_node, max(1, (B.nsites + 1) / 2));
-    hipLaunchKernelGGL((k_link<1536, 8, 4>), dim3(blocks), dim3(512), 0, st, B);
+    hipLaunchKernelGGL((k_link<1536, 8, 256>), dim3(blocks), dim3(512), 0, st, B);
   }
   mark(ev, 1, st);
   hipLaunchKernelGGL(k_uf_union, dim3(L.blocks_op), dim3(256), 0, st, B);

@@ -159,9 +159,10 @@ int lg_exchange_state(lg_batch* h, int32_t* chain_at, uint64_t* accepted, uint64
 /* ---- launch settings ----------------------------------------------------------
  * group: lanes per bond in the diagonal update (4, 8, 16, 32 or 64); blocks:
  * grid of the per-operator and per-node kernels; link: linking kernel (0 one
- * thread per operator, 512 or 1536 one wave per site staging that many
- * events in LDS); compress: 1 a union-find compression pass before the
- * estimator, 0 the estimator finds the roots itself.  By default they follow
+ * thread per operator; 128, 512 or 1536 a group of threads per site staging
+ * that many events in LDS: 16 threads, one wave, four waves); compress: 1 a
+ * union-find compression pass before the estimator, 0 the estimator finds
+ * the roots itself.  By default they follow
  * the expected operator counts and the number of chains, and lg_batch_tune
  * times the candidates.  set: fields < 0 keep their value; the settings then
  * stay as given (no automatic choice) until the next tune.  Results do not

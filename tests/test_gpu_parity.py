@@ -97,7 +97,8 @@ LAUNCH_TRAJ = [
     (O.CHAIN, 8, 600.0, 1.0, 1.0, 12),
     (O.CHAIN, 4, 2000.0, 1.0, 0.9, 6),     # about 2500 events per site: beyond the 1536-event staging
 ]
-LAUNCHES = [dict(link=0), dict(link=512), dict(link=1536), dict(link=512, group=4, blocks=7),
+LAUNCHES = [dict(link=0), dict(link=128), dict(link=512), dict(link=1536), dict(link=512, group=4, blocks=7),
+            dict(link=128, group=8, blocks=5),
             dict(link=0, group=16, blocks=64), dict(link=1536, group=32),
             dict(link=512, compress=1), dict(link=0, compress=0, blocks=9), dict(link=1536, compress=1, group=8)]
 
@@ -171,13 +172,16 @@ def test_batch_equals_single_chains(loopgpu):
         loopgpu.params(O.SQUARE, 6, 1.0, 1.0, 1.4, seed=11, chain_id=3),
     ]
     n = 30
-    b = loopgpu.Batch(plist)
-    rb = b.run(n)
-    for i, p in enumerate(plist):
-        print("batch chain", i, flush=True)
-        rs = loopgpu.Chain(p).sweep(n)
-        np.testing.assert_array_equal(rb[:, i, EXACT], rs[:, EXACT])
-        np.testing.assert_allclose(rb[:, i, FLOAT], rs[:, FLOAT], rtol=RTOL, atol=1e-9)
+    singles = [loopgpu.Chain(p).sweep(n) for p in plist]
+    for link in (None, 0, 128, 512, 1536):  # the default and every linking kernel (sites of several chains per block)
+        b = loopgpu.Batch(plist)
+        if link is not None:
+            b.set_launch(link=link)
+        rb = b.run(n)
+        for i, rs in enumerate(singles):
+            print("batch chain", i, "link", link, flush=True)
+            np.testing.assert_array_equal(rb[:, i, EXACT], rs[:, EXACT])
+            np.testing.assert_allclose(rb[:, i, FLOAT], rs[:, FLOAT], rtol=RTOL, atol=1e-9)
 
 
 def test_graph_replay_matches_plain_launches(loopgpu):
