@@ -208,6 +208,10 @@ LG_HD int site_bonds(const ChainDev& c, int s, int* bl) {
   if (c.kind == 0) {
     bl[0] = s;
     bl[1] = (s == 0) ? c.L - 1 : s - 1;
+    // unused; written on both paths so that the list stays in registers
+    // (stores to different entries per path are merged into an indexed
+    // store, which puts the array in scratch memory)
+    bl[2] = bl[3] = -1;
     return 2;
   }
   int x = s / c.L, y = s - x * c.L;

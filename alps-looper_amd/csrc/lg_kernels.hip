@@ -201,10 +201,8 @@ __device__ __forceinline__ void scan_advance_epoch(const DevBatch& B) {
 // ---------------------------------------------------------------------------
 // (a) diagonal update, pass 1: count and store accepted candidates
 // ---------------------------------------------------------------------------
-// (at least 6 waves per SIMD: latency-bound; measured faster so even when
-// the bound cost a few spilled registers)
 template <int G>
-__global__ void __launch_bounds__(256, 6) k_diag_count(DevBatch B) {
+__global__ void __launch_bounds__(256) k_diag_count(DevBatch B) {
   if (blockIdx.x == 0 && threadIdx.x == 0) scan_advance_epoch(B);  // for k_scan, which follows
   if (overflowed(B)) return;
   const int groups_per_block = blockDim.x / G;
