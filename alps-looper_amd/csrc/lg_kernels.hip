@@ -6,8 +6,8 @@
 // next one:
 //   diag_count   per bond: Poisson candidates from Philox, acceptance against
 //                the previous world lines, kept off-diagonal operators counted
-//   scan_*       bond offsets of the new operator list and segment offsets of
-//                the new world lines (max(events,1) per site)
+//   scan         (single pass) bond offsets of the new operator list and
+//                segment offsets of the new world lines (max(events,1) per site)
 //   diag_fill    per bond: kept operators (regraphed) merged with accepted
 //                candidates in time order
 //   link         per site: rank of each event on the site's world line
@@ -486,8 +486,8 @@ __global__ void __launch_bounds__(256) k_diag_fill(DevBatch B) {
 }
 
 // ---------------------------------------------------------------------------
-// (b) world-line linking, one wave per site.  The events of a site are the
-// operators of its bonds (each bond list sorted by tau); an event's rank on
+// (b) world-line linking, a group of threads per site.  The events of a
+// site are the operators of its bonds (each bond list sorted by tau); an event's rank on
 // the site is its index in its own list plus, in each other list, the number
 // of operators before it (ties ordered by bond index), and the spin of the
 // segment below it flips with the off-diagonal operators counted the same
