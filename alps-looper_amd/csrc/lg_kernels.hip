@@ -487,11 +487,11 @@ __global__ void __launch_bounds__(256) k_diag_fill(DevBatch B) {
 
 // ---------------------------------------------------------------------------
 // (b) world-line linking, a group of threads per site.  The events of a
-// site are the operators of its bonds (each bond list sorted by tau); an event's rank on
-// the site is its index in its own list plus, in each other list, the number
-// of operators before it (ties ordered by bond index), and the spin of the
-// segment below it flips with the off-diagonal operators counted the same
-// way.  The lists (times and off-diagonal prefix counts) are staged in LDS,
+// site are the operators of its bonds (each bond list sorted by tau); an
+// event's rank on the site is its index in its own list plus, in each other
+// list, the number of operators before it (ties ordered by bond index), and
+// the spin of the segment below it flips with the off-diagonal operators
+// counted the same way.  The lists (times and off-diagonal prefix counts) are staged in LDS,
 // ranked there, and the site's segment arrays are written in node order
 // through the inverse permutation (whole cache lines).  A site with more
 // than CAP events searches the lists in global memory and writes directly.
