@@ -903,16 +903,25 @@ __device__ __forceinline__ int chain_of_node(const DevBatch& B, int node) {
 // A segment adds its length to one class slot of its root (ClusterAcc), its
 // tau = 0 boundary (first segment of a site) to one class counter and its
 // winding contributions to one packed 64-bit counter.  Each block handles
-// chunks of EST_CHUNK consecutive nodes and merges the segments that share a
-// root in an LDS hash table (open addressing, HT slots) before touching global
+// chunks of 256 * chunk_items(M) consecutive nodes and merges the segments
+// that share a root in an LDS hash table (open addressing, HT slots) before touching global
 // memory: the large loops of a cold antiferromagnet (a few clusters holding
 // most segments) then cost one global update per chunk instead of one per
 // segment.  Segments whose root finds no slot update global memory directly.
 // ---------------------------------------------------------------------------
 constexpr int EST_THREADS = 256;
-constexpr int EST_ITEMS = 16;
-constexpr int EST_U = 4;         // items per group (EST_ITEMS / EST_U groups)
-constexpr int EST_CHUNK = EST_THREADS * EST_ITEMS;
+constexpr int EST_U = 4;         // items per thread and group
+
+// Items per thread of the per-node chunked kernels (estimate, collector): 16,
+// or 8 or 4 when 16 would give fewer than 1024 chunks.  A small configuration
+// otherwise occupies a few blocks, each walking long serial item lists; a
+// large one keeps the big chunks that share LDS merges across more segments.
+// A function of the segment count only, so every kernel of a sweep agrees.
+__device__ __forceinline__ int chunk_items(int M) {
+  int it = 16;
+  while (it > 4 && (long long)M < 1024LL * 256 * it) it >>= 1;
+  return it;
+}
 constexpr int HT_BITS = 9;
 constexpr int HT = 1 << HT_BITS;
 constexpr int HT_PROBES = 8;
@@ -977,15 +986,16 @@ __global__ void __launch_bounds__(EST_THREADS) k_estimate(DevBatch B) {
   }
   __syncthreads();
   const int M = B.totals[1];
-  const int nchunks = (M + EST_CHUNK - 1) / EST_CHUNK;
+  const int items = chunk_items(M), echunk = items * EST_THREADS;
+  const int nchunks = (M + echunk - 1) / echunk;
   int clo, chi;
   xcd_share(nchunks, clo, chi);
   for (int chunk = clo; chunk < chi; ++chunk) {
     // the thread's items in groups of EST_U: every memory step of a group
     // (parents, the walks to the roots, the segments' own data, the events
     // below them) is issued for all its items before any is waited for
-    for (int g0 = 0; g0 < EST_ITEMS; g0 += EST_U) {
-      const int ib = chunk * EST_CHUNK + g0 * EST_THREADS + tid;
+    for (int g0 = 0; g0 < items; g0 += EST_U) {
+      const int ib = chunk * echunk + g0 * EST_THREADS + tid;
       if (__builtin_amdgcn_readfirstlane(ib) >= M) break;  // lane 0 holds the wave's lowest node
       int r[EST_U];
       uint32_t f[EST_U];
@@ -1144,15 +1154,14 @@ __global__ void __launch_bounds__(256) k_field_apply(DevBatch B) {
 // ---------------------------------------------------------------------------
 // (d) operator flip, then the improved-estimator collector
 // (susceptibility.h:177-198, stiffness.h:117-121) over the roots.
-// Collection is chunked: chunk j covers nodes [j*COL_CHUNK, (j+1)*COL_CHUNK)
-// and belongs to the chain of its first node; its sum over the roots of that
+// Collection is chunked: chunk j covers nodes [j*C, (j+1)*C), C = 256 *
+// chunk_items(M), and belongs to the chain of its first node; its sum over the roots of that
 // chain is written to col_part[j] (no atomics, fixed order), roots of other
 // chains in the chunk (chunks straddling a chain boundary) are added to
 // rec_acc atomically.  end_sweep adds the partials of each chain in order.
 // ---------------------------------------------------------------------------
 constexpr int COL_THREADS = 256;
-constexpr int COL_ITEMS = 16;
-constexpr int COL_CHUNK = COL_THREADS * COL_ITEMS;
+constexpr int COL_MIN_CHUNK = COL_THREADS * 4;  // smallest collector chunk (chunk_items)
 constexpr int NCOL = NUM_COLLECTOR_TERMS;
 
 // Collector terms of one cluster.  h, flip: the chain's field and the
@@ -1211,11 +1220,12 @@ __global__ void __launch_bounds__(COL_THREADS) k_flip_collect(DevBatch B, int cl
   // contiguous in node order, so a chunk meets few of them)
   constexpr int CSLOTS = 8;
   __shared__ double slot[CSLOTS][NCOL];
-  const int nchunks = (M + COL_CHUNK - 1) / COL_CHUNK;
+  const int citems = chunk_items(M), cchunk = citems * COL_THREADS;
+  const int nchunks = (M + cchunk - 1) / cchunk;
   int clo, chi;
   xcd_share(nchunks, clo, chi);
   for (int chunk = clo; chunk < chi; ++chunk) {
-    const int base = chunk * COL_CHUNK;
+    const int base = chunk * cchunk;
     const int c0 = node_chain(B, base);
     const int c0_end = (c0 + 1 < B.nchains) ? B.chain_node0[c0 + 1] : M;  // nodes of chain c0 end here
     for (int s = threadIdx.x; s < CSLOTS * NCOL; s += COL_THREADS) slot[s / NCOL][s % NCOL] = 0;
@@ -1235,7 +1245,7 @@ __global__ void __launch_bounds__(COL_THREADS) k_flip_collect(DevBatch B, int cl
         for (int q = 0; q < NCOL; ++q) unsafeAtomicAdd(rec + term_slot(q), w[q]);
       }
     };
-    for (int it = 0; it < COL_ITEMS; ++it) {
+    for (int it = 0; it < citems; ++it) {
       const int i = base + it * COL_THREADS + threadIdx.x;
       if (i < M && B.parent[i] == i) {
         const int c = (i < c0_end) ? c0 : node_chain(B, i);
@@ -1310,7 +1320,8 @@ __global__ void __launch_bounds__(END_THREADS) k_end_sweep(DevBatch B) {
   for (int q = 0; q < NCOL; ++q) v[q] = 0;
   if (!bad) {
     const int start = B.soff_new[P.site_base], end = B.soff_new[P.site_base + P.N];
-    const int j0 = (start + COL_CHUNK - 1) / COL_CHUNK, j1 = (end + COL_CHUNK - 1) / COL_CHUNK;
+    const int cchunk = chunk_items(B.totals[1]) * COL_THREADS;  // as in k_flip_collect
+    const int j0 = (start + cchunk - 1) / cchunk, j1 = (end + cchunk - 1) / cchunk;
     for (int j = j0 + tid; j < j1; j += END_THREADS)
       for (int q = 0; q < NCOL; ++q) v[q] += B.col_part[(size_t)j * NCOL + q];
   }
@@ -1443,7 +1454,7 @@ static void diag_dispatch(const DevBatch& B, const LaunchCfg& L, hipStream_t st,
   }
 }
 
-int col_chunks(int node_cap) { return (node_cap + COL_CHUNK - 1) / COL_CHUNK; }
+int col_chunks(int node_cap) { return (node_cap + COL_MIN_CHUNK - 1) / COL_MIN_CHUNK; }
 
 int scan_blocks(const DevBatch& B) {
   return (B.nbonds + SCAN_CHUNK - 1) / SCAN_CHUNK + (B.nsites + SCAN_CHUNK - 1) / SCAN_CHUNK;

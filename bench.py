@@ -20,14 +20,22 @@ server: a burst of jobs measures the saturated rate, then Poisson arrivals at
 and p50/p99 job latency (submit -> result).  Job sweep counts are multiplied
 by --sweep-scale (0.1 by default, so that the run takes minutes).
 
+--workload c5 runs the 512x512 square lattice at beta=512.  --workload
+chain128 / chain1024 run the reference's own published benchmark parameters
+of the chain (L=128, T=0.01 and L=1024, T=0.001).
+
 CPU baselines run the oracle's looper-mode restatement: one core for the
-one-chain workload, a thread pool over the host cores for scan and serve.
+one-chain workloads, a thread pool over the host cores for scan and serve;
+for the chain workloads the reference's own serial program (built from its
+sources into oracle/_ref) on one core.
 """
 
 import argparse
 import json
 import os
 import platform
+import re
+import subprocess
 import sys
 import time
 
@@ -118,6 +126,70 @@ def cpu_baseline(kind, L, beta, seconds):
                    "from the empty operator string: %d sweeps untimed, then %d timed sweeps "
                    "(mean n_op %.0f) in %.1f s; CPU: %s" % (n_warm, n_timed, nops / n_timed, tsum, cpu_model())),
     }
+
+
+def reference_program_baseline(L, T, nop_mean, seconds):
+    """The reference's own serial kernel (standalone/loop.C, compiled from its
+    sources into oracle/_ref) on 1 host core for the same chain: it prints its
+    own sweep rate ("Speed = ... MCS/sec", standalone/loop.C); updates/s is
+    that rate times the GPU run's mean operator count of the same model and
+    temperature.  The sweep count is sized from a short first run so that the
+    sample takes about `seconds`."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "standalone_loop")
+    if not os.path.exists(exe):
+        return None
+    def speed(n):
+        r = subprocess.run([exe, "-l", str(L), "-t", repr(T), "-n", str(n)], capture_output=True,
+                           text=True, timeout=600)
+        m = re.search(r"Speed = ([0-9.eE+-]+) MCS/sec", r.stdout + r.stderr)  # printed on stderr
+        return float(m.group(1)) if m else None
+    probe = speed(64)
+    if not probe:
+        return None
+    n = max(64, int(probe * seconds / 1.125))  # the program adds n/8 thermalization sweeps
+    v = speed(n)
+    return {
+        "value": v * nop_mean,
+        "unit": "spin-operator updates/s",
+        "cores": 1,
+        "kind": "reference",
+        "sample": ("the reference's standalone/loop.C (c++ -O3, oracle/_ref/standalone_loop) -l %d -t %g -n %d: "
+                   "%.1f sweeps/s on its own clock, times the GPU run's mean n_op %.0f; CPU: %s"
+                   % (L, T, n, v, nop_mean, cpu_model())),
+        "sweeps_per_s": v,
+    }
+
+
+def run_chain(args, lg, L, T):
+    """The reference's published benchmark configurations of the chain
+    (standalone/README.en: "-l 128 -t 0.01" and "-l 1024 -t 0.001"), one chain,
+    plain GPU path"""
+    beta = 1.0 / T
+    ch = lg.Chain(lg.params(lg.CHAIN, L, beta, 1.0, 1.0, seed=23125))
+    chunk = 1000
+    log("chain L=%d beta=%g: warmup %d sweeps" % (L, beta, args.warmup))
+    run_chunked(lambda k: ch.sweep(k), args.warmup, chunk)
+    log("timed %d sweeps" % args.steps)
+    t0 = time.perf_counter()
+    rec = run_chunked(lambda k: ch.sweep(k), args.steps, chunk)
+    dt = time.perf_counter() - t0
+    nop_mean = float(rec[:, 0].mean())
+    prof = ch.profile(args.profile_sweeps)
+    out = {
+        "metric": METRIC, "value": float(rec[:, 0].sum()) / dt, "unit": "spin-operator updates/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * dt / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "chain L=%d periodic S=1/2 Heisenberg AF, J=1, T=%g (beta=%g), one chain, "
+                               "plain GPU path (the reference's published benchmark parameters)" % (L, T, beta),
+                   "lattice": "chain lattice", "L": L, "beta": beta, "J": 1.0, "chains": 1, "seed": 23125},
+        "sweeps_per_s": args.steps / dt,
+        "n_op_mean": nop_mean,
+        "energy_density": float(rec[:, 2].mean() / L),
+        "kernel_ms": prof,
+        "roofline": roofline(prof, nop_mean),
+        "published_reference_sweeps_per_s": {128: 412.0, 1024: 3.30}.get(L),
+    }
+    return out, ("reference", (L, T, nop_mean))
 
 
 def measured_traffic(name, kernel, nop_mean):
@@ -419,7 +491,7 @@ def main():
     ap.add_argument("--profile-sweeps", type=int, default=100)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["c2", "c5", "scan", "serve"], default="c2")
+    ap.add_argument("--workload", choices=["c2", "c5", "scan", "serve", "chain128", "chain1024"], default="c2")
     ap.add_argument("--serve-seconds", type=float, default=60.0, help="serve: seconds of Poisson arrivals")
     ap.add_argument("--serve-burst", type=int, default=48, help="serve: jobs of the saturation burst")
     ap.add_argument("--sweep-scale", type=float, default=0.1, help="serve: factor on every job's sweep count")
@@ -447,11 +519,20 @@ def main():
         out, cpu_cfg = run_c2(args, lg, 512, 512.0, None)
     elif args.workload == "scan":
         out, cpu_cfg = run_scan(args, lg)
+    elif args.workload == "chain128":
+        out, cpu_cfg = run_chain(args, lg, 128, 0.01)
+    elif args.workload == "chain1024":
+        out, cpu_cfg = run_chain(args, lg, 1024, 0.001)
     else:
         out, cpu_cfg = run_serve(args, lg)
     if cpu_cfg and not args.no_cpu:
         log("cpu baseline %.0f s" % args.cpu_seconds)
-        if cpu_cfg[0] == "scan":
+        if cpu_cfg[0] == "reference":
+            out["cpu_baseline"] = reference_program_baseline(*cpu_cfg[1], args.cpu_seconds)
+            if out["cpu_baseline"] is None:  # reference program not built here: the oracle port
+                L, T, _ = cpu_cfg[1]
+                out["cpu_baseline"] = cpu_baseline(0, L, 1.0 / T, args.cpu_seconds)
+        elif cpu_cfg[0] == "scan":
             out["cpu_baseline"] = cpu_pool_baseline(cpu_cfg[1], args.cpu_seconds,
                                                     "the 512 chains of the scan in order")
         elif cpu_cfg[0] == "serve":
