@@ -570,8 +570,7 @@ __global__ void __launch_bounds__(64 * WAVES) k_link(DevBatch B) {
   __shared__ double lt_all[SITES][CAP];           // staged times, list order; then local parents
   __shared__ uint16_t lpre_all[SITES][CAP + 4];   // off-diagonal prefix counts; then element ranks
   __shared__ uint16_t inv_all[SITES][CAP];        // rank -> staged element
-  __shared__ uint8_t lsp_all[SITES][CAP];         // staged element -> spin of the segment below
-  __shared__ uint8_t lfl_all[SITES][CAP];         // staged element -> event flags
+  __shared__ uint8_t lfl_all[SITES][CAP];         // staged element -> event flags | spin below << 7
   __shared__ int wn0[SITES], wn1[SITES], wtot[SITES], wnbd[SITES];
   __shared__ int wbond[SITES][4], woff[SITES][4];
   // site slot of the step; with whole waves per site it is wave-uniform (read
@@ -581,7 +580,6 @@ __global__ void __launch_bounds__(64 * WAVES) k_link(DevBatch B) {
   double* lt = lt_all[w];
   uint16_t* lpre = lpre_all[w];
   uint16_t* inv = inv_all[w];
-  uint8_t* lsp = lsp_all[w];
   uint8_t* lfl = lfl_all[w];
   int* lpar = reinterpret_cast<int*>(&lt_all[0][0]);  // SITES * CAP * 2 ints
   int slo, shi;
@@ -674,8 +672,7 @@ __global__ void __launch_bounds__(64 * WAVES) k_link(DevBatch B) {
         if (side) { B.l1[k] = node; B.u1[k] = inner ? ~up : up; } else { B.l0[k] = node; B.u0[k] = up; }
         if (lds) {
           inv[rank] = (uint16_t)e;
-          lsp[e] = sp;
-          lfl[e] = fl;
+          lfl[e] = (uint8_t)(fl | (sp << 7));  // flags use bits 0-5
         } else {
           B.evt_new[node] = tau;
           B.segspin_new[node] = sp;
@@ -691,8 +688,9 @@ __global__ void __launch_bounds__(64 * WAVES) k_link(DevBatch B) {
       for (int r = sl; r < tot; r += TPS) {
         const int e = inv[r];
         B.evt_new[n0 + r] = lt[e];
-        B.segspin_new[n0 + r] = lsp[e];
-        B.evflags[n0 + r] = lfl[e];
+        const uint8_t f = lfl[e];
+        B.segspin_new[n0 + r] = f >> 7;
+        B.evflags[n0 + r] = f & 0x7f;
         if (B.nchains > 1) B.node_site[n0 + r] = gs;
         lpre[e] = (uint16_t)r;
       }
