@@ -126,6 +126,36 @@ def test_launch_settings_leave_trajectories_bit_exact(loopgpu, cfg):
         np.testing.assert_array_equal(gs, cs)
 
 
+LARGE = [
+    # square L, beta, sweeps: the last sweeps hold 2.6M and 4.4-5.2M segments,
+    # where the estimator and the collector take 8 and 16 items per thread
+    (128, 96.0, 3),
+    (128, 160.0, 4),
+]
+
+
+@pytest.mark.parametrize("cfg", LARGE, ids=lambda c: "L%d_b%g" % c[:2])
+def test_large_configurations_match_oracle(loopgpu, cfg):
+    """sizes of the benchmark configuration (larger estimator and collector
+    chunks than the small cases above), with and without the compress pass"""
+    L, beta, n = cfg
+    print("large", cfg, flush=True)
+    c = O.PhiloxChain(O.SQUARE, L, beta, 1.0, 1.0, seed=5, chain_id=2)
+    rc = c.sweep(n)
+    cs, cbo, ct, cty = c.state()
+    assert 2 * rc[-1, 0] > 2.2e6          # past the 4-item chunks
+    for comp in (1, 0):
+        g = loopgpu.Chain(loopgpu.params(O.SQUARE, L, beta, 1.0, 1.0, seed=5, chain_id=2))
+        g.set_launch(compress=comp)
+        _cmp_records(g.sweep(n), rc, "large %s compress %d" % (cfg, comp))
+        gs, gbo, gt, gty = g.get_config()
+        np.testing.assert_array_equal(gbo, cbo)
+        np.testing.assert_array_equal(gt, ct)
+        np.testing.assert_array_equal(gty, cty)
+        np.testing.assert_array_equal(gs, cs)
+        g.close()
+
+
 def test_compress_mode_switches_keep_trajectory(loopgpu):
     """switching between the compress pass and the estimator's own root walk
     between sweeps (accumulators cleared on the switch) keeps the oracle's
