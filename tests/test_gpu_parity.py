@@ -127,10 +127,12 @@ def test_launch_settings_leave_trajectories_bit_exact(loopgpu, cfg):
 
 
 LARGE = [
-    # square L, beta, sweeps: the last sweeps hold 2.6M and 4.4-5.2M segments,
-    # where the estimator and the collector take 8 and 16 items per thread
-    (128, 96.0, 3),
-    (128, 160.0, 4),
+    # square L, beta, sweeps, default linking kernel: the last sweeps hold
+    # 2.6M, 4.4-5.2M and 7M segments, where the estimator and the collector
+    # take 8 and 16 items per thread; about 640 events per site at beta = 256
+    (128, 96.0, 3, 512),
+    (128, 160.0, 4, 512),
+    (128, 256.0, 3, 1536),
 ]
 
 
@@ -138,7 +140,7 @@ LARGE = [
 def test_large_configurations_match_oracle(loopgpu, cfg):
     """sizes of the benchmark configuration (larger estimator and collector
     chunks than the small cases above), with and without the compress pass"""
-    L, beta, n = cfg
+    L, beta, n, link = cfg
     print("large", cfg, flush=True)
     c = O.PhiloxChain(O.SQUARE, L, beta, 1.0, 1.0, seed=5, chain_id=2)
     rc = c.sweep(n)
@@ -146,6 +148,7 @@ def test_large_configurations_match_oracle(loopgpu, cfg):
     assert 2 * rc[-1, 0] > 2.2e6          # past the 4-item chunks
     for comp in (1, 0):
         g = loopgpu.Chain(loopgpu.params(O.SQUARE, L, beta, 1.0, 1.0, seed=5, chain_id=2))
+        assert g.launch()["link"] == link
         g.set_launch(compress=comp)
         _cmp_records(g.sweep(n), rc, "large %s compress %d" % (cfg, comp))
         gs, gbo, gt, gty = g.get_config()
