@@ -36,8 +36,15 @@ std::string num_label(double v) {
 }
 }  // namespace
 
-Correlation::Correlation(int lattice, int L, bool correlations, bool structure_factor, int origin)
-    : corr_(correlations), sfac_(structure_factor), origin_(origin) {
+Correlation::Correlation(int lattice, int L, bool correlations, bool structure_factor, int origin,
+                         std::vector<Custom> custom)
+    : corr_(correlations), sfac_(structure_factor), origin_(origin), custom_(std::move(custom)) {
+  need_corr_ = corr_;
+  need_sfac_ = sfac_;
+  for (const auto& c : custom_) {
+    if (c.kind == 2) need_corr_ = true;
+    if (c.kind == 3) need_sfac_ = true;
+  }
   nx_ = (lattice == LG_LATTICE_CHAIN) ? 1 : L;
   ny_ = L;
   n_ = nx_ * ny_;
@@ -46,11 +53,12 @@ Correlation::Correlation(int lattice, int L, bool correlations, bool structure_f
     if (nx_ == 1) return "( " + std::to_string(s) + " )";
     return "( " + std::to_string(s / ny_) + "," + std::to_string(s % ny_) + " )";
   };
-  if (corr_) {
+  for (int r = 0; r < n_; ++r) site_.push_back(coord(r));
+  if (need_corr_) {
     const std::string o = coord(origin_ < 0 ? 0 : origin_);
     for (int r = 0; r < n_; ++r) dist_.push_back(o + " -- " + coord(r));
   }
-  if (sfac_) {
+  if (need_sfac_) {
     for (int a = 0; a < nx_; ++a)
       for (int b = 0; b < ny_; ++b) {
         if (nx_ == 1) mom_.push_back("( " + num_label(momentum(b, ny_)) + " )");
@@ -72,7 +80,7 @@ double Correlation::momentum(int n, int L) {
 
 void Correlation::compute(const uint8_t* spins, double* ucorr, double* scorr, double* sfac) const {
   const int nx = nx_, ny = ny_, n = n_;
-  if (corr_ && origin_ >= 0) {
+  if (need_corr_ && origin_ >= 0) {
     // INITIAL_SITE: correlations of the origin with every site
     const int so = 1 - 2 * (spins[origin_] & 1);
     const int go = ((origin_ / ny + origin_ % ny) & 1) ? -1 : 1;
@@ -82,7 +90,7 @@ void Correlation::compute(const uint8_t* spins, double* ucorr, double* scorr, do
       ucorr[r] = 0.25 * so * sr;
       scorr[r] = 0.25 * go * gr * so * sr;
     }
-  } else if (corr_) {
+  } else if (need_corr_) {
     const int W = (ny + 63) / 64;
     std::vector<uint64_t> row((size_t)nx * W, 0), rot((size_t)nx * W);
     for (int x = 0; x < nx; ++x)
@@ -108,7 +116,7 @@ void Correlation::compute(const uint8_t* spins, double* ucorr, double* scorr, do
       }
     }
   }
-  if (sfac_) {
+  if (need_sfac_) {
     // F(a, y) = sum_x v(x, y) e^(i 2 pi a x / nx), then over y
     std::vector<double> fr((size_t)nx * ny), fi((size_t)nx * ny);
     for (int a = 0; a < nx; ++a)
@@ -148,6 +156,22 @@ void Correlation::measure(const uint8_t* spins, std::map<std::string, Observable
     }
   if (sfac_)
     for (int k = 0; k < n_; ++k) m["Spin Structure Factor[" + mom_[k] + "]"] << f[k];
+  // custom measurements of Sz (custom.h:147-250): Sz = 1/2 - spin per site;
+  // Sz:Sz correlations and structure factor are the spin correlations and
+  // the spin structure factor above
+  for (const auto& c : custom_) {
+    if (c.kind == 0) {
+      double v = 0;
+      for (int r = 0; r < n_; ++r) v += 0.5 - (spins[r] & 1);
+      m[c.name] << v / n_;
+    } else if (c.kind == 1) {
+      for (int r = 0; r < n_; ++r) m[c.name + "[" + site_[r] + "]"] << 0.5 - (spins[r] & 1);
+    } else if (c.kind == 2) {
+      for (int d = 0; d < n_; ++d) m[c.name + "[" + dist_[d] + "]"] << u[d];
+    } else {
+      for (int k = 0; k < n_; ++k) m[c.name + "[" + mom_[k] + "]"] << f[k];
+    }
+  }
 }
 
 std::vector<std::string> Correlation::names() const {
@@ -159,6 +183,12 @@ std::vector<std::string> Correlation::names() const {
     }
   if (sfac_)
     for (const auto& k : mom_) v.push_back("Spin Structure Factor[" + k + "]");
+  for (const auto& c : custom_) {
+    if (c.kind == 0) v.push_back(c.name);
+    const auto& labels = c.kind == 1 ? site_ : (c.kind == 2 ? dist_ : mom_);
+    if (c.kind != 0)
+      for (const auto& l : labels) v.push_back(c.name + "[" + l + "]");
+  }
   return v;
 }
 

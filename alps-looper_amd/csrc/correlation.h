@@ -15,9 +15,13 @@ namespace looper_amd {
 
 class Correlation {
  public:
+  // custom measurement of Sz: kind 0 average, 1 local, 2 correlations (Sz:Sz),
+  // 3 structure factor (Sz:Sz); name as given in MEASURE_...[name]
+  struct Custom { int kind; std::string name; };
   // origin < 0: every pair, by displacement; else the INITIAL_SITE's pairs
-  Correlation(int lattice, int L, bool correlations, bool structure_factor, int origin);
-  bool enabled() const { return corr_ || sfac_; }
+  Correlation(int lattice, int L, bool correlations, bool structure_factor, int origin,
+              std::vector<Custom> custom = {});
+  bool enabled() const { return corr_ || sfac_ || !custom_.empty(); }
   // per-site / per-displacement / per-momentum values (N each)
   void compute(const uint8_t* spins, double* ucorr, double* scorr, double* sfac) const;
   void measure(const uint8_t* spins, std::map<std::string, Observable>& m) const;
@@ -26,7 +30,10 @@ class Correlation {
 
  private:
   bool corr_, sfac_;
+  bool need_corr_, need_sfac_;  // also for custom measurements
   int origin_;
+  std::vector<Custom> custom_;
+  std::vector<std::string> site_;
   int nx_ = 1, ny_ = 1, n_ = 1;
   std::vector<std::string> dist_, mom_;
   std::vector<double> cos_, sin_;

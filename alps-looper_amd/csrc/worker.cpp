@@ -55,6 +55,11 @@ class Parameters {
     add(cur);
   }
   bool defined(const std::string& k) const { return kv_.count(k) > 0; }
+  std::vector<std::string> keys() const {
+    std::vector<std::string> v;
+    for (const auto& kv : kv_) v.push_back(kv.first);
+    return v;
+  }
   std::string str(const std::string& k, const std::string& def = "") const {
     auto it = kv_.find(k);
     if (it == kv_.end()) return def;
@@ -228,12 +233,30 @@ class loop_worker {
     if (flag("MEASURE[Green Function]"))
       std::fprintf(stderr, "WARNING: Green funciton measurement is not implemented yet\n");
     const bool mc = flag("MEASURE[Correlations]"), ms = flag("MEASURE[Structure Factor]");
-    if (mc || ms) {
+    // ALPS custom measurements (custom.h:63-250) in their forms for the
+    // S=1/2 site operator Sz: MEASURE_AVERAGE / MEASURE_LOCAL [name] = Sz,
+    // MEASURE_CORRELATIONS / MEASURE_STRUCTURE_FACTOR [name] = "Sz:Sz"
+    std::vector<Correlation::Custom> custom;
+    const char* kinds[] = {"MEASURE_AVERAGE[", "MEASURE_LOCAL[", "MEASURE_CORRELATIONS[", "MEASURE_STRUCTURE_FACTOR["};
+    for (const auto& k : p.keys()) {
+      for (int t = 0; t < 4; ++t) {
+        const std::string pre = kinds[t];
+        if (k.compare(0, pre.size(), pre) != 0 || k.back() != ']') continue;
+        const std::string name = k.substr(pre.size(), k.size() - pre.size() - 1);
+        std::string e = p.str(k);
+        e.erase(std::remove(e.begin(), e.end(), ' '), e.end());
+        if ((t < 2 && e == "Sz") || (t >= 2 && e == "Sz:Sz")) custom.push_back({t, name});
+        else
+          std::fprintf(stderr, "WARNING: custom measurement %s = %s is not supported (only Sz and Sz:Sz); skipped\n",
+                       k.c_str(), e.c_str());
+      }
+    }
+    if (mc || ms || !custom.empty()) {
       if (lp_.L < 2 || lp_.L % 2) throw std::invalid_argument("L must be even");
       const int origin = p.defined("INITIAL_SITE") ? (int)std::llround(p.num("INITIAL_SITE")) : -1;
       if (p.defined("INITIAL_SITE") && (origin < 0 || origin >= n_sites_))
         throw std::invalid_argument("INITIAL_SITE out of range");
-      corr_.reset(new Correlation(lp_.lattice, lp_.L, mc, ms, origin));
+      corr_.reset(new Correlation(lp_.lattice, lp_.L, mc, ms, origin, custom));
       spins_.resize(n_sites_);
       spins8_.resize(n_sites_);
     }

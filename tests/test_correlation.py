@@ -177,3 +177,44 @@ def test_worker_correlations_match_exact_diagonalisation(loopgpu, kind, L, T, or
     # the scalar observables are unchanged by the extra measurements
     m, e, _ = w.result("Energy Density")
     assert stats.sigmas(m, e, ref["Energy"] / N) < 4.0
+
+
+@pytest.mark.gpu
+def test_worker_custom_measurements_of_sz(loopgpu):
+    """MEASURE_AVERAGE / MEASURE_LOCAL [name] = Sz and MEASURE_CORRELATIONS /
+    MEASURE_STRUCTURE_FACTOR [name] = "Sz:Sz" (custom.h:147-250), with the
+    parameters of the reference's chain L=4, T=0.5 run: against exact
+    diagonalisation, against that run's output and against the built-in
+    correlation estimators; other operator expressions are skipped."""
+    print("worker custom measurements chain 4 0.5", flush=True)
+    txt = ('LATTICE = "chain lattice"\nL = 4\nT = 0.5\nSWEEPS = 40000\nTHERMALIZATION = 2000\nSEED = 23125\n'
+           'MEASURE[Correlations] = true\nMEASURE[Structure Factor] = true\n')
+    txt += "".join('%s = "%s"\n' % (k, v) for k, v in FX["params"].items() if k.startswith("MEASURE_"))
+    w = loopgpu.Worker(txt)
+    w.run()
+    ref = ed.diagonalize("chain", 4, 0.5)
+    dist, mom = C.labels("chain", 4, None)
+    res = FX["results"]
+    m, e, _ = w.result("Sz Average")
+    assert stats.sigmas(m, e, 0.0) < 4.0
+    assert stats.sigmas(m, e, *res["Sz Average"]) < 4.0
+    for i in range(4):
+        name = "Sz Local[( %d )]" % i
+        m, e, _ = w.result(name)
+        assert stats.sigmas(m, e, 0.0) < 4.0, name
+        assert stats.sigmas(m, e, *res[name]) < 4.0, name
+    for custom, builtin, labels, exact in [("Sz Correlations", "Spin Correlations", dist, ref["Spin Correlations"]),
+                                           ("Sz Structure Factor", "Spin Structure Factor", mom, ref["Spin Structure Factor"])]:
+        for lab, v in zip(labels, exact):
+            name = "%s[%s]" % (custom, lab)
+            got = w.result(name)
+            assert got == w.result("%s[%s]" % (builtin, lab)), name
+            m, e, _ = got
+            if e == 0:
+                assert m == pytest.approx(v, abs=1e-12), name
+            else:
+                assert stats.sigmas(m, e, v) < 4.0, (name, m, e, v)
+                assert stats.sigmas(m, e, *res[name]) < 4.0, name
+    assert "Exchange" not in res
+    with pytest.raises(loopgpu.LoopGPUError):
+        w.result("Exchange")
