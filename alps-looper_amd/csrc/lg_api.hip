@@ -329,8 +329,9 @@ int Batch::allocate() {
   if ((rc = alloc(u1, op_cap))) return rc;
   if ((rc = alloc(evflags, node_cap))) return rc;
   if ((rc = alloc(chain_node0, cap_chains))) return rc;
-  // node_site only serves batches of more than one chain
-  if ((rc = alloc(node_site, cap_chains > 1 ? node_cap : 1))) return rc;
+  // node_site: every segment's site in batches of more than one chain, the
+  // first segment's site of each site for one chain
+  if ((rc = alloc(node_site, node_cap))) return rc;
   if ((rc = alloc(parent, node_cap))) return rc;
   if ((rc = alloc(fbit, node_cap))) return rc;
   if ((rc = alloc(acc, node_cap))) return rc;

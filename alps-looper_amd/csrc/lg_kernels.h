@@ -49,7 +49,7 @@ struct DevBatch {
   int* u1;                     // [new ops] upper-leg node on the target site
   uint8_t* evflags;            // [nodes]
   int* chain_node0;            // [nchains] first segment node of each chain (new configuration)
-  int* node_site;              // [nodes] site of each segment (written for batches of more than one chain)
+  int* node_site;              // [nodes] site of each segment (batches of several chains) or of each site's first segment
   int* parent;                 // [nodes]
   uint8_t* fbit;               // [nodes] flip bit of the node's cluster
   ClusterAcc* acc;             // [nodes] (roots only)

@@ -115,22 +115,16 @@ __device__ __forceinline__ bool overflowed(const DevBatch& B) {
   return __hip_atomic_load(B.flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
 }
 
-// Site of each segment node (node_site) is stored only for batches of more
-// than one chain; a single chain finds its sites by binary search over the
-// segment offsets, which only the first segment of each site needs.
+// Site of each segment node (node_site): stored for every segment in batches
+// of more than one chain, and for the first segment of each site (the only
+// one that needs it) when the batch is one chain.
 __device__ __forceinline__ int node_chain(const DevBatch& B, int node) {
   return B.nchains == 1 ? 0 : B.site_chain[B.node_site[node]];
 }
 
 // site (global index) whose first segment is `node`
 __device__ __forceinline__ int first_node_site(const DevBatch& B, int node) {
-  if (B.nchains > 1) return B.node_site[node];
-  int lo = 0, n = B.nsites;
-  while (n > 1) {
-    const int h = n >> 1;
-    if (B.soff_new[lo + h] <= node) { lo += h; n -= h; } else { n = h; }
-  }
-  return lo;
+  return B.node_site[node];
 }
 
 // Lower bound of each lane's time in a site's event times t[0..m) (ascending),
@@ -421,7 +415,7 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan(DevBatch B) {
           B.segspin_new[node] = B.spins_old[P.old_site_base + (i - P.site_base)];
           B.evflags[node] = (uint8_t)(EV_EMPTY | EV_FIRST | (gauge(P, i - P.site_base) < 0 ? EV_GAUGE : 0));
           B.parent[node] = node;
-          if (B.nchains > 1) B.node_site[node] = i;
+          B.node_site[node] = i;
         }
       }
     }
@@ -678,7 +672,7 @@ __global__ void __launch_bounds__(64 * WAVES) k_link(DevBatch B) {
           B.segspin_new[node] = sp;
           B.evflags[node] = fl;
           B.parent[node] = node;
-          if (B.nchains > 1) B.node_site[node] = gs;
+          if (B.nchains > 1 || rank == 0) B.node_site[node] = gs;
         }
       }
     }
@@ -691,7 +685,7 @@ __global__ void __launch_bounds__(64 * WAVES) k_link(DevBatch B) {
         const uint8_t f = lfl[e];
         B.segspin_new[n0 + r] = f >> 7;
         B.evflags[n0 + r] = f & 0x7f;
-        if (B.nchains > 1) B.node_site[n0 + r] = gs;
+        if (B.nchains > 1 || r == 0) B.node_site[n0 + r] = gs;
         lpre[e] = (uint16_t)r;
       }
     }
@@ -787,7 +781,7 @@ __global__ void __launch_bounds__(256) k_link_ops(DevBatch B) {
       B.evflags[node] = (uint8_t)((info & 1) | (side ? 0 : EV_SRC) | dflag | (rank == 0 ? EV_FIRST : 0) |
                                   (gauge(P, s) < 0 ? EV_GAUGE : 0));
       B.parent[node] = node;
-      if (B.nchains > 1) B.node_site[node] = gs;
+      if (B.nchains > 1 || rank == 0) B.node_site[node] = gs;
       if (side) { B.l1[k] = node; B.u1[k] = up; } else { B.l0[k] = node; B.u0[k] = up; }
     }
   }
@@ -995,13 +989,26 @@ __global__ void __launch_bounds__(EST_THREADS) k_estimate(DevBatch B) {
     for (int g0 = 0; g0 < items; g0 += EST_U) {
       const int ib = chunk * echunk + g0 * EST_THREADS + tid;
       if (__builtin_amdgcn_readfirstlane(ib) >= M) break;  // lane 0 holds the wave's lowest node
-      int r[EST_U];
-      uint32_t f[EST_U];
+      // items past the end read node M - 1 and are dropped before any store,
+      // so that every load is issued without a branch around it
+      int nd[EST_U], r[EST_U];
+      uint8_t fl[EST_U], sp[EST_U], flo[EST_U];
+      double thi[EST_U], tlo[EST_U];
+#pragma unroll
+      for (int u = 0; u < EST_U; ++u) nd[u] = min(ib + u * EST_THREADS, M - 1);
 #pragma unroll
       for (int u = 0; u < EST_U; ++u) {
-        const int i = ib + u * EST_THREADS;
-        r[u] = i < M ? B.parent[i] : -1;
+        r[u] = B.parent[nd[u]];
+        fl[u] = B.evflags[nd[u]];
+        sp[u] = B.segspin_new[nd[u]];
+        thi[u] = B.evt_new[nd[u]];
+        // the segment below: the previous node, except for a site's first
+        // segment (replaced below)
+        const int pv = max(nd[u] - 1, 0);
+        tlo[u] = B.evt_new[pv];
+        flo[u] = B.evflags[pv];
       }
+      uint32_t f[EST_U];
       if (WALK) {
         // walks to the roots, one hop of every item at a time
         int p0[EST_U];
@@ -1010,7 +1017,7 @@ __global__ void __launch_bounds__(EST_THREADS) k_estimate(DevBatch B) {
         for (bool more = true; more;) {
           int q[EST_U];
 #pragma unroll
-          for (int u = 0; u < EST_U; ++u) q[u] = r[u] >= 0 ? B.parent[r[u]] : r[u];
+          for (int u = 0; u < EST_U; ++u) q[u] = B.parent[r[u]];
           more = false;
 #pragma unroll
           for (int u = 0; u < EST_U; ++u)
@@ -1018,46 +1025,50 @@ __global__ void __launch_bounds__(EST_THREADS) k_estimate(DevBatch B) {
         }
 #pragma unroll
         for (int u = 0; u < EST_U; ++u) {
-          const int i = ib + u * EST_THREADS;
-          if (i >= M) continue;
-          if (r[u] != p0[u]) B.parent[i] = r[u];
-          const int c = chain_of_node(B, i);
+          const int c = chain_of_node(B, nd[u]);
           const ChainDev& P = B.chains[c];
           f[u] = P.h != 0.0 ? 0u
                : (philox(B.sweep[c], STAGE_FLIP, (uint32_t)(r[u] - B.chain_node0[c]), 0u, P.key0, P.key1).x & 1u);
+        }
+#pragma unroll
+        for (int u = 0; u < EST_U; ++u) {
+          const int i = ib + u * EST_THREADS;
+          if (i >= M) continue;
+          if (r[u] != p0[u]) B.parent[i] = r[u];
           B.fbit[i] = (uint8_t)f[u];
         }
       } else {
 #pragma unroll
-        for (int u = 0; u < EST_U; ++u) f[u] = r[u] >= 0 ? B.fbit[r[u]] : 0u;  // drawn for the root by k_uf_compress
+        for (int u = 0; u < EST_U; ++u) f[u] = B.fbit[r[u]];  // drawn for the root by k_uf_compress
 #pragma unroll
         for (int u = 0; u < EST_U; ++u) {
           const int i = ib + u * EST_THREADS;
           if (i < M && i != r[u]) B.fbit[i] = (uint8_t)f[u];
         }
       }
-      uint8_t fl[EST_U], sp[EST_U], flo[EST_U];
-      double thi[EST_U], tlo[EST_U];
-      int gs[EST_U], prev[EST_U];
+      // a site's first segment wraps through tau = 1: the segment below it is
+      // the site's last
+      // (one wave-uniform branch; the loads in it are issued for every lane,
+      // at node 0 / site 0 for lanes without a first segment)
+      int gs[EST_U] = {};
+      bool anyfirst = false;
 #pragma unroll
-      for (int u = 0; u < EST_U; ++u) {
-        const int i = ib + u * EST_THREADS;
-        fl[u] = i < M ? B.evflags[i] : (uint8_t)EV_EMPTY;
-        sp[u] = i < M ? B.segspin_new[i] : 0;
-        thi[u] = i < M ? B.evt_new[i] : 0.0;
-      }
+      for (int u = 0; u < EST_U; ++u) anyfirst |= (fl[u] & EV_FIRST) != 0;
+      if (__ballot(anyfirst)) {
 #pragma unroll
-      for (int u = 0; u < EST_U; ++u) {
-        const int i = ib + u * EST_THREADS;
-        gs[u] = (i < M && (fl[u] & EV_FIRST)) ? first_node_site(B, i) : 0;
-        // the segment below the site's first event wraps through tau = 1
-        prev[u] = (fl[u] & EV_FIRST) ? B.soff_new[gs[u] + 1] - 1 : i - 1;
-      }
+        for (int u = 0; u < EST_U; ++u) {
+          const int g = first_node_site(B, (fl[u] & EV_FIRST) ? nd[u] : 0);
+          gs[u] = (fl[u] & EV_FIRST) ? g : 0;
+        }
+        int pv[EST_U];
 #pragma unroll
-      for (int u = 0; u < EST_U; ++u) {
-        const bool need = !(fl[u] & EV_EMPTY);
-        tlo[u] = need ? B.evt_new[prev[u]] : 0.0;
-        flo[u] = need ? B.evflags[prev[u]] : (uint8_t)0;
+        for (int u = 0; u < EST_U; ++u) pv[u] = B.soff_new[gs[u] + 1] - 1;
+#pragma unroll
+        for (int u = 0; u < EST_U; ++u) {
+          const double t = B.evt_new[pv[u]];
+          const uint8_t fo = B.evflags[pv[u]];
+          if ((fl[u] & (EV_FIRST | EV_EMPTY)) == EV_FIRST) { tlo[u] = t; flo[u] = fo; }
+        }
       }
 #pragma unroll
       for (int u = 0; u < EST_U; ++u) {
@@ -1160,6 +1171,7 @@ __global__ void __launch_bounds__(256) k_field_apply(DevBatch B) {
 // ---------------------------------------------------------------------------
 constexpr int COL_THREADS = 256;
 constexpr int COL_MIN_CHUNK = COL_THREADS * 4;  // smallest collector chunk (chunk_items)
+constexpr int COL_U = 4;                         // items per lane in flight (divides chunk_items)
 constexpr int NCOL = NUM_COLLECTOR_TERMS;
 
 // Collector terms of one cluster.  h, flip: the chain's field and the
@@ -1210,9 +1222,20 @@ __global__ void __launch_bounds__(COL_THREADS) k_flip_collect(DevBatch B, int cl
   const int M = B.totals[1];
   int klo, khi;
   xcd_share(nops, klo, khi);
-  for (int k = klo + threadIdx.x; k < khi; k += blockDim.x) {
-    const int l0 = B.l0[k], u0 = B.u0[k];
-    if (B.fbit[l0] ^ B.fbit[u0]) B.oi_new[k] ^= 1;
+  // COL_U operators per lane at a time, every load issued before any store
+  // (the last operator of the share stands in for those past its end)
+  for (int k0 = klo + threadIdx.x; k0 < khi; k0 += COL_U * COL_THREADS) {
+    int kk[COL_U], a[COL_U], b[COL_U], t[COL_U];
+    uint8_t fa[COL_U], fb[COL_U];
+#pragma unroll
+    for (int u = 0; u < COL_U; ++u) kk[u] = min(k0 + u * COL_THREADS, khi - 1);
+#pragma unroll
+    for (int u = 0; u < COL_U; ++u) { a[u] = B.l0[kk[u]]; b[u] = B.u0[kk[u]]; t[u] = B.oi_new[kk[u]]; }
+#pragma unroll
+    for (int u = 0; u < COL_U; ++u) { fa[u] = B.fbit[a[u]]; fb[u] = B.fbit[b[u]]; }
+#pragma unroll
+    for (int u = 0; u < COL_U; ++u)
+      if (k0 + u * COL_THREADS < khi && (fa[u] ^ fb[u])) B.oi_new[kk[u]] = t[u] ^ 1;
   }
   // per-chunk sums by chain: slot k holds chain c_first + k (chains are
   // contiguous in node order, so a chunk meets few of them)
@@ -1243,9 +1266,32 @@ __global__ void __launch_bounds__(COL_THREADS) k_flip_collect(DevBatch B, int cl
         for (int q = 0; q < NCOL; ++q) unsafeAtomicAdd(rec + term_slot(q), w[q]);
       }
     };
-    for (int it = 0; it < citems; ++it) {
-      const int i = base + it * COL_THREADS + threadIdx.x;
-      if (i < M && B.parent[i] == i) {
+    // COL_U nodes per lane at a time: parents, then the accumulators and
+    // flip bits of the roots among them (a non-root reads the chunk's first
+    // node's, a cached line, and drops it), then the sums
+    for (int it0 = 0; it0 < citems; it0 += COL_U) {
+      const int ib = base + it0 * COL_THREADS + threadIdx.x;
+      if (__builtin_amdgcn_readfirstlane(ib) >= M) break;  // lane 0 holds the wave's lowest node
+      int nd[COL_U], par[COL_U];
+      bool root[COL_U];
+#pragma unroll
+      for (int u = 0; u < COL_U; ++u) nd[u] = min(ib + u * COL_THREADS, M - 1);
+#pragma unroll
+      for (int u = 0; u < COL_U; ++u) par[u] = B.parent[nd[u]];
+#pragma unroll
+      for (int u = 0; u < COL_U; ++u) root[u] = ib + u * COL_THREADS < M && par[u] == nd[u];
+      ClusterAcc ac[COL_U];
+      uint8_t fr[COL_U];
+#pragma unroll
+      for (int u = 0; u < COL_U; ++u) {
+        const int j = root[u] ? nd[u] : base;
+        ac[u] = B.acc[j];
+        fr[u] = B.fbit[j];
+      }
+#pragma unroll
+      for (int u = 0; u < COL_U; ++u) {
+        if (!root[u]) continue;
+        const int i = nd[u];
         const int c = (i < c0_end) ? c0 : node_chain(B, i);
         if (c != cl) {
           if (cl >= 0) flush(cl);
@@ -1254,7 +1300,7 @@ __global__ void __launch_bounds__(COL_THREADS) k_flip_collect(DevBatch B, int cl
           cl = c;
         }
         double v[NCOL];
-        cluster_terms(B.acc[i], B.chains[c].h, B.fbit[i], v);
+        cluster_terms(ac[u], B.chains[c].h, fr[u], v);
         if (clear) B.acc[i] = ClusterAcc{{0, 0, 0, 0}, {0, 0, 0, 0}, 0, 0};
 #pragma unroll
         for (int q = 0; q < NCOL; ++q) w[q] += v[q];
