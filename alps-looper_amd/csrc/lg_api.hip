@@ -653,16 +653,19 @@ int Batch::tune(uint32_t sweeps_per_trial) {
   const int groups[] = {4, 8, 16, 32, 64};
   const int mults[] = {2, 4, 8, 16, 32, 64};
   const int links[] = {0, 128, 512, 1536};
+  const int flushes[] = {16, 8, 4};
   // group size first with the current grids, then the grids with the best
-  // group, then the linking kernel, then the compress pass
-  const int trials[] = {5, 6, 4, 2};
-  for (int pass = 0; pass < 4; ++pass) {
+  // group, then the linking kernel, then the compress pass, then the
+  // estimator's flush interval
+  const int trials[] = {5, 6, 4, 2, 3};
+  for (int pass = 0; pass < 5; ++pass) {
     for (int i = 0; i < trials[pass]; ++i) {
       LaunchCfg trial = best;
       if (pass == 0) trial.group = groups[i];
       else if (pass == 1) trial.blocks_op = trial.blocks_node = cus * mults[i];
       else if (pass == 2) trial.link_cap = links[i];
-      else trial.compress = i;
+      else if (pass == 3) trial.compress = i;
+      else trial.est_flush = flushes[i];
       rc = apply_cfg(trial);
       if (rc) return rc;
       rc = synchronize();
@@ -683,6 +686,7 @@ void Batch::get_launch(lg_launch* out) const {
   out->blocks = cfg.blocks_op;
   out->link = cfg.link_cap;
   out->compress = cfg.compress;
+  out->flush = cfg.est_flush;
 }
 
 int Batch::set_launch(const lg_launch& l) {
@@ -704,6 +708,10 @@ int Batch::set_launch(const lg_launch& l) {
   if (l.compress >= 0) {
     if (l.compress > 1) return set_error(LG_ERR_ARG, "launch compress must be 0 or 1");
     c.compress = l.compress;
+  }
+  if (l.flush >= 0) {
+    if (l.flush != 4 && l.flush != 8 && l.flush != 16) return set_error(LG_ERR_ARG, "launch flush must be 4, 8 or 16");
+    c.est_flush = l.flush;
   }
   tuned = true;
   return apply_cfg(c);

@@ -963,7 +963,7 @@ __device__ __forceinline__ void ht_add(int* hkey, double (*hL)[HT], unsigned int
 }
 
 template <bool WALK>
-__global__ void __launch_bounds__(EST_THREADS) k_estimate(DevBatch B) {
+__global__ void __launch_bounds__(EST_THREADS) k_estimate(DevBatch B, int flush_items) {
   if (overflowed(B)) return;
   __shared__ int hkey[HT];
   __shared__ double hL[4][HT];
@@ -988,133 +988,138 @@ __global__ void __launch_bounds__(EST_THREADS) k_estimate(DevBatch B) {
     // below them) is issued for all its items before any is waited for
     for (int g0 = 0; g0 < items; g0 += EST_U) {
       const int ib = chunk * echunk + g0 * EST_THREADS + tid;
-      if (__builtin_amdgcn_readfirstlane(ib) >= M) break;  // lane 0 holds the wave's lowest node
-      // items past the end read node M - 1 and are dropped before any store,
-      // so that every load is issued without a branch around it
-      int nd[EST_U], r[EST_U];
-      uint8_t fl[EST_U], sp[EST_U], flo[EST_U];
-      double thi[EST_U], tlo[EST_U];
+      if (__builtin_amdgcn_readfirstlane(ib) < M) {  // lane 0 holds the wave's lowest node
+        // items past the end read node M - 1 and are dropped before any store,
+        // so that every load is issued without a branch around it
+        int nd[EST_U], r[EST_U];
+        uint8_t fl[EST_U], sp[EST_U], flo[EST_U];
+        double thi[EST_U], tlo[EST_U];
 #pragma unroll
-      for (int u = 0; u < EST_U; ++u) nd[u] = min(ib + u * EST_THREADS, M - 1);
-#pragma unroll
-      for (int u = 0; u < EST_U; ++u) {
-        r[u] = B.parent[nd[u]];
-        fl[u] = B.evflags[nd[u]];
-        sp[u] = B.segspin_new[nd[u]];
-        thi[u] = B.evt_new[nd[u]];
-        // the segment below: the previous node, except for a site's first
-        // segment (replaced below)
-        const int pv = max(nd[u] - 1, 0);
-        tlo[u] = B.evt_new[pv];
-        flo[u] = B.evflags[pv];
-      }
-      uint32_t f[EST_U];
-      if (WALK) {
-        // walks to the roots, one hop of every item at a time
-        int p0[EST_U];
-#pragma unroll
-        for (int u = 0; u < EST_U; ++u) p0[u] = r[u];
-        for (bool more = true; more;) {
-          int q[EST_U];
-#pragma unroll
-          for (int u = 0; u < EST_U; ++u) q[u] = B.parent[r[u]];
-          more = false;
-#pragma unroll
-          for (int u = 0; u < EST_U; ++u)
-            if (q[u] != r[u]) { r[u] = q[u]; more = true; }
-        }
+        for (int u = 0; u < EST_U; ++u) nd[u] = min(ib + u * EST_THREADS, M - 1);
 #pragma unroll
         for (int u = 0; u < EST_U; ++u) {
-          const int c = chain_of_node(B, nd[u]);
-          const ChainDev& P = B.chains[c];
-          f[u] = P.h != 0.0 ? 0u
-               : (philox(B.sweep[c], STAGE_FLIP, (uint32_t)(r[u] - B.chain_node0[c]), 0u, P.key0, P.key1).x & 1u);
+          r[u] = B.parent[nd[u]];
+          fl[u] = B.evflags[nd[u]];
+          sp[u] = B.segspin_new[nd[u]];
+          thi[u] = B.evt_new[nd[u]];
+          // the segment below: the previous node, except for a site's first
+          // segment (replaced below)
+          const int pv = max(nd[u] - 1, 0);
+          tlo[u] = B.evt_new[pv];
+          flo[u] = B.evflags[pv];
+        }
+        uint32_t f[EST_U];
+        if (WALK) {
+          // walks to the roots, one hop of every item at a time
+          int p0[EST_U];
+#pragma unroll
+          for (int u = 0; u < EST_U; ++u) p0[u] = r[u];
+          for (bool more = true; more;) {
+            int q[EST_U];
+#pragma unroll
+            for (int u = 0; u < EST_U; ++u) q[u] = B.parent[r[u]];
+            more = false;
+#pragma unroll
+            for (int u = 0; u < EST_U; ++u)
+              if (q[u] != r[u]) { r[u] = q[u]; more = true; }
+          }
+#pragma unroll
+          for (int u = 0; u < EST_U; ++u) {
+            const int c = chain_of_node(B, nd[u]);
+            const ChainDev& P = B.chains[c];
+            f[u] = P.h != 0.0 ? 0u
+                 : (philox(B.sweep[c], STAGE_FLIP, (uint32_t)(r[u] - B.chain_node0[c]), 0u, P.key0, P.key1).x & 1u);
+          }
+#pragma unroll
+          for (int u = 0; u < EST_U; ++u) {
+            const int i = ib + u * EST_THREADS;
+            if (i >= M) continue;
+            if (r[u] != p0[u]) B.parent[i] = r[u];
+            B.fbit[i] = (uint8_t)f[u];
+          }
+        } else {
+#pragma unroll
+          for (int u = 0; u < EST_U; ++u) f[u] = B.fbit[r[u]];  // drawn for the root by k_uf_compress
+#pragma unroll
+          for (int u = 0; u < EST_U; ++u) {
+            const int i = ib + u * EST_THREADS;
+            if (i < M && i != r[u]) B.fbit[i] = (uint8_t)f[u];
+          }
+        }
+        // a site's first segment wraps through tau = 1: the segment below it is
+        // the site's last
+        // (one wave-uniform branch; the loads in it are issued for every lane,
+        // at node 0 / site 0 for lanes without a first segment)
+        int gs[EST_U] = {};
+        bool anyfirst = false;
+#pragma unroll
+        for (int u = 0; u < EST_U; ++u) anyfirst |= (fl[u] & EV_FIRST) != 0;
+        if (__ballot(anyfirst)) {
+#pragma unroll
+          for (int u = 0; u < EST_U; ++u) {
+            const int g = first_node_site(B, (fl[u] & EV_FIRST) ? nd[u] : 0);
+            gs[u] = (fl[u] & EV_FIRST) ? g : 0;
+          }
+          int pv[EST_U];
+#pragma unroll
+          for (int u = 0; u < EST_U; ++u) pv[u] = B.soff_new[gs[u] + 1] - 1;
+#pragma unroll
+          for (int u = 0; u < EST_U; ++u) {
+            const double t = B.evt_new[pv[u]];
+            const uint8_t fo = B.evflags[pv[u]];
+            if ((fl[u] & (EV_FIRST | EV_EMPTY)) == EV_FIRST) { tlo[u] = t; flo[u] = fo; }
+          }
         }
 #pragma unroll
         for (int u = 0; u < EST_U; ++u) {
           const int i = ib + u * EST_THREADS;
           if (i >= M) continue;
-          if (r[u] != p0[u]) B.parent[i] = r[u];
-          B.fbit[i] = (uint8_t)f[u];
-        }
-      } else {
-#pragma unroll
-        for (int u = 0; u < EST_U; ++u) f[u] = B.fbit[r[u]];  // drawn for the root by k_uf_compress
-#pragma unroll
-        for (int u = 0; u < EST_U; ++u) {
-          const int i = ib + u * EST_THREADS;
-          if (i < M && i != r[u]) B.fbit[i] = (uint8_t)f[u];
-        }
-      }
-      // a site's first segment wraps through tau = 1: the segment below it is
-      // the site's last
-      // (one wave-uniform branch; the loads in it are issued for every lane,
-      // at node 0 / site 0 for lanes without a first segment)
-      int gs[EST_U] = {};
-      bool anyfirst = false;
-#pragma unroll
-      for (int u = 0; u < EST_U; ++u) anyfirst |= (fl[u] & EV_FIRST) != 0;
-      if (__ballot(anyfirst)) {
-#pragma unroll
-        for (int u = 0; u < EST_U; ++u) {
-          const int g = first_node_site(B, (fl[u] & EV_FIRST) ? nd[u] : 0);
-          gs[u] = (fl[u] & EV_FIRST) ? g : 0;
-        }
-        int pv[EST_U];
-#pragma unroll
-        for (int u = 0; u < EST_U; ++u) pv[u] = B.soff_new[gs[u] + 1] - 1;
-#pragma unroll
-        for (int u = 0; u < EST_U; ++u) {
-          const double t = B.evt_new[pv[u]];
-          const uint8_t fo = B.evflags[pv[u]];
-          if ((fl[u] & (EV_FIRST | EV_EMPTY)) == EV_FIRST) { tlo[u] = t; flo[u] = fo; }
+          const int spin = sp[u];
+          const int sig = 1 - 2 * spin;
+          const bool first = (fl[u] & EV_FIRST) != 0;
+          int dw0 = 0, dw1 = 0;
+          double len = 1.0;
+          if (!(fl[u] & EV_EMPTY)) {
+            len = first ? ((thi[u] + 1.0) - tlo[u]) : (thi[u] - tlo[u]);
+            // this segment is the lower leg of the event above it and the upper
+            // leg of the event below it (stiffness.h:94-105, source legs only)
+            if (fl[u] & EV_SRC) { if (fl[u] & EV_DIR1) dw1 += sig; else dw0 += sig; }
+            if (flo[u] & EV_SRC) { if (flo[u] & EV_DIR1) dw1 -= sig; else dw0 -= sig; }
+          }
+          if (first) B.spins_new[gs[u]] = (uint8_t)(spin ^ f[u]);
+          B.segspin_new[i] = (uint8_t)(spin ^ f[u]);
+          const int cls = (spin << 1) | ((fl[u] & EV_GAUGE) ? 1 : 0);  // seg_class
+          double sL[4] = {0, 0, 0, 0};
+          sL[cls] = len;
+          ht_add(hkey, hL, hn, hw, B, r[u], sL, first ? (1 << (8 * cls)) : 0, pack_winding(dw0, dw1));
         }
       }
+      // the table goes to global memory every flush_items items (a multiple
+      // of EST_U) and at the chunk's end; a block-uniform condition, so every
+      // wave takes the barriers
+      if ((g0 + EST_U) % flush_items != 0 && g0 + EST_U < items) continue;
+      __syncthreads();
+      for (int s = tid; s < HT; s += EST_THREADS) {
+        const int r = hkey[s];
+        if (r < 0) continue;
+        ClusterAcc* a = B.acc + r;
 #pragma unroll
-      for (int u = 0; u < EST_U; ++u) {
-        const int i = ib + u * EST_THREADS;
-        if (i >= M) continue;
-        const int spin = sp[u];
-        const int sig = 1 - 2 * spin;
-        const bool first = (fl[u] & EV_FIRST) != 0;
-        int dw0 = 0, dw1 = 0;
-        double len = 1.0;
-        if (!(fl[u] & EV_EMPTY)) {
-          len = first ? ((thi[u] + 1.0) - tlo[u]) : (thi[u] - tlo[u]);
-          // this segment is the lower leg of the event above it and the upper
-          // leg of the event below it (stiffness.h:94-105, source legs only)
-          if (fl[u] & EV_SRC) { if (fl[u] & EV_DIR1) dw1 += sig; else dw0 += sig; }
-          if (flo[u] & EV_SRC) { if (flo[u] & EV_DIR1) dw1 -= sig; else dw0 -= sig; }
+        for (int k = 0; k < 4; ++k)
+          if (hL[k][s] != 0.0) { unsafeAtomicAdd(&a->L[k], hL[k][s]); hL[k][s] = 0; }
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+          const unsigned int v = hn[q][s];
+          if (v) {
+            if (v & 0xffffu) atomicAdd(&a->n[2 * q], (int)(v & 0xffffu));
+            if (v >> 16) atomicAdd(&a->n[2 * q + 1], (int)(v >> 16));
+            hn[q][s] = 0;
+          }
         }
-        if (first) B.spins_new[gs[u]] = (uint8_t)(spin ^ f[u]);
-        B.segspin_new[i] = (uint8_t)(spin ^ f[u]);
-        const int cls = (spin << 1) | ((fl[u] & EV_GAUGE) ? 1 : 0);  // seg_class
-        double sL[4] = {0, 0, 0, 0};
-        sL[cls] = len;
-        ht_add(hkey, hL, hn, hw, B, r[u], sL, first ? (1 << (8 * cls)) : 0, pack_winding(dw0, dw1));
+        if (hw[s]) { atomicAdd((unsigned long long*)&a->w, hw[s]); hw[s] = 0; }
+        hkey[s] = -1;
       }
+      __syncthreads();
     }
-    __syncthreads();
-    for (int s = tid; s < HT; s += EST_THREADS) {
-      const int r = hkey[s];
-      if (r < 0) continue;
-      ClusterAcc* a = B.acc + r;
-#pragma unroll
-      for (int k = 0; k < 4; ++k)
-        if (hL[k][s] != 0.0) { unsafeAtomicAdd(&a->L[k], hL[k][s]); hL[k][s] = 0; }
-#pragma unroll
-      for (int q = 0; q < 2; ++q) {
-        const unsigned int v = hn[q][s];
-        if (v) {
-          if (v & 0xffffu) atomicAdd(&a->n[2 * q], (int)(v & 0xffffu));
-          if (v >> 16) atomicAdd(&a->n[2 * q + 1], (int)(v >> 16));
-          hn[q][s] = 0;
-        }
-      }
-      if (hw[s]) { atomicAdd((unsigned long long*)&a->w, hw[s]); hw[s] = 0; }
-      hkey[s] = -1;
-    }
-    __syncthreads();
   }
 }
 
@@ -1548,10 +1553,10 @@ void launch_cluster_stage(const DevBatch& B, const LaunchCfg& L, hipStream_t st,
   if (L.compress) {
     hipLaunchKernelGGL(k_uf_compress, dim3(L.blocks_node), dim3(256), 0, st, B);
     mark(ev, 3, st);
-    hipLaunchKernelGGL(k_estimate<false>, dim3(L.blocks_node), dim3(EST_THREADS), 0, st, B);
+    hipLaunchKernelGGL(k_estimate<false>, dim3(L.blocks_node), dim3(EST_THREADS), 0, st, B, L.est_flush);
   } else {
     mark(ev, 3, st);
-    hipLaunchKernelGGL(k_estimate<true>, dim3(L.blocks_node), dim3(EST_THREADS), 0, st, B);
+    hipLaunchKernelGGL(k_estimate<true>, dim3(L.blocks_node), dim3(EST_THREADS), 0, st, B, L.est_flush);
   }
   mark(ev, 4, st);
   if (B.any_field) {

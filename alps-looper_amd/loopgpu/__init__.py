@@ -39,17 +39,17 @@ EXPORTED = [
 
 class LGLaunch(ctypes.Structure):
     _fields_ = [("group", ctypes.c_int32), ("blocks", ctypes.c_int32), ("link", ctypes.c_int32),
-                ("compress", ctypes.c_int32)]
+                ("compress", ctypes.c_int32), ("flush", ctypes.c_int32)]
 
 
 def _launch_get(fn, h):
     out = LGLaunch()
     _check(fn(h, ctypes.byref(out)))
-    return {"group": out.group, "blocks": out.blocks, "link": out.link, "compress": out.compress}
+    return {"group": out.group, "blocks": out.blocks, "link": out.link, "compress": out.compress, "flush": out.flush}
 
 
-def _launch_set(fn, h, group, blocks, link, compress):
-    v = [-1 if x is None else x for x in (group, blocks, link, compress)]
+def _launch_set(fn, h, group, blocks, link, compress, flush):
+    v = [-1 if x is None else x for x in (group, blocks, link, compress, flush)]
     _check(fn(h, ctypes.byref(LGLaunch(*v))))
 
 
@@ -194,8 +194,8 @@ class Chain:
         """current launch settings (include/loopgpu.h lg_launch)"""
         return _launch_get(lib.lg_chain_get_launch, self.h)
 
-    def set_launch(self, group=None, blocks=None, link=None, compress=None):
-        _launch_set(lib.lg_chain_set_launch, self.h, group, blocks, link, compress)
+    def set_launch(self, group=None, blocks=None, link=None, compress=None, flush=None):
+        _launch_set(lib.lg_chain_set_launch, self.h, group, blocks, link, compress, flush)
 
     def info(self):
         ns, nb, no, sw = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int64(), ctypes.c_uint64()
@@ -256,8 +256,8 @@ class Batch:
     def launch(self):
         return _launch_get(lib.lg_batch_get_launch, self.h)
 
-    def set_launch(self, group=None, blocks=None, link=None, compress=None):
-        _launch_set(lib.lg_batch_set_launch, self.h, group, blocks, link, compress)
+    def set_launch(self, group=None, blocks=None, link=None, compress=None, flush=None):
+        _launch_set(lib.lg_batch_set_launch, self.h, group, blocks, link, compress, flush)
 
     def set_graph(self, enable):
         _check(lib.lg_batch_set_graph(self.h, 1 if enable else 0))

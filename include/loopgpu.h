@@ -162,7 +162,8 @@ int lg_exchange_state(lg_batch* h, int32_t* chain_at, uint64_t* accepted, uint64
  * thread per operator; 128, 512 or 1536 a group of threads per site staging
  * that many events in LDS: 16 threads, one wave, four waves); compress: 1 a
  * union-find compression pass before the estimator, 0 the estimator finds
- * the roots itself.  By default they follow
+ * the roots itself; flush: segments per estimator lane (4, 8 or 16) between
+ * flushes of its LDS cluster table to global memory.  By default they follow
  * the expected operator counts and the number of chains, and lg_batch_tune
  * times the candidates.  set: fields < 0 keep their value; the settings then
  * stay as given (no automatic choice) until the next tune.  Results do not
@@ -173,6 +174,7 @@ typedef struct lg_launch {
   int32_t blocks;
   int32_t link;
   int32_t compress;
+  int32_t flush;
 } lg_launch;
 int lg_chain_get_launch(lg_chain* h, lg_launch* out);
 int lg_chain_set_launch(lg_chain* h, const lg_launch* l);

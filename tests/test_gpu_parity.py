@@ -100,7 +100,8 @@ LAUNCH_TRAJ = [
 LAUNCHES = [dict(link=0), dict(link=128), dict(link=512), dict(link=1536), dict(link=512, group=4, blocks=7),
             dict(link=128, group=8, blocks=5),
             dict(link=0, group=16, blocks=64), dict(link=1536, group=32),
-            dict(link=512, compress=1), dict(link=0, compress=0, blocks=9), dict(link=1536, compress=1, group=8)]
+            dict(link=512, compress=1), dict(link=0, compress=0, blocks=9), dict(link=1536, compress=1, group=8),
+            dict(flush=8), dict(flush=4, compress=0), dict(flush=4, link=128, blocks=5)]
 
 
 @pytest.mark.parametrize("cfg", LAUNCH_TRAJ, ids=lambda c: "k%d_L%d_b%g" % c[:3])
@@ -191,7 +192,7 @@ def test_compress_mode_switches_keep_trajectory(loopgpu):
 
 def test_launch_setting_errors(loopgpu):
     g = loopgpu.Chain(loopgpu.params(O.CHAIN, 8, 1.0))
-    for bad in (dict(group=3), dict(link=100), dict(blocks=0), dict(compress=2)):
+    for bad in (dict(group=3), dict(link=100), dict(blocks=0), dict(compress=2), dict(flush=2), dict(flush=12)):
         with pytest.raises(loopgpu.LoopGPUError) as e:
             g.set_launch(**bad)
         assert e.value.code == -1
