@@ -313,6 +313,7 @@ def run_scan(args, lg):
         "config": {"workload": "512-chain temperature scan, square 32x32, T_i=0.05*40^(i/511), batched + HIP graph",
                    "chains": len(plist), "L": 32},
         "plain_value": plain, "batched_over_plain": batched / plain,
+        "launch": b.launch(),
         "kernel_ms": prof, "roofline": rl,
     }
     return out, ("scan", [(1, 32, 1.0 / T, 1.0, 23125) for T in (0.05 * 40.0 ** (i / 511) for i in range(512))])
