@@ -178,6 +178,30 @@ __device__ __forceinline__ int group_spin_prev(const DevBatch& B, int base, int 
   return B.segspin_old[base + (j == m ? 0 : j)];
 }
 
+// Both sites' previous world lines staged in LDS (a wave per bond, lines of
+// at most DIAG_LDS events): each lane's time is searched in both lines at
+// once, one LDS read per line and step, instead of a window of shuffles per
+// 64 events.  Same results as group_spin_prev.
+constexpr int DIAG_LDS = 384;
+
+__device__ __forceinline__ void lds_spins_prev(const DevBatch& B, const double* t0, int base0, int m0,
+                                               const double* t1, int base1, int m1, double tau, int& c0,
+                                               int& c1) {
+  int lo0 = 0, n0 = m0, lo1 = 0, n1 = m1;
+  while (n0 > 0 || n1 > 0) {  // same trip count on every lane (m0, m1 are the bond's)
+    const int h0 = n0 >> 1, h1 = n1 >> 1;
+    const double v0 = t0[min(lo0 + h0, DIAG_LDS - 1)], v1 = t1[min(lo1 + h1, DIAG_LDS - 1)];
+    if (n0 > 0) { if (v0 < tau) { lo0 += h0 + 1; n0 -= h0 + 1; } else { n0 = h0; } }
+    if (n1 > 0) { if (v1 < tau) { lo1 += h1 + 1; n1 -= h1 + 1; } else { n1 = h1; } }
+  }
+  const bool eq0 = lo0 < m0 && t0[min(lo0, DIAG_LDS - 1)] == tau;
+  const bool eq1 = lo1 < m1 && t1[min(lo1, DIAG_LDS - 1)] == tau;
+  const int s0 = B.segspin_old[base0 + (lo0 == m0 ? 0 : lo0)];
+  const int s1 = B.segspin_old[base1 + (lo1 == m1 ? 0 : lo1)];
+  c0 = eq0 ? -1 : s0;
+  c1 = eq1 ? -1 : s1;
+}
+
 // Status word of a scan chunk: launch epoch (30 bits), state (1 sum, 2
 // inclusive prefix), value.
 __device__ __forceinline__ unsigned long long scan_word(uint32_t epoch, int state, int v) {
@@ -232,6 +256,23 @@ __global__ void __launch_bounds__(256) k_diag_count(DevBatch B) {
       const int base0 = B.soff_old[P.old_site_base + s0], m0 = B.soff_old[P.old_site_base + s0 + 1] - base0;
       const int base1 = B.soff_old[P.old_site_base + s1], m1 = B.soff_old[P.old_site_base + s1 + 1] - base1;
       int cur0 = 0, cur1 = 0;  // world-line cursors (candidate times ascend)
+      // a wave per bond: the two lines in LDS when they fit (a wave-uniform
+      // choice; the wave's own slice, so no block barrier)
+      double* lt0 = nullptr;
+      double* lt1 = nullptr;
+      if constexpr (G == 64) {
+        __shared__ double lines[256 / 64][2][DIAG_LDS];
+        if (m0 <= DIAG_LDS && m1 <= DIAG_LDS) {
+          lt0 = lines[threadIdx.x >> 6][0];
+          lt1 = lines[threadIdx.x >> 6][1];
+          __builtin_amdgcn_wave_barrier();  // the previous bond's searches are done
+          for (int q = gl; q < m0; q += 64) lt0[q] = B.evt_old[base0 + q];
+          for (int q = gl; q < m1; q += 64) lt1[q] = B.evt_old[base1 + q];
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+          __builtin_amdgcn_wave_barrier();
+          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+      }
       uint64_t Tb = 0;
       uint32_t kb = 0;
       while (true) {
@@ -243,8 +284,13 @@ __global__ void __launch_bounds__(256) k_diag_count(DevBatch B) {
         // lanes past the end of [0,1) search for -1 (found at once)
         const double tau = valid ? (double)T * TGRID_INV : -1.0;
         const int g = choose_graph(x.z, P);
-        const int c0 = group_spin_prev<G>(B, base0, m0, tau, cur0);
-        const int c1 = group_spin_prev<G>(B, base1, m1, tau, cur1);
+        int c0, c1;
+        if (lt0) {
+          lds_spins_prev(B, lt0, base0, m0, lt1, base1, m1, tau, c0, c1);
+        } else {
+          c0 = group_spin_prev<G>(B, base0, m0, tau, cur0);
+          c1 = group_spin_prev<G>(B, base1, m1, tau, cur1);
+        }
         const bool accept = valid && (c0 >= 0) && (c1 >= 0) && (((g & 1) ^ c0 ^ c1) != 0);
         uint64_t m = gballot<G>(accept);
         if (accept) {

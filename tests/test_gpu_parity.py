@@ -101,7 +101,8 @@ LAUNCHES = [dict(link=0), dict(link=128), dict(link=512), dict(link=1536), dict(
             dict(link=128, group=8, blocks=5),
             dict(link=0, group=16, blocks=64), dict(link=1536, group=32),
             dict(link=512, compress=1), dict(link=0, compress=0, blocks=9), dict(link=1536, compress=1, group=8),
-            dict(flush=8), dict(flush=4, compress=0), dict(flush=4, link=128, blocks=5)]
+            dict(flush=8), dict(flush=4, compress=0), dict(flush=4, link=128, blocks=5),
+            dict(group=64, link=0), dict(group=64, link=512, compress=0)]
 
 
 @pytest.mark.parametrize("cfg", LAUNCH_TRAJ, ids=lambda c: "k%d_L%d_b%g" % c[:3])
