@@ -478,6 +478,8 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan(DevBatch B) {
 // ---------------------------------------------------------------------------
 // (a) diagonal update, pass 2: merge kept operators and accepted candidates
 // ---------------------------------------------------------------------------
+constexpr int FILL_LDS = 256;  // times per list staged in LDS (a wave per bond)
+
 template <int G>
 __global__ void __launch_bounds__(256) k_diag_fill(DevBatch B) {
   if (overflowed(B)) return;
@@ -499,29 +501,50 @@ __global__ void __launch_bounds__(256) k_diag_fill(DevBatch B) {
     const long long cb = P.cand_base + (long long)b * P.cand_cap;
     const double* ct = B.cand_t + cb;
     const int info_b = gb << 3;
-    // kept operators: regraph (graph_impl.h:323-327) and place
-    for (int r = o0 + gl; r < o1; r += G) {
-      if (!(B.oi_old[r] & 1)) continue;
-      const int kr = B.kpre[r];
-      const double tau = B.ot_old[r];
-      U4 x = philox(sweep, STAGE_REGRAPH, (uint32_t)b, (uint32_t)kr, P.key0, P.key1);
-      int g = ((uint64_t)x.x < P.toff) ? 0 : 1;
-      int pos = nb + kr + lower_bound_d(ct, acc, tau);
-      B.ot_new[pos] = tau;
-      B.oi_new[pos] = info_b | (g << 1) | 1;
-      B.kc_new[pos] = kr;
+    // tc: accepted candidates' times; to: the old operators' times
+    auto place = [&](const double* tc, const double* to) {
+      // kept operators: regraph (graph_impl.h:323-327) and place
+      for (int r = o0 + gl; r < o1; r += G) {
+        if (!(B.oi_old[r] & 1)) continue;
+        const int kr = B.kpre[r];
+        const double tau = B.ot_old[r];
+        U4 x = philox(sweep, STAGE_REGRAPH, (uint32_t)b, (uint32_t)kr, P.key0, P.key1);
+        int g = ((uint64_t)x.x < P.toff) ? 0 : 1;
+        int pos = nb + kr + lower_bound_d(tc, acc, tau);
+        B.ot_new[pos] = tau;
+        B.oi_new[pos] = info_b | (g << 1) | 1;
+        B.kc_new[pos] = kr;
+      }
+      // accepted candidates: new diagonal operators
+      for (int j = gl; j < acc; j += G) {
+        const double tau = tc[j];
+        const int g = B.cand_g[cb + j];
+        int r = lower_bound_d(to, o1 - o0, tau) + o0;
+        int kk = (r < o1) ? B.kpre[r] : kept;
+        int pos = nb + j + kk;
+        B.ot_new[pos] = tau;
+        B.oi_new[pos] = info_b | (g << 1);
+        B.kc_new[pos] = kk;
+      }
+    };
+    // a wave per bond: both lists searched in LDS when they fit (the wave's
+    // own slice, no block barrier)
+    if constexpr (G == 64) {
+      __shared__ double lists[256 / 64][2][FILL_LDS];
+      if (acc <= FILL_LDS && o1 - o0 <= FILL_LDS) {
+        double* lc = lists[threadIdx.x >> 6][0];
+        double* lo = lists[threadIdx.x >> 6][1];
+        __builtin_amdgcn_wave_barrier();  // the previous bond's searches are done
+        for (int q = gl; q < acc; q += 64) lc[q] = ct[q];
+        for (int q = gl; q < o1 - o0; q += 64) lo[q] = B.ot_old[o0 + q];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        place(lc, lo);
+        continue;
+      }
     }
-    // accepted candidates: new diagonal operators
-    for (int j = gl; j < acc; j += G) {
-      const double tau = ct[j];
-      const int g = B.cand_g[cb + j];
-      int r = lower_bound_d(B.ot_old + o0, o1 - o0, tau) + o0;
-      int kk = (r < o1) ? B.kpre[r] : kept;
-      int pos = nb + j + kk;
-      B.ot_new[pos] = tau;
-      B.oi_new[pos] = info_b | (g << 1);
-      B.kc_new[pos] = kk;
-    }
+    place(ct, B.ot_old + o0);
   }
 }
 
