@@ -626,6 +626,8 @@ __device__ __forceinline__ void lds_unite(int* P, int a, int b) {
 // its local root; k_uf_union skips them (u1 is stored complemented).  Roots
 // stay cluster minima, so the partition and every result are those of the
 // plain union-find.
+constexpr int LINK_U = 4;  // staged elements per thread in flight
+
 template <int CAP, int WAVES, int TPS>
 __global__ void __launch_bounds__(64 * WAVES) k_link(DevBatch B) {
   if (overflowed(B)) return;
@@ -683,14 +685,48 @@ __global__ void __launch_bounds__(64 * WAVES) k_link(DevBatch B) {
       spin0 = B.spins_old[P.old_site_base + s];
       gflag = gauge(P, s) < 0 ? EV_GAUGE : 0;
       if (tot <= CAP) {
+        // LINK_U elements per thread at a time, all loads before any store:
+        // time, off-diagonal prefix count and off-diagonal bit (kept in the
+        // flags byte until the element's flags replace it)
+        for (int e0 = sl; e0 < tot; e0 += LINK_U * TPS) {
+          int ek[LINK_U], ej[LINK_U];
+          double et[LINK_U];
+          int ekc[LINK_U], eoi[LINK_U];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          if (j >= S.nbd) break;
-          for (int q = sl; q < S.len[j]; q += TPS) {
-            lt[S.off[j] + q] = B.ot_new[S.lo[j] + q];
-            lpre[S.off[j] + j + q] = (uint16_t)B.kc_new[S.lo[j] + q];
+          for (int u = 0; u < LINK_U; ++u) {
+            const int e = min(e0 + u * TPS, tot - 1);
+            int j = 0;
+#pragma unroll
+            for (int jj = 1; jj < 4; ++jj)
+              if (jj < S.nbd && e >= S.off[jj]) j = jj;
+            ej[u] = j;
+            ek[u] = (j == 0 ? S.lo[0] - S.off[0] : j == 1 ? S.lo[1] - S.off[1] : j == 2 ? S.lo[2] - S.off[2]
+                                                                                 : S.lo[3] - S.off[3]) + e;
           }
-          if (sl == 0) lpre[S.off[j] + j + S.len[j]] = (uint16_t)list_pre_global(B, S, j, S.len[j]);
+#pragma unroll
+          for (int u = 0; u < LINK_U; ++u) {
+            et[u] = B.ot_new[ek[u]];
+            ekc[u] = B.kc_new[ek[u]];
+            eoi[u] = B.oi_new[ek[u]];
+          }
+#pragma unroll
+          for (int u = 0; u < LINK_U; ++u) {
+            const int e = e0 + u * TPS;
+            if (e >= tot) break;
+            lt[e] = et[u];
+            lpre[e + ej[u]] = (uint16_t)ekc[u];
+            lfl[e] = (uint8_t)(eoi[u] & 1);
+          }
+        }
+        // each list's total after its last element, one thread per list
+        if (sl < S.nbd) {
+          const int j = sl;  // selected with constant indices (S stays in registers)
+          const int len = j == 0 ? S.len[0] : j == 1 ? S.len[1] : j == 2 ? S.len[2] : S.len[3];
+          const int off = j == 0 ? S.off[0] : j == 1 ? S.off[1] : j == 2 ? S.off[2] : S.off[3];
+          const int lo = j == 0 ? S.lo[0] : j == 1 ? S.lo[1] : j == 2 ? S.lo[2] : S.lo[3];
+          const int last = len > 0 ? lo + len - 1 : 0;  // (operator 0 stands in for an empty list)
+          const int v = B.kc_new[last] + (B.oi_new[last] & 1);
+          lpre[off + j + len] = (uint16_t)(len > 0 ? v : 0);
         }
       }
       if (sl == 0) {
@@ -726,7 +762,7 @@ __global__ void __launch_bounds__(64 * WAVES) k_link(DevBatch B) {
         int rank, par;
         if (lds) rank_event<true>(B, S, i, bi, p, tau, lt, lpre, rank, par);
         else rank_event<false>(B, S, i, bi, p, tau, nullptr, nullptr, rank, par);
-        const uint8_t fl = (uint8_t)((B.oi_new[k] & 1) | (side ? 0 : EV_SRC) |
+        const uint8_t fl = (uint8_t)(((lds ? lfl[e] : B.oi_new[k]) & 1) | (side ? 0 : EV_SRC) |
                                      (bond_dir(*Pp, bi) ? EV_DIR1 : 0) | (rank == 0 ? EV_FIRST : 0) | gflag);
         const uint8_t sp = spin0 ^ (uint8_t)(par & 1);
         const int node = n0 + rank;
