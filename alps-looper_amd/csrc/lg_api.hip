@@ -415,9 +415,9 @@ int Batch::set_layout(std::vector<ChainDev>& ds, const std::vector<uint64_t>& sw
     const double per_site = nsites > 0 ? ev / nsites : 0;
     c.link_cap = per_site < 48 ? 0 : (per_site < 96 ? 128 : (per_site <= 400 ? 512 : 1536));
     // the estimator's LDS table: hot configurations (many one- and
-    // two-segment clusters, which overflow the table) flush it every 4 items
-    // per lane (512-chain scan: estimate 0.214 -> 0.181 ms); cold ones once per
-    // chunk (square 128^2, beta=128: 0.178 ms against 0.201 at 8 and 0.282 at 4)
+    // two-segment clusters, which overflow the table) flush it every 1024
+    // segments (512-chain scan: estimate 0.172 -> 0.166 ms); cold ones once
+    // per chunk (square 128^2, beta=128: 0.154 ms against 0.19 at 4 or 8)
     c.est_flush = per_site < 48 ? 4 : 16;
     // the estimator finds the roots itself (a pass over the segments less,
     // but the collector then clears every root's accumulator) for one large

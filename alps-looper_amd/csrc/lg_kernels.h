@@ -87,8 +87,8 @@ struct LaunchCfg {
   // and flip bits in one load), 0 the estimator walks to the roots itself and
   // the collector clears the accumulators it has read
   int compress = 1;
-  // estimator: items per lane between flushes of its LDS cluster table to
-  // global memory (4, 8 or 16; 16 = once per chunk)
+  // estimator: flush of its LDS cluster table to global memory every
+  // 256 * est_flush segments (4, 8 or 16; 16 = once per chunk)
   int est_flush = 16;
 };
 

@@ -1005,9 +1005,14 @@ __device__ __forceinline__ int chain_of_node(const DevBatch& B, int node) {
 // memory: the large loops of a cold antiferromagnet (a few clusters holding
 // most segments) then cost one global update per chunk instead of one per
 // segment.  Segments whose root finds no slot update global memory directly.
+// A block of 1024 threads takes a chunk (at most 4 items per thread, 2 in
+// flight): every chunk of a large configuration is resident at once, so the
+// kernel lasts about one block's serial chain of groups, which more threads
+// per chunk shorten; with 16 waves per block only two blocks fit a CU, which
+// leaves LDS for a 1024-slot table (53 KB).
 // ---------------------------------------------------------------------------
-constexpr int EST_THREADS = 256;
-constexpr int EST_U = 4;         // items per thread and group
+constexpr int EST_THREADS = 1024;
+constexpr int EST_U = 2;         // items per thread and group
 
 // Items per thread of the per-node chunked kernels (estimate, collector): 16,
 // or 8 or 4 when 16 would give fewer than 1024 chunks.  A small configuration
@@ -1019,7 +1024,7 @@ __device__ __forceinline__ int chunk_items(int M) {
   while (it > 4 && (long long)M < 1024LL * 256 * it) it >>= 1;
   return it;
 }
-constexpr int HT_BITS = 9;
+constexpr int HT_BITS = 10;
 constexpr int HT = 1 << HT_BITS;
 constexpr int HT_PROBES = 8;
 
@@ -1083,7 +1088,9 @@ __global__ void __launch_bounds__(EST_THREADS) k_estimate(DevBatch B, int flush_
   }
   __syncthreads();
   const int M = B.totals[1];
-  const int items = chunk_items(M), echunk = items * EST_THREADS;
+  // the chunk of 256 * chunk_items(M) segments over the block's threads
+  const int items = chunk_items(M) * 256 / EST_THREADS, echunk = items * EST_THREADS;
+  const int flush_every = max(1, flush_items * 256 / EST_THREADS);  // items per thread
   const int nchunks = (M + echunk - 1) / echunk;
   int clo, chi;
   xcd_share(nchunks, clo, chi);
@@ -1094,13 +1101,18 @@ __global__ void __launch_bounds__(EST_THREADS) k_estimate(DevBatch B, int flush_
     for (int g0 = 0; g0 < items; g0 += EST_U) {
       const int ib = chunk * echunk + g0 * EST_THREADS + tid;
       if (__builtin_amdgcn_readfirstlane(ib) < M) {  // lane 0 holds the wave's lowest node
-        // items past the end read node M - 1 and are dropped before any store,
-        // so that every load is issued without a branch around it
+        // items past the chunk or the end read a valid node and are dropped
+        // before any store, so that every load is issued without a branch
+        bool ok[EST_U];
         int nd[EST_U], r[EST_U];
         uint8_t fl[EST_U], sp[EST_U], flo[EST_U];
         double thi[EST_U], tlo[EST_U];
 #pragma unroll
-        for (int u = 0; u < EST_U; ++u) nd[u] = min(ib + u * EST_THREADS, M - 1);
+        for (int u = 0; u < EST_U; ++u) {
+          const int i = ib + u * EST_THREADS;
+          ok[u] = g0 + u < items && i < M;
+          nd[u] = ok[u] ? i : min(ib, M - 1);
+        }
 #pragma unroll
         for (int u = 0; u < EST_U; ++u) {
           r[u] = B.parent[nd[u]];
@@ -1138,7 +1150,7 @@ __global__ void __launch_bounds__(EST_THREADS) k_estimate(DevBatch B, int flush_
 #pragma unroll
           for (int u = 0; u < EST_U; ++u) {
             const int i = ib + u * EST_THREADS;
-            if (i >= M) continue;
+            if (!ok[u]) continue;
             if (r[u] != p0[u]) B.parent[i] = r[u];
             B.fbit[i] = (uint8_t)f[u];
           }
@@ -1148,7 +1160,7 @@ __global__ void __launch_bounds__(EST_THREADS) k_estimate(DevBatch B, int flush_
 #pragma unroll
           for (int u = 0; u < EST_U; ++u) {
             const int i = ib + u * EST_THREADS;
-            if (i < M && i != r[u]) B.fbit[i] = (uint8_t)f[u];
+            if (ok[u] && i != r[u]) B.fbit[i] = (uint8_t)f[u];
           }
         }
         // a site's first segment wraps through tau = 1: the segment below it is
@@ -1178,7 +1190,7 @@ __global__ void __launch_bounds__(EST_THREADS) k_estimate(DevBatch B, int flush_
 #pragma unroll
         for (int u = 0; u < EST_U; ++u) {
           const int i = ib + u * EST_THREADS;
-          if (i >= M) continue;
+          if (!ok[u]) continue;
           const int spin = sp[u];
           const int sig = 1 - 2 * spin;
           const bool first = (fl[u] & EV_FIRST) != 0;
@@ -1199,10 +1211,10 @@ __global__ void __launch_bounds__(EST_THREADS) k_estimate(DevBatch B, int flush_
           ht_add(hkey, hL, hn, hw, B, r[u], sL, first ? (1 << (8 * cls)) : 0, pack_winding(dw0, dw1));
         }
       }
-      // the table goes to global memory every flush_items items (a multiple
-      // of EST_U) and at the chunk's end; a block-uniform condition, so every
-      // wave takes the barriers
-      if ((g0 + EST_U) % flush_items != 0 && g0 + EST_U < items) continue;
+      // the table goes to global memory every 256 * flush_items segments (at
+      // the end of a group) and at the chunk's end; a block-uniform
+      // condition, so every wave takes the barriers
+      if ((g0 + EST_U) % flush_every != 0 && g0 + EST_U < items) continue;
       __syncthreads();
       for (int s = tid; s < HT; s += EST_THREADS) {
         const int r = hkey[s];

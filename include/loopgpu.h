@@ -162,8 +162,8 @@ int lg_exchange_state(lg_batch* h, int32_t* chain_at, uint64_t* accepted, uint64
  * thread per operator; 128, 512 or 1536 a group of threads per site staging
  * that many events in LDS: 16 threads, one wave, four waves); compress: 1 a
  * union-find compression pass before the estimator, 0 the estimator finds
- * the roots itself; flush: segments per estimator lane (4, 8 or 16) between
- * flushes of its LDS cluster table to global memory.  By default they follow
+ * the roots itself; flush: the estimator flushes its LDS cluster table to
+ * global memory every 256 * flush segments of its chunk (4, 8 or 16).  By default they follow
  * the expected operator counts and the number of chains, and lg_batch_tune
  * times the candidates.  set: fields < 0 keep their value; the settings then
  * stay as given (no automatic choice) until the next tune.  Results do not
