@@ -1651,17 +1651,21 @@ void launch_cluster_stage(const DevBatch& B, const LaunchCfg& L, hipStream_t st,
   // by the expected events per site: one thread per operator (few events),
   // 16 threads per site staging 128 events with 16 sites per block (29 KB of
   // LDS), a wave per site staging 512 events with 4 sites per block (29 KB),
-  // or 1536 events with 2 sites per block and 4 waves per site (43 KB)
+  // or 1536 events with 2 sites per block and 4 waves per site (43 KB).  The
+  // 128- and 512-event kernels take one block step per block (square 128^2,
+  // beta=128: 0.119 ms with two steps per block, 0.101 with one); the
+  // 1536-event kernel keeps the per-node grid (square 512^2, beta=512: 5.76
+  // ms at 8 steps per block, 5.95 at one)
   if (L.link_cap == 0) {
     hipLaunchKernelGGL(k_link_ops, dim3(L.blocks_op), dim3(256), 0, st, B);
   } else if (L.link_cap <= 128) {
-    int blocks = min(L.blocks_node, max(1, (B.nsites + 15) / 16));
+    const int blocks = max(1, (B.nsites + 15) / 16);
     hipLaunchKernelGGL((k_link<128, 4, 16>), dim3(blocks), dim3(256), 0, st, B);
   } else if (L.link_cap <= 512) {
-    int blocks = min(L.blocks_node, max(1, (B.nsites + 3) / 4));
+    const int blocks = max(1, (B.nsites + 3) / 4);
     hipLaunchKernelGGL((k_link<512, 4, 64>), dim3(blocks), dim3(256), 0, st, B);
   } else {
-    int blocks = min(L.blocks_node, max(1, (B.nsites + 1) / 2));
+    const int blocks = min(L.blocks_node, max(1, (B.nsites + 1) / 2));
     hipLaunchKernelGGL((k_link<1536, 8, 256>), dim3(blocks), dim3(512), 0, st, B);
   }
   mark(ev, 1, st);
