@@ -1292,7 +1292,7 @@ __global__ void __launch_bounds__(256) k_field_apply(DevBatch B) {
 // rec_acc atomically.  end_sweep adds the partials of each chain in order.
 // ---------------------------------------------------------------------------
 constexpr int COL_THREADS = 256;
-constexpr int COL_MIN_CHUNK = COL_THREADS * 4;  // smallest collector chunk (chunk_items)
+constexpr int COL_MIN_CHUNK = 256 * 4;  // smallest collector chunk (256 * chunk_items)
 constexpr int COL_U = 4;                         // items per lane in flight (divides chunk_items)
 constexpr int NCOL = NUM_COLLECTOR_TERMS;
 
@@ -1363,7 +1363,8 @@ __global__ void __launch_bounds__(COL_THREADS) k_flip_collect(DevBatch B, int cl
   // contiguous in node order, so a chunk meets few of them)
   constexpr int CSLOTS = 8;
   __shared__ double slot[CSLOTS][NCOL];
-  const int citems = chunk_items(M), cchunk = citems * COL_THREADS;
+  // chunks of 256 * chunk_items(M) nodes over the block's threads
+  const int cchunk = chunk_items(M) * 256, citems = cchunk / COL_THREADS;
   const int nchunks = (M + cchunk - 1) / cchunk;
   int clo, chi;
   xcd_share(nchunks, clo, chi);
@@ -1395,13 +1396,16 @@ __global__ void __launch_bounds__(COL_THREADS) k_flip_collect(DevBatch B, int cl
       const int ib = base + it0 * COL_THREADS + threadIdx.x;
       if (__builtin_amdgcn_readfirstlane(ib) >= M) break;  // lane 0 holds the wave's lowest node
       int nd[COL_U], par[COL_U];
-      bool root[COL_U];
+      bool ok[COL_U], root[COL_U];
 #pragma unroll
-      for (int u = 0; u < COL_U; ++u) nd[u] = min(ib + u * COL_THREADS, M - 1);
+      for (int u = 0; u < COL_U; ++u) {
+        ok[u] = it0 + u < citems && ib + u * COL_THREADS < M;  // in the chunk and the batch
+        nd[u] = ok[u] ? ib + u * COL_THREADS : min(ib, M - 1);
+      }
 #pragma unroll
       for (int u = 0; u < COL_U; ++u) par[u] = B.parent[nd[u]];
 #pragma unroll
-      for (int u = 0; u < COL_U; ++u) root[u] = ib + u * COL_THREADS < M && par[u] == nd[u];
+      for (int u = 0; u < COL_U; ++u) root[u] = ok[u] && par[u] == nd[u];
       ClusterAcc ac[COL_U];
       uint8_t fr[COL_U];
 #pragma unroll
@@ -1486,7 +1490,7 @@ __global__ void __launch_bounds__(END_THREADS) k_end_sweep(DevBatch B) {
   for (int q = 0; q < NCOL; ++q) v[q] = 0;
   if (!bad) {
     const int start = B.soff_new[P.site_base], end = B.soff_new[P.site_base + P.N];
-    const int cchunk = chunk_items(B.totals[1]) * COL_THREADS;  // as in k_flip_collect
+    const int cchunk = chunk_items(B.totals[1]) * 256;  // as in k_flip_collect
     const int j0 = (start + cchunk - 1) / cchunk, j1 = (end + cchunk - 1) / cchunk;
     for (int j = j0 + tid; j < j1; j += END_THREADS)
       for (int q = 0; q < NCOL; ++q) v[q] += B.col_part[(size_t)j * NCOL + q];
