@@ -420,10 +420,11 @@ int Batch::set_layout(std::vector<ChainDev>& ds, const std::vector<uint64_t>& sw
     // per chunk (square 128^2, beta=128: 0.154 ms against 0.19 at 4 or 8)
     c.est_flush = per_site < 48 ? 4 : 16;
     // the estimator finds the roots itself (a pass over the segments less,
-    // but the collector then clears every root's accumulator) for one large
-    // chain; the compress pass for many chains (mostly small and hot, many
-    // small clusters) and for small configurations, where it measured faster
-    c.compress = (chains.size() > 1 || ev < 7.0e6) ? 1 : 0;
+    // but the collector then clears every root's accumulator): with the
+    // 1024-thread estimator it measured faster than the compress pass for one
+    // chain at every size tried (square 128^2, beta=128: 0.660 against 0.678
+    // ms; 64^2, beta=64: 0.222 against 0.226) and for the 512-chain scan
+    c.compress = 0;
     // grid of the per-operator and per-node kernels: a finer split of large
     // configurations balances the blocks' uneven work (C5: 2048 -> 16384
     // blocks took the sweep from 41.0 to 38.1 ms; C2 is fastest at 2048)

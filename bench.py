@@ -186,7 +186,7 @@ def run_chain(args, lg, L, T):
         "n_op_mean": nop_mean,
         "energy_density": float(rec[:, 2].mean() / L),
         "kernel_ms": prof,
-        "roofline": roofline(prof, nop_mean),
+        "roofline": roofline(prof, nop_mean, compress=ch.launch()["compress"]),
         "published_reference_sweeps_per_s": {128: 412.0, 1024: 3.30}.get(L),
     }
     return out, ("reference", (L, T, nop_mean))
@@ -204,16 +204,27 @@ def measured_traffic(name, kernel, nop_mean):
         return None
 
 
-def roofline(prof, nop_mean, traffic_profile=None):
+def kernel_bytes(compress):
+    """algorithmic bytes per operator of each kernel: without the compress
+    pass the estimator's own walk does the compression"""
+    b = dict(BYTES_PER_OP)
+    if not compress:
+        b["estimate"] += b["uf_compress"]
+        b["uf_compress"] = 0.0
+    return b
+
+
+def roofline(prof, nop_mean, traffic_profile=None, compress=1):
+    kb = kernel_bytes(compress)
     dom = max(prof, key=lambda k: prof[k])
     ms = prof[dom]
-    bytes_launch = BYTES_PER_OP[dom] * nop_mean
+    bytes_launch = kb[dom] * nop_mean
     achieved = bytes_launch / (ms * 1e-3) / 1e9
     return {
         "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
         "frac": achieved / HBM_PEAK_GBS,
         "traffic": measured_traffic(traffic_profile, dom, nop_mean) if traffic_profile else None,
-        "kernel": dom, "kernel_ms": ms, "bytes_per_op": BYTES_PER_OP[dom],
+        "kernel": dom, "kernel_ms": ms, "bytes_per_op": kb[dom],
         "bytes_per_launch": bytes_launch,
     }
 
@@ -223,13 +234,14 @@ STAGES = {"diagonal update": ("diag_count", "scan", "diag_fill"),
           "estimators": ("estimate",), "flip + collect": ("flip_collect", "end_sweep")}
 
 
-def stage_gbs(prof, nop_mean):
+def stage_gbs(prof, nop_mean, compress=1):
     """algorithmic GB/s of each stage (per-stage bytes per operator-sweep)"""
+    kb = kernel_bytes(compress)
     out = {}
     for st, ks in STAGES.items():
         ms = sum(prof[k] for k in ks)
-        b = sum(BYTES_PER_OP[k] for k in ks) * nop_mean
-        out[st] = {"ms": ms, "bytes_per_op": sum(BYTES_PER_OP[k] for k in ks), "gbs": b / (ms * 1e-3) / 1e9}
+        b = sum(kb[k] for k in ks) * nop_mean
+        out[st] = {"ms": ms, "bytes_per_op": sum(kb[k] for k in ks), "gbs": b / (ms * 1e-3) / 1e9}
     return out
 
 
@@ -250,7 +262,8 @@ def run_c2(args, lg, L=128, beta=128.0, traffic_profile="c2_square128_hbm_traffi
     value = nop_total / dt
     log("profile %d sweeps" % args.profile_sweeps)
     prof = ch.profile(args.profile_sweeps)
-    rl = roofline(prof, nop_mean, traffic_profile)
+    comp = ch.launch()["compress"]
+    rl = roofline(prof, nop_mean, traffic_profile, comp)
     out = {
         "metric": METRIC, "value": value, "unit": "spin-operator updates/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -264,7 +277,7 @@ def run_c2(args, lg, L=128, beta=128.0, traffic_profile="c2_square128_hbm_traffi
         "energy_density": float(rec[:, 2].mean() / (L * L)),
         "sweep_hbm_frac": value * SWEEP_BYTES_PER_OP / (HBM_PEAK_GBS * 1e9),
         "kernel_ms": prof,
-        "stages": stage_gbs(prof, nop_mean),
+        "stages": stage_gbs(prof, nop_mean, comp),
         "roofline": rl,
     }
     return out, (kind, L, beta)
@@ -304,7 +317,7 @@ def run_scan(args, lg):
     plain = nops / (time.perf_counter() - t0)
     log("profile")
     prof = b.profile(args.profile_sweeps)
-    rl = roofline(prof, nop_total / args.steps)
+    rl = roofline(prof, nop_total / args.steps, compress=b.launch()["compress"])
     out = {
         "metric": METRIC, "value": batched, "unit": "spin-operator updates/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,

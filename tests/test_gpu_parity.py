@@ -169,7 +169,7 @@ def test_compress_mode_switches_keep_trajectory(loopgpu):
     c = O.PhiloxChain(O.SQUARE, 8, 4.0, 1.0, 1.0, seed=8, chain_id=5)
     rc = c.sweep(n)
     g = loopgpu.Chain(loopgpu.params(O.SQUARE, 8, 4.0, 1.0, 1.0, seed=8, chain_id=5))
-    assert g.launch()["compress"] == 1          # a small configuration: compress pass by default
+    assert g.launch()["compress"] == 0          # by default the estimator walks to the roots itself
     parts = []
     for m in modes:
         g.set_launch(compress=m)
@@ -179,7 +179,7 @@ def test_compress_mode_switches_keep_trajectory(loopgpu):
              loopgpu.params(O.SQUARE, 6, 2.0, 1.0, 0.8, seed=4, chain_id=1),
              loopgpu.params(O.CHAIN, 10, 1.0, 1.0, 1.2, seed=4, chain_id=3)]
     b = loopgpu.Batch(plist)
-    assert b.launch()["compress"] == 1          # many chains: compress pass by default
+    assert b.launch()["compress"] == 0          # for batches too
     parts = []
     for m in modes:
         b.set_launch(compress=m)
