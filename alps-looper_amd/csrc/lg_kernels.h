@@ -90,6 +90,9 @@ struct LaunchCfg {
   // estimator: flush of its LDS cluster table to global memory every
   // 256 * est_flush segments (4, 8 or 16; 16 = once per chunk)
   int est_flush = 16;
+  // diagonal update, fill pass with a wave per bond: times per list staged
+  // in LDS (256, 512 or 1024; the block holds 64 bytes per time)
+  int fill_cap = 256;
 };
 
 constexpr int NUM_SWEEP_KERNELS = 9;

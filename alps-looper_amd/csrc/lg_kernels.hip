@@ -478,10 +478,12 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan(DevBatch B) {
 // ---------------------------------------------------------------------------
 // (a) diagonal update, pass 2: merge kept operators and accepted candidates
 // ---------------------------------------------------------------------------
-constexpr int FILL_LDS = 256;  // times per list staged in LDS (a wave per bond)
+// times per list staged in LDS (a wave per bond): fill_cap, chosen per
+// launch from the expected operators per bond; dynamic LDS of
+// 256 / 64 waves * 2 lists * fill_cap doubles
 
 template <int G>
-__global__ void __launch_bounds__(256) k_diag_fill(DevBatch B) {
+__global__ void __launch_bounds__(256) k_diag_fill(DevBatch B, int fill_cap) {
   if (overflowed(B)) return;
   const int groups_per_block = blockDim.x / G;
   const int gl = glane<G>();
@@ -530,10 +532,10 @@ __global__ void __launch_bounds__(256) k_diag_fill(DevBatch B) {
     // a wave per bond: both lists searched in LDS when they fit (the wave's
     // own slice, no block barrier)
     if constexpr (G == 64) {
-      __shared__ double lists[256 / 64][2][FILL_LDS];
-      if (acc <= FILL_LDS && o1 - o0 <= FILL_LDS) {
-        double* lc = lists[threadIdx.x >> 6][0];
-        double* lo = lists[threadIdx.x >> 6][1];
+      extern __shared__ double fill_lists[];
+      if (acc <= fill_cap && o1 - o0 <= fill_cap) {
+        double* lc = fill_lists + (threadIdx.x >> 6) * 2 * fill_cap;
+        double* lo = lc + fill_cap;
         __builtin_amdgcn_wave_barrier();  // the previous bond's searches are done
         for (int q = gl; q < acc; q += 64) lc[q] = ct[q];
         for (int q = gl; q < o1 - o0; q += 64) lo[q] = B.ot_old[o0 + q];
@@ -1610,7 +1612,10 @@ static void launch_diag(const DevBatch& B, const LaunchCfg& L, hipStream_t st, b
   int blocks = (B.nbonds + groups_per_block - 1) / groups_per_block;
   if (blocks > L.max_blocks_bond) blocks = L.max_blocks_bond;
   if (blocks < 1) blocks = 1;
-  if (fill) hipLaunchKernelGGL(k_diag_fill<G>, dim3(blocks), dim3(256), 0, st, B);
+  if (fill) {
+    const size_t lds = G == 64 ? (size_t)(256 / 64) * 2 * L.fill_cap * sizeof(double) : 0;
+    hipLaunchKernelGGL(k_diag_fill<G>, dim3(blocks), dim3(256), lds, st, B, L.fill_cap);
+  }
   else hipLaunchKernelGGL(k_diag_count<G>, dim3(blocks), dim3(256), 0, st, B);
 }
 
