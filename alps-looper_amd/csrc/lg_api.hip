@@ -404,12 +404,13 @@ int Batch::set_layout(std::vector<ChainDev>& ds, const std::vector<uint64_t>& sw
   if (!tuned) cfg.group = G;
   // expected events per site (two legs per operator; operators per bond about
   // beta * (offset + half the largest diagonal energy)) picks the staging size
-  double ev = 0, ops_bond = 0;
+  double ev = 0, ops_bond = 0, nb_total = 0;
   for (size_t c = 0; c < chains.size(); ++c) {
     const double per_bond = chains[c].offset_total / chains[c].Nb +
                             0.5 * (std::fabs(params[c].jz) / 4 + std::fabs(params[c].jxy) / 2);
     ev += 2.0 * chains[c].beta * chains[c].Nb * per_bond;
-    ops_bond = std::max(ops_bond, chains[c].beta * per_bond);
+    ops_bond += chains[c].Nb * chains[c].beta * per_bond;
+    nb_total += chains[c].Nb;
   }
   LaunchCfg c = cfg;
   // the fill pass stages a bond's lists in LDS when they fit: room for the
@@ -417,6 +418,10 @@ int Batch::set_layout(std::vector<ChainDev>& ds, const std::vector<uint64_t>& sw
   // holds limits its occupancy: square 512^2, beta=512, fill 4.05 -> 3.0 ms
   // with 512 times, while 512 would cost square 128^2 0.045 -> 0.061 ms).
   // Not a launch setting: results and the tuner's choices do not depend on it.
+  // Sized for the bond-weighted mean, so that a few large jobs in a mixed
+  // batch fall back to global memory instead of costing every bond
+  // occupancy.
+  if (nb_total > 0) ops_bond /= nb_total;
   c.fill_cap = 1.25 * ops_bond <= 256 ? 256 : (1.25 * ops_bond <= 512 ? 512 : 1024);
   if (!tuned) {
     const double per_site = nsites > 0 ? ev / nsites : 0;
