@@ -1,6 +1,6 @@
 """Extract golden vectors from the reference's own test outputs into small JSON
-fixtures (run once in a container that has the reference checkout; the GPU box
-never reads the reference).
+fixtures (run once where the reference checkout is present; the tests read
+only the fixtures, never the reference).
 
 Sources (data files of the reference's tests):
   test/union_find.op            100 random unions and the resulting forest
