@@ -3,13 +3,16 @@
     python alps-looper_amd/profiling.py trace  RESULTS.db [--last N]
     python alps-looper_amd/profiling.py pmc    FETCH.db WRITE.db COPY_FETCH.db COPY_WRITE.db \
                                                --copy-mib 1024 [--last N] [--nop NOP] [--json OUT]
+    python alps-looper_amd/profiling.py counters RESULTS.db [RESULTS.db ...] [--last N]
 
 `trace` prints per-kernel call counts and average durations (the top_kernels
 view); `pmc` prints HBM bytes per dispatch from FETCH_SIZE and WRITE_SIZE,
 each scaled by the factor that makes the plain copy kernel (k_copy_stream,
 reads and writes COPY_MIB MiB per dispatch) read exactly its algorithmic
-bytes.  --last N keeps the last N dispatches of each kernel (the equilibrated
-sweeps at the end of a run).
+bytes.  `counters` prints the mean per dispatch of every counter in the
+given databases (one counter-collection run each), kernel by kernel, and the
+wait and LDS ratios when their counters are present.  --last N keeps the last
+N dispatches of each kernel (the equilibrated sweeps at the end of a run).
 """
 
 import argparse
@@ -47,9 +50,21 @@ def pmc(db, counter, last=None):
     return {k: sum(v[-last:] if last else v) / len(v[-last:] if last else v) for k, v in per.items()}
 
 
+def counters(dbs, last=None):
+    """kernel -> counter -> mean value per dispatch, over the given databases"""
+    out = {}
+    for db in dbs:
+        con = sqlite3.connect(db)
+        names = [r[0] for r in con.execute("select distinct counter_name from counters_collection")]
+        for c in names:
+            for k, v in pmc(db, c, last).items():
+                out.setdefault(k, {})[c] = v
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("mode", choices=["trace", "pmc"])
+    ap.add_argument("mode", choices=["trace", "pmc", "counters"])
     ap.add_argument("dbs", nargs="+")
     ap.add_argument("--last", type=int, default=None)
     ap.add_argument("--copy-mib", type=float, default=1024)
@@ -66,6 +81,28 @@ def main():
                 continue
             print("| %s | %d | %.2f | %.1f%% |" % (k, x["calls"], x["avg_us"], 100 * x["avg_us"] / tot))
         print("| sum of sweep kernels | | %.2f | |" % tot)
+        return
+    if a.mode == "counters":
+        C = counters(a.dbs, a.last)
+        names = sorted({c for k in C for c in C[k]})
+        ratios = [("wait/cycles", "SQ_WAIT_ANY", "SQ_WAVE_CYCLES"),
+                  ("LDS bank conflict/LDS inst", "SQ_LDS_BANK_CONFLICT", "SQ_INSTS_LDS"),
+                  ("L2 hit", "TCC_HIT_sum", None)]
+        extra = [r for r in ratios if r[1] in names and (r[2] is None or r[2] in names)]
+        print("| kernel | " + " | ".join(names + [r[0] for r in extra]) + " |")
+        print("|---|" + "---|" * (len(names) + len(extra)))
+        for k in sorted(C, key=lambda k: k):
+            if k.startswith("__amd") or k == "copy_stream":
+                continue
+            row = ["%.4g" % C[k].get(c, float("nan")) for c in names]
+            for _, num, den in extra:
+                if den is None:  # hit rate
+                    h, m = C[k].get("TCC_HIT_sum", 0), C[k].get("TCC_MISS_sum", 0)
+                    row.append("%.2f" % (h / (h + m)) if h + m else "-")
+                else:
+                    d = C[k].get(den, 0)
+                    row.append("%.2f" % (C[k].get(num, 0) / d) if d else "-")
+            print("| %s | %s |" % (k, " | ".join(row)))
         return
     fetch, write, cfetch, cwrite = a.dbs[:4]
     F = pmc(fetch, "FETCH_SIZE", a.last)
