@@ -161,11 +161,11 @@ int lg_exchange_state(lg_batch* h, int32_t* chain_at, uint64_t* accepted, uint64
  * grid of the per-operator and per-node kernels; link: linking kernel (0 one
  * thread per operator; 128, 512 or 1536 a group of threads per site staging
  * that many events in LDS: 16 threads, one wave, four waves); compress: 1 a
- * union-find compression pass before the estimator, 0 the estimator finds
- * the roots itself; flush: the estimator flushes its LDS cluster table to
- * global memory every 256 * flush segments of its chunk (4, 8 or 16).  By default they follow
- * the expected operator counts and the number of chains, and lg_batch_tune
- * times the candidates.  set: fields < 0 keep their value; the settings then
+ * union-find compression pass before the estimator, 0 (the default) the
+ * estimator finds the roots itself; flush: the estimator flushes its LDS
+ * cluster table to global memory every 256 * flush segments of its chunk
+ * (4, 8 or 16).  By default they follow the expected operator counts, and
+ * lg_batch_tune times the candidates.  set: fields < 0 keep their value; the settings then
  * stay as given (no automatic choice) until the next tune.  Results do not
  * depend on them.  No counterpart in the reference (launch configuration of
  * the device kernels). */
