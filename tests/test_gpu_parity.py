@@ -94,7 +94,8 @@ LAUNCH_TRAJ = [
     # than the 512-event staging (the linking kernel's global-memory path)
     (O.SQUARE, 8, 6.0, 1.0, 0.8, 20),
     (O.CHAIN, 32, 60.0, 1.0, 1.0, 20),
-    (O.CHAIN, 8, 600.0, 1.0, 1.0, 12),
+    (O.CHAIN, 8, 600.0, 1.0, 1.0, 12),     # about 375 operators per bond: the fill pass's 512-time stage
+    (O.CHAIN, 6, 1200.0, 1.0, 1.0, 8),     # about 750 per bond: its 1024-time stage (64 KB of LDS)
     (O.CHAIN, 4, 2000.0, 1.0, 0.9, 6),     # about 2500 events per site: beyond the 1536-event staging
 ]
 LAUNCHES = [dict(link=0), dict(link=128), dict(link=512), dict(link=1536), dict(link=512, group=4, blocks=7),
