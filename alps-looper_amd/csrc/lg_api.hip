@@ -86,7 +86,8 @@ ChainDev make_chain_dev(const lg_params& p) {
   double W = v[0] + v[1] + v[2] + v[3];
   d.beta = p.beta;
   d.has_weight = W > 0;
-  d.scale = (W > 0) ? std::ldexp(1.0, 52) / (p.beta * W) : 0.0;
+  d.wsum = W;
+  d.scale = candidate_scale(p.beta, W);
   d.offset_total = d.Nb * off;
   int last = -1;
   for (int g = 0; g < 4; ++g) if (v[g] > 0) last = g;
@@ -188,6 +189,8 @@ int Batch::init(const lg_params* p, int n) {
   std::vector<ChainDev> ds(n);
   for (int c = 0; c < n; ++c) ds[c] = make_chain_dev(p[c]);
   params.assign(p, p + n);
+  cap_beta = 0;
+  for (int c = 0; c < n; ++c) cap_beta = std::max(cap_beta, p[c].beta);
   rc = set_layout(ds, std::vector<uint64_t>(n, 0));
   if (rc) return rc;
   return reset_state();
@@ -367,7 +370,7 @@ int Batch::allocate() {
 }
 
 // New bases (contiguous in the given order), chain tables, sweep counters.
-int Batch::set_layout(std::vector<ChainDev>& ds, const std::vector<uint64_t>& sweeps) {
+int Batch::set_layout(std::vector<ChainDev>& ds, const std::vector<uint64_t>& sweeps, bool check_old) {
   long long sites = 0, bonds = 0, cand = 0;
   for (auto& d : ds) {
     d.site_base = (int)sites;
@@ -379,6 +382,26 @@ int Batch::set_layout(std::vector<ChainDev>& ds, const std::vector<uint64_t>& sw
   }
   if ((int)ds.size() > cap_chains || sites > budget.sites || bonds > budget.bonds || cand > cand_total)
     return set_error(LG_ERR_CAPACITY, "chains exceed the batch capacity");
+  if (check_old) {
+    // every chain's previous configuration lies inside the extent of the
+    // arrays it is read from, and no two chains read the same sites or bonds
+    std::vector<std::pair<long long, long long>> sr, br;
+    for (const auto& d : ds) {
+      if (d.old_site_base < 0 || d.old_bond_base < 0 || d.old_site_base + (long long)d.N > old_ext_sites ||
+          d.old_bond_base + (long long)d.Nb > old_ext_bonds)
+        return set_error(LG_ERR_STATE, "relayout: a chain's previous configuration lies outside the arrays");
+      sr.push_back({d.old_site_base, d.old_site_base + d.N});
+      br.push_back({d.old_bond_base, d.old_bond_base + d.Nb});
+    }
+    for (auto* r : {&sr, &br}) {
+      std::sort(r->begin(), r->end());
+      for (size_t i = 1; i < r->size(); ++i)
+        if ((*r)[i].first < (*r)[i - 1].second)
+          return set_error(LG_ERR_STATE, "relayout: chains' previous configurations overlap");
+    }
+    if (sites > cap_sites || bonds > cap_bonds)
+      return set_error(LG_ERR_STATE, "relayout: new layout beyond the site or bond arrays");
+  }
   chains = ds;
   nsites = (int)sites;
   nbonds = (int)bonds;
@@ -547,7 +570,7 @@ int Batch::relayout(const std::vector<int>& keep, const std::vector<lg_params>& 
   }
   params = ps;
   swept_since_layout = false;
-  return set_layout(ds, sw);
+  return set_layout(ds, sw, true);
 }
 
 void Batch::invalidate_graphs() {
@@ -581,6 +604,7 @@ DevBatch Batch::dev(int from) const {
   B.nslots = nslots; B.xslot = xslot; B.xbeta = xbeta; B.xscale = xscale; B.xlnr = xlnr;
   B.xacc = xacc; B.xstep = xstep; B.xkey0 = xkey0; B.xkey1 = xkey1;
   B.op_cap = op_cap; B.node_cap = node_cap;
+  B.sched = d_sched;
   return B;
 }
 
@@ -673,14 +697,14 @@ int Batch::tune(uint32_t sweeps_per_trial) {
   if (rc) return rc;
   LaunchCfg best = cfg;
   double best_t = 1e300;
-  const int groups[] = {4, 8, 16, 32, 64};
+  const int groups[] = {1, 2, 4, 8, 16, 32, 64};
   const int mults[] = {2, 4, 8, 16, 32, 64};
-  const int links[] = {0, 128, 512, 1536};
+  const int links[] = {0, 1, 128, 512, 1536};
   const int flushes[] = {16, 8, 4};
   // group size first with the current grids, then the grids with the best
   // group, then the linking kernel, then the compress pass, then the
   // estimator's flush interval
-  const int trials[] = {5, 6, 4, 2, 3};
+  const int trials[] = {7, 6, 5, 2, 3};
   for (int pass = 0; pass < 5; ++pass) {
     for (int i = 0; i < trials[pass]; ++i) {
       LaunchCfg trial = best;
@@ -715,8 +739,9 @@ void Batch::get_launch(lg_launch* out) const {
 int Batch::set_launch(const lg_launch& l) {
   LaunchCfg c = cfg;
   if (l.group >= 0) {
-    if (l.group != 4 && l.group != 8 && l.group != 16 && l.group != 32 && l.group != 64)
-      return set_error(LG_ERR_ARG, "launch group must be 4, 8, 16, 32 or 64");
+    if (l.group != 1 && l.group != 2 && l.group != 4 && l.group != 8 && l.group != 16 && l.group != 32 &&
+        l.group != 64)
+      return set_error(LG_ERR_ARG, "launch group must be 1, 2, 4, 8, 16, 32 or 64");
     c.group = l.group;
   }
   if (l.blocks >= 0) {
@@ -724,8 +749,8 @@ int Batch::set_launch(const lg_launch& l) {
     c.blocks_op = c.blocks_node = l.blocks;
   }
   if (l.link >= 0) {
-    if (l.link != 0 && l.link != 128 && l.link != 512 && l.link != 1536)
-      return set_error(LG_ERR_ARG, "launch link must be 0, 128, 512 or 1536");
+    if (l.link != 0 && l.link != 1 && l.link != 128 && l.link != 512 && l.link != 1536)
+      return set_error(LG_ERR_ARG, "launch link must be 0, 1, 128, 512 or 1536");
     c.link_cap = l.link;
   }
   if (l.compress >= 0) {
@@ -869,7 +894,7 @@ int Batch::set_config(int c, const int32_t* spins_in, const int32_t* bond_offset
   uint32_t sw = (uint32_t)sweep;
   LG_HIP(hipMemcpy(d_sweep, &sw, sizeof(uint32_t), hipMemcpyHostToDevice));
   host_sweep[c] = sweep;
-  return LG_OK;
+  return sync_schedule();
 }
 
 int Batch::get_config(int c, int32_t* spins, int32_t* bond_offsets, double* times, int32_t* types) {
@@ -988,7 +1013,7 @@ int Batch::load_configs(const std::vector<HostConfig>& cfg) {
   old_ext_sites = nsites;
   old_ext_bonds = nbonds;
   old_ext_nodes = node_base;
-  return LG_OK;
+  return sync_schedule();
 }
 
 void Batch::adopt(Batch& o) {
@@ -997,6 +1022,71 @@ void Batch::adopt(Batch& o) {
   o.allocations.clear();
   o.graph_exec[0] = o.graph_exec[1] = nullptr;
   o.stream = nullptr;
+}
+
+int Batch::sync_schedule() {
+  bool any = false;
+  for (size_t c = 0; c < chains.size(); ++c) {
+    ChainDev& d = chains[c];
+    if (d.sched_n <= 0) continue;
+    d.beta = 1.0 / schedule_temperature(sched_host.data() + 2 * d.sched_off, d.sched_n, (int)host_sweep[c]);
+    d.scale = candidate_scale(d.beta, d.wsum);
+    any = true;
+  }
+  if (!any) return LG_OK;
+  LG_HIP(hipStreamSynchronize(stream));
+  LG_HIP(hipMemcpy(d_chains, chains.data(), chains.size() * sizeof(ChainDev), hipMemcpyHostToDevice));
+  invalidate_graphs();
+  return LG_OK;
+}
+
+int Batch::set_schedule(int n, const int32_t* steps, const double* temps) {
+  if (chains.size() != 1) return set_error(LG_ERR_UNSUPPORTED, "an annealing schedule needs a single-chain handle");
+  if (n < 0 || (n > 0 && (!steps || !temps))) return set_error(LG_ERR_ARG, "invalid schedule");
+  const lg_params p = params[0];
+  double bmax = p.beta;
+  for (int k = 0; k < n; ++k) {
+    if (steps[k] < 1 || (k > 0 && steps[k] <= steps[k - 1]))
+      return set_error(LG_ERR_ARG, "schedule steps must be ascending and >= 1");
+    if (!(temps[k] > 0) || !std::isfinite(temps[k])) return set_error(LG_ERR_ARG, "schedule temperatures must be positive");
+    bmax = std::max(bmax, 1.0 / temps[k]);
+  }
+  if (n > 0 && 1.0 / temps[n - 1] != p.beta)
+    return set_error(LG_ERR_ARG, "the schedule's last temperature must be the chain's (final) temperature");
+  if (bmax > cap_beta) {
+    // capacity (candidate slots, operators) for the largest beta of the
+    // schedule; the configuration and sweep count carry over
+    std::vector<HostConfig> cfgs(1);
+    int rc = chain_config(0, cfgs[0].spins, cfgs[0].bo, cfgs[0].times, cfgs[0].types);
+    if (rc) return rc;
+    cfgs[0].sweep = host_sweep[0];
+    lg_params q = p;
+    q.beta = bmax;
+    Batch nb;
+    nb.use_graph = use_graph;
+    if ((rc = nb.init(&q, 1))) return rc;
+    nb.params[0] = p;
+    if ((rc = nb.load_configs(cfgs))) return rc;
+    adopt(nb);
+  }
+  sched_host.clear();
+  for (int k = 0; k < n; ++k) { sched_host.push_back((double)steps[k]); sched_host.push_back(temps[k]); }
+  if (n > 0) {
+    if (d_sched) { (void)hipFree(d_sched); allocations.erase(std::find(allocations.begin(), allocations.end(), (void*)d_sched)); d_sched = nullptr; }
+    int rc = alloc(d_sched, sched_host.size());
+    if (rc) return rc;
+    LG_HIP(hipMemcpy(d_sched, sched_host.data(), sched_host.size() * sizeof(double), hipMemcpyHostToDevice));
+  }
+  chains[0].sched_n = n;
+  chains[0].sched_off = 0;
+  if (n == 0) {  // fixed temperature again
+    chains[0].beta = p.beta;
+    chains[0].scale = candidate_scale(p.beta, chains[0].wsum);
+    LG_HIP(hipMemcpy(d_chains, chains.data(), sizeof(ChainDev), hipMemcpyHostToDevice));
+    invalidate_graphs();
+    return LG_OK;
+  }
+  return sync_schedule();
 }
 
 // Cluster stage on a given configuration, written into the "new" buffers as
@@ -1148,6 +1238,11 @@ int lg_chain_destroy(lg_chain* h) {
   return LG_OK;
 }
 
+int lg_chain_set_schedule(lg_chain* h, int32_t n, const int32_t* steps, const double* temperatures) {
+  if (!h) return set_error(LG_ERR_ARG, "null handle");
+  return h->b.set_schedule(n, steps, temperatures);
+}
+
 int lg_cluster_config(const lg_params* p, const int32_t* spins, const int32_t* bond_offsets, const double* times,
                       const int32_t* types, uint64_t sweep, int32_t* labels, int32_t* spins_out,
                       int32_t* types_out, int64_t* n_clusters) {
@@ -1255,10 +1350,20 @@ int lg_batch_add(lg_batch* h, const lg_params* p, int32_t n) {
   return LG_OK;
 }
 
-static const char kChainMagic[8] = {'L', 'G', 'C', 'H', 'A', 'I', 'N', '1'};
+// Chain checkpoint blob, in the content order of the reference worker's save
+// (path_integral.C:111-124: mcs, spins, operators; an operator is its type,
+// location and time, operator.h:108-109,138-139, location_impl.h:43):
+//   "LGCHAIN2", sweep (uint64; the Monte Carlo step count and the Philox
+//   counter), n_sites (uint64), spins (int32 each, 0 = up), n_ops (uint64),
+//   operators in imaginary-time order (ties by bond, then by their order on
+//   the bond) as {int32 type = offdiagonal | graph << 2, int32 location =
+//   bond << 1 | 1, double time}, then the chain's parameters (lg_params:
+//   lattice, couplings, temperature, seed and chain id), which load checks.
+static const char kChainMagic[8] = {'L', 'G', 'C', 'H', 'A', 'I', 'N', '2'};
+static const size_t kOpRecord = 16;
 
 static size_t chain_blob_size(const lg::ChainDev& d, long long nops) {
-  return 8 + sizeof(lg_params) + 8 + (size_t)d.N * 4 + (size_t)(d.Nb + 1) * 4 + (size_t)nops * 12;
+  return 8 + 8 + 8 + (size_t)d.N * 4 + 8 + (size_t)nops * kOpRecord + sizeof(lg_params);
 }
 
 int lg_chain_state_size(lg_chain* h, size_t* bytes) {
@@ -1279,47 +1384,81 @@ int lg_chain_save(lg_chain* h, void* buf, size_t len) {
   const lg::ChainDev& d = h->b.chains[0];
   const long long nops = (long long)times.size();
   if (len < chain_blob_size(d, nops)) return set_error(LG_ERR_ARG, "checkpoint buffer too small");
+  std::vector<int> bond(nops), order(nops);
+  for (int b = 0; b < d.Nb; ++b)
+    for (int k = bo[b]; k < bo[b + 1]; ++k) bond[k] = b;
+  for (long long k = 0; k < nops; ++k) order[k] = (int)k;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    if (times[a] != times[b]) return times[a] < times[b];
+    return bond[a] < bond[b];
+  });
   char* o = (char*)buf;
-  std::memcpy(o, kChainMagic, 8); o += 8;
-  std::memcpy(o, &h->b.params[0], sizeof(lg_params)); o += sizeof(lg_params);
-  const uint64_t sw = h->b.host_sweep[0];
-  std::memcpy(o, &sw, 8); o += 8;
-  std::memcpy(o, spins.data(), (size_t)d.N * 4); o += (size_t)d.N * 4;
-  std::memcpy(o, bo.data(), (size_t)(d.Nb + 1) * 4); o += (size_t)(d.Nb + 1) * 4;
-  if (nops) {
-    std::memcpy(o, times.data(), (size_t)nops * 8); o += (size_t)nops * 8;
-    std::memcpy(o, types.data(), (size_t)nops * 4);
+  auto put = [&](const void* p, size_t n) { std::memcpy(o, p, n); o += n; };
+  put(kChainMagic, 8);
+  const uint64_t sw = h->b.host_sweep[0], ns = (uint64_t)d.N, no = (uint64_t)nops;
+  put(&sw, 8);
+  put(&ns, 8);
+  put(spins.data(), (size_t)d.N * 4);
+  put(&no, 8);
+  for (long long i = 0; i < nops; ++i) {
+    const int k = order[i];
+    const int32_t type = types[k], loc = (bond[k] << 1) | 1;
+    put(&type, 4);
+    put(&loc, 4);
+    put(&times[k], 8);
   }
+  put(&h->b.params[0], sizeof(lg_params));
   return LG_OK;
 }
 
 int lg_chain_load(lg_chain* h, const void* buf, size_t len) {
   if (!h || !buf) return set_error(LG_ERR_ARG, "null pointer");
   const lg::ChainDev& d = h->b.chains[0];
-  const size_t head = 8 + sizeof(lg_params) + 8 + (size_t)d.N * 4 + (size_t)(d.Nb + 1) * 4;
-  if (len < head) return set_error(LG_ERR_ARG, "checkpoint too short");
   const char* in = (const char*)buf;
-  if (std::memcmp(in, kChainMagic, 8) != 0) return set_error(LG_ERR_ARG, "not a chain checkpoint");
+  const char* end = in + len;
+  auto get = [&](void* p, size_t n) {
+    if ((size_t)(end - in) < n) return false;
+    std::memcpy(p, in, n);
+    in += n;
+    return true;
+  };
+  char magic[8];
+  uint64_t sw = 0, ns = 0, no = 0;
+  if (!get(magic, 8) || std::memcmp(magic, kChainMagic, 8) != 0) return set_error(LG_ERR_ARG, "not a chain checkpoint");
+  if (!get(&sw, 8) || !get(&ns, 8)) return set_error(LG_ERR_ARG, "checkpoint truncated");
+  if (ns != (uint64_t)d.N) return set_error(LG_ERR_ARG, "checkpoint of a different lattice");
+  std::vector<int32_t> spins(d.N);
+  if (!get(spins.data(), (size_t)d.N * 4) || !get(&no, 8)) return set_error(LG_ERR_ARG, "checkpoint truncated");
+  if (no > (uint64_t)(end - in) / kOpRecord) return set_error(LG_ERR_ARG, "checkpoint truncated");
+  std::vector<int32_t> type(no), bond(no);
+  std::vector<double> time(no);
+  for (uint64_t i = 0; i < no; ++i) {
+    int32_t loc = 0;
+    get(&type[i], 4);
+    get(&loc, 4);
+    get(&time[i], 8);
+    bond[i] = loc >> 1;
+    if ((loc & 1) != 1 || bond[i] < 0 || bond[i] >= d.Nb) return set_error(LG_ERR_ARG, "checkpoint operator location invalid");
+  }
   lg_params q;
-  std::memcpy(&q, in + 8, sizeof(lg_params));
+  if (!get(&q, sizeof(lg_params))) return set_error(LG_ERR_ARG, "checkpoint truncated");
   const lg_params& p = h->b.params[0];
   if (q.lattice != p.lattice || q.L != p.L || q.beta != p.beta || q.jxy != p.jxy || q.jz != p.jz ||
-      q.force_scatter != p.force_scatter || q.seed != p.seed || q.chain_id != p.chain_id)
+      q.force_scatter != p.force_scatter || q.seed != p.seed || q.chain_id != p.chain_id || q.h != p.h)
     return set_error(LG_ERR_ARG, "checkpoint of a different chain (parameters, seed or chain id)");
-  const char* o = in + 8 + sizeof(lg_params);
-  uint64_t sw = 0;
-  std::memcpy(&sw, o, 8); o += 8;
-  std::vector<int32_t> spins(d.N), bo(d.Nb + 1);
-  std::memcpy(spins.data(), o, (size_t)d.N * 4); o += (size_t)d.N * 4;
-  std::memcpy(bo.data(), o, (size_t)(d.Nb + 1) * 4); o += (size_t)(d.Nb + 1) * 4;
-  const long long nops = bo[d.Nb];
-  if (nops < 0 || len < head + (size_t)nops * 12) return set_error(LG_ERR_ARG, "checkpoint truncated");
-  std::vector<double> times(nops);
-  std::vector<int32_t> types(nops);
-  if (nops) {
-    std::memcpy(times.data(), o, (size_t)nops * 8); o += (size_t)nops * 8;
-    std::memcpy(types.data(), o, (size_t)nops * 4);
+  // bond-major order for the device: a stable sort by bond keeps each bond's
+  // operators in their imaginary-time order
+  std::vector<int> order(no);
+  for (uint64_t i = 0; i < no; ++i) order[i] = (int)i;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return bond[a] < bond[b]; });
+  std::vector<int32_t> bo(d.Nb + 1, 0), types(no);
+  std::vector<double> times(no);
+  for (uint64_t i = 0; i < no; ++i) {
+    ++bo[bond[i] + 1];
+    types[i] = type[order[i]];
+    times[i] = time[order[i]];
   }
+  for (int b = 0; b < d.Nb; ++b) bo[b + 1] += bo[b];
   return h->b.set_config(0, spins.data(), bo.data(), times.data(), types.data(), sw);
 }
 

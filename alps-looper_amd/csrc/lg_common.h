@@ -55,7 +55,36 @@ struct ChainDev {
   uint32_t key0, key1;               // Philox key = (seed, chain id)
   int has_weight;
   double h;                          // longitudinal field (site term -h S^z)
+  double wsum;                       // sum_g v_g of a bond (candidate rate beta * wsum)
+  int sched_n, sched_off;            // annealing schedule: entries and offset in the batch's table (0: fixed beta)
 };
+
+// Temperature of Monte Carlo step `step` on an annealing schedule of n
+// (start step, temperature) pairs stored as doubles, start steps ascending,
+// the last pair (first step at the final temperature, final temperature):
+// the piecewise-linear interpolation of looper/temperature.h:93-113 (the
+// first entry's temperature up to its step, the final one after the last).
+// Step 0 takes the first entry's temperature (the reference rejects step 0
+// when a schedule is set, temperature.h:96-100, and its worker asks for it
+// on its first sweep).  The arithmetic is written out so that host and
+// device evaluate it identically.
+LG_HD double schedule_temperature(const double* sched, int n, int step) {
+#ifdef __HIPCC__
+#pragma clang fp contract(off)
+#endif
+  if (step < 1) step = 1;
+  int k = 0;
+  while (k < n && sched[2 * k] < (double)step) ++k;
+  if (k == 0) return sched[1];
+  if (k == n) return sched[2 * n - 1];
+  const double s0 = sched[2 * k - 2], t0 = sched[2 * k - 1], s1 = sched[2 * k], t1 = sched[2 * k + 1];
+  return t0 + ((double)step - s0) * (t1 - t0) / (s1 - s0);
+}
+
+// candidate-time scale 2^52 / (beta * sum_g v_g) of a chain at inverse temperature beta
+LG_HD double candidate_scale(double beta, double wsum) {
+  return wsum > 0 ? 4503599627370496.0 / (beta * wsum) : 0.0;
+}
 
 // Philox4x32-10 (Salmon et al., SC'11): 10 rounds, multipliers D2511F53 /
 // CD9E8D57, Weyl increments 9E3779B9 / BB67AE85.

@@ -73,6 +73,13 @@ class Batch {
   // take over o's device resources and settings (o is left empty)
   void adopt(Batch& o);
   bool is_exchange() const { return nslots > 0; }
+  // annealing schedule of a single-chain batch: n (start step, temperature)
+  // pairs (looper/temperature.h); n == 0 removes it.  Rebuilds the device
+  // memory when the schedule's largest beta needs more capacity.
+  int set_schedule(int n, const int32_t* steps, const double* temps);
+  // inverse temperature of every scheduled chain for its current sweep count,
+  // uploaded to the device descriptors
+  int sync_schedule();
   int cluster_only(const int32_t* spins, const int32_t* bond_offsets, const double* times,
                    const int32_t* types, uint64_t sweep, int32_t* labels, int32_t* spins_out,
                    int32_t* types_out, int64_t* n_clusters);
@@ -103,7 +110,9 @@ class Batch {
   template <class T>
   int alloc(T*& ptr, size_t n);
   int reserve(const Capacity& cap, bool dynamic);
-  int set_layout(std::vector<ChainDev>& ds, const std::vector<uint64_t>& sweeps);
+  // new bases (contiguous in the given order), chain tables, sweep counters;
+  // check_old: also check the previous bases a relayout leaves the chains with
+  int set_layout(std::vector<ChainDev>& ds, const std::vector<uint64_t>& sweeps, bool check_old = false);
   int allocate();
   int reset_state();
   void release();
@@ -157,6 +166,10 @@ class Batch {
   unsigned long long* xacc = nullptr;
   uint32_t* xstep = nullptr;
   uint32_t xkey0 = 0, xkey1 = 0;
+  // annealing schedules (one chain): table of (start step, temperature) pairs
+  std::vector<double> sched_host;
+  double* d_sched = nullptr;
+  double cap_beta = 0;          // inverse temperature the capacity was sized for
 };
 
 }  // namespace lg

@@ -74,6 +74,9 @@ struct DevBatch {
   unsigned long long* xacc;    // [nslots-1] accepted swaps per pair
   uint32_t* xstep;             // exchange steps done
   uint32_t xkey0, xkey1;
+  // annealing schedules: (start step, temperature) pairs of the chains with
+  // sched_n > 0, at their sched_off (k_end_sweep sets beta for the next sweep)
+  const double* sched;
   int op_cap, node_cap;
 };
 
@@ -82,7 +85,7 @@ struct LaunchCfg {
   int max_blocks_bond = 1 << 20;
   int blocks_op = 1024;        // grid of the per-operator kernels (grid-stride)
   int blocks_node = 1024;      // grid of the per-node kernels (grid-stride)
-  int link_cap = 512;          // linking: 0 one thread per operator, else a wave per site staging 512 or 1536 events
+  int link_cap = 512;          // linking: 0 one thread per operator, 1 one thread per site, else a group of threads per site staging 128, 512 or 1536 events
   // roots: 1 a compression pass before the estimator (which then reads roots
   // and flip bits in one load), 0 the estimator walks to the roots itself and
   // the collector clears the accumulators it has read

@@ -894,6 +894,86 @@ __global__ void __launch_bounds__(256) k_link_ops(DevBatch B) {
   }
 }
 
+// Linking, one thread per site: the site's bond lists (two or four, each in
+// time order) merged in (time, bond) order -- the canonical event order of a
+// site -- writing the site's segments in node order and the legs of each
+// event; the segment spin flips at every off-diagonal event, so no prefix
+// counts or searches are needed.  For sites with a few events each: a
+// thread's work is serial in its site's events.  The list heads are kept in
+// named registers and selected with constant indices (a dynamically indexed
+// register array would live in scratch memory).
+__global__ void __launch_bounds__(256) k_link_site(DevBatch B) {
+  if (overflowed(B)) return;
+  int slo, shi;
+  xcd_share(B.nsites, slo, shi);
+  for (int gs = slo + threadIdx.x; gs < shi; gs += blockDim.x) {
+    const ChainDev& P = B.chains[B.site_chain[gs]];
+    const int s = gs - P.site_base;
+    int bl[4];
+    const int nbd = site_bonds(P, s, bl);
+    int p0, p1, p2 = 0, p3 = 0, e0, e1, e2 = 0, e3 = 0;
+    {
+      const int b0 = P.bond_base + bl[0], b1 = P.bond_base + bl[1];
+      p0 = B.bo_new[b0]; e0 = B.bo_new[b0 + 1];
+      p1 = B.bo_new[b1]; e1 = B.bo_new[b1 + 1];
+      if (nbd == 4) {
+        const int b2 = P.bond_base + bl[2], b3 = P.bond_base + bl[3];
+        p2 = B.bo_new[b2]; e2 = B.bo_new[b2 + 1];
+        p3 = B.bo_new[b3]; e3 = B.bo_new[b3 + 1];
+      }
+    }
+    if (p0 == e0 && p1 == e1 && p2 == e2 && p3 == e3) continue;  // event-free: written by k_scan
+    // side (0 source, 1 target) and direction flags of each list's bond
+    uint8_t f0, f1, f2 = 0, f3 = 0;
+    {
+      int a, b;
+      bond_sites(P, bl[0], a, b);
+      f0 = (uint8_t)((a == s ? EV_SRC : 0) | (bond_dir(P, bl[0]) ? EV_DIR1 : 0));
+      bond_sites(P, bl[1], a, b);
+      f1 = (uint8_t)((a == s ? EV_SRC : 0) | (bond_dir(P, bl[1]) ? EV_DIR1 : 0));
+      if (nbd == 4) {
+        bond_sites(P, bl[2], a, b);
+        f2 = (uint8_t)((a == s ? EV_SRC : 0) | (bond_dir(P, bl[2]) ? EV_DIR1 : 0));
+        bond_sites(P, bl[3], a, b);
+        f3 = (uint8_t)((a == s ? EV_SRC : 0) | (bond_dir(P, bl[3]) ? EV_DIR1 : 0));
+      }
+    }
+    double t0 = p0 < e0 ? B.ot_new[p0] : 3.0, t1 = p1 < e1 ? B.ot_new[p1] : 3.0;
+    double t2 = p2 < e2 ? B.ot_new[p2] : 3.0, t3 = p3 < e3 ? B.ot_new[p3] : 3.0;
+    const int n0 = B.soff_new[gs], n1 = B.soff_new[gs + 1];
+    const uint8_t gflag = gauge(P, s) < 0 ? EV_GAUGE : 0;
+    const bool all_sites = B.nchains > 1;
+    uint8_t spin = B.spins_old[P.old_site_base + s];
+    for (int node = n0; node < n1; ++node) {
+      // the head with the least (time, bond)
+      int i = 0;
+      double t = t0;
+      int bi = bl[0];
+      if (t1 < t || (t1 == t && bl[1] < bi)) { i = 1; t = t1; bi = bl[1]; }
+      if (nbd == 4) {
+        if (t2 < t || (t2 == t && bl[2] < bi)) { i = 2; t = t2; bi = bl[2]; }
+        if (t3 < t || (t3 == t && bl[3] < bi)) { i = 3; t = t3; bi = bl[3]; }
+      }
+      const int k = i == 0 ? p0 : i == 1 ? p1 : i == 2 ? p2 : p3;
+      const uint8_t fl = i == 0 ? f0 : i == 1 ? f1 : i == 2 ? f2 : f3;
+      const int off = B.oi_new[k] & 1;
+      const int up = (node + 1 == n1) ? n0 : node + 1;  // segment above the event
+      B.evt_new[node] = t;
+      B.segspin_new[node] = spin;
+      B.evflags[node] = (uint8_t)(off | fl | (node == n0 ? EV_FIRST : 0) | gflag);
+      B.parent[node] = node;
+      if (all_sites || node == n0) B.node_site[node] = gs;
+      if (fl & EV_SRC) { B.l0[k] = node; B.u0[k] = up; } else { B.l1[k] = node; B.u1[k] = up; }
+      spin ^= (uint8_t)off;
+      // advance the chosen list
+      if (i == 0) { ++p0; t0 = p0 < e0 ? B.ot_new[p0] : 3.0; }
+      else if (i == 1) { ++p1; t1 = p1 < e1 ? B.ot_new[p1] : 3.0; }
+      else if (i == 2) { ++p2; t2 = p2 < e2 ? B.ot_new[p2] : 3.0; }
+      else { ++p3; t3 = p3 < e3 ? B.ot_new[p3] : 3.0; }
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // (c) lock-free union-find on segments: hook the larger root under the
 // smaller with atomicCAS, so every root is its cluster's minimum node (the
@@ -1520,6 +1600,14 @@ __global__ void __launch_bounds__(END_THREADS) k_end_sweep(DevBatch B) {
     out[R_BETA] = P.beta;
     out[R_SWEEP] = (double)B.sweep[c];
     B.sweep[c] += 1;
+    // annealing: the next sweep's inverse temperature and candidate scale
+    // (looper/temperature.h:93-113 at the next Monte Carlo step)
+    if (P.sched_n > 0) {
+      ChainDev* W = B.chains + c;
+      const double beta = 1.0 / schedule_temperature(B.sched + 2 * P.sched_off, P.sched_n, (int)B.sweep[c]);
+      W->beta = beta;
+      W->scale = candidate_scale(beta, P.wsum);
+    }
     __threadfence();
     is_last = (atomicAdd(B.end_count, 1) == (int)gridDim.x - 1);
   }
@@ -1621,6 +1709,8 @@ static void launch_diag(const DevBatch& B, const LaunchCfg& L, hipStream_t st, b
 
 static void diag_dispatch(const DevBatch& B, const LaunchCfg& L, hipStream_t st, bool fill) {
   switch (L.group) {
+    case 1: launch_diag<1>(B, L, st, fill); break;
+    case 2: launch_diag<2>(B, L, st, fill); break;
     case 4: launch_diag<4>(B, L, st, fill); break;
     case 8: launch_diag<8>(B, L, st, fill); break;
     case 16: launch_diag<16>(B, L, st, fill); break;
@@ -1667,6 +1757,9 @@ void launch_cluster_stage(const DevBatch& B, const LaunchCfg& L, hipStream_t st,
   // ms at 8 steps per block, 5.95 at one)
   if (L.link_cap == 0) {
     hipLaunchKernelGGL(k_link_ops, dim3(L.blocks_op), dim3(256), 0, st, B);
+  } else if (L.link_cap == 1) {
+    const int blocks = max(1, min(L.blocks_op, (B.nsites + 255) / 256));
+    hipLaunchKernelGGL(k_link_site, dim3(blocks), dim3(256), 0, st, B);
   } else if (L.link_cap <= 128) {
     const int blocks = max(1, (B.nsites + 15) / 16);
     hipLaunchKernelGGL((k_link<128, 4, 16>), dim3(blocks), dim3(256), 0, st, B);

@@ -5,6 +5,8 @@
 //  * parameters by the reference's names (alps::Parameters semantics for the
 //    subset this path supports): LATTICE, L, W, T / TEMPERATURE / BETA /
 //    INVERSE_TEMPERATURE (temperature.h:43-79), J / Jxy / Jz, MODEL, local_S,
+//    annealing T_START_i / T_DURATION_i and their TEMPERATURE_, BETA_,
+//    INVERSE_TEMPERATURE_ forms (temperature.h:56-81),
 //    SWEEPS (default "[65536:]") and THERMALIZATION (default SWEEPS.min/8,
 //    montecarlo.h:41-45), SEED, FORCE_SCATTER (weight_impl.h:169), h,
 //    DISABLE_IMPROVED_ESTIMATOR, MEASURE[Correlations] / MEASURE[Structure
@@ -185,6 +187,51 @@ static int parse_sweeps(const std::string& s, uint64_t& mn, uint64_t& mx) {
   return 0;
 }
 
+// looper::temperature::init (temperature.h:43-81): the final temperature
+// from T, TEMPERATURE, BETA or INVERSE_TEMPERATURE, and the annealing
+// schedule T_START_i / T_DURATION_i (or the TEMPERATURE_, BETA_ and
+// INVERSE_TEMPERATURE_ forms), i = 0, 1, ... until a pair is missing or not
+// positive, each starting where the previous one ended.  steps/temps receive
+// the (start step, temperature) pairs, the last one (first step at the final
+// temperature, final temperature).
+double parse_temperature(const Parameters& p, std::vector<int32_t>& steps, std::vector<double>& temps) {
+  double T;
+  if (p.defined("T")) T = p.num("T");
+  else if (p.defined("TEMPERATURE")) T = p.num("TEMPERATURE");
+  else if (p.defined("BETA")) T = 1 / p.num("BETA");
+  else if (p.defined("INVERSE_TEMPERATURE")) T = 1 / p.num("INVERSE_TEMPERATURE");
+  else throw std::invalid_argument("temperature is not initialized");
+  if (!(T > 0)) throw std::invalid_argument("negative temperature");
+  steps.clear();
+  temps.clear();
+  int s_prev = 1;
+  for (int i = 0;; ++i) {
+    const std::string n = std::to_string(i);
+    int td = -1;
+    double ts = -1;
+    if (p.defined("T_DURATION_" + n) && p.defined("T_START_" + n)) {
+      td = (int)p.num("T_DURATION_" + n);
+      ts = p.num("T_START_" + n);
+    } else if (p.defined("TEMPERATURE_DURATION_" + n) && p.defined("TEMPERATURE_START_" + n)) {
+      td = (int)p.num("TEMPERATURE_DURATION_" + n);
+      ts = p.num("TEMPERATURE_START_" + n);
+    } else if (p.defined("BETA_DURATION_" + n) && p.defined("BETA_START_" + n)) {
+      td = (int)p.num("BETA_DURATION_" + n);
+      ts = 1 / p.num("BETA_START_" + n);
+    } else if (p.defined("INVERSE_TEMPERATURE_DURATION_" + n) && p.defined("INVERSE_TEMPERATURE_START_" + n)) {
+      td = (int)p.num("INVERSE_TEMPERATURE_DURATION_" + n);
+      ts = 1 / p.num("INVERSE_TEMPERATURE_START_" + n);
+    }
+    if (td <= 0 || !(ts > 0)) break;
+    steps.push_back(s_prev);
+    temps.push_back(ts);
+    s_prev += td;
+  }
+  steps.push_back(s_prev);
+  temps.push_back(T);
+  return T;
+}
+
 class loop_worker {
  public:
   explicit loop_worker(const Parameters& p) {
@@ -204,14 +251,8 @@ class loop_worker {
     lp_.h = p.num("h", 0.0);
     // improved estimators unless a field or DISABLE_IMPROVED_ESTIMATOR (path_integral.C:217)
     improved_ = lp_.h == 0.0 && !p.defined("DISABLE_IMPROVED_ESTIMATOR");
-    // temperature.h:43-55
-    double T;
-    if (p.defined("T")) T = p.num("T");
-    else if (p.defined("TEMPERATURE")) T = p.num("TEMPERATURE");
-    else if (p.defined("BETA")) T = 1 / p.num("BETA");
-    else if (p.defined("INVERSE_TEMPERATURE")) T = 1 / p.num("INVERSE_TEMPERATURE");
-    else throw std::invalid_argument("temperature is not initialized");
-    if (!(T > 0)) throw std::invalid_argument("negative temperature");
+    // temperature and annealing schedule (temperature.h:43-81)
+    const double T = parse_temperature(p, sched_steps_, sched_temps_);
     lp_.beta = 1 / T;
     double J = p.num("J", 1.0);
     lp_.jxy = p.num("Jxy", J);
@@ -222,6 +263,9 @@ class loop_worker {
     if (parse_sweeps(p.str("SWEEPS", "[65536:]"), sweeps_min_, sweeps_max_))
       throw std::invalid_argument("malformed SWEEPS");
     therm_ = p.defined("THERMALIZATION") ? (uint64_t)std::llround(p.num("THERMALIZATION")) : (sweeps_min_ >> 3);
+    // path_integral.C:212-213 (annealing_steps = first step at the final temperature - 1)
+    if ((uint64_t)(sched_steps_.back() - 1) > therm_)
+      throw std::invalid_argument("longer annealing steps than thermalization");
     n_sites_ = (lp_.lattice == LG_LATTICE_CHAIN) ? lp_.L : lp_.L * lp_.L;
     dim_ = (lp_.lattice == LG_LATTICE_CHAIN) ? 1 : 2;
     // correlation.h:74-108; the Green function is disabled there as well
@@ -261,9 +305,12 @@ class loop_worker {
       spins8_.resize(n_sites_);
     }
     int rc = lg_chain_create(&lp_, &h_);
+    if (rc == LG_OK && sched_steps_.size() > 1)
+      rc = lg_chain_set_schedule(h_, (int32_t)sched_steps_.size(), sched_steps_.data(), sched_temps_.data());
     if (rc != LG_OK) {
       char buf[512];
       lg_last_error(buf, sizeof buf);
+      if (h_) { lg_chain_destroy(h_); h_ = nullptr; }
       if (rc == LG_ERR_ARG) throw std::invalid_argument(buf);
       throw std::runtime_error(buf);
     }
@@ -288,10 +335,14 @@ class loop_worker {
   uint64_t mcs() const { return mcs_; }
 
   // one Monte Carlo step (path_integral.C:323-351); sweeps are fetched from the
-  // device in chunks, so run() usually only consumes a buffered record
-  void run(std::map<std::string, Observable>& obs) {
+  // device in chunks, so run() usually only consumes a buffered record.  A
+  // chunk never goes beyond the `budget` steps the caller still takes, the
+  // end of thermalization or the end of the run, so the device's
+  // configuration is this step's whenever the caller stops (save() needs it)
+  void run(std::map<std::string, Observable>& obs, uint64_t budget = 1) {
     if (next_ == buf_.size()) {
-      uint64_t left = (mcs_ < therm_) ? therm_ - mcs_ : (therm_ + sweeps_max_ - mcs_);
+      uint64_t left = (mcs_ < therm_) ? therm_ - mcs_ : (therm_ + sweeps_min_ - mcs_);
+      left = std::min<uint64_t>(left, budget);
       // correlations are measured on every sweep's spins: one sweep at a time
       if (corr_ && is_thermalized()) left = 1;
       uint32_t k = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(left, chunk_));
@@ -322,52 +373,82 @@ class loop_worker {
     ++mcs_;
   }
 
-  // checkpoint (path_integral.C:111-124): counters, spins, operators
-  std::vector<uint8_t> save() const {
-    if (next_ != buf_.size()) throw std::logic_error("save() with buffered sweeps; call at a chunk boundary");
-    int32_t ns = 0, nb = 0;
-    int64_t nops = 0;
-    uint64_t sw = 0;
-    lg_chain_info(h_, &ns, &nb, &nops, &sw);
-    std::vector<int32_t> spins(ns), bo(nb + 1), types(nops);
-    std::vector<double> times(nops);
-    lg_chain_get_config(h_, spins.data(), bo.data(), times.data(), types.data());
+  // checkpoint: the Monte Carlo counters (mc_steps::save, montecarlo.h:66:
+  // mcs, SWEEPS range, thermalization), then the chain's configuration in the
+  // reference's content order (path_integral.C:111-124: spins, operators;
+  // lg_chain_save), then the measurements so far (the observable set that
+  // parapack checkpoints beside the worker), so that a resumed worker ends
+  // with the results of an uninterrupted one.
+  std::vector<uint8_t> save(const std::map<std::string, Observable>& obs) const {
+    if (next_ != buf_.size()) throw std::logic_error("save() with sweeps fetched but not yet run");
+    size_t n = 0;
+    int rc = lg_chain_state_size(h_, &n);
+    std::vector<uint8_t> chain(n);
+    if (rc == LG_OK) rc = lg_chain_save(h_, chain.data(), n);
+    if (rc != LG_OK) {
+      char b[512];
+      lg_last_error(b, sizeof b);
+      throw std::runtime_error(b);
+    }
     std::vector<uint8_t> out;
-    auto put = [&](const void* p, size_t n) { const uint8_t* b = (const uint8_t*)p; out.insert(out.end(), b, b + n); };
-    const char magic[8] = {'L', 'G', 'C', 'H', 'K', 'P', 'T', '1'};
+    auto put = [&](const void* p, size_t k) { const uint8_t* b = (const uint8_t*)p; out.insert(out.end(), b, b + k); };
+    const char magic[8] = {'L', 'G', 'W', 'O', 'R', 'K', 'R', '2'};
     put(magic, 8);
-    put(&mcs_, 8); put(&therm_, 8); put(&sweeps_min_, 8); put(&sweeps_max_, 8); put(&sw, 8);
-    put(&ns, 4); put(&nb, 4); put(&nops, 8);
-    put(spins.data(), 4 * ns); put(bo.data(), 4 * (nb + 1));
-    put(times.data(), 8 * nops); put(types.data(), 4 * nops);
+    put(&mcs_, 8); put(&sweeps_min_, 8); put(&sweeps_max_, 8); put(&therm_, 8);
+    const uint64_t cl = chain.size();
+    put(&cl, 8);
+    put(chain.data(), chain.size());
+    const uint64_t nobs = obs.size();
+    put(&nobs, 8);
+    for (const auto& kv : obs) {
+      const uint64_t nl = kv.first.size();
+      put(&nl, 8);
+      put(kv.first.data(), nl);
+      std::vector<double> v;
+      kv.second.save(v);
+      const uint64_t nv = v.size();
+      put(&nv, 8);
+      put(v.data(), 8 * nv);
+    }
     return out;
   }
-  void load(const uint8_t* p, size_t len) {
+  void load(const uint8_t* p, size_t len, std::map<std::string, Observable>& obs) {
     size_t off = 0;
     auto get = [&](void* d, size_t n) {
-      if (off + n > len) throw std::invalid_argument("truncated checkpoint");
+      if (n > len - off) throw std::invalid_argument("truncated checkpoint");
       std::memcpy(d, p + off, n);
       off += n;
     };
     char magic[8];
     get(magic, 8);
-    if (std::memcmp(magic, "LGCHKPT1", 8) != 0) throw std::invalid_argument("not a loopgpu checkpoint");
-    uint64_t sw;
-    int32_t ns, nb;
-    int64_t nops;
-    get(&mcs_, 8); get(&therm_, 8); get(&sweeps_min_, 8); get(&sweeps_max_, 8); get(&sw, 8);
-    get(&ns, 4); get(&nb, 4); get(&nops, 8);
-    if (ns != n_sites_) throw std::invalid_argument("checkpoint lattice does not match the parameters");
-    std::vector<int32_t> spins(ns), bo(nb + 1), types(nops);
-    std::vector<double> times(nops);
-    get(spins.data(), 4 * ns); get(bo.data(), 4 * (nb + 1));
-    get(times.data(), 8 * nops); get(types.data(), 4 * nops);
-    int rc = lg_chain_set_config(h_, spins.data(), bo.data(), times.data(), types.data(), sw);
+    if (std::memcmp(magic, "LGWORKR2", 8) != 0) throw std::invalid_argument("not a loopgpu worker checkpoint");
+    uint64_t mcs, smin, smax, therm, cl;
+    get(&mcs, 8); get(&smin, 8); get(&smax, 8); get(&therm, 8); get(&cl, 8);
+    if (cl > len - off) throw std::invalid_argument("truncated checkpoint");
+    int rc = lg_chain_load(h_, p + off, cl);
     if (rc != LG_OK) {
       char b[512];
       lg_last_error(b, sizeof b);
       throw std::invalid_argument(b);
     }
+    off += cl;
+    uint64_t nobs;
+    get(&nobs, 8);
+    std::map<std::string, Observable> o;
+    for (uint64_t i = 0; i < nobs; ++i) {
+      uint64_t nl, nv;
+      get(&nl, 8);
+      if (nl > len - off) throw std::invalid_argument("truncated checkpoint");
+      std::string name((const char*)p + off, nl);
+      off += nl;
+      get(&nv, 8);
+      if (nv > (len - off) / 8 || nv < 7) throw std::invalid_argument("truncated checkpoint");
+      std::vector<double> v(nv);
+      get(v.data(), 8 * nv);
+      o[name].load(v.data());
+    }
+    mcs_ = mcs; sweeps_min_ = smin; sweeps_max_ = smax; therm_ = therm;
+    obs.swap(o);
     buf_.clear();
     next_ = 0;
   }
@@ -462,6 +543,8 @@ class loop_worker {
   }
 
   lg_params lp_{};
+  std::vector<int32_t> sched_steps_;   // annealing schedule (start step, temperature); last = final
+  std::vector<double> sched_temps_;
   bool improved_ = true;
   lg_chain* h_ = nullptr;
   uint64_t mcs_ = 0, therm_ = 0, sweeps_min_ = 0, sweeps_max_ = 0;
@@ -558,7 +641,7 @@ int lg_worker_run(lg_worker* h, uint64_t max_steps, uint64_t* done) {
   if (!h) return werr(LG_ERR_ARG, "null handle");
   uint64_t n = 0;
   try {
-    while (h->w->can_work() && n < max_steps) { h->w->run(h->obs); ++n; }
+    while (h->w->can_work() && n < max_steps) { h->w->run(h->obs, max_steps - n); ++n; }
   } catch (const std::exception& e) {
     if (done) *done = n;
     return werr(LG_ERR_DEVICE, e.what());
@@ -598,7 +681,7 @@ int lg_worker_report(lg_worker* h, char* buf, size_t len, size_t* needed) {
 int lg_worker_save(lg_worker* h, void* buf, size_t* len) {
   if (!h || !len) return werr(LG_ERR_ARG, "null pointer");
   try {
-    std::vector<uint8_t> b = h->w->save();
+    std::vector<uint8_t> b = h->w->save(h->obs);
     if (buf && *len >= b.size()) std::memcpy(buf, b.data(), b.size());
     bool fits = buf && *len >= b.size();
     *len = b.size();
@@ -611,7 +694,7 @@ int lg_worker_save(lg_worker* h, void* buf, size_t* len) {
 int lg_worker_load(lg_worker* h, const void* buf, size_t len) {
   if (!h || !buf) return werr(LG_ERR_ARG, "null pointer");
   try {
-    h->w->load((const uint8_t*)buf, len);
+    h->w->load((const uint8_t*)buf, len, h->obs);
     return LG_OK;
   } catch (const std::invalid_argument& e) {
     return werr(LG_ERR_ARG, e.what());
@@ -629,6 +712,25 @@ int lg_worker_set_chunk(lg_worker* h, uint32_t sweeps) {
 int lg_worker_destroy(lg_worker* h) {
   delete h;
   return LG_OK;
+}
+
+int lg_temperature_schedule(const char* params, int32_t max_n, int32_t* n, int32_t* steps, double* temperatures) {
+  if (!params || !n) return werr(LG_ERR_ARG, "null pointer");
+  try {
+    looper_amd::Parameters p(params);
+    std::vector<int32_t> st;
+    std::vector<double> tp;
+    looper_amd::parse_temperature(p, st, tp);
+    *n = (int32_t)st.size();
+    if ((int32_t)st.size() > max_n) return werr(LG_ERR_ARG, "schedule longer than max_n");
+    for (size_t k = 0; k < st.size(); ++k) {
+      if (steps) steps[k] = st[k];
+      if (temperatures) temperatures[k] = tp[k];
+    }
+    return LG_OK;
+  } catch (const std::exception& e) {
+    return werr(LG_ERR_ARG, e.what());
+  }
 }
 
 }  // extern "C"

@@ -26,7 +26,7 @@ REC_SIZE = len(REC_FIELDS)
 EXPORTED = [
     "lg_last_error", "lg_device_count",
     "lg_chain_create", "lg_chain_sweep", "lg_chain_info", "lg_chain_get_config",
-    "lg_chain_set_config", "lg_chain_destroy", "lg_cluster_config",
+    "lg_chain_set_config", "lg_chain_destroy", "lg_chain_set_schedule", "lg_cluster_config",
     "lg_batch_create", "lg_batch_run", "lg_batch_tune", "lg_batch_set_graph",
     "lg_batch_chain_info", "lg_batch_destroy",
     "lg_chain_get_launch", "lg_chain_set_launch", "lg_batch_get_launch", "lg_batch_set_launch",
@@ -81,6 +81,7 @@ def _load():
     lib.lg_chain_get_config.argtypes = [vp, vp, vp, vp, vp]
     lib.lg_chain_set_config.argtypes = [vp, vp, vp, vp, vp, ctypes.c_uint64]
     lib.lg_chain_destroy.argtypes = [vp]
+    lib.lg_chain_set_schedule.argtypes = [vp, ctypes.c_int32, vp, vp]
     lib.lg_cluster_config.argtypes = [P(LGParams), vp, vp, vp, vp, ctypes.c_uint64, vp, vp, vp, P(ctypes.c_int64)]
     lib.lg_batch_create.argtypes = [P(LGParams), ctypes.c_int32, P(vp)]
     lib.lg_batch_run.argtypes = [vp, ctypes.c_uint32, vp]
@@ -113,6 +114,7 @@ def _load():
     lib.lg_worker_set_chunk.argtypes = [vp, ctypes.c_uint32]
     lib.lg_worker_destroy.argtypes = [vp]
     lib.lg_spin_correlations.argtypes = [ctypes.c_int32, ctypes.c_int32, vp, ctypes.c_int32, vp, vp, vp]
+    lib.lg_temperature_schedule.argtypes = [ctypes.c_char_p, ctypes.c_int32, P(ctypes.c_int32), vp, vp]
     return lib
 
 
@@ -217,6 +219,14 @@ class Chain:
         times = np.ascontiguousarray(times, np.float64)
         types = np.ascontiguousarray(types, np.int32)
         _check(lib.lg_chain_set_config(self.h, _ptr(spins), _ptr(bo), _ptr(times), _ptr(types), int(sweep)))
+
+    def set_schedule(self, steps, temperatures):
+        """annealing schedule (include/loopgpu.h lg_chain_set_schedule)"""
+        st = np.ascontiguousarray(steps, np.int32)
+        tp = np.ascontiguousarray(temperatures, np.float64)
+        if st.shape != tp.shape:
+            raise LoopGPUError(-1, "steps and temperatures differ in length")
+        _check(lib.lg_chain_set_schedule(self.h, len(st), _ptr(st) if len(st) else None, _ptr(tp) if len(tp) else None))
 
     def close(self):
         if self.h:
@@ -431,7 +441,7 @@ EXPORTED += ["lg_server_last_error", "lg_server_start", "lg_server_submit", "lg_
              "lg_server_cancel", "lg_server_stats_get", "lg_server_stop",
              "lg_worker_last_error", "lg_worker_create", "lg_worker_run", "lg_worker_status", "lg_worker_result",
              "lg_worker_report", "lg_worker_save", "lg_worker_load", "lg_worker_set_chunk", "lg_worker_destroy",
-             "lg_spin_correlations"]
+             "lg_spin_correlations", "lg_temperature_schedule"]
 
 
 def spin_correlations(kind, L, spins, origin=-1):
@@ -447,6 +457,17 @@ def spin_correlations(kind, L, spins, origin=-1):
     if rc != 0:
         raise LoopGPUError(rc, "lg_spin_correlations: invalid arguments")
     return u, s, f
+
+
+def temperature_schedule(params):
+    """(steps, temperatures) of the schedule a parameter text defines
+    (include/loopgpu_worker.h lg_temperature_schedule)"""
+    n = ctypes.c_int32()
+    cap = 64
+    st = np.zeros(cap, np.int32)
+    tp = np.zeros(cap, np.float64)
+    _wcheck(lib.lg_temperature_schedule(params.encode(), cap, ctypes.byref(n), st.ctypes.data, tp.ctypes.data))
+    return st[:n.value].copy(), tp[:n.value].copy()
 
 
 def _scheck(rc):

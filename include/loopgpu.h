@@ -104,10 +104,26 @@ int lg_chain_get_config(lg_chain* h, int32_t* spins, int32_t* bond_offsets, doub
 int lg_chain_set_config(lg_chain* h, const int32_t* spins, const int32_t* bond_offsets,
                         const double* times, const int32_t* types, uint64_t sweep);
 int lg_chain_destroy(lg_chain* h);
-/* Checkpoint blob of a chain (the worker's save/load, path_integral.C:111-124:
- * mcs, spins, operators; here also the parameters, whose seed and chain id
- * with the sweep count are the Philox key and counter, so a loaded chain
- * continues the same trajectory).  load rejects a blob of another chain. */
+/* Annealing (reference: looper::temperature, looper/temperature.h:56-113, as
+ * loop_worker::run applies it, path_integral.C:325): n pairs (steps[k],
+ * temperatures[k]) with steps ascending from 1, the last pair giving the
+ * first step at the chain's own final temperature 1/p->beta.  Sweep (Monte
+ * Carlo step) m runs at 1/T(m), T piecewise linear between the pairs
+ * (temperatures[0] up to steps[0], the final temperature from the last step
+ * on; step 0 takes temperatures[0]).  The inverse temperature advances on the
+ * device, so captured sweeps stay valid; records carry each sweep's beta.
+ * Capacity grows to the schedule's largest beta (the configuration is kept).
+ * n = 0 returns to the fixed temperature.  Single chains only. */
+int lg_chain_set_schedule(lg_chain* h, int32_t n, const int32_t* steps, const double* temperatures);
+/* Checkpoint blob of a chain, in the content order of the worker's save/load
+ * (path_integral.C:111-124): "LGCHAIN2", mcs (uint64 sweep count), spins
+ * (uint64 count, int32 each, 0 = up), operators (uint64 count, then in
+ * imaginary-time order {int32 type = offdiagonal | graph << 2, int32
+ * location = bond << 1 | 1, double time}, operator.h:108-109,138-139,
+ * location_impl.h:43), then the parameters (lg_params), whose seed and chain
+ * id with the sweep count are the Philox key and counter, so a loaded chain
+ * continues the same trajectory.  load rejects a blob of another chain.
+ * Little-endian host byte order. */
 int lg_chain_state_size(lg_chain* h, size_t* bytes);
 int lg_chain_save(lg_chain* h, void* buf, size_t len);
 int lg_chain_load(lg_chain* h, const void* buf, size_t len);
@@ -157,10 +173,11 @@ int lg_exchange_create(const lg_params* p, const double* betas, int32_t n, uint3
 int lg_exchange_state(lg_batch* h, int32_t* chain_at, uint64_t* accepted, uint64_t* steps);
 
 /* ---- launch settings ----------------------------------------------------------
- * group: lanes per bond in the diagonal update (4, 8, 16, 32 or 64); blocks:
+ * group: lanes per bond in the diagonal update (1, 2, 4, 8, 16, 32 or 64); blocks:
  * grid of the per-operator and per-node kernels; link: linking kernel (0 one
- * thread per operator; 128, 512 or 1536 a group of threads per site staging
- * that many events in LDS: 16 threads, one wave, four waves); compress: 1 a
+ * thread per operator; 1 one thread per site merging its bond lists; 128, 512
+ * or 1536 a group of threads per site staging that many events in LDS: 16
+ * threads, one wave, four waves); compress: 1 a
  * union-find compression pass before the estimator, 0 (the default) the
  * estimator finds the roots itself; flush: the estimator flushes its LDS
  * cluster table to global memory every 256 * flush segments of its chunk

@@ -15,7 +15,11 @@
  * Parameters are text in the reference's .ip syntax ("KEY = value;" or one
  * per line; expressions over other parameters allowed), keys: LATTICE
  * ("chain lattice" | "square lattice"), L, W, T | TEMPERATURE | BETA |
- * INVERSE_TEMPERATURE, J, Jxy, Jz, MODEL (= spin), local_S (= 1/2), SWEEPS
+ * INVERSE_TEMPERATURE, the annealing schedule T_START_i / T_DURATION_i (or
+ * TEMPERATURE_START_i / TEMPERATURE_DURATION_i, BETA_START_i /
+ * BETA_DURATION_i, INVERSE_TEMPERATURE_START_i / INVERSE_TEMPERATURE_DURATION_i;
+ * temperature.h:56-113, no longer than THERMALIZATION, path_integral.C:
+ * 212-213), J, Jxy, Jz, MODEL (= spin), local_S (= 1/2), SWEEPS
  * (default "[65536:]"), THERMALIZATION (default SWEEPS.min/8), SEED,
  * FORCE_SCATTER, CHAIN_ID, h, DISABLE_IMPROVED_ESTIMATOR,
  * MEASURE[Correlations], MEASURE[Structure Factor], INITIAL_SITE
@@ -49,13 +53,30 @@ int lg_worker_status(lg_worker* h, uint64_t* mcs, int* thermalized, double* prog
 int lg_worker_result(lg_worker* h, const char* name, double* mean, double* error, double* tau);
 /* "[results]" block, one "Name: mean +/- error; tau = t" line per observable */
 int lg_worker_report(lg_worker* h, char* buf, size_t len, size_t* needed);
-/* checkpoint blob: MC counters, spins, operators (types and times) */
+/* checkpoint blob: "LGWORKR2", the Monte Carlo counters (mcs, SWEEPS min
+ * and max, THERMALIZATION; mc_steps::save, montecarlo.h:66), the chain's
+ * blob (lg_chain_save: mcs, spins, operators -- the content of
+ * loop_worker::save, path_integral.C:111-124), then every observable's
+ * binning state (what parapack checkpoints beside the worker), so a worker
+ * loaded from it ends with the results of an uninterrupted run.  Valid at
+ * any step count: lg_worker_run never advances the device beyond the steps
+ * it was asked for.  *len receives the size (buf may be NULL to query it). */
 int lg_worker_save(lg_worker* h, void* buf, size_t* len);
 int lg_worker_load(lg_worker* h, const void* buf, size_t len);
 /* sweeps fetched from the device per batch of run() calls (default 1024;
  * 1 while correlations are measured, which need every sweep's spins) */
 int lg_worker_set_chunk(lg_worker* h, uint32_t sweeps);
 int lg_worker_destroy(lg_worker* h);
+
+/* The temperature schedule a parameter set defines (reference:
+ * looper::temperature::init, temperature.h:43-81, as test/temperature.C
+ * prints it): *n (start step, temperature) pairs, the last one (first step
+ * at the final temperature, final temperature); n = 1 without annealing.
+ * Initial temperature = temperatures[0], annealing steps = steps[n-1] - 1.
+ * The worker runs Monte Carlo step m at 1 / T(m) (lg_chain_set_schedule).
+ * Host computation, no device needed.  LG_ERR_ARG when no temperature is
+ * defined or more than max_n pairs result (*n still receives the count). */
+int lg_temperature_schedule(const char* params, int32_t max_n, int32_t* n, int32_t* steps, double* temperatures);
 
 /* The correlation estimators of one tau = 0 configuration (reference:
  * looper::correlation::normal_estimator::collector::commit,

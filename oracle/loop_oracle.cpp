@@ -425,6 +425,26 @@ struct Alias {
 };
 
 // ---------------------------------------------------------------------------
+// Annealing schedule, looper/temperature.h:93-113: (start step, temperature)
+// pairs with ascending steps, the last one (first step at the final
+// temperature, final temperature).  T(s) is the first temperature up to the
+// first step, linear in between, the final one after the last step; s < 1
+// is taken as 1 (the reference's operator() rejects it).
+// ---------------------------------------------------------------------------
+struct Schedule {
+  std::vector<int> step;
+  std::vector<double> temp;
+  bool on() const { return !step.empty(); }
+  double T(int s) const {
+    if (s < 1) s = 1;
+    const size_t k = std::lower_bound(step.begin(), step.end(), s) - step.begin();
+    if (k == 0) return temp.front();
+    if (k == step.size()) return temp.back();
+    return temp[k - 1] + (s - step[k - 1]) * (temp[k] - temp[k - 1]) / (step[k] - step[k - 1]);
+  }
+};
+
+// ---------------------------------------------------------------------------
 // Looper-mode worker: path_integral.C:380-864, serial, FIELD=SIGN=false,
 // IMPROVE=true, S=1/2 (max_virtual_sites == 1).
 // ---------------------------------------------------------------------------
@@ -452,6 +472,7 @@ struct LooperChain {
   uint64_t mcs = 0;
 
   double h = 0;   // longitudinal field, site term -h S^z
+  Schedule sched; // annealing: beta = 1 / T(mcs) before each sweep (path_integral.C:325)
 
   void init(int kind, int L, double b, double jxy, double jz, double fs, uint32_t seed, double field = 0) {
     lat.init(kind, L);
@@ -474,6 +495,7 @@ struct LooperChain {
 
   void sweep(double* rec) {
     const int N = lat.N;
+    if (sched.on()) beta = 1.0 / sched.T((int)mcs);
     auto rnd = [&]() { return u01(eng); };
     std::swap(ops, ops_p);
     ops.clear();
@@ -835,6 +857,7 @@ struct PhiloxChain {
   Thresholds th;
   double beta = 1, scale = 0, offset_total = 0, W = 0;
   double h = 0;   // longitudinal field
+  Schedule sched; // annealing: beta and candidate scale from T(sweep) before each sweep
   uint32_t key0 = 0, key1 = 0;
   uint32_t sweep = 0;
   std::vector<int> spins;                 // tau = 0 spins
@@ -908,6 +931,10 @@ struct PhiloxChain {
 
   void sweep_once(double* rec) {
     const int N = lat.N, Nb = lat.Nb;
+    if (sched.on()) {
+      beta = 1.0 / sched.T((int)sweep);
+      scale = (W > 0) ? std::ldexp(1.0, 52) / (beta * W) : 0;
+    }
     // (a) diagonal update, per bond
     std::vector<int> nboff(Nb + 1, 0);
     std::vector<double> ntime;
@@ -1226,5 +1253,22 @@ void or_philox_load(void* h, const int* spins, const int* boff, const double* ti
   c->reset_prev_from_ops();
 }
 void or_philox_destroy(void* h) { delete (PhiloxChain*)h; }
+
+// ---- annealing -------------------------------------------------------------
+static Schedule make_schedule(int n, const int* steps, const double* temps) {
+  Schedule s;
+  s.step.assign(steps, steps + n);
+  s.temp.assign(temps, temps + n);
+  return s;
+}
+double or_schedule_temperature(int n, const int* steps, const double* temps, int step) {
+  return make_schedule(n, steps, temps).T(step);
+}
+void or_philox_set_schedule(void* h, int n, const int* steps, const double* temps) {
+  ((PhiloxChain*)h)->sched = make_schedule(n, steps, temps);
+}
+void or_looper_set_schedule(void* h, int n, const int* steps, const double* temps) {
+  ((LooperChain*)h)->sched = make_schedule(n, steps, temps);
+}
 
 }  // extern "C"

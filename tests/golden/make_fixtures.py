@@ -8,6 +8,7 @@ Sources (data files of the reference's tests):
   test/weight.op                XXZ graph weights (standard and FORCE_SCATTER=0.1)
   test/spinmodel_helper.op      chain lattice bond list (ALPS conventions)
   test/virtual_lattice.op       square lattice L=2 vertices, edges, gauge
+  test/temperature.ip/.op       annealing schedules: parameter sets and T(step)
   standalone/loop.op            standalone kernel output (L=8, T=0.2)
   loop.op                       end-to-end run: ED and path-integral QMC, chain L=4, T=0.25;
                                 and the path-integral run with MEASURE[Correlations] and
@@ -177,8 +178,49 @@ def loop_op_correlations():
     raise RuntimeError("correlation block not found in loop.op")
 
 
+def temperature():
+    """parameter sets of test/temperature.ip (globals apply to the following
+    {...} blocks, "#clear;" drops them) with the initial / final temperature,
+    annealing steps and T(step) lines test/temperature.op prints for each"""
+    sets, glob, cur = [], {}, None
+    for raw in read("test/temperature.ip"):
+        line = raw.strip()
+        if line.startswith("#clear"):
+            glob = {}
+            continue
+        if not line:
+            continue
+        for tok in re.split(r"(\{|\})", line):
+            tok = tok.strip()
+            if tok == "{":
+                cur = dict(glob)
+            elif tok == "}":
+                sets.append(cur)
+                cur = None
+            else:
+                for stmt in tok.split(";"):
+                    if "=" in stmt:
+                        k, v = (x.strip() for x in stmt.split("=", 1))
+                        (cur if cur is not None else glob)[k] = v
+    out, blk = [], None
+    for l in read("test/temperature.op"):
+        if l.startswith("initial temperature"):
+            blk = {"initial": float(l.split("=")[1]), "T": []}
+            out.append(blk)
+        elif l.startswith("final temperature"):
+            blk["final"] = float(l.split("=")[1])
+        elif l.startswith("annealing steps"):
+            blk["annealing_steps"] = int(l.split("=")[1])
+        elif l.strip():
+            a, b = l.split()
+            blk["T"].append([int(a), float(b)])
+    assert len(sets) == len(out)
+    return [dict(o, params=p) for p, o in zip(sets, out)]
+
+
 def main():
     fx = {
+        "temperature": temperature(),
         "union_find": union_find(),
         "compatibility": compatibility(),
         "weights": weights(),

@@ -53,6 +53,10 @@ lib.or_philox_state.argtypes = [vp, vp, vp, vp, vp]
 lib.or_philox_roots.argtypes = [vp, vp]
 lib.or_philox_load.argtypes = [vp, vp, vp, vp, vp, U]
 lib.or_philox_destroy.argtypes = [vp]
+lib.or_schedule_temperature.argtypes = [I, vp, vp, I]
+lib.or_schedule_temperature.restype = D
+lib.or_philox_set_schedule.argtypes = [vp, I, vp, vp]
+lib.or_looper_set_schedule.argtypes = [vp, I, vp, vp]
 
 R = lib.or_record_size()
 # record fields (the device library's lg_sweep_record layout)
@@ -95,6 +99,19 @@ def det_log(x):
     return y
 
 
+def _schedule_arrays(steps, temps):
+    st = np.ascontiguousarray(steps, np.int32)
+    tp = np.ascontiguousarray(temps, np.float64)
+    assert st.shape == tp.shape
+    return st, tp
+
+
+def schedule_temperature(steps, temps, step):
+    """T(step) of an annealing schedule (looper/temperature.h:93-113)"""
+    st, tp = _schedule_arrays(steps, temps)
+    return lib.or_schedule_temperature(len(st), _p(st), _p(tp), int(step))
+
+
 def lattice(kind, L):
     N, Nb, dim = I(), I(), I()
     lib.or_lattice_info(kind, L, ctypes.byref(N), ctypes.byref(Nb), ctypes.byref(dim))
@@ -123,6 +140,10 @@ class LooperChain:
         lib.or_looper_sweep(self.h, n, _p(rec))
         return rec
 
+    def set_schedule(self, steps, temps):
+        st, tp = _schedule_arrays(steps, temps)
+        lib.or_looper_set_schedule(self.h, len(st), _p(st), _p(tp))
+
     def __del__(self):
         lib.or_looper_destroy(self.h)
 
@@ -141,6 +162,10 @@ class PhiloxChain:
 
     def set_beta(self, beta):
         lib.or_philox_set_beta(self.h, ctypes.c_double(beta))
+
+    def set_schedule(self, steps, temps):
+        st, tp = _schedule_arrays(steps, temps)
+        lib.or_philox_set_schedule(self.h, len(st), _p(st), _p(tp))
 
     def state(self):
         n = lib.or_philox_num_ops(self.h)

@@ -69,3 +69,30 @@ def test_worker_with_field_matches_exact_diagonalisation(loopgpu):
                      ("Magnetization^2", ref["Magnetization^2"]), ("Susceptibility", ref["Susceptibility"])]:
         m, err, _ = w.result(name)
         assert stats.sigmas(m, err, rk) < 4.0, (name, m, err, rk)
+
+
+def test_worker_disabled_improved_estimator_without_field(loopgpu):
+    """DISABLE_IMPROVED_ESTIMATOR without a field (path_integral.C:217): the
+    normal estimators of the same observables (susceptibility.h:258-339), no
+    generalized or stiffness observables, against exact diagonalisation; the
+    improved run of the same chain measures the same energies"""
+    import ed
+    L, T = 8, 0.4
+    base = "LATTICE = \"chain lattice\"\nL = %d\nT = %g\nSWEEPS = 100000\nTHERMALIZATION = 2000\nSEED = 29" % (L, T)
+    w = loopgpu.Worker(base + "\nDISABLE_IMPROVED_ESTIMATOR = 1")
+    w.run()
+    rep = w.report()
+    assert "Generalized" not in rep and "Stiffness" not in rep
+    ref = ed.diagonalize("chain", L, T, 1.0, 1.0)
+    for name in ("Energy", "Magnetization^2", "Susceptibility", "Staggered Magnetization^2",
+                 "Staggered Susceptibility", "|Staggered Magnetization|", "Specific Heat"):
+        rk = ref[name]
+        m, err, _ = w.result(name)
+        assert stats.sigmas(m, err, rk) < 4.0, (name, m, err, rk)
+    m, _, _ = w.result("Magnetization")
+    assert abs(m) < 0.1
+    # the estimator choice does not change the Markov chain: identical energies
+    v = loopgpu.Worker(base)
+    v.run()
+    assert v.result("Energy") == w.result("Energy")
+    assert v.result("Generalized Susceptibility")[0] > 0

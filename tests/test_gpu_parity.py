@@ -98,7 +98,8 @@ LAUNCH_TRAJ = [
     (O.CHAIN, 6, 1200.0, 1.0, 1.0, 8),     # about 750 per bond: its 1024-time stage (64 KB of LDS)
     (O.CHAIN, 4, 2000.0, 1.0, 0.9, 6),     # about 2500 events per site: beyond the 1536-event staging
 ]
-LAUNCHES = [dict(link=0), dict(link=128), dict(link=512), dict(link=1536), dict(link=512, group=4, blocks=7),
+LAUNCHES = [dict(link=0), dict(link=1), dict(link=128), dict(link=512), dict(link=1536),
+            dict(link=1, group=8, blocks=3), dict(link=1, compress=1), dict(link=512, group=4, blocks=7),
             dict(link=128, group=8, blocks=5),
             dict(link=0, group=16, blocks=64), dict(link=1536, group=32),
             dict(link=512, compress=1), dict(link=0, compress=0, blocks=9), dict(link=1536, compress=1, group=8),
@@ -209,7 +210,7 @@ def test_batch_equals_single_chains(loopgpu):
     ]
     n = 30
     singles = [loopgpu.Chain(p).sweep(n) for p in plist]
-    for link in (None, 0, 128, 512, 1536):  # the default and every linking kernel (sites of several chains per block)
+    for link in (None, 0, 1, 128, 512, 1536):  # the default and every linking kernel (sites of several chains per block)
         b = loopgpu.Batch(plist)
         if link is not None:
             b.set_launch(link=link)
@@ -284,3 +285,50 @@ def test_batch_add_keeps_every_trajectory(loopgpu):
         rs = loopgpu.Chain(p).sweep(20)
         np.testing.assert_array_equal(r2[:, 2 + j, EXACT], rs[:, EXACT])
         np.testing.assert_allclose(r2[:, 2 + j, FLOAT], rs[:, FLOAT], rtol=RTOL, atol=1e-9)
+
+
+def _decode_chain_blob(blob, n_sites):
+    """the chain checkpoint's fields (include/loopgpu.h lg_chain_save)"""
+    import struct
+    assert blob[:8] == b"LGCHAIN2"
+    sweep, ns = struct.unpack_from("<QQ", blob, 8)
+    assert ns == n_sites
+    off = 24
+    spins = np.frombuffer(blob, np.int32, ns, off)
+    off += 4 * ns
+    (nops,) = struct.unpack_from("<Q", blob, off)
+    off += 8
+    rec = np.frombuffer(blob, np.dtype([("type", "<i4"), ("loc", "<i4"), ("time", "<f8")]), nops, off)
+    return sweep, spins, rec
+
+
+def test_kill_and_resume_matches_uninterrupted_oracle(loopgpu):
+    """a chain saved, destroyed and resumed in a new handle continues the
+    oracle's uninterrupted trajectory bit for bit; the blob holds mcs, spins
+    and the operators in imaginary-time order (path_integral.C:111-124)"""
+    for kind, L, beta, jz in ((O.SQUARE, 8, 5.0, 0.8), (O.CHAIN, 32, 12.0, 1.0)):
+        print("kill and resume", kind, L, beta, flush=True)
+        p = loopgpu.params(kind, L, beta, 1.0, jz, seed=12, chain_id=9)
+        c = O.PhiloxChain(kind, L, beta, 1.0, jz, seed=12, chain_id=9)
+        rc = c.sweep(50)
+        a = loopgpu.Chain(p)
+        ra = a.sweep(23)
+        blob = a.save()
+        cfg = a.get_config()
+        a.close()
+        del a
+        sweep, spins, rec = _decode_chain_blob(blob, len(cfg[0]))
+        assert sweep == 23
+        np.testing.assert_array_equal(spins, cfg[0])
+        bond = np.repeat(np.arange(len(cfg[1]) - 1), np.diff(cfg[1]))
+        order = np.lexsort((np.arange(len(bond)), bond, cfg[2]))      # time, then bond, then position
+        np.testing.assert_array_equal(rec["time"], cfg[2][order])
+        np.testing.assert_array_equal(rec["type"], cfg[3][order])
+        np.testing.assert_array_equal(rec["loc"], 2 * bond[order] + 1)
+        assert np.all(np.diff(rec["time"]) >= 0)
+        b = loopgpu.Chain(p)
+        b.load(blob)
+        rb = b.sweep(27)
+        _cmp_records(np.concatenate([ra, rb]), rc, "kill and resume %s" % ((kind, L),))
+        for x, y in zip(b.get_config(), c.state()):
+            np.testing.assert_array_equal(x, y)

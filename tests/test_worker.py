@@ -55,23 +55,31 @@ def test_worker_matches_reference_run_and_ed(loopgpu):
 
 
 @pytest.mark.gpu
-def test_worker_checkpoint_resume_is_exact(loopgpu):
+def test_worker_checkpoint_at_any_step_resumes_exactly(loopgpu):
+    """saved at a step count off the fetch chunk (and again at the end of
+    thermalization), a worker loaded from the blob ends with every result of
+    an uninterrupted run, bit for bit"""
     p = 'LATTICE = "square lattice"\nL = 6\nT = 0.3\nSWEEPS = 3000\nTHERMALIZATION = 200\nSEED = 9'
-    a = loopgpu.Worker(p)
-    a.set_chunk(100)
-    a.run(1000)
-    blob = a.save()
-    a.run()
-    b = loopgpu.Worker(p)
-    b.load(blob)
-    b.set_chunk(100)
-    # b has not seen the first 800 measurements: compare the raw trajectory
-    # through a third worker that ran straight through
     c = loopgpu.Worker(p)
     c.set_chunk(100)
     c.run()
-    ma, ea, _ = a.result("Energy")
-    mc, ec, _ = c.result("Energy")
-    assert ma == mc and ea == ec
-    b.run()
-    assert b.status()["mcs"] == c.status()["mcs"]
+    for first in (1037, 200, 7):
+        print("worker checkpoint at step", first, flush=True)
+        a = loopgpu.Worker(p)
+        a.set_chunk(100)
+        assert a.run(first) == first
+        assert a.status()["mcs"] == first
+        blob = a.save()
+        b = loopgpu.Worker(p)
+        b.load(blob)
+        assert b.status()["mcs"] == first
+        b.set_chunk(64)
+        b.run()
+        assert b.status()["mcs"] == c.status()["mcs"] == 3200
+        for name in ("Energy", "Staggered Magnetization^2", "Susceptibility", "Number of Clusters", "Stiffness",
+                     "Specific Heat", "Binder Ratio of Staggered Magnetization"):
+            assert b.result(name) == c.result(name), (first, name)
+        assert b.report() == c.report()
+    with pytest.raises(loopgpu.LoopGPUError) as e:
+        b.load(blob[:100])
+    assert e.value.code == -1
