@@ -62,6 +62,10 @@ static void xxz_weights(double jxy_in, double jz, double a, double v[4], double*
 int check_params(const lg_params& p) {
   if (p.lattice != LG_LATTICE_CHAIN && p.lattice != LG_LATTICE_SQUARE)
     return set_error(LG_ERR_ARG, "lattice must be chain (0) or square (1)");
+  if (p.two_s < 0 || p.two_s > 2)
+    return set_error(LG_ERR_ARG, "two_s (2S) must be 1 (S=1/2; 0 means 1) or 2 (S=1)");
+  if (p.two_s == 2 && p.lattice != LG_LATTICE_CHAIN)
+    return set_error(LG_ERR_ARG, "S=1 (virtual sites) is built for the chain lattice only");
   if (p.L < 2 || (p.L & 1))
     return set_error(LG_ERR_ARG, "L must be even and >= 2 (bipartite, sign-free lattice)");
   if (p.lattice == LG_LATTICE_SQUARE && (long long)p.L * p.L * 2 > (1ll << 27))
@@ -78,8 +82,10 @@ ChainDev make_chain_dev(const lg_params& p) {
   std::memset(&d, 0, sizeof(d));
   d.kind = p.lattice;
   d.L = p.L;
-  d.N = (p.lattice == LG_LATTICE_CHAIN) ? p.L : p.L * p.L;
-  d.Nb = (p.lattice == LG_LATTICE_CHAIN) ? p.L : 2 * p.L * p.L;
+  // S=1: two virtual sites per real site, four virtual bonds per real bond
+  d.nv = (p.two_s == 2) ? 2 : 1;
+  d.N = d.nv * ((p.lattice == LG_LATTICE_CHAIN) ? p.L : p.L * p.L);
+  d.Nb = d.nv * d.nv * ((p.lattice == LG_LATTICE_CHAIN) ? p.L : 2 * p.L * p.L);
   d.dim = (p.lattice == LG_LATTICE_CHAIN) ? 1 : 2;
   double v[4], off;
   xxz_weights(p.jxy, p.jz, p.force_scatter, v, &off);
@@ -446,9 +452,14 @@ int Batch::set_layout(std::vector<ChainDev>& ds, const std::vector<uint64_t>& sw
   // occupancy.
   if (nb_total > 0) ops_bond /= nb_total;
   c.fill_cap = 1.25 * ops_bond <= 256 ? 256 : (1.25 * ops_bond <= 512 ? 512 : 1024);
+  bool split = false;
+  for (const auto& d : chains) split = split || d.nv > 1;
   if (!tuned) {
     const double per_site = nsites > 0 ? ev / nsites : 0;
     c.link_cap = per_site < 48 ? 0 : (per_site < 96 ? 128 : (per_site <= 400 ? 512 : 1536));
+    // virtual sites: only the per-operator and per-site kernels write their
+    // separate top segments
+    if (split) c.link_cap = per_site < 8 ? 0 : 1;
     // the estimator's LDS table: hot configurations (many one- and
     // two-segment clusters, which overflow the table) flush it every 1024
     // segments (512-chain scan: estimate 0.172 -> 0.166 ms); cold ones once
@@ -600,7 +611,11 @@ DevBatch Batch::dev(int from) const {
   B.sweep = d_sweep; B.rec_acc = rec_acc; B.rec_ring = rec_ring;
   B.ring_pos = ring_pos; B.end_count = end_count; B.ring_size = ring_size;
   B.any_field = 0;
-  for (const auto& d : chains) if (d.h != 0.0) B.any_field = 1;
+  B.any_split = 0;
+  for (const auto& d : chains) {
+    if (d.h != 0.0) B.any_field = 1;
+    if (d.nv > 1) B.any_split = 1;
+  }
   B.nslots = nslots; B.xslot = xslot; B.xbeta = xbeta; B.xscale = xscale; B.xlnr = xlnr;
   B.xacc = xacc; B.xstep = xstep; B.xkey0 = xkey0; B.xkey1 = xkey1;
   B.op_cap = op_cap; B.node_cap = node_cap;
@@ -710,7 +725,10 @@ int Batch::tune(uint32_t sweeps_per_trial) {
       LaunchCfg trial = best;
       if (pass == 0) trial.group = groups[i];
       else if (pass == 1) trial.blocks_op = trial.blocks_node = cus * mults[i];
-      else if (pass == 2) trial.link_cap = links[i];
+      else if (pass == 2) {
+        if (links[i] > 1 && has_virtual_sites()) continue;  // these kernels assume wrapping segments
+        trial.link_cap = links[i];
+      }
       else if (pass == 3) trial.compress = i;
       else trial.est_flush = flushes[i];
       rc = apply_cfg(trial);
@@ -751,6 +769,8 @@ int Batch::set_launch(const lg_launch& l) {
   if (l.link >= 0) {
     if (l.link != 0 && l.link != 1 && l.link != 128 && l.link != 512 && l.link != 1536)
       return set_error(LG_ERR_ARG, "launch link must be 0, 1, 128, 512 or 1536");
+    if (l.link > 1 && has_virtual_sites())
+      return set_error(LG_ERR_ARG, "chains with virtual sites (S=1) take linking kernel 0 or 1");
     c.link_cap = l.link;
   }
   if (l.compress >= 0) {
@@ -820,13 +840,15 @@ void host_site_arrays(const ChainDev& d, const std::vector<int>& spins, const st
   std::vector<int> bond_of(ot.size());
   for (int b = 0; b < d.Nb; ++b) for (int k = bo[b]; k < bo[b + 1]; ++k) bond_of[k] = b;
   soff.assign(N + 1, 0);
+  const bool split = d.nv > 1;  // virtual sites: m events, m + 1 segments, the top one ending at tau = 1
   for (int s = 0; s < N; ++s) {
     std::sort(ev[s].begin(), ev[s].end(), [&](int a, int b) {
       if (ot[a] != ot[b]) return ot[a] < ot[b];
       if (bond_of[a] != bond_of[b]) return bond_of[a] < bond_of[b];
       return a < b;
     });
-    soff[s + 1] = soff[s] + std::max<int>((int)ev[s].size(), 1);
+    const int m = (int)ev[s].size();
+    soff[s + 1] = soff[s] + (m == 0 ? 1 : (split ? m + 1 : m));
   }
   evt.assign(soff[N], EMPTY_TIME);
   segspin.assign(soff[N], 0);
@@ -838,6 +860,10 @@ void host_site_arrays(const ChainDev& d, const std::vector<int>& spins, const st
       evt[soff[s] + j] = ot[k];
       segspin[soff[s] + j] = (uint8_t)c;
       c ^= offd[k];
+    }
+    if (split) {
+      evt[soff[s + 1] - 1] = 1.0;
+      segspin[soff[s + 1] - 1] = (uint8_t)c;
     }
   }
 }
@@ -1014,6 +1040,30 @@ int Batch::load_configs(const std::vector<HostConfig>& cfg) {
   old_ext_bonds = nbonds;
   old_ext_nodes = node_base;
   return sync_schedule();
+}
+
+int Batch::reload(const std::vector<lg_params>& ps, const std::vector<HostConfig>& cfg) {
+  if (ps.size() != cfg.size()) return set_error(LG_ERR_ARG, "reload: one configuration per chain");
+  std::vector<ChainDev> ds;
+  std::vector<uint64_t> sw;
+  for (size_t c = 0; c < ps.size(); ++c) {
+    int rc = check_params(ps[c]);
+    if (rc) return rc;
+    ds.push_back(make_chain_dev(ps[c]));
+    sw.push_back(cfg[c].sweep);
+  }
+  LG_HIP(hipStreamSynchronize(stream));
+  params = ps;
+  int rc = set_layout(ds, sw);
+  if (rc) return rc;
+  // the data is uploaded at the new bases: nothing reads the old ones
+  for (auto& d : chains) { d.old_site_base = d.site_base; d.old_bond_base = d.bond_base; }
+  if (!chains.empty())
+    LG_HIP(hipMemcpy(d_chains, chains.data(), chains.size() * sizeof(ChainDev), hipMemcpyHostToDevice));
+  if ((rc = load_configs(cfg))) return rc;
+  swept_since_layout = true;
+  invalidate_graphs();
+  return LG_OK;
 }
 
 void Batch::adopt(Batch& o) {

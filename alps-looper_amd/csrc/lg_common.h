@@ -15,7 +15,9 @@
 //    (operator legs touching s, ordered by (tau, bond, rank)), node soff[s]+j
 //    is the segment just below event j; node soff[s] is the wrapping segment
 //    that also holds the tau = 0 boundary.  A site without events owns one
-//    node.  Roots are the minimum node index of each cluster.
+//    node.  A virtual site (S=1) does not wrap: node soff[s] starts at tau = 0
+//    and node soff[s]+m is its top segment up to tau = 1.  Roots are the
+//    minimum node index of each cluster.
 #pragma once
 
 #include <stdint.h>
@@ -32,18 +34,22 @@ constexpr uint64_t TGRID = 1ull << 52;
 constexpr double TGRID_INV = 1.0 / 4503599627370496.0;
 constexpr double EMPTY_TIME = 2.0;  // ev_time of the single node of an event-free site
 
-enum : uint32_t { STAGE_DIAG = 1, STAGE_REGRAPH = 2, STAGE_FLIP = 3, STAGE_EXCHANGE = 4 };
+enum : uint32_t { STAGE_DIAG = 1, STAGE_REGRAPH = 2, STAGE_FLIP = 3, STAGE_EXCHANGE = 4, STAGE_BOUNDARY = 5 };
 
 // ev_flags bits
-// EV_FIRST: lowest segment of its site (wraps through tau = 1); EV_GAUGE: the
-// site's gauge factor is -1
-enum : uint8_t { EV_OFFDIAG = 1, EV_SRC = 2, EV_DIR1 = 4, EV_EMPTY = 8, EV_FIRST = 16, EV_GAUGE = 32 };
+// EV_FIRST: lowest segment of its site (wraps through tau = 1, unless
+// EV_NOWRAP); EV_GAUGE: the site's gauge factor is -1; EV_NOWRAP: the site's
+// segments do not wrap (virtual sites: the lowest segment starts at tau = 0
+// and the site owns a separate top segment ending at tau = 1, whose time
+// entry is 1.0)
+enum : uint8_t { EV_OFFDIAG = 1, EV_SRC = 2, EV_DIR1 = 4, EV_EMPTY = 8, EV_FIRST = 16, EV_GAUGE = 32, EV_NOWRAP = 64 };
 
 // Per-chain descriptor (device resident, rebuilt by the host when the set of
 // chains in a batch changes).
 struct ChainDev {
   int kind;            // 0 chain, 1 square
-  int L, N, Nb, dim;
+  int L, N, Nb, dim;   // N, Nb: (virtual) sites and bonds of the lattice the kernels sweep
+  int nv;              // virtual sites per real site (2S; 1 for S=1/2, 2 for S=1 on the chain)
   int site_base, bond_base;          // layout of the configuration being built
   int old_site_base, old_bond_base;  // layout of the previous configuration
   int cand_cap;                      // candidate slots per bond
@@ -218,8 +224,16 @@ LG_HD int choose_graph(uint32_t x, const ChainDev& c) {
 // chain: bond b = (b, b+1 mod L); site s has bonds {s (source), s-1 (target)}
 // square: site s = x*L + y; bond 2s = (s, s+y^) [dir 1], 2s+1 = (s, s+x^) [dir 0];
 //         site s has bonds {2s, 2s+1 (source), 2(s-y^), 2(s-x^)+1 (target)}
+// S=1 chain (two virtual sites per real site, path_integral.C:589-615):
+//         virtual site v = 2s + a; real bond i = (i, i+1) carries the four
+//         virtual bonds 4i + 2a + c = (2i + a, 2(i+1) + c); virtual site v has
+//         bonds {4s+2a, 4s+2a+1 (source), 4(s-1)+a, 4(s-1)+2+a (target)}
 LG_HD void bond_sites(const ChainDev& c, int b, int& s0, int& s1) {
-  if (c.kind == 0) {
+  if (c.kind == 0 && c.nv == 2) {
+    const int i = b >> 2, a = (b >> 1) & 1, d = b & 1;
+    s0 = 2 * i + a;
+    s1 = 2 * ((i + 1 == c.L) ? 0 : i + 1) + d;
+  } else if (c.kind == 0) {
     s0 = b;
     s1 = (b + 1 == c.L) ? 0 : b + 1;
   } else {
@@ -232,28 +246,43 @@ LG_HD void bond_sites(const ChainDev& c, int b, int& s0, int& s1) {
 
 LG_HD int bond_dir(const ChainDev& c, int b) { return (c.kind == 0) ? 0 : ((b & 1) ? 0 : 1); }
 
-// bonds incident to site s; returns their count (2 or 4)
+// bonds incident to site s; returns their count (2 or 4).  The four entries
+// are chosen into scalars and stored once: stores of different entries on
+// different paths get merged into an indexed store, which puts the caller's
+// array in scratch memory.
 LG_HD int site_bonds(const ChainDev& c, int s, int* bl) {
-  if (c.kind == 0) {
-    bl[0] = s;
-    bl[1] = (s == 0) ? c.L - 1 : s - 1;
-    // unused; written on both paths so that the list stays in registers
-    // (stores to different entries per path are merged into an indexed
-    // store, which puts the array in scratch memory)
-    bl[2] = bl[3] = -1;
-    return 2;
+  int b0, b1, b2, b3, n;
+  if (c.kind == 0 && c.nv == 2) {
+    const int r = s >> 1, a = s & 1, rm = (r == 0) ? c.L - 1 : r - 1;
+    b0 = 4 * r + 2 * a;
+    b1 = b0 + 1;
+    b2 = 4 * rm + a;
+    b3 = b2 + 2;
+    n = 4;
+  } else if (c.kind == 0) {
+    b0 = s;
+    b1 = (s == 0) ? c.L - 1 : s - 1;
+    b2 = b3 = -1;  // unused
+    n = 2;
+  } else {
+    const int x = s / c.L, y = s - x * c.L;
+    const int ym = (y == 0) ? c.L - 1 : y - 1;
+    const int xm = (x == 0) ? c.L - 1 : x - 1;
+    b0 = 2 * s;
+    b1 = 2 * s + 1;
+    b2 = 2 * (x * c.L + ym);
+    b3 = 2 * (xm * c.L + y) + 1;
+    n = 4;
   }
-  int x = s / c.L, y = s - x * c.L;
-  int ym = (y == 0) ? c.L - 1 : y - 1;
-  int xm = (x == 0) ? c.L - 1 : x - 1;
-  bl[0] = 2 * s;
-  bl[1] = 2 * s + 1;
-  bl[2] = 2 * (x * c.L + ym);
-  bl[3] = 2 * (xm * c.L + y) + 1;
-  return 4;
+  bl[0] = b0;
+  bl[1] = b1;
+  bl[2] = b2;
+  bl[3] = b3;
+  return n;
 }
 
 LG_HD int gauge(const ChainDev& c, int s) {
+  if (c.kind == 0 && c.nv == 2) return ((s >> 1) & 1) ? -1 : 1;   // the real site's
   if (c.kind == 0) return (s & 1) ? -1 : 1;
   int x = s / c.L, y = s - x * c.L;
   return ((x + y) & 1) ? -1 : 1;

@@ -70,9 +70,18 @@ class Batch {
     uint64_t sweep = 0;
   };
   int load_configs(const std::vector<HostConfig>& cfg);
+  // Replace the set of chains by `ps` with configurations `cfg` (chains that
+  // start fresh have all spins up and no operators), uploaded compactly to
+  // the batch's existing device memory: the path for chains that resume from
+  // a saved configuration (serving preemption).
+  int reload(const std::vector<lg_params>& ps, const std::vector<HostConfig>& cfg);
   // take over o's device resources and settings (o is left empty)
   void adopt(Batch& o);
   bool is_exchange() const { return nslots > 0; }
+  bool has_virtual_sites() const {
+    for (const auto& d : chains) if (d.nv > 1) return true;
+    return false;
+  }
   // annealing schedule of a single-chain batch: n (start step, temperature)
   // pairs (looper/temperature.h); n == 0 removes it.  Rebuilds the device
   // memory when the schedule's largest beta needs more capacity.

@@ -66,6 +66,7 @@ struct DevBatch {
   int ring_size;
   // replica exchange between neighbouring temperature slots (nslots 0: off)
   int any_field;               // some chain has a longitudinal field
+  int any_split;               // some chain has virtual sites (S=1): segments without wrap, boundary permutation
   int nslots;
   int* xslot;                  // [nslots] chain holding each temperature
   const double* xbeta;         // [nslots] inverse temperatures

@@ -401,7 +401,8 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan(DevBatch B) {
       } else {
         int m = site_events(B, base + k);
         empty[k] = (m == 0);
-        v = empty[k] ? 1 : m;
+        // a virtual site's segments do not wrap: m events, m + 1 segments
+        v = empty[k] ? 1 : (B.chains[B.site_chain[base + k]].nv > 1 ? m + 1 : m);
       }
     }
     vals[k] = v;
@@ -459,7 +460,8 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan(DevBatch B) {
           int node = off;
           B.evt_new[node] = EMPTY_TIME;
           B.segspin_new[node] = B.spins_old[P.old_site_base + (i - P.site_base)];
-          B.evflags[node] = (uint8_t)(EV_EMPTY | EV_FIRST | (gauge(P, i - P.site_base) < 0 ? EV_GAUGE : 0));
+          B.evflags[node] = (uint8_t)(EV_EMPTY | EV_FIRST | (gauge(P, i - P.site_base) < 0 ? EV_GAUGE : 0) |
+                                      (P.nv > 1 ? EV_NOWRAP : 0));
           B.parent[node] = node;
           B.node_site[node] = i;
         }
@@ -882,13 +884,23 @@ __global__ void __launch_bounds__(256) k_link_ops(DevBatch B) {
       const int gs = P.site_base + s;
       const int n0 = B.soff_new[gs], n1 = B.soff_new[gs + 1];
       const int node = n0 + rank;
-      const int up = (node + 1 == n1) ? n0 : node + 1;  // segment above the event
+      const bool split = P.nv > 1;
+      const int up = (node + 1 == n1) ? n0 : node + 1;  // segment above the event (split: never wraps)
+      const uint8_t gflag = gauge(P, s) < 0 ? EV_GAUGE : 0;
+      const uint8_t sp = B.spins_old[P.old_site_base + s] ^ (uint8_t)(par & 1);
       B.evt_new[node] = tau;
-      B.segspin_new[node] = B.spins_old[P.old_site_base + s] ^ (uint8_t)(par & 1);
-      B.evflags[node] = (uint8_t)((info & 1) | (side ? 0 : EV_SRC) | dflag | (rank == 0 ? EV_FIRST : 0) |
-                                  (gauge(P, s) < 0 ? EV_GAUGE : 0));
+      B.segspin_new[node] = sp;
+      B.evflags[node] = (uint8_t)((info & 1) | (side ? 0 : EV_SRC) | dflag | (rank == 0 ? EV_FIRST : 0) | gflag |
+                                  (split && rank == 0 ? EV_NOWRAP : 0));
       B.parent[node] = node;
       if (B.nchains > 1 || rank == 0) B.node_site[node] = gs;
+      if (split && up == n1 - 1) {  // the last event: the site's top segment, up to tau = 1
+        B.evt_new[up] = 1.0;
+        B.segspin_new[up] = sp ^ (uint8_t)(info & 1);
+        B.evflags[up] = gflag;
+        B.parent[up] = up;
+        if (B.nchains > 1) B.node_site[up] = gs;
+      }
       if (side) { B.l1[k] = node; B.u1[k] = up; } else { B.l0[k] = node; B.u0[k] = up; }
     }
   }
@@ -902,76 +914,88 @@ __global__ void __launch_bounds__(256) k_link_ops(DevBatch B) {
 // thread's work is serial in its site's events.  The list heads are kept in
 // named registers and selected with constant indices (a dynamically indexed
 // register array would live in scratch memory).
+__device__ __forceinline__ void link_site_events(const DevBatch& B, int gs) {
+  const ChainDev& P = B.chains[B.site_chain[gs]];
+  const int s = gs - P.site_base;
+  int bl[4];
+  const int nbd = site_bonds(P, s, bl);
+  int p0, p1, p2 = 0, p3 = 0, e0, e1, e2 = 0, e3 = 0;
+  {
+    const int b0 = P.bond_base + bl[0], b1 = P.bond_base + bl[1];
+    p0 = B.bo_new[b0]; e0 = B.bo_new[b0 + 1];
+    p1 = B.bo_new[b1]; e1 = B.bo_new[b1 + 1];
+    if (nbd == 4) {
+      const int b2 = P.bond_base + bl[2], b3 = P.bond_base + bl[3];
+      p2 = B.bo_new[b2]; e2 = B.bo_new[b2 + 1];
+      p3 = B.bo_new[b3]; e3 = B.bo_new[b3 + 1];
+    }
+  }
+  if (p0 == e0 && p1 == e1 && p2 == e2 && p3 == e3) return;  // event-free: written by k_scan
+  // side (0 source, 1 target) and direction flags of each list's bond
+  uint8_t f0, f1, f2 = 0, f3 = 0;
+  {
+    int a, b;
+    bond_sites(P, bl[0], a, b);
+    f0 = (uint8_t)((a == s ? EV_SRC : 0) | (bond_dir(P, bl[0]) ? EV_DIR1 : 0));
+    bond_sites(P, bl[1], a, b);
+    f1 = (uint8_t)((a == s ? EV_SRC : 0) | (bond_dir(P, bl[1]) ? EV_DIR1 : 0));
+    if (nbd == 4) {
+      bond_sites(P, bl[2], a, b);
+      f2 = (uint8_t)((a == s ? EV_SRC : 0) | (bond_dir(P, bl[2]) ? EV_DIR1 : 0));
+      bond_sites(P, bl[3], a, b);
+      f3 = (uint8_t)((a == s ? EV_SRC : 0) | (bond_dir(P, bl[3]) ? EV_DIR1 : 0));
+    }
+  }
+  double t0 = p0 < e0 ? B.ot_new[p0] : 3.0, t1 = p1 < e1 ? B.ot_new[p1] : 3.0;
+  double t2 = p2 < e2 ? B.ot_new[p2] : 3.0, t3 = p3 < e3 ? B.ot_new[p3] : 3.0;
+  const int n0 = B.soff_new[gs], n1 = B.soff_new[gs + 1];
+  const uint8_t gflag = gauge(P, s) < 0 ? EV_GAUGE : 0;
+  const bool all_sites = B.nchains > 1;
+  const bool split = P.nv > 1;     // virtual site: m + 1 segments, no wrap
+  const int ne = split ? n1 - 1 : n1;
+  uint8_t spin = B.spins_old[P.old_site_base + s];
+  for (int node = n0; node < ne; ++node) {
+    // the head with the least (time, bond)
+    int i = 0;
+    double t = t0;
+    int bi = bl[0];
+    if (t1 < t || (t1 == t && bl[1] < bi)) { i = 1; t = t1; bi = bl[1]; }
+    if (nbd == 4) {
+      if (t2 < t || (t2 == t && bl[2] < bi)) { i = 2; t = t2; bi = bl[2]; }
+      if (t3 < t || (t3 == t && bl[3] < bi)) { i = 3; t = t3; bi = bl[3]; }
+    }
+    const int k = i == 0 ? p0 : i == 1 ? p1 : i == 2 ? p2 : p3;
+    const uint8_t fl = i == 0 ? f0 : i == 1 ? f1 : i == 2 ? f2 : f3;
+    const int off = B.oi_new[k] & 1;
+    const int up = (node + 1 == n1) ? n0 : node + 1;  // segment above the event (split: never wraps)
+    B.evt_new[node] = t;
+    B.segspin_new[node] = spin;
+    B.evflags[node] = (uint8_t)(off | fl | (node == n0 ? EV_FIRST | (split ? EV_NOWRAP : 0) : 0) | gflag);
+    B.parent[node] = node;
+    if (all_sites || node == n0) B.node_site[node] = gs;
+    if (fl & EV_SRC) { B.l0[k] = node; B.u0[k] = up; } else { B.l1[k] = node; B.u1[k] = up; }
+    spin ^= (uint8_t)off;
+    // advance the chosen list
+    if (i == 0) { ++p0; t0 = p0 < e0 ? B.ot_new[p0] : 3.0; }
+    else if (i == 1) { ++p1; t1 = p1 < e1 ? B.ot_new[p1] : 3.0; }
+    else if (i == 2) { ++p2; t2 = p2 < e2 ? B.ot_new[p2] : 3.0; }
+    else { ++p3; t3 = p3 < e3 ? B.ot_new[p3] : 3.0; }
+  }
+  if (split) {  // the top segment, from the last event up to tau = 1
+    B.evt_new[ne] = 1.0;
+    B.segspin_new[ne] = spin;
+    B.evflags[ne] = gflag;
+    B.parent[ne] = ne;
+    if (all_sites) B.node_site[ne] = gs;
+  }
+}
+
+// one thread per site, grid-stride over XCD-contiguous shares
 __global__ void __launch_bounds__(256) k_link_site(DevBatch B) {
   if (overflowed(B)) return;
   int slo, shi;
   xcd_share(B.nsites, slo, shi);
-  for (int gs = slo + threadIdx.x; gs < shi; gs += blockDim.x) {
-    const ChainDev& P = B.chains[B.site_chain[gs]];
-    const int s = gs - P.site_base;
-    int bl[4];
-    const int nbd = site_bonds(P, s, bl);
-    int p0, p1, p2 = 0, p3 = 0, e0, e1, e2 = 0, e3 = 0;
-    {
-      const int b0 = P.bond_base + bl[0], b1 = P.bond_base + bl[1];
-      p0 = B.bo_new[b0]; e0 = B.bo_new[b0 + 1];
-      p1 = B.bo_new[b1]; e1 = B.bo_new[b1 + 1];
-      if (nbd == 4) {
-        const int b2 = P.bond_base + bl[2], b3 = P.bond_base + bl[3];
-        p2 = B.bo_new[b2]; e2 = B.bo_new[b2 + 1];
-        p3 = B.bo_new[b3]; e3 = B.bo_new[b3 + 1];
-      }
-    }
-    if (p0 == e0 && p1 == e1 && p2 == e2 && p3 == e3) continue;  // event-free: written by k_scan
-    // side (0 source, 1 target) and direction flags of each list's bond
-    uint8_t f0, f1, f2 = 0, f3 = 0;
-    {
-      int a, b;
-      bond_sites(P, bl[0], a, b);
-      f0 = (uint8_t)((a == s ? EV_SRC : 0) | (bond_dir(P, bl[0]) ? EV_DIR1 : 0));
-      bond_sites(P, bl[1], a, b);
-      f1 = (uint8_t)((a == s ? EV_SRC : 0) | (bond_dir(P, bl[1]) ? EV_DIR1 : 0));
-      if (nbd == 4) {
-        bond_sites(P, bl[2], a, b);
-        f2 = (uint8_t)((a == s ? EV_SRC : 0) | (bond_dir(P, bl[2]) ? EV_DIR1 : 0));
-        bond_sites(P, bl[3], a, b);
-        f3 = (uint8_t)((a == s ? EV_SRC : 0) | (bond_dir(P, bl[3]) ? EV_DIR1 : 0));
-      }
-    }
-    double t0 = p0 < e0 ? B.ot_new[p0] : 3.0, t1 = p1 < e1 ? B.ot_new[p1] : 3.0;
-    double t2 = p2 < e2 ? B.ot_new[p2] : 3.0, t3 = p3 < e3 ? B.ot_new[p3] : 3.0;
-    const int n0 = B.soff_new[gs], n1 = B.soff_new[gs + 1];
-    const uint8_t gflag = gauge(P, s) < 0 ? EV_GAUGE : 0;
-    const bool all_sites = B.nchains > 1;
-    uint8_t spin = B.spins_old[P.old_site_base + s];
-    for (int node = n0; node < n1; ++node) {
-      // the head with the least (time, bond)
-      int i = 0;
-      double t = t0;
-      int bi = bl[0];
-      if (t1 < t || (t1 == t && bl[1] < bi)) { i = 1; t = t1; bi = bl[1]; }
-      if (nbd == 4) {
-        if (t2 < t || (t2 == t && bl[2] < bi)) { i = 2; t = t2; bi = bl[2]; }
-        if (t3 < t || (t3 == t && bl[3] < bi)) { i = 3; t = t3; bi = bl[3]; }
-      }
-      const int k = i == 0 ? p0 : i == 1 ? p1 : i == 2 ? p2 : p3;
-      const uint8_t fl = i == 0 ? f0 : i == 1 ? f1 : i == 2 ? f2 : f3;
-      const int off = B.oi_new[k] & 1;
-      const int up = (node + 1 == n1) ? n0 : node + 1;  // segment above the event
-      B.evt_new[node] = t;
-      B.segspin_new[node] = spin;
-      B.evflags[node] = (uint8_t)(off | fl | (node == n0 ? EV_FIRST : 0) | gflag);
-      B.parent[node] = node;
-      if (all_sites || node == n0) B.node_site[node] = gs;
-      if (fl & EV_SRC) { B.l0[k] = node; B.u0[k] = up; } else { B.l1[k] = node; B.u1[k] = up; }
-      spin ^= (uint8_t)off;
-      // advance the chosen list
-      if (i == 0) { ++p0; t0 = p0 < e0 ? B.ot_new[p0] : 3.0; }
-      else if (i == 1) { ++p1; t1 = p1 < e1 ? B.ot_new[p1] : 3.0; }
-      else if (i == 2) { ++p2; t2 = p2 < e2 ? B.ot_new[p2] : 3.0; }
-      else { ++p3; t3 = p3 < e3 ? B.ot_new[p3] : 3.0; }
-    }
-  }
+  for (int gs = slo + threadIdx.x; gs < shi; gs += blockDim.x) link_site_events(B, gs);
 }
 
 // ---------------------------------------------------------------------------
@@ -1027,6 +1051,36 @@ __global__ void __launch_bounds__(256) k_uf_union(DevBatch B) {
       uf_unite(B.parent, l0, u1);
       uf_unite(B.parent, l1, u0);
     }
+  }
+}
+
+// tau boundary of the virtual sites of an S=1 chain (path_integral.C:589-615,
+// the S > 1/2 branch of the symmetrization): per real site, the tau = 0 ends
+// (lowest segments) of its two virtual sites are joined to their tau = 1 ends
+// (top segments) by a permutation that maps equal spins onto each other,
+// uniform among those that do (looper::partitioned_random_shuffle,
+// looper/permutation.h:88-112).  With equal spins at tau = 0 it is the
+// identity or the exchange with probability 1/2 each, decided by bit 0 of
+// Philox(sweep, STAGE_BOUNDARY, real site, 0); otherwise the spins allow one
+// permutation.  (The spins at tau = 1 are those at tau = 0 in some order:
+// every update keeps each real site's magnetization periodic.)
+__global__ void __launch_bounds__(256) k_boundary(DevBatch B) {
+  if (overflowed(B)) return;
+  const int stride = gridDim.x * blockDim.x;
+  for (int gs = blockIdx.x * blockDim.x + threadIdx.x; gs < B.nsites; gs += stride) {
+    const int c = B.site_chain[gs];
+    const ChainDev& P = B.chains[c];
+    if (P.nv != 2) continue;
+    const int v = gs - P.site_base;
+    if (v & 1) continue;  // one thread per real site: its virtual site 0
+    const int a0 = B.soff_new[gs], a1 = B.soff_new[gs + 1], a2 = B.soff_new[gs + 2];
+    const int bot0 = a0, top0 = a1 - 1, bot1 = a1, top1 = a2 - 1;
+    const int s0 = B.segspin_new[bot0], s1 = B.segspin_new[bot1], e0 = B.segspin_new[top0];
+    const bool swap = (s0 == s1)
+        ? (philox(B.sweep[c], STAGE_BOUNDARY, (uint32_t)(v >> 1), 0u, P.key0, P.key1).x & 1u) != 0
+        : (s0 != e0);
+    uf_unite(B.parent, bot0, swap ? top1 : top0);
+    uf_unite(B.parent, bot1, swap ? top0 : top1);
   }
 }
 
@@ -1266,7 +1320,13 @@ __global__ void __launch_bounds__(EST_THREADS) k_estimate(DevBatch B, int flush_
           for (int u = 0; u < EST_U; ++u) {
             const double t = B.evt_new[pv[u]];
             const uint8_t fo = B.evflags[pv[u]];
-            if ((fl[u] & (EV_FIRST | EV_EMPTY)) == EV_FIRST) { tlo[u] = t; flo[u] = fo; }
+            if ((fl[u] & (EV_FIRST | EV_EMPTY)) == EV_FIRST) {
+              // wrapping through tau = 1: the segment below is the site's last;
+              // a virtual site's lowest segment starts at tau = 0, no event below
+              const bool nowrap = (fl[u] & EV_NOWRAP) != 0;
+              tlo[u] = nowrap ? 0.0 : t;
+              flo[u] = nowrap ? (uint8_t)0 : fo;
+            }
           }
         }
 #pragma unroll
@@ -1279,7 +1339,7 @@ __global__ void __launch_bounds__(EST_THREADS) k_estimate(DevBatch B, int flush_
           int dw0 = 0, dw1 = 0;
           double len = 1.0;
           if (!(fl[u] & EV_EMPTY)) {
-            len = first ? ((thi[u] + 1.0) - tlo[u]) : (thi[u] - tlo[u]);
+            len = (first && !(fl[u] & EV_NOWRAP)) ? ((thi[u] + 1.0) - tlo[u]) : (thi[u] - tlo[u]);
             // this segment is the lower leg of the event above it and the upper
             // leg of the event below it (stiffness.h:94-105, source legs only)
             if (fl[u] & EV_SRC) { if (fl[u] & EV_DIR1) dw1 += sig; else dw0 += sig; }
@@ -1771,6 +1831,10 @@ void launch_cluster_stage(const DevBatch& B, const LaunchCfg& L, hipStream_t st,
     hipLaunchKernelGGL((k_link<1536, 8, 256>), dim3(blocks), dim3(512), 0, st, B);
   }
   mark(ev, 1, st);
+  if (B.any_split) {
+    const int blocks = max(1, min(L.blocks_op, (B.nsites + 255) / 256));
+    hipLaunchKernelGGL(k_boundary, dim3(blocks), dim3(256), 0, st, B);
+  }
   hipLaunchKernelGGL(k_uf_union, dim3(L.blocks_op), dim3(256), 0, st, B);
   mark(ev, 2, st);
   if (L.compress) {

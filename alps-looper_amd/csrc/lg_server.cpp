@@ -33,9 +33,17 @@ struct Job {
   lg_job job{};
   int status = LG_JOB_QUEUED;
   clock_type::time_point t_submit, t_admit, t_done;
+  bool admitted_once = false;
   uint64_t done = 0;
   lg::Footprint fp;
   looper_amd::Observable obs[LG_JOB_NOBS];
+  // preemption: requested for a running job; its configuration (spins,
+  // operators, sweep count = Philox counter) is saved at the next chunk
+  // boundary and loaded back when it is admitted again
+  bool preempt = false;
+  bool saved = false;
+  lg::Batch::HostConfig cfg;
+  uint64_t preemptions = 0;
 };
 
 void add_fp(lg::Footprint& a, const lg::Footprint& b, int sgn) {
@@ -60,7 +68,7 @@ struct lg_server {
   std::thread th;
   bool stop = false;
   std::string fatal;
-  uint64_t next_id = 1, submitted = 0, total_updates = 0, launches = 0;
+  uint64_t next_id = 1, submitted = 0, total_updates = 0, launches = 0, preempted = 0;
   double busy_s = 0;
   std::vector<double> latencies;
 
@@ -108,14 +116,30 @@ struct lg_server {
     while (true) {
       std::vector<int> keep;
       std::vector<lg_params> add;
-      std::vector<uint64_t> next;
-      bool changed = false;
+      std::vector<uint64_t> next, added;
+      bool changed = false, resumed = false;
       {
         std::unique_lock<std::mutex> lk(mu);
         cv_work.wait(lk, [&] { return stop || !queue.empty() || !active.empty(); });
         if (stop) return;
         for (size_t i = 0; i < active.size(); ++i) {
           Job& j = jobs[active[i]];
+          if (j.status == LG_JOB_RUNNING && j.preempt) {
+            // save the configuration the job has reached and queue it again
+            // (behind the jobs already waiting); its measurements stay
+            int rc = batch.chain_config((int)i, j.cfg.spins, j.cfg.bo, j.cfg.times, j.cfg.types);
+            if (rc) {
+              fail_all("preempt: " + lg::g_last_error_copy());
+              return;
+            }
+            j.cfg.sweep = batch.host_sweep[i];
+            j.saved = true;
+            j.preempt = false;
+            j.status = LG_JOB_QUEUED;
+            ++j.preemptions;
+            ++preempted;
+            queue.push_back(j.id);
+          }
           if (j.status == LG_JOB_RUNNING) {
             keep.push_back((int)i);
             next.push_back(active[i]);
@@ -132,17 +156,56 @@ struct lg_server {
             add_fp(used, j.fp, +1);
             ++admitted_;
             j.status = LG_JOB_RUNNING;
-            j.t_admit = clock_type::now();
+            if (!j.admitted_once) j.t_admit = clock_type::now();
+            j.admitted_once = true;
             add.push_back(j.job.params);
             next.push_back(j.id);
+            added.push_back(j.id);
+            resumed = resumed || j.saved;
             it = queue.erase(it);
             changed = true;
           } else {
             ++it;
           }
         }
+        if (resumed) {
+          // a job resumes from its saved configuration: every resident chain's
+          // configuration goes through the host and the batch is reloaded
+          // compactly (a rare event; admissions of fresh jobs stay on the
+          // device)
+          std::vector<lg::Batch::HostConfig> cfgs(next.size());
+          std::vector<lg_params> ps;
+          for (size_t k = 0; k < keep.size(); ++k) {
+            int rc = batch.chain_config(keep[k], cfgs[k].spins, cfgs[k].bo, cfgs[k].times, cfgs[k].types);
+            if (rc) {
+              fail_all("resume: " + lg::g_last_error_copy());
+              return;
+            }
+            cfgs[k].sweep = batch.host_sweep[keep[k]];
+            ps.push_back(batch.params[keep[k]]);
+          }
+          for (size_t a = 0; a < added.size(); ++a) {
+            Job& j = jobs[added[a]];
+            lg::Batch::HostConfig& h = cfgs[keep.size() + a];
+            if (j.saved) {
+              h = std::move(j.cfg);
+              j.cfg = lg::Batch::HostConfig();
+              j.saved = false;
+            } else {
+              const lg::ChainDev d = lg::make_chain_dev(j.job.params);
+              h.spins.assign(d.N, 0);  // all up, no operators (path_integral.C:225)
+              h.bo.assign(d.Nb + 1, 0);
+            }
+            ps.push_back(j.job.params);
+          }
+          int rc = batch.reload(ps, cfgs);
+          if (rc) {
+            fail_all("resume: " + lg::g_last_error_copy());
+            return;
+          }
+        }
       }
-      if (changed) {
+      if (changed && !resumed) {
         int rc = batch.relayout(keep, add);
         if (rc) {
           std::lock_guard<std::mutex> lk(mu);
@@ -300,6 +363,21 @@ int lg_server_cancel(lg_server* s, uint64_t id) {
   return LG_OK;
 }
 
+int lg_server_preempt(lg_server* s, uint64_t id) {
+  if (!s) return serr(LG_ERR_ARG, "null pointer");
+  std::lock_guard<std::mutex> lk(s->mu);
+  auto it = s->jobs.find(id);
+  if (it == s->jobs.end()) return serr(LG_ERR_ARG, "unknown job id");
+  Job& j = it->second;
+  if (j.status == LG_JOB_RUNNING) {
+    j.preempt = true;
+    s->cv_work.notify_all();
+    return LG_OK;
+  }
+  if (j.status == LG_JOB_QUEUED) return LG_OK;
+  return serr(LG_ERR_STATE, "job is done or cancelled");
+}
+
 int lg_server_stats_get(lg_server* s, lg_server_stats* st) {
   if (!s || !st) return serr(LG_ERR_ARG, "null pointer");
   std::lock_guard<std::mutex> lk(s->mu);
@@ -327,6 +405,7 @@ int lg_server_stats_get(lg_server* s, lg_server_stats* st) {
   st->total_updates = s->total_updates;
   st->updates_per_s = s->busy_s > 0 ? s->total_updates / s->busy_s : 0.0;
   st->sweep_launches = s->launches;
+  st->preemptions = s->preempted;
   return LG_OK;
 }
 

@@ -243,8 +243,15 @@ class loop_worker {
     if (lp_.lattice == LG_LATTICE_SQUARE && p.defined("W") && (int)std::llround(p.num("W")) != lp_.L)
       throw std::invalid_argument("square lattice with W != L is not supported");
     if (p.str("MODEL", "spin") != "spin") throw std::invalid_argument("only MODEL = spin is supported");
-    if (p.defined("local_S") && std::fabs(p.num("local_S") - 0.5) > 1e-12)
-      throw std::invalid_argument("only local_S = 1/2 is supported");
+    // spin size (model_parameter.h): S=1/2, or S=1 on the chain through two
+    // virtual sites per real site (path_integral.C:589-615)
+    lp_.two_s = 1;
+    if (p.defined("local_S")) {
+      const double S = p.num("local_S");
+      if (std::fabs(S - 1.0) < 1e-12 && lp_.lattice == LG_LATTICE_CHAIN) lp_.two_s = 2;
+      else if (std::fabs(S - 0.5) > 1e-12)
+        throw std::invalid_argument("local_S must be 1/2, or 1 on the chain lattice");
+    }
     for (const char* k : {"Gamma", "D"})
       if (p.defined(k) && p.num(k) != 0.0)
         throw std::invalid_argument(std::string("parameter ") + k + " != 0 (transverse field / single-ion terms) is not supported");
@@ -296,6 +303,7 @@ class loop_worker {
       }
     }
     if (mc || ms || !custom.empty()) {
+      if (lp_.two_s == 2) throw std::invalid_argument("correlation measurements are built for S=1/2 only");
       if (lp_.L < 2 || lp_.L % 2) throw std::invalid_argument("L must be even");
       const int origin = p.defined("INITIAL_SITE") ? (int)std::llround(p.num("INITIAL_SITE")) : -1;
       if (p.defined("INITIAL_SITE") && (origin < 0 || origin >= n_sites_))

@@ -58,6 +58,7 @@ class LGParams(ctypes.Structure):
         ("lattice", ctypes.c_int32), ("L", ctypes.c_int32), ("beta", ctypes.c_double),
         ("jxy", ctypes.c_double), ("jz", ctypes.c_double), ("force_scatter", ctypes.c_double),
         ("seed", ctypes.c_uint32), ("chain_id", ctypes.c_uint32), ("h", ctypes.c_double),
+        ("two_s", ctypes.c_int32),
     ]
 
 
@@ -145,15 +146,17 @@ def kernel_names():
     return [lib.lg_profile_kernel_name(i).decode() for i in range(lib.lg_profile_num_kernels())]
 
 
-def params(lattice, L, beta, jxy=1.0, jz=None, seed=23125, chain_id=0, force_scatter=0.0, h=0.0):
+def params(lattice, L, beta, jxy=1.0, jz=None, seed=23125, chain_id=0, force_scatter=0.0, h=0.0, two_s=1):
     if jz is None:
         jz = jxy
     return LGParams(int(lattice), int(L), float(beta), float(jxy), float(jz), float(force_scatter),
-                    int(seed), int(chain_id), float(h))
+                    int(seed), int(chain_id), float(h), int(two_s))
 
 
-def lattice_size(lattice, L):
-    return (L, L) if lattice == CHAIN else (L * L, 2 * L * L)
+def lattice_size(lattice, L, two_s=1):
+    """(sites, bonds) the device sweeps: virtual ones for S=1 (two per real site)"""
+    nv = 2 if two_s == 2 else 1
+    return (nv * L, nv * nv * L) if lattice == CHAIN else (nv * L * L, nv * nv * 2 * L * L)
 
 
 def _ptr(a):
@@ -167,7 +170,7 @@ class Chain:
         self.p = p
         self.h = ctypes.c_void_p()
         _check(lib.lg_chain_create(ctypes.byref(p), ctypes.byref(self.h)))
-        self.n_sites, self.n_bonds = lattice_size(p.lattice, p.L)
+        self.n_sites, self.n_bonds = lattice_size(p.lattice, p.L, p.two_s)
 
     def sweep(self, n, records=True):
         rec = np.zeros((n, REC_SIZE)) if records else None
@@ -420,7 +423,8 @@ class ServerStats(ctypes.Structure):
     _fields_ = [("submitted", ctypes.c_uint64), ("done", ctypes.c_uint64), ("running", ctypes.c_uint64),
                 ("queued", ctypes.c_uint64), ("p50_latency_s", ctypes.c_double), ("p99_latency_s", ctypes.c_double),
                 ("mean_latency_s", ctypes.c_double), ("updates_per_s", ctypes.c_double), ("busy_s", ctypes.c_double),
-                ("total_updates", ctypes.c_uint64), ("sweep_launches", ctypes.c_uint64)]
+                ("total_updates", ctypes.c_uint64), ("sweep_launches", ctypes.c_uint64),
+                ("preemptions", ctypes.c_uint64)]
 
 
 def _server_bind():
@@ -432,13 +436,14 @@ def _server_bind():
     lib.lg_server_poll.argtypes = [vp, ctypes.c_uint64, P(JobResult)]
     lib.lg_server_wait.argtypes = [vp, ctypes.c_uint64, ctypes.c_int64, P(JobResult)]
     lib.lg_server_cancel.argtypes = [vp, ctypes.c_uint64]
+    lib.lg_server_preempt.argtypes = [vp, ctypes.c_uint64]
     lib.lg_server_stats_get.argtypes = [vp, P(ServerStats)]
     lib.lg_server_stop.argtypes = [vp]
 
 
 _server_bind()
 EXPORTED += ["lg_server_last_error", "lg_server_start", "lg_server_submit", "lg_server_poll", "lg_server_wait",
-             "lg_server_cancel", "lg_server_stats_get", "lg_server_stop",
+             "lg_server_cancel", "lg_server_preempt", "lg_server_stats_get", "lg_server_stop",
              "lg_worker_last_error", "lg_worker_create", "lg_worker_run", "lg_worker_status", "lg_worker_result",
              "lg_worker_report", "lg_worker_save", "lg_worker_load", "lg_worker_set_chunk", "lg_worker_destroy",
              "lg_spin_correlations", "lg_temperature_schedule"]
@@ -511,6 +516,10 @@ class Server:
 
     def cancel(self, jid):
         _scheck(lib.lg_server_cancel(self.h, jid))
+
+    def preempt(self, jid):
+        """save the running job at the next chunk boundary and queue it again"""
+        _scheck(lib.lg_server_preempt(self.h, jid))
 
     def stats(self):
         s = ServerStats()

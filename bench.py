@@ -338,11 +338,12 @@ def run_scan(args, lg):
     batched = nop_total / dt
     nop_sweep = nop_total / args.steps
     # sustained rate over a longer stretch (the same graph replay)
-    log("scan: sustained %d batched sweeps" % args.sustain)
-    t0 = time.perf_counter()
-    rs = run_chunked(lambda k: b.run(k), args.sustain, 500)
-    dts = time.perf_counter() - t0
-    sustained = float(rs[:, :, 0].sum()) / dts
+    sustained = None
+    if args.sustain > 0:
+        log("scan: sustained %d batched sweeps" % args.sustain)
+        t0 = time.perf_counter()
+        rs = run_chunked(lambda k: b.run(k), args.sustain, 500)
+        sustained = float(rs[:, :, 0].sum()) / (time.perf_counter() - t0)
     log("scan: profile %d sweeps" % args.profile_sweeps)
     prof = b.profile(args.profile_sweeps)
     launch = b.launch()

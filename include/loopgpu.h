@@ -41,9 +41,15 @@ enum { LG_LATTICE_CHAIN = 0, LG_LATTICE_SQUARE = 1 };
 
 /* Parameters of one Markov chain.  Mirrors the worker's parameters:
  * LATTICE ("chain lattice" / "square lattice"), L, T or BETA, J or Jxy/Jz,
- * SEED, FORCE_SCATTER (loop_config.h, lattice.h:304-307, temperature.h:43-79,
- * weight_impl.h:165-188); the spin model is S=1/2 XXZ,
- * H = sum_b [Jz Sz Sz + Jxy/2 (S+S- + S-S+)].
+ * SEED, FORCE_SCATTER, local_S (loop_config.h, lattice.h:304-307,
+ * temperature.h:43-79, weight_impl.h:165-188); the spin model is the XXZ
+ * model H = sum_b [Jz Sz Sz + Jxy/2 (S+S- + S-S+)] with spin S = two_s / 2.
+ * S=1 (two_s = 2, chain lattice only) is swept as looper sweeps it: two
+ * virtual S=1/2 sites per real site (virtual site 2s + a), four virtual bonds
+ * per real bond (bond 4i + 2a + c joins 2i + a and 2(i+1) + c), and a random
+ * spin-conserving permutation joining the tau = 1 and tau = 0 ends of each real
+ * site's virtual world lines (path_integral.C:589-615); sites, bonds, spins
+ * and operators at the ABI are then the virtual ones.
  * chain_id selects an independent random stream for the same seed. */
 typedef struct lg_params {
   int32_t lattice;       /* LG_LATTICE_CHAIN or LG_LATTICE_SQUARE */
@@ -55,6 +61,7 @@ typedef struct lg_params {
   uint32_t seed;         /* SEED */
   uint32_t chain_id;     /* independent stream index */
   double h;              /* longitudinal field h (site term -h S^z, looper's FIELD path) */
+  int32_t two_s;         /* 2S: 1 (S=1/2; 0 is taken as 1) or 2 (S=1, chain lattice) */
 } lg_params;
 
 /* Per-sweep collector sums (all improved estimators) -- see the collector of

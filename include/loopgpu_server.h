@@ -68,6 +68,7 @@ typedef struct lg_server_stats {
   double busy_s;           /* wall time with at least one job resident */
   uint64_t total_updates;
   uint64_t sweep_launches; /* batched sweeps executed */
+  uint64_t preemptions;    /* jobs saved and re-queued by lg_server_preempt */
 } lg_server_stats;
 
 int lg_server_last_error(char* buf, size_t len);
@@ -78,6 +79,15 @@ int lg_server_poll(lg_server* s, uint64_t id, lg_job_result* res);
 /* block until the job is done or cancelled, or timeout_ms elapses (0: poll) */
 int lg_server_wait(lg_server* s, uint64_t id, int64_t timeout_ms, lg_job_result* res);
 int lg_server_cancel(lg_server* s, uint64_t id);
+/* Preemption: a running job leaves the device at the next chunk boundary
+ * with its configuration saved (spins, operators and sweep count, the content
+ * of the reference worker's checkpoint, path_integral.C:111-124) and its
+ * measurements kept, and waits in the queue behind the jobs already there;
+ * when admitted again it continues from that configuration, so its
+ * trajectory and results are those of an uninterrupted run.  Its device
+ * footprint is free meanwhile.  A queued job is left as it is; a done or
+ * cancelled one gives LG_ERR_STATE. */
+int lg_server_preempt(lg_server* s, uint64_t id);
 int lg_server_stats_get(lg_server* s, lg_server_stats* st);
 int lg_server_stop(lg_server* s);
 

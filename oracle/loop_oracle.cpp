@@ -73,7 +73,7 @@ inline u32x4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
 
 // Counter layout shared with the device: (sweep, stage, element, sub-index),
 // key = (seed, chain id).
-enum { STAGE_DIAG = 1, STAGE_REGRAPH = 2, STAGE_FLIP = 3 };
+enum { STAGE_DIAG = 1, STAGE_REGRAPH = 2, STAGE_FLIP = 3, STAGE_BOUNDARY = 5 };
 
 // ---------------------------------------------------------------------------
 // Deterministic natural logarithm built only from IEEE-754 +,-,*,/ (no fused
@@ -150,11 +150,27 @@ inline uint64_t exp_gap(const u32x4& x, double scale) {
 // ---------------------------------------------------------------------------
 struct Lattice {
   int kind = 0, L = 0, N = 0, Nb = 0, dim = 1;
+  int nv = 1;  // virtual sites per real site (2S)
   std::vector<int> src, tgt, dir, gauge;
   std::vector<std::vector<int>> site_bonds;  // bonds incident to each site
-  void init(int k, int l) {
-    kind = k; L = l;
-    if (kind == 0) {
+  // nv = 2 (S=1, chain only): looper's virtual lattice for S > 1/2
+  // (lattice.h:104-175, virtual_graph): real site s -> virtual sites 2s,
+  // 2s+1; real bond i = (i, i+1) -> the four virtual bonds 4i + 2a + c =
+  // (2i + a, 2(i+1) + c); every virtual site carries its real site's gauge
+  void init(int k, int l, int n_virtual = 1) {
+    kind = k; L = l; nv = n_virtual;
+    if (kind == 0 && nv == 2) {
+      N = 2 * L; Nb = 4 * L; dim = 1;
+      src.resize(Nb); tgt.resize(Nb); dir.assign(Nb, 0);
+      for (int i = 0; i < L; ++i)
+        for (int a = 0; a < 2; ++a)
+          for (int c = 0; c < 2; ++c) {
+            src[4 * i + 2 * a + c] = 2 * i + a;
+            tgt[4 * i + 2 * a + c] = 2 * ((i + 1) % L) + c;
+          }
+      gauge.resize(N);
+      for (int v = 0; v < N; ++v) gauge[v] = ((v / 2) % 2 == 0) ? 1 : -1;
+    } else if (kind == 0) {
       N = L; Nb = L; dim = 1;
       src.resize(Nb); tgt.resize(Nb); dir.assign(Nb, 0);
       for (int b = 0; b < Nb; ++b) { src[b] = b; tgt[b] = (b + 1) % L; }
@@ -445,6 +461,48 @@ struct Schedule {
 };
 
 // ---------------------------------------------------------------------------
+// Boundary permutation for S > 1/2 (looper/permutation.h): a random
+// permutation of a real site's virtual sites under which every tau = 0 spin
+// meets an equal tau = 1 spin.
+//   shuffle_range:  looper::random_shuffle (permutation.h:36-45), swapping
+//                   each position with one drawn from the rest, (int)(n*u);
+//   partition_by:   looper::guided_sort_01 (permutation.h:51-80): entries at
+//                   positions whose guide is 1 in the front part exchanged
+//                   with entries at positions whose guide is 0 in the back,
+//                   from both ends inwards;
+//   partitioned_shuffle: looper::partitioned_random_shuffle
+//                   (permutation.h:88-112): partition the identity by the
+//                   tau = 1 spins, shuffle each part, partition by the tau = 0
+//                   spins.
+// ---------------------------------------------------------------------------
+template <class U01>
+void shuffle_range(int* first, int n, U01& u01) {
+  for (; n > 1; ++first, --n) std::swap(first[0], first[(int)(n * u01())]);
+}
+
+void partition_by(int* perm, int n, const int* guide) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    while (lo < hi && guide[lo] == 0) ++lo;
+    while (lo < hi && guide[hi] == 1) --hi;
+    if (lo >= hi) return;
+    std::swap(perm[lo], perm[hi]);
+    ++lo;
+    --hi;
+  }
+}
+
+template <class U01>
+void partitioned_shuffle(int* perm, int n, const int* guide0, const int* guide1, U01& u01) {
+  partition_by(perm, n, guide1);
+  int c = 0;
+  for (int i = 0; i < n; ++i) c += guide1[i] == 0;
+  shuffle_range(perm, c, u01);
+  shuffle_range(perm + c, n - c, u01);
+  partition_by(perm, n, guide0);
+}
+
+// ---------------------------------------------------------------------------
 // Looper-mode worker: path_integral.C:380-864, serial, FIELD=SIGN=false,
 // IMPROVE=true, S=1/2 (max_virtual_sites == 1).
 // ---------------------------------------------------------------------------
@@ -474,8 +532,9 @@ struct LooperChain {
   double h = 0;   // longitudinal field, site term -h S^z
   Schedule sched; // annealing: beta = 1 / T(mcs) before each sweep (path_integral.C:325)
 
-  void init(int kind, int L, double b, double jxy, double jz, double fs, uint32_t seed, double field = 0) {
-    lat.init(kind, L);
+  void init(int kind, int L, double b, double jxy, double jz, double fs, uint32_t seed, double field = 0,
+            int nv = 1) {
+    lat.init(kind, L, nv);
     w = xxz_weights(jxy, jz, fs);
     beta = b;
     h = field;
@@ -540,8 +599,19 @@ struct LooperChain {
       fid = r.fid; current[s0] = r.curr0; current[s1] = r.curr1;
       oi.loop0 = r.loop0; oi.loop1 = r.loop1;
     }
-    // symmetrize (583-588)
-    for (int s = 0; s < N; ++s) uf_unify(frag, s, current[s]);
+    // symmetrize (583-615): S=1/2 closes each world line through tau = 1;
+    // virtual sites join a real site's tau = 0 ends to its tau = 1 ends
+    // through a random spin-conserving permutation
+    if (lat.nv == 1) {
+      for (int s = 0; s < N; ++s) uf_unify(frag, s, current[s]);
+    } else {
+      std::vector<int> perm(lat.nv);
+      for (int off = 0; off < N; off += lat.nv) {
+        for (int i = 0; i < lat.nv; ++i) perm[i] = i;
+        partitioned_shuffle(perm.data(), lat.nv, &spins[off], &spins_c[off], rnd);
+        for (int i = 0; i < lat.nv; ++i) uf_unify(frag, off + i, current[off + perm[i]]);
+      }
+    }
     // cluster ids (626-648)
     int nc = uf_set_id(frag, 0, fid, 0);
     uf_copy_id(frag, 0, fid);
@@ -876,9 +946,9 @@ struct PhiloxChain {
   int last_nodes = 0;
 
   void init(int kind, int L, double b, double jxy, double jz, double fs, uint32_t seed, uint32_t chain,
-            double field = 0) {
+            double field = 0, int nv = 1) {
     h = field;
-    lat.init(kind, L);
+    lat.init(kind, L, nv);
     w = xxz_weights(jxy, jz, fs);
     th = make_thresholds(w);
     beta = b;
@@ -903,7 +973,7 @@ struct PhiloxChain {
     pm.assign(lat.N, 0);
     for (int s = 0; s < lat.N; ++s) {
       pm[s] = se.off[s + 1] - se.off[s];
-      psoff[s + 1] = psoff[s] + std::max(pm[s], 1);
+      psoff[s + 1] = psoff[s] + segments(pm[s]);
     }
     pev_time.assign(psoff[lat.N], 0);
     pseg_spin.assign(psoff[lat.N], 0);
@@ -916,8 +986,15 @@ struct PhiloxChain {
         pseg_spin[psoff[s] + j] = c;
         c ^= (otype[k] & 1);
       }
+      if (split()) { pev_time[psoff[s] + pm[s]] = 1.0; pseg_spin[psoff[s] + pm[s]] = c; }
     }
   }
+
+  // virtual sites: a site with m events owns m + 1 segments (from tau = 0,
+  // between events, up to tau = 1); otherwise the lowest segment wraps
+  // through tau = 1 and m segments remain (one for a site without events)
+  bool split() const { return lat.nv > 1; }
+  int segments(int m) const { return m == 0 ? 1 : (split() ? m + 1 : m); }
 
   // spin at tau on site s in the previous configuration; returns -1 on a tie
   int spin_at(int s, double tau) const {
@@ -926,7 +1003,7 @@ struct PhiloxChain {
     const double* t = &pev_time[base];
     int j = (int)(std::lower_bound(t, t + m, tau) - t);
     if (j < m && t[j] == tau) return -1;
-    return pseg_spin[base + (j == m ? 0 : j)];
+    return pseg_spin[base + (j == m ? (split() ? m : 0) : j)];
   }
 
   void sweep_once(double* rec) {
@@ -985,7 +1062,7 @@ struct PhiloxChain {
     // (b) world-line linking: segments as nodes
     SiteEvents se = build_site_events(lat, nops, bond.data(), otime.data());
     std::vector<int> soff(N + 1, 0), m(N);
-    for (int s = 0; s < N; ++s) { m[s] = se.off[s + 1] - se.off[s]; soff[s + 1] = soff[s] + std::max(m[s], 1); }
+    for (int s = 0; s < N; ++s) { m[s] = se.off[s + 1] - se.off[s]; soff[s + 1] = soff[s] + segments(m[s]); }
     int M = soff[N];
     std::vector<int> seg_spin(M), seg_site(M);
     std::vector<double> ev_time(M, 0);
@@ -998,6 +1075,11 @@ struct PhiloxChain {
         seg_spin[soff[s] + j] = c;
         seg_site[soff[s] + j] = s;
         c ^= (otype[k] & 1);
+      }
+      if (split()) {  // top segment, from the last event up to tau = 1
+        ev_time[soff[s] + m[s]] = 1.0;
+        seg_spin[soff[s] + m[s]] = c;
+        seg_site[soff[s] + m[s]] = s;
       }
     }
     auto node = [&](int s, int j) { return soff[s] + j; };
@@ -1023,6 +1105,21 @@ struct PhiloxChain {
       else if (g == 0) { unite(l0, l1); unite(u0, u1); }
       else { unite(l0, u1); unite(l1, u0); }
     }
+    // virtual sites (S=1): each real site's tau = 0 ends (lowest segments)
+    // meet its tau = 1 ends (top segments) through a spin-conserving
+    // permutation; with equal tau = 0 spins the exchange is taken iff bit 0 of
+    // Philox(sweep, STAGE_BOUNDARY, real site, 0) is set
+    if (split()) {
+      for (int r = 0; r < N / 2; ++r) {
+        const int v0 = 2 * r, v1 = 2 * r + 1;
+        const int bot0 = soff[v0], top0 = soff[v0 + 1] - 1, bot1 = soff[v1], top1 = soff[v1 + 1] - 1;
+        const bool swap = (seg_spin[bot0] == seg_spin[bot1])
+            ? (philox4x32_10(sweep, STAGE_BOUNDARY, (uint32_t)r, 0, key0, key1).v[0] & 1) != 0
+            : seg_spin[bot0] != seg_spin[top0];
+        unite(bot0, swap ? top1 : top0);
+        unite(bot1, swap ? top0 : top1);
+      }
+    }
     for (int i = 0; i < M; ++i) parent[i] = find(i);
     last_root = parent;
     last_nodes = M;
@@ -1038,23 +1135,33 @@ struct PhiloxChain {
         est_end(e, 1.0, gs, c);
         continue;
       }
-      for (int j = 0; j < m[s]; ++j) {
+      const int nseg = segments(m[s]);
+      for (int j = 0; j < nseg; ++j) {
         int i = soff[s] + j;
         Estimate& e = acc[parent[i]];
         int c = seg_spin[i];
-        double len = (j == 0) ? (ev_time[soff[s]] + 1.0 - ev_time[soff[s] + m[s] - 1])
-                              : (ev_time[i] - ev_time[i - 1]);
+        // segment j lies between events j-1 and j (wrapping: segment 0 spans
+        // the last event to tau = 1 and tau = 0 to the first event; split:
+        // segment 0 starts at tau = 0 and segment m ends at tau = 1)
+        const double tlo = (j > 0) ? ev_time[i - 1] : (split() ? 0.0 : ev_time[soff[s] + m[s] - 1] - 1.0);
+        const double thi = ev_time[i];
+        double len = (j == 0 && !split()) ? (thi + 1.0 - ev_time[soff[s] + m[s] - 1]) : (thi - tlo);
         if (j == 0) {
           e.usize0 += 0.5; e.umag0 += 0.5 - c; e.ssize0 += gs * 0.5; e.smag0 += gs * (0.5 - c);
         }
         e.usize += 0.5 * len; e.umag += (0.5 - c) * len;
         e.ssize += gs * 0.5 * len; e.smag += gs * (0.5 - c) * len;
-        // winding: this segment is the lower leg of event j and the upper leg of event j-1
-        int ka = se.ev_op[se.off[s] + j], sa = se.ev_side[se.off[s] + j];
-        int jb = (j == 0) ? m[s] - 1 : j - 1;
-        int kb = se.ev_op[se.off[s] + jb], sb = se.ev_side[se.off[s] + jb];
-        if (sa == 0) for (int d = 0; d < lat.dim; ++d) e.winding[d] += (1 - 2 * c) * lat.vr(bond[ka], d);
-        if (sb == 0) for (int d = 0; d < lat.dim; ++d) e.winding[d] -= (1 - 2 * c) * lat.vr(bond[kb], d);
+        // winding: this segment is the lower leg of event j (if any) and the
+        // upper leg of event j-1 (if any; event m-1 for a wrapping segment 0)
+        if (j < m[s]) {
+          int ka = se.ev_op[se.off[s] + j], sa = se.ev_side[se.off[s] + j];
+          if (sa == 0) for (int d = 0; d < lat.dim; ++d) e.winding[d] += (1 - 2 * c) * lat.vr(bond[ka], d);
+        }
+        if (j > 0 || !split()) {
+          int jb = (j == 0) ? m[s] - 1 : j - 1;
+          int kb = se.ev_op[se.off[s] + jb], sb = se.ev_side[se.off[s] + jb];
+          if (sb == 0) for (int d = 0; d < lat.dim; ++d) e.winding[d] -= (1 - 2 * c) * lat.vr(bond[kb], d);
+        }
       }
     }
     for (int i = 0; i < R_COUNT; ++i) rec[i] = 0;
@@ -1131,14 +1238,14 @@ void or_thresholds(double jxy, double jz, double force_scatter, uint64_t* thr4, 
   *toff = t.toff;
 }
 
-int or_lattice_info(int kind, int L, int* N, int* Nb, int* dim) {
-  if ((kind != 0 && kind != 1) || L < 2) return -1;
-  Lattice lat; lat.init(kind, L);
+int or_lattice_info(int kind, int L, int nv, int* N, int* Nb, int* dim) {
+  if ((kind != 0 && kind != 1) || L < 2 || nv < 1 || nv > 2 || (nv == 2 && kind != 0)) return -1;
+  Lattice lat; lat.init(kind, L, nv);
   *N = lat.N; *Nb = lat.Nb; *dim = lat.dim;
   return 0;
 }
-int or_lattice_bonds(int kind, int L, int* src, int* tgt, int* gauge) {
-  Lattice lat; lat.init(kind, L);
+int or_lattice_bonds(int kind, int L, int nv, int* src, int* tgt, int* gauge) {
+  Lattice lat; lat.init(kind, L, nv);
   for (int b = 0; b < lat.Nb; ++b) { src[b] = lat.src[b]; tgt[b] = lat.tgt[b]; }
   for (int s = 0; s < lat.N; ++s) gauge[s] = lat.gauge[s];
   return 0;
@@ -1160,9 +1267,10 @@ void or_uf_pairs(int n, int npairs, const int* pairs, int variant, int* parent, 
 }
 
 // ---- looper mode ---------------------------------------------------------
-void* or_looper_create(int kind, int L, double beta, double jxy, double jz, double fs, uint32_t seed, double h) {
+void* or_looper_create(int kind, int L, double beta, double jxy, double jz, double fs, uint32_t seed, double h,
+                       int nv) {
   LooperChain* c = new LooperChain();
-  c->init(kind, L, beta, jxy, jz, fs, seed, h);
+  c->init(kind, L, beta, jxy, jz, fs, seed, h, nv);
   return c;
 }
 void or_looper_sweep(void* h, int n, double* rec) {
@@ -1210,9 +1318,9 @@ int or_segment_of_elements(int kind, int L, int nops, const int* bond, const dou
 
 // ---- philox mode -------------------------------------------------------------
 void* or_philox_create(int kind, int L, double beta, double jxy, double jz, double fs, double h, uint32_t seed,
-                       uint32_t chain) {
+                       uint32_t chain, int nv) {
   PhiloxChain* c = new PhiloxChain();
-  c->init(kind, L, beta, jxy, jz, fs, seed, chain, h);
+  c->init(kind, L, beta, jxy, jz, fs, seed, chain, h, nv);
   return c;
 }
 void or_philox_sweep(void* h, int n, double* rec) {
@@ -1253,6 +1361,31 @@ void or_philox_load(void* h, const int* spins, const int* boff, const double* ti
   c->reset_prev_from_ops();
 }
 void or_philox_destroy(void* h) { delete (PhiloxChain*)h; }
+
+// ---- boundary permutation (test/permutation.C's two tests) -----------------
+// mt19937(seed), uniforms x / 2^32; test 1: trial1 shuffles of 0..n-1 into
+// out1; test 2: per trial, conf0[i] = (int)(2u), conf1 = shuffled conf0, perm =
+// the partitioned shuffle of the identity guided by (conf0, conf1)
+void or_permutation_test(uint32_t seed, int n, int trial1, int trial2, int* out1, int* conf0, int* conf1,
+                         int* perm) {
+  std::mt19937 eng(seed);
+  auto u01 = [&]() { return (double)eng() / 4294967296.0; };
+  for (int t = 0; t < trial1; ++t) {
+    int* r = out1 + t * n;
+    for (int j = 0; j < n; ++j) r[j] = j;
+    shuffle_range(r, n, u01);
+  }
+  for (int t = 0; t < trial2; ++t) {
+    int* c0 = conf0 + t * n;
+    int* c1 = conf1 + t * n;
+    int* p = perm + t * n;
+    for (int i = 0; i < n; ++i) c0[i] = (int)(2 * u01());
+    for (int i = 0; i < n; ++i) c1[i] = c0[i];
+    shuffle_range(c1, n, u01);
+    for (int j = 0; j < n; ++j) p[j] = j;
+    partitioned_shuffle(p, n, c0, c1, u01);
+  }
+}
 
 // ---- annealing -------------------------------------------------------------
 static Schedule make_schedule(int n, const int* steps, const double* temps) {

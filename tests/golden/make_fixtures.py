@@ -9,6 +9,10 @@ Sources (data files of the reference's tests):
   test/spinmodel_helper.op      chain lattice bond list (ALPS conventions)
   test/virtual_lattice.op       square lattice L=2 vertices, edges, gauge
   test/temperature.ip/.op       annealing schedules: parameter sets and T(step)
+  test/permutation.op           random_shuffle and partitioned_random_shuffle output
+                                (mt19937 seed 23094)
+  test/virtual_lattice.op       the virtual lattice of a square L=2 lattice with S=3/2 on
+                                site 0 and S=1 elsewhere (virtual site and bond order)
   standalone/loop.op            standalone kernel output (L=8, T=0.2)
   loop.op                       end-to-end run: ED and path-integral QMC, chain L=4, T=0.25;
                                 and the path-integral run with MEASURE[Correlations] and
@@ -218,8 +222,55 @@ def temperature():
     return [dict(o, params=p) for p, o in zip(sets, out)]
 
 
+def permutation():
+    """test/permutation.op: 20 shuffles of 0..9, then 5 partitioned shuffles
+    with their guides (conf0 at tau = 0, conf1 at tau = 1)"""
+    lines = read("test/permutation.op")
+    shuffles, parts, cur = [], [], {}
+    for l in lines:
+        f = l.split()
+        if not f:
+            continue
+        if f[0] in ("conf0", "conf1", "perm"):
+            cur[f[0]] = [int(x) for x in f[1:]]
+            if f[0] == "perm":
+                parts.append(cur)
+                cur = {}
+        elif all(x.isdigit() for x in f) and len(f) == 10:
+            shuffles.append([int(x) for x in f])
+    assert len(shuffles) == 20 and len(parts) == 5
+    return {"seed": 23094, "n": 10, "shuffles": shuffles, "partitioned": parts}
+
+
+def virtual_lattice():
+    """the first virtual graph of test/virtual_lattice.op (square L=2, site 0
+    of type 1 with S=3/2, the other sites S=1): its real edges, the virtual
+    edges in order, the virtual sites of each real site and the gauge"""
+    lines = read("test/virtual_lattice.op")
+    real, virt, gauges, site_map = [], [], [], []
+    block = 0
+    for l in lines:
+        m = re.match(r'\s*<EDGE source="(\d+)" target="(\d+)"', l)
+        if l.startswith("<GRAPH"):
+            block += 1
+        elif m and block == 1:
+            real.append([int(m.group(1)) - 1, int(m.group(2)) - 1])
+        elif m and block == 2:
+            virt.append([int(m.group(1)) - 1, int(m.group(2)) - 1])
+        elif block == 2 and re.match(r"^-?1( -?1)+\s*$", l) and not gauges:
+            gauges = [int(x) for x in l.split()]
+        mm = re.match(r"\s*(\d+) -> \[(\d+),(\d+)\]", l)
+        if mm and block == 2 and len(site_map) < 4:
+            site_map.append([int(mm.group(2)), int(mm.group(3))])
+    assert len(real) == 8 and len(virt) == 40 and len(gauges) == 9
+    return {"real_edges": real, "two_s": [3, 2, 2, 2], "virtual_edges": virt, "virtual_gauge": gauges,
+            "site_map": site_map}
+
+
 def main():
     fx = {
+        "permutation": permutation(),
+        "virtual_lattice": virtual_lattice(),
         "temperature": temperature(),
         "union_find": union_find(),
         "compatibility": compatibility(),

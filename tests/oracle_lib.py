@@ -31,11 +31,11 @@ lib.or_det_exp.argtypes = [vp, vp, I]
 lib.or_exp_gap.argtypes = [vp, D, vp]
 lib.or_xxz_weights.argtypes = [D, D, D, vp, vp, vp]
 lib.or_thresholds.argtypes = [D, D, D, vp, vp]
-lib.or_lattice_info.argtypes = [I, I, vp, vp, vp]
-lib.or_lattice_bonds.argtypes = [I, I, vp, vp, vp]
+lib.or_lattice_info.argtypes = [I, I, I, vp, vp, vp]
+lib.or_lattice_bonds.argtypes = [I, I, I, vp, vp, vp]
 lib.or_uf_pairs.argtypes = [I, I, vp, I, vp, vp, vp]
 lib.or_looper_create.restype = vp
-lib.or_looper_create.argtypes = [I, I, D, D, D, D, U, D]
+lib.or_looper_create.argtypes = [I, I, D, D, D, D, U, D, I]
 lib.or_looper_sweep.argtypes = [vp, I, vp]
 lib.or_looper_destroy.argtypes = [vp]
 lib.or_standalone_run.argtypes = [U, D, U, U, vp, vp]
@@ -44,7 +44,7 @@ lib.or_looper_labels.argtypes = [I, I, I, vp, vp, vp, I, vp]
 lib.or_segment_of_elements.argtypes = [I, I, I, vp, vp, vp]
 lib.or_segment_of_elements.restype = I
 lib.or_philox_create.restype = vp
-lib.or_philox_create.argtypes = [I, I, D, D, D, D, D, U, U]
+lib.or_philox_create.argtypes = [I, I, D, D, D, D, D, U, U, I]
 lib.or_philox_sweep.argtypes = [vp, I, vp]
 lib.or_philox_set_beta.argtypes = [vp, D]
 lib.or_philox_num_ops.argtypes = [vp]
@@ -53,6 +53,7 @@ lib.or_philox_state.argtypes = [vp, vp, vp, vp, vp]
 lib.or_philox_roots.argtypes = [vp, vp]
 lib.or_philox_load.argtypes = [vp, vp, vp, vp, vp, U]
 lib.or_philox_destroy.argtypes = [vp]
+lib.or_permutation_test.argtypes = [U, I, I, I, vp, vp, vp, vp]
 lib.or_schedule_temperature.argtypes = [I, vp, vp, I]
 lib.or_schedule_temperature.restype = D
 lib.or_philox_set_schedule.argtypes = [vp, I, vp, vp]
@@ -106,19 +107,31 @@ def _schedule_arrays(steps, temps):
     return st, tp
 
 
+def permutation_test(seed, n, trial1, trial2):
+    """the oracle's restatement of looper::random_shuffle and
+    partitioned_random_shuffle driven as test/permutation.C drives them:
+    (shuffles [trial1, n], conf0, conf1, perm [trial2, n])"""
+    o1 = np.zeros(trial1 * n, np.int32)
+    c0, c1, pm = (np.zeros(trial2 * n, np.int32) for _ in range(3))
+    lib.or_permutation_test(seed, n, trial1, trial2, _p(o1), _p(c0), _p(c1), _p(pm))
+    return o1.reshape(trial1, n), c0.reshape(trial2, n), c1.reshape(trial2, n), pm.reshape(trial2, n)
+
+
 def schedule_temperature(steps, temps, step):
     """T(step) of an annealing schedule (looper/temperature.h:93-113)"""
     st, tp = _schedule_arrays(steps, temps)
     return lib.or_schedule_temperature(len(st), _p(st), _p(tp), int(step))
 
 
-def lattice(kind, L):
+def lattice(kind, L, nv=1):
+    """(sites, bonds, source, target, gauge); nv = 2: the virtual lattice of
+    the S=1 chain (two virtual sites per real site)"""
     N, Nb, dim = I(), I(), I()
-    lib.or_lattice_info(kind, L, ctypes.byref(N), ctypes.byref(Nb), ctypes.byref(dim))
+    assert lib.or_lattice_info(kind, L, nv, ctypes.byref(N), ctypes.byref(Nb), ctypes.byref(dim)) == 0
     src = np.zeros(Nb.value, np.int32)
     tgt = np.zeros(Nb.value, np.int32)
     gauge = np.zeros(N.value, np.int32)
-    lib.or_lattice_bonds(kind, L, _p(src), _p(tgt), _p(gauge))
+    lib.or_lattice_bonds(kind, L, nv, _p(src), _p(tgt), _p(gauge))
     return N.value, Nb.value, src, tgt, gauge
 
 
@@ -131,9 +144,9 @@ def xxz_weights(jxy, jz, fs=0.0):
 
 
 class LooperChain:
-    def __init__(self, kind, L, beta, jxy=1.0, jz=None, seed=23125, fs=0.0, field=0.0):
+    def __init__(self, kind, L, beta, jxy=1.0, jz=None, seed=23125, fs=0.0, field=0.0, two_s=1):
         jz = jxy if jz is None else jz
-        self.h = lib.or_looper_create(kind, L, beta, jxy, jz, fs, seed, field)
+        self.h = lib.or_looper_create(kind, L, beta, jxy, jz, fs, seed, field, 2 if two_s == 2 else 1)
 
     def sweep(self, n):
         rec = np.zeros((n, R))
@@ -149,11 +162,12 @@ class LooperChain:
 
 
 class PhiloxChain:
-    def __init__(self, kind, L, beta, jxy=1.0, jz=None, seed=23125, chain_id=0, fs=0.0, field=0.0):
+    def __init__(self, kind, L, beta, jxy=1.0, jz=None, seed=23125, chain_id=0, fs=0.0, field=0.0, two_s=1):
         jz = jxy if jz is None else jz
         self.kind, self.L = kind, L
-        self.N, self.Nb, _, _, _ = lattice(kind, L)
-        self.h = lib.or_philox_create(kind, L, beta, jxy, jz, fs, field, seed, chain_id)
+        nv = 2 if two_s == 2 else 1
+        self.N, self.Nb, _, _, _ = lattice(kind, L, nv)
+        self.h = lib.or_philox_create(kind, L, beta, jxy, jz, fs, field, seed, chain_id, nv)
 
     def sweep(self, n):
         rec = np.zeros((n, R))

@@ -100,6 +100,7 @@ LAUNCH_TRAJ = [
 ]
 LAUNCHES = [dict(link=0), dict(link=1), dict(link=128), dict(link=512), dict(link=1536),
             dict(link=1, group=8, blocks=3), dict(link=1, compress=1), dict(link=512, group=4, blocks=7),
+            dict(group=1), dict(group=2, link=1), dict(group=1, link=0, blocks=5), dict(group=2, compress=1),
             dict(link=128, group=8, blocks=5),
             dict(link=0, group=16, blocks=64), dict(link=1536, group=32),
             dict(link=512, compress=1), dict(link=0, compress=0, blocks=9), dict(link=1536, compress=1, group=8),

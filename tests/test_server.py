@@ -107,3 +107,51 @@ def test_served_jobs_match_exact_diagonalisation(loopgpu):
                       ("Staggered Susceptibility", ref["Staggered Susceptibility"])]:
             assert stats.sigmas(r["mean"][k], r["error"][k], rk) < 3.5, (L, T, jz, k, r["mean"][k], r["error"][k], rk)
     s.stop()
+
+
+@pytest.mark.gpu
+def test_preempted_job_resumes_with_the_uninterrupted_result(loopgpu):
+    """a job preempted twice (its configuration saved, its device memory given
+    to a job that was waiting) ends with the result of the same job run
+    without interruption; the waiting job runs meanwhile"""
+    lg = loopgpu
+    A = _job(lg, lg.SQUARE, 12, 6.0, 0.8, 300)
+    B = _job(lg, lg.CHAIN, 128, 40.0, 1.0, 301)
+    ref = lg.Server(max_jobs=8)
+    ra = ref.wait(ref.submit(A, sweeps=20000, thermalization=2000))
+    rb_ref = ref.wait(ref.submit(B, sweeps=1500, thermalization=100))
+    ref.stop()
+    # room for one of the two jobs at a time (operator footprints about 3100
+    # and 6900)
+    s = lg.Server(max_jobs=8, max_ops=8000, max_chunk=16)
+    ja = s.submit(A, sweeps=20000, thermalization=2000)
+    time.sleep(0.3)
+    jb = s.submit(B, sweeps=1500, thermalization=100)
+    assert s.poll(jb)["status"] == lg.JOB_QUEUED        # B waits for A's memory
+    done_before = s.poll(ja)["sweeps_done"]
+    s.preempt(ja)
+    rb = s.wait(jb)
+    assert rb["status"] == lg.JOB_DONE
+    mid = s.poll(ja)
+    assert mid["status"] in (lg.JOB_QUEUED, lg.JOB_RUNNING) and mid["sweeps_done"] < 22000
+    s.preempt(ja)                                       # again, whatever its state
+    r = s.wait(ja)
+    st = s.stats()
+    s.stop()
+    assert r["status"] == lg.JOB_DONE and r["sweeps_done"] == 22000
+    assert st["preemptions"] >= 1 and done_before < 22000
+    for res, want in ((r, ra), (rb, rb_ref)):
+        for k in EXACT:
+            assert res["mean"][k] == want["mean"][k], k
+            assert res["error"][k] == want["error"][k], k
+        for k in FLOAT:
+            assert res["mean"][k] == pytest.approx(want["mean"][k], rel=1e-9, abs=1e-12), k
+    with pytest.raises(lg.LoopGPUError) as e:
+        s2 = lg.Server(max_jobs=2)
+        j = s2.submit(_job(lg, lg.CHAIN, 8, 1.0, 1.0, 5), sweeps=10)
+        s2.wait(j)
+        try:
+            s2.preempt(j)
+        finally:
+            s2.stop()
+    assert e.value.code == -4
