@@ -16,7 +16,7 @@ FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function", "-Wno-unused-value", "-Wno-unused-result",
 ]
-SOURCES = ["lg_kernels.hip", "lg_api.hip", "worker.cpp", "correlation.cpp", "lg_server.cpp"]
+SOURCES = ["lg_kernels.hip", "lg_api.hip", "lg_noalloc.hip", "worker.cpp", "correlation.cpp", "lg_server.cpp"]
 
 
 def run(cmd, verbose):
