@@ -343,6 +343,8 @@ int Batch::allocate() {
   if ((rc = alloc(node_site, node_cap))) return rc;
   if ((rc = alloc(parent, node_cap))) return rc;
   if ((rc = alloc(fbit, node_cap))) return rc;
+  if ((rc = alloc(fmask, node_cap / 64 + 1))) return rc;
+  if ((rc = alloc(rmask, node_cap / 64 + 1))) return rc;
   if ((rc = alloc(acc, node_cap))) return rc;
   // the estimator accumulates into zeroed accumulators; the collector
   // clears each root's after reading it
@@ -606,7 +608,7 @@ DevBatch Batch::dev(int from) const {
   B.counts = counts; B.acc_count = acc_count; B.kpre = kpre;
   B.cand_t = cand_t; B.cand_g = cand_g; B.kc_new = kc_new;
   B.l0 = l0; B.l1 = l1; B.u0 = u0; B.u1 = u1; B.evflags = evflags; B.chain_node0 = chain_node0; B.node_site = node_site;
-  B.parent = parent; B.fbit = fbit; B.acc = acc;
+  B.parent = parent; B.fbit = fbit; B.fmask = fmask; B.rmask = rmask; B.acc = acc;
   B.scan_status = scan_status; B.scan_ctl = scan_ctl; B.col_part = col_part; B.totals = totals; B.flags = flags;
   B.sweep = d_sweep; B.rec_acc = rec_acc; B.rec_ring = rec_ring;
   B.ring_pos = ring_pos; B.end_count = end_count; B.ring_size = ring_size;

@@ -49,7 +49,6 @@ enum : uint8_t { EV_OFFDIAG = 1, EV_SRC = 2, EV_DIR1 = 4, EV_EMPTY = 8, EV_FIRST
 struct ChainDev {
   int kind;            // 0 chain, 1 square
   int L, N, Nb, dim;   // N, Nb: (virtual) sites and bonds of the lattice the kernels sweep
-  int nv;              // virtual sites per real site (2S; 1 for S=1/2, 2 for S=1 on the chain)
   int site_base, bond_base;          // layout of the configuration being built
   int old_site_base, old_bond_base;  // layout of the previous configuration
   int cand_cap;                      // candidate slots per bond
@@ -63,6 +62,7 @@ struct ChainDev {
   double h;                          // longitudinal field (site term -h S^z)
   double wsum;                       // sum_g v_g of a bond (candidate rate beta * wsum)
   int sched_n, sched_off;            // annealing schedule: entries and offset in the batch's table (0: fixed beta)
+  int nv;                            // virtual sites per real site (2S; 1 for S=1/2, 2 for S=1 on the chain)
 };
 
 // Temperature of Monte Carlo step `step` on an annealing schedule of n
@@ -229,13 +229,15 @@ LG_HD int choose_graph(uint32_t x, const ChainDev& c) {
 //         virtual bonds 4i + 2a + c = (2i + a, 2(i+1) + c); virtual site v has
 //         bonds {4s+2a, 4s+2a+1 (source), 4(s-1)+a, 4(s-1)+2+a (target)}
 LG_HD void bond_sites(const ChainDev& c, int b, int& s0, int& s1) {
-  if (c.kind == 0 && c.nv == 2) {
-    const int i = b >> 2, a = (b >> 1) & 1, d = b & 1;
-    s0 = 2 * i + a;
-    s1 = 2 * ((i + 1 == c.L) ? 0 : i + 1) + d;
-  } else if (c.kind == 0) {
-    s0 = b;
-    s1 = (b + 1 == c.L) ? 0 : b + 1;
+  if (c.kind == 0) {
+    // v = log2(virtual sites per site): bond 4i + 2a + d joins virtual site
+    // a of site i to virtual site d of site i+1 (v = 1); bond i joins sites
+    // i and i+1 (v = 0: a = d = 0).  One formula keeps the kernels' register
+    // count that of the plain chain.
+    const int v = (c.nv == 2) ? 1 : 0;
+    const int i = b >> (2 * v), a = (b >> 1) & v, d = b & v;
+    s0 = (i << v) + a;
+    s1 = (((i + 1 == c.L) ? 0 : i + 1) << v) + d;
   } else {
     int s = b >> 1, x = s / c.L, y = s - x * c.L;
     s0 = s;

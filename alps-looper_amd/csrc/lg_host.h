@@ -152,6 +152,8 @@ class Batch {
   int* node_site = nullptr;
   int* parent = nullptr;
   uint8_t* fbit = nullptr;
+  unsigned long long* fmask = nullptr;
+  unsigned long long* rmask = nullptr;
   ClusterAcc* acc = nullptr;
   unsigned long long* scan_status = nullptr;
   int* scan_ctl = nullptr;

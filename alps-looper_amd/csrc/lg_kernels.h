@@ -51,7 +51,7 @@ struct DevBatch {
   int* chain_node0;            // [nchains] first segment node of each chain (new configuration)
   int* node_site;              // [nodes] site of each segment (batches of several chains) or of each site's first segment
   int* parent;                 // [nodes]
-  uint8_t* fbit;               // [nodes] flip bit of the node's cluster
+  uint8_t* fbit;               // [nodes] flip bit of a cluster at its root (compress mode, field path)
   ClusterAcc* acc;             // [nodes] (roots only)
   unsigned long long* scan_status;  // per scan chunk: epoch | state | value
   int* scan_ctl;                    // [0] chunk ticket, [1] launch epoch
@@ -66,7 +66,6 @@ struct DevBatch {
   int ring_size;
   // replica exchange between neighbouring temperature slots (nslots 0: off)
   int any_field;               // some chain has a longitudinal field
-  int any_split;               // some chain has virtual sites (S=1): segments without wrap, boundary permutation
   int nslots;
   int* xslot;                  // [nslots] chain holding each temperature
   const double* xbeta;         // [nslots] inverse temperatures
@@ -75,10 +74,15 @@ struct DevBatch {
   unsigned long long* xacc;    // [nslots-1] accepted swaps per pair
   uint32_t* xstep;             // exchange steps done
   uint32_t xkey0, xkey1;
+  int op_cap, node_cap;
+  // (fields added later sit here, so that the kernels' argument loads of the
+  // fields above keep their layout and the kernels their register counts)
   // annealing schedules: (start step, temperature) pairs of the chains with
   // sched_n > 0, at their sched_off (k_end_sweep sets beta for the next sweep)
   const double* sched;
-  int op_cap, node_cap;
+  unsigned long long* fmask;   // [nodes / 64 + 1] flip bit of every node's cluster, 64 nodes per word
+  unsigned long long* rmask;   // [nodes / 64 + 1] 1 for the roots, 64 nodes per word
+  int any_split;               // some chain has virtual sites (S=1): segments without wrap, boundary permutation
 };
 
 struct LaunchCfg {

@@ -844,9 +844,54 @@ __global__ void __launch_bounds__(64 * WAVES) k_link(DevBatch B) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// (c) lock-free union-find on segments: hook the larger root under the
+// smaller with atomicCAS, so every root is its cluster's minimum node (the
+// hooking rule of ECL-CC, Jaiganesh & Burtscher, HPDC'18).  parent[x] <= x
+// always holds.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int ld_parent(int* P, int x) {
+  return __hip_atomic_load(P + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Plain walk to the root: shortening the path with stores while walking cost
+// more write traffic than the longer walks it saves (k_estimate points every
+// node at its root once all unions are done).
+__device__ __forceinline__ int uf_find(int* P, int v) {
+  int cur = v, next;
+  while (cur != (next = ld_parent(P, cur))) cur = next;
+  return cur;
+}
+
+__device__ __forceinline__ void uf_unite(int* P, int a, int b) {
+  int ra = uf_find(P, a), rb = uf_find(P, b);
+  while (ra != rb) {
+    if (ra < rb) { int t = ra; ra = rb; rb = t; }
+    // ra > rb: hook ra under rb
+    int old = atomicCAS(P + ra, ra, rb);
+    if (old == ra) return;
+    ra = uf_find(P, old);
+    rb = uf_find(P, rb);
+  }
+}
+
+// every segment its own root, before the unions of k_link_ops
+__global__ void __launch_bounds__(256) k_node_init(DevBatch B) {
+  if (overflowed(B)) return;
+  const int M = B.totals[1];
+  int ilo, ihi;
+  xcd_share(M, ilo, ihi);
+  for (int i = ilo + threadIdx.x; i < ihi; i += blockDim.x) B.parent[i] = i;
+}
+
 // Linking, one thread per operator: for each leg, the rank of the event on
 // its site by binary search in the site's other bond lists (global memory).
 // Used when sites hold few events (a wave per site would idle most lanes).
+// The thread then unites its operator's legs by its graph (the k_uf_union
+// rules) right away: k_node_init made every segment a root beforehand, and
+// the linking itself writes no parent, so the unions of different threads
+// only meet in the lock-free union-find.  This saves the separate union
+// pass, which re-read the four legs and the type of every operator.
 __global__ void __launch_bounds__(256) k_link_ops(DevBatch B) {
   if (overflowed(B)) return;
   const int nops = B.totals[0];
@@ -862,6 +907,7 @@ __global__ void __launch_bounds__(256) k_link_ops(DevBatch B) {
     const uint8_t dflag = bond_dir(P, b) ? EV_DIR1 : 0;
     int s0, s1;
     bond_sites(P, b, s0, s1);
+    int lo_leg[2], up_leg[2];  // lower and upper segment of each side
 #pragma unroll
     for (int side = 0; side < 2; ++side) {
       const int s = side ? s1 : s0;
@@ -892,16 +938,29 @@ __global__ void __launch_bounds__(256) k_link_ops(DevBatch B) {
       B.segspin_new[node] = sp;
       B.evflags[node] = (uint8_t)((info & 1) | (side ? 0 : EV_SRC) | dflag | (rank == 0 ? EV_FIRST : 0) | gflag |
                                   (split && rank == 0 ? EV_NOWRAP : 0));
-      B.parent[node] = node;
       if (B.nchains > 1 || rank == 0) B.node_site[node] = gs;
       if (split && up == n1 - 1) {  // the last event: the site's top segment, up to tau = 1
         B.evt_new[up] = 1.0;
         B.segspin_new[up] = sp ^ (uint8_t)(info & 1);
         B.evflags[up] = gflag;
-        B.parent[up] = up;
         if (B.nchains > 1) B.node_site[up] = gs;
       }
       if (side) { B.l1[k] = node; B.u1[k] = up; } else { B.l0[k] = node; B.u0[k] = up; }
+      lo_leg[side] = node;
+      up_leg[side] = up;
+    }
+    const int g = (info >> 1) & 3;
+    const int l0 = lo_leg[0], u0 = up_leg[0], l1 = lo_leg[1], u1 = up_leg[1];
+    if (g & 2) {            // frozen: all four legs (graph_impl.h:282-283)
+      uf_unite(B.parent, l0, l1);
+      uf_unite(B.parent, u0, u1);
+      uf_unite(B.parent, l0, u0);
+    } else if (g == 0) {    // horizontal cut (graph_impl.h:284-288)
+      uf_unite(B.parent, l0, l1);
+      uf_unite(B.parent, u0, u1);
+    } else {                // cross (graph_impl.h:289-293)
+      uf_unite(B.parent, l0, u1);
+      uf_unite(B.parent, l1, u0);
     }
   }
 }
@@ -996,37 +1055,6 @@ __global__ void __launch_bounds__(256) k_link_site(DevBatch B) {
   int slo, shi;
   xcd_share(B.nsites, slo, shi);
   for (int gs = slo + threadIdx.x; gs < shi; gs += blockDim.x) link_site_events(B, gs);
-}
-
-// ---------------------------------------------------------------------------
-// (c) lock-free union-find on segments: hook the larger root under the
-// smaller with atomicCAS, so every root is its cluster's minimum node (the
-// hooking rule of ECL-CC, Jaiganesh & Burtscher, HPDC'18).  parent[x] <= x
-// always holds.
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ int ld_parent(int* P, int x) {
-  return __hip_atomic_load(P + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Plain walk to the root: shortening the path with stores while walking cost
-// more write traffic than the longer walks it saves (k_estimate points every
-// node at its root once all unions are done).
-__device__ __forceinline__ int uf_find(int* P, int v) {
-  int cur = v, next;
-  while (cur != (next = ld_parent(P, cur))) cur = next;
-  return cur;
-}
-
-__device__ __forceinline__ void uf_unite(int* P, int a, int b) {
-  int ra = uf_find(P, a), rb = uf_find(P, b);
-  while (ra != rb) {
-    if (ra < rb) { int t = ra; ra = rb; rb = t; }
-    // ra > rb: hook ra under rb
-    int old = atomicCAS(P + ra, ra, rb);
-    if (old == ra) return;
-    ra = uf_find(P, old);
-    rb = uf_find(P, rb);
-  }
 }
 
 __global__ void __launch_bounds__(256) k_uf_union(DevBatch B) {
@@ -1133,6 +1161,9 @@ __device__ __forceinline__ int chain_of_node(const DevBatch& B, int node) {
 // and fresh allocations are cleared.  This saves a pass over the nodes; the
 // clearing costs a store per root, so batches of many small clusters keep
 // the compress pass.
+// Each wave publishes the flip bits and root bits of its 64 consecutive
+// segments as one word each (fmask, rmask) for the collector and the
+// operator flips.
 // A segment adds its length to one class slot of its root (ClusterAcc), its
 // tau = 0 boundary (first segment of a site) to one class counter and its
 // winding contributions to one packed 64-bit counter.  Each block handles
@@ -1209,7 +1240,8 @@ __device__ __forceinline__ void ht_add(int* hkey, double (*hL)[HT], unsigned int
 }
 
 template <bool WALK>
-__global__ void __launch_bounds__(EST_THREADS) k_estimate(DevBatch B, int flush_items) {
+// (at most 64 VGPRs: two blocks of 16 waves per CU)
+__global__ void __launch_bounds__(EST_THREADS) __attribute__((amdgpu_waves_per_eu(8))) k_estimate(DevBatch B, int flush_items) {
   if (overflowed(B)) return;
   __shared__ int hkey[HT];
   __shared__ double hL[4][HT];
@@ -1286,17 +1318,22 @@ __global__ void __launch_bounds__(EST_THREADS) k_estimate(DevBatch B, int flush_
 #pragma unroll
           for (int u = 0; u < EST_U; ++u) {
             const int i = ib + u * EST_THREADS;
-            if (!ok[u]) continue;
-            if (r[u] != p0[u]) B.parent[i] = r[u];
-            B.fbit[i] = (uint8_t)f[u];
+            if (ok[u] && r[u] != p0[u]) B.parent[i] = r[u];
           }
         } else {
 #pragma unroll
           for (int u = 0; u < EST_U; ++u) f[u] = B.fbit[r[u]];  // drawn for the root by k_uf_compress
+        }
+        // flip and root bits of the wave's 64 consecutive nodes (the first a
+        // multiple of 64), one word each, for the collector
 #pragma unroll
-          for (int u = 0; u < EST_U; ++u) {
-            const int i = ib + u * EST_THREADS;
-            if (ok[u] && i != r[u]) B.fbit[i] = (uint8_t)f[u];
+        for (int u = 0; u < EST_U; ++u) {
+          const unsigned long long fm = __ballot(ok[u] && f[u] != 0);
+          const unsigned long long rm = __ballot(ok[u] && r[u] == nd[u]);
+          const int w0 = __builtin_amdgcn_readfirstlane(ib) + u * EST_THREADS;
+          if (lane_id() == 0 && g0 + u < items && w0 < M) {
+            B.fmask[w0 >> 6] = fm;
+            B.rmask[w0 >> 6] = rm;
           }
         }
         // a site's first segment wraps through tau = 1: the segment below it is
@@ -1418,7 +1455,7 @@ __global__ void __launch_bounds__(256) k_field_apply(DevBatch B) {
     if (B.chains[c].h == 0.0) continue;
     const uint8_t f = B.fbit[B.parent[i]];
     if (!f) continue;
-    B.fbit[i] = 1;
+    atomicOr(&B.fmask[i >> 6], 1ull << (i & 63));
     B.segspin_new[i] ^= 1;
     if (B.evflags[i] & EV_FIRST) B.spins_new[first_node_site(B, i)] ^= 1;
   }
@@ -1432,6 +1469,10 @@ __global__ void __launch_bounds__(256) k_field_apply(DevBatch B) {
 // chain is written to col_part[j] (no atomics, fixed order), roots of other
 // chains in the chunk (chunks straddling a chain boundary) are added to
 // rec_acc atomically.  end_sweep adds the partials of each chain in order.
+// An operator flips iff the clusters of its source-side legs l0 and u0 flip
+// differently; both bits come from the flip bitmap k_estimate wrote (a
+// 64-times smaller array than a byte per node, so the gathers stay in L2),
+// and the roots come from its root bitmap instead of a parent load per node.
 // ---------------------------------------------------------------------------
 constexpr int COL_THREADS = 256;
 constexpr int COL_MIN_CHUNK = 256 * 4;  // smallest collector chunk (256 * chunk_items)
@@ -1490,16 +1531,17 @@ __global__ void __launch_bounds__(COL_THREADS) k_flip_collect(DevBatch B, int cl
   // (the last operator of the share stands in for those past its end)
   for (int k0 = klo + threadIdx.x; k0 < khi; k0 += COL_U * COL_THREADS) {
     int kk[COL_U], a[COL_U], b[COL_U], t[COL_U];
-    uint8_t fa[COL_U], fb[COL_U];
+    unsigned long long fa[COL_U], fb[COL_U];
 #pragma unroll
     for (int u = 0; u < COL_U; ++u) kk[u] = min(k0 + u * COL_THREADS, khi - 1);
 #pragma unroll
     for (int u = 0; u < COL_U; ++u) { a[u] = B.l0[kk[u]]; b[u] = B.u0[kk[u]]; t[u] = B.oi_new[kk[u]]; }
 #pragma unroll
-    for (int u = 0; u < COL_U; ++u) { fa[u] = B.fbit[a[u]]; fb[u] = B.fbit[b[u]]; }
+    for (int u = 0; u < COL_U; ++u) { fa[u] = B.fmask[a[u] >> 6]; fb[u] = B.fmask[b[u] >> 6]; }
 #pragma unroll
     for (int u = 0; u < COL_U; ++u)
-      if (k0 + u * COL_THREADS < khi && (fa[u] ^ fb[u])) B.oi_new[kk[u]] = t[u] ^ 1;
+      if (k0 + u * COL_THREADS < khi && (((fa[u] >> (a[u] & 63)) ^ (fb[u] >> (b[u] & 63))) & 1))
+        B.oi_new[kk[u]] = t[u] ^ 1;
   }
   // per-chunk sums by chain: slot k holds chain c_first + k (chains are
   // contiguous in node order, so a chunk meets few of them)
@@ -1537,7 +1579,8 @@ __global__ void __launch_bounds__(COL_THREADS) k_flip_collect(DevBatch B, int cl
     for (int it0 = 0; it0 < citems; it0 += COL_U) {
       const int ib = base + it0 * COL_THREADS + threadIdx.x;
       if (__builtin_amdgcn_readfirstlane(ib) >= M) break;  // lane 0 holds the wave's lowest node
-      int nd[COL_U], par[COL_U];
+      int nd[COL_U];
+      unsigned long long rw[COL_U];
       bool ok[COL_U], root[COL_U];
 #pragma unroll
       for (int u = 0; u < COL_U; ++u) {
@@ -1545,16 +1588,16 @@ __global__ void __launch_bounds__(COL_THREADS) k_flip_collect(DevBatch B, int cl
         nd[u] = ok[u] ? ib + u * COL_THREADS : min(ib, M - 1);
       }
 #pragma unroll
-      for (int u = 0; u < COL_U; ++u) par[u] = B.parent[nd[u]];
+      for (int u = 0; u < COL_U; ++u) rw[u] = B.rmask[nd[u] >> 6];
 #pragma unroll
-      for (int u = 0; u < COL_U; ++u) root[u] = ok[u] && par[u] == nd[u];
+      for (int u = 0; u < COL_U; ++u) root[u] = ok[u] && ((rw[u] >> (nd[u] & 63)) & 1);
       ClusterAcc ac[COL_U];
       uint8_t fr[COL_U];
 #pragma unroll
       for (int u = 0; u < COL_U; ++u) {
         const int j = root[u] ? nd[u] : base;
         ac[u] = B.acc[j];
-        fr[u] = B.fbit[j];
+        fr[u] = (uint8_t)((B.fmask[j >> 6] >> (j & 63)) & 1);
       }
 #pragma unroll
       for (int u = 0; u < COL_U; ++u) {
@@ -1816,6 +1859,7 @@ void launch_cluster_stage(const DevBatch& B, const LaunchCfg& L, hipStream_t st,
   // 1536-event kernel keeps the per-node grid (square 512^2, beta=512: 5.76
   // ms at 8 steps per block, 5.95 at one)
   if (L.link_cap == 0) {
+    hipLaunchKernelGGL(k_node_init, dim3(L.blocks_node), dim3(256), 0, st, B);
     hipLaunchKernelGGL(k_link_ops, dim3(L.blocks_op), dim3(256), 0, st, B);
   } else if (L.link_cap == 1) {
     const int blocks = max(1, min(L.blocks_op, (B.nsites + 255) / 256));
@@ -1835,7 +1879,8 @@ void launch_cluster_stage(const DevBatch& B, const LaunchCfg& L, hipStream_t st,
     const int blocks = max(1, min(L.blocks_op, (B.nsites + 255) / 256));
     hipLaunchKernelGGL(k_boundary, dim3(blocks), dim3(256), 0, st, B);
   }
-  hipLaunchKernelGGL(k_uf_union, dim3(L.blocks_op), dim3(256), 0, st, B);
+  // (the per-operator linking kernel unites as it links)
+  if (L.link_cap != 0) hipLaunchKernelGGL(k_uf_union, dim3(L.blocks_op), dim3(256), 0, st, B);
   mark(ev, 2, st);
   if (L.compress) {
     hipLaunchKernelGGL(k_uf_compress, dim3(L.blocks_node), dim3(256), 0, st, B);

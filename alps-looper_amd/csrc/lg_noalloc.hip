@@ -39,7 +39,7 @@ constexpr size_t al(size_t x) { return (x + 255) & ~(size_t)255; }
 // byte offsets of the regions in the array
 struct Layout {
   size_t chains, bond_chain, site_chain, spins_old, spins_new, counts, bo_new, ot_new, oi_new, kc_new;
-  size_t l0, l1, u0, u1, soff_new, evt_new, segspin_new, evflags, node_site, parent, fbit, acc;
+  size_t l0, l1, u0, u1, soff_new, evt_new, segspin_new, evflags, node_site, parent, fbit, fmask, rmask, acc;
   size_t chain_node0, scan_status, ctl, col_part, rec_acc, rec_ring, canon, roots, total;
 };
 
@@ -68,6 +68,8 @@ constexpr Layout make_layout() {
   L.node_site = take(4 * NA_NODES);
   L.parent = take(4 * NA_NODES);
   L.fbit = take(NA_NODES);
+  L.fmask = take(8 * (NA_NODES / 64 + 1));
+  L.rmask = take(8 * (NA_NODES / 64 + 1));
   L.acc = take(sizeof(ClusterAcc) * NA_NODES);
   L.chain_node0 = take(4);
   L.scan_status = take(8 * NA_SCAN);
@@ -152,6 +154,8 @@ DevBatch na_dev(unsigned char* base, int N, int Nb, bool field) {
   B.node_site = (int*)(base + L.node_site);
   B.parent = (int*)(base + L.parent);
   B.fbit = (uint8_t*)(base + L.fbit);
+  B.fmask = (unsigned long long*)(base + L.fmask);
+  B.rmask = (unsigned long long*)(base + L.rmask);
   B.acc = (ClusterAcc*)(base + L.acc);
   B.scan_status = (unsigned long long*)(base + L.scan_status);
   B.scan_ctl = ctl;

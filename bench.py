@@ -197,7 +197,18 @@ def cpu_pool_baseline(chains, seconds, what, untimed=8, max_sweeps=64):
     }
 
 
-def measured_traffic(name, kernel, nop_mean):
+def device_kernels(kernel, link):
+    """device kernels behind a profiled sweep stage under linking setting
+    `link` (0: k_node_init + k_link_ops, which also unites; 1: k_link_site;
+    else the LDS-staged k_link), by their short names"""
+    if kernel == "link":
+        return ("node_init", "link_ops") if link == 0 else ("link_site",) if link == 1 else ("link",)
+    if kernel == "uf_union":
+        return () if link == 0 else ("uf_union",)
+    return (kernel,)
+
+
+def measured_traffic(name, kernel, nop_mean, link=0):
     """HBM bytes per launch of `kernel` from the committed FETCH_SIZE/WRITE_SIZE
     summary (profiles/<name>.json, copy-kernel calibrated), scaled to nop_mean"""
     if not name:
@@ -206,23 +217,28 @@ def measured_traffic(name, kernel, nop_mean):
     try:
         with open(path) as f:
             d = json.load(f)
-        return d["kernels"][kernel]["bytes_per_op"] * nop_mean
+        ks = device_kernels(kernel, link)
+        return sum(d["kernels"][k]["bytes_per_op"] for k in ks) * nop_mean if ks else None
     except (OSError, KeyError, TypeError):
         return None
 
 
-def kernel_bytes(compress):
+def kernel_bytes(compress, link=0):
     """algorithmic bytes per operator of each kernel: without the compress
-    pass the estimator's own walk does the compression"""
+    pass the estimator's own walk does the compression; the per-operator
+    linking kernel (link 0) also does the unions"""
     b = dict(BYTES_PER_OP)
     if not compress:
         b["estimate"] += b["uf_compress"]
         b["uf_compress"] = 0.0
+    if link == 0:
+        b["link"] += b["uf_union"]
+        b["uf_union"] = 0.0
     return b
 
 
-def roofline(prof, nop_mean, traffic_profile=None, compress=1):
-    kb = kernel_bytes(compress)
+def roofline(prof, nop_mean, traffic_profile=None, compress=1, link=0):
+    kb = kernel_bytes(compress, link)
     dom = max(prof, key=lambda k: prof[k])
     ms = prof[dom]
     bytes_launch = kb[dom] * nop_mean
@@ -230,9 +246,9 @@ def roofline(prof, nop_mean, traffic_profile=None, compress=1):
     return {
         "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
         "frac": achieved / HBM_PEAK_GBS,
-        "traffic": measured_traffic(traffic_profile, dom, nop_mean),
-        "kernel": dom, "kernel_ms": ms, "bytes_per_op": kb[dom],
-        "bytes_per_launch": bytes_launch,
+        "traffic": measured_traffic(traffic_profile, dom, nop_mean, link),
+        "kernel": dom, "device_kernels": list(device_kernels(dom, link)), "kernel_ms": ms,
+        "bytes_per_op": kb[dom], "bytes_per_launch": bytes_launch,
     }
 
 
@@ -241,9 +257,9 @@ STAGES = {"diagonal update": ("diag_count", "scan", "diag_fill"),
           "estimators": ("estimate",), "flip + collect": ("flip_collect", "end_sweep")}
 
 
-def stage_gbs(prof, nop_mean, compress=1):
+def stage_gbs(prof, nop_mean, compress=1, link=0):
     """algorithmic GB/s of each stage (per-stage bytes per operator-sweep)"""
-    kb = kernel_bytes(compress)
+    kb = kernel_bytes(compress, link)
     out = {}
     for st, ks in STAGES.items():
         ms = sum(prof[k] for k in ks)
@@ -281,7 +297,7 @@ def run_single(args, lg, kind, L, beta, traffic_profile=None, workload=None):
     value = nop_total / dt
     log("profile %d sweeps" % args.profile_sweeps)
     prof = ch.profile(args.profile_sweeps)
-    comp = ch.launch()["compress"]
+    comp, lk = ch.launch()["compress"], ch.launch()["link"]
     N = L if kind == lg.CHAIN else L * L
     cfg = {"workload": workload or ("%s periodic S=1/2 Heisenberg AF, J=1, beta=%g, one chain, plain GPU path"
                                     % ("chain L=%d" % L if kind == lg.CHAIN else "square %dx%d" % (L, L), beta)),
@@ -295,8 +311,8 @@ def run_single(args, lg, kind, L, beta, traffic_profile=None, workload=None):
         "energy_density": float(rec[:, 2].mean() / N),
         "sweep_hbm_frac": value * SWEEP_BYTES_PER_OP / (HBM_PEAK_GBS * 1e9),
         "kernel_ms": prof,
-        "stages": stage_gbs(prof, nop_mean, comp),
-        "roofline": roofline(prof, nop_mean, traffic_profile, comp),
+        "stages": stage_gbs(prof, nop_mean, comp, lk),
+        "roofline": roofline(prof, nop_mean, traffic_profile, comp, lk),
     })
     if not args.no_cpu:
         log("cpu baseline %.0f s" % args.cpu_seconds)
@@ -320,8 +336,27 @@ def scan_params(lg, n=512, L=32):
 
 def run_scan(args, lg):
     plist = scan_params(lg)
-    b = lg.Batch(plist)
     equil = args.equilibrate if args.equilibrate is not None else 400
+    # plain path first: the same chains one chain (and one launch per stage)
+    # at a time -- every 8th chain of the scan, equilibrated like the batch --
+    # so that the batch's profiled sweeps are the workload's last launches
+    sub = plist[::8]
+    plain_sweeps = 20
+    log("scan: plain path, %d chains, %d equilibration + %d timed sweeps each" % (len(sub), equil, plain_sweeps))
+    chains = []
+    for c in sub:
+        ch = lg.Chain(c)
+        ch.sweep(equil, records=False)
+        chains.append(ch)
+    t0 = time.perf_counter()
+    nops = 0.0
+    for ch in chains:
+        nops += float(ch.sweep(plain_sweeps)[:, 0].sum())
+    tp = time.perf_counter() - t0
+    plain = nops / tp
+    for ch in chains:
+        ch.close()
+    b = lg.Batch(plist)
     log("scan: equilibrate %d batched sweeps" % equil)
     run_chunked(lambda k: b.run(k, records=False), equil, 200)
     log("scan: tune")
@@ -347,24 +382,6 @@ def run_scan(args, lg):
     log("scan: profile %d sweeps" % args.profile_sweeps)
     prof = b.profile(args.profile_sweeps)
     launch = b.launch()
-    # plain path: the same chains one chain (and one launch per stage) at a
-    # time -- every 8th chain of the scan, equilibrated like the batch
-    sub = plist[::8]
-    plain_sweeps = 20
-    log("scan: plain path, %d chains, %d equilibration + %d timed sweeps each" % (len(sub), equil, plain_sweeps))
-    chains = []
-    for c in sub:
-        ch = lg.Chain(c)
-        ch.sweep(equil, records=False)
-        chains.append(ch)
-    t0 = time.perf_counter()
-    nops = 0.0
-    for ch in chains:
-        nops += float(ch.sweep(plain_sweeps)[:, 0].sum())
-    tp = time.perf_counter() - t0
-    plain = nops / tp
-    for ch in chains:
-        ch.close()
     cfg = {"workload": "512-chain temperature scan: S=1/2 Heisenberg AF on the 32x32 periodic square lattice, "
                        "J=1, T_i=0.05*40^(i/511), chain i keyed (23125, i); batched (one launch per stage for all "
                        "chains), HIP-graph sweep, autotuned launch settings",
@@ -383,9 +400,9 @@ def run_scan(args, lg):
         "energy_density_coldest_hottest": [float(E[0]), float(E[-1])],
         "launch": launch,
         "kernel_ms": prof,
-        "stages": stage_gbs(prof, nop_sweep, launch["compress"]),
+        "stages": stage_gbs(prof, nop_sweep, launch["compress"], launch["link"]),
         "sweep_hbm_frac": batched * SWEEP_BYTES_PER_OP / (HBM_PEAK_GBS * 1e9),
-        "roofline": roofline(prof, nop_sweep, "c3_scan512_hbm_traffic", launch["compress"]),
+        "roofline": roofline(prof, nop_sweep, "c3_scan512_hbm_traffic", launch["compress"], launch["link"]),
     })
     b.close()
     if not args.no_cpu:
