@@ -52,7 +52,7 @@ struct ChainDev {
   int site_base, bond_base;          // layout of the configuration being built
   int old_site_base, old_bond_base;  // layout of the previous configuration
   int cand_cap;                      // candidate slots per bond
-  long long cand_base;               // first candidate slot of bond 0
+  long long cand_base;               // the chain's candidate slots: slot j of bond b at cand_base + j * Nb + b
   double beta, scale;                // scale = 2^52 / (beta * sum_g v_g)
   double offset_total;               // sum_b offset_b (energy.h / weight_impl.h)
   uint64_t thr[3];                   // graph thresholds: g = first g with x < thr[g], else 3

@@ -84,6 +84,15 @@ __device__ __forceinline__ int lower_bound_d(const double* a, int n, double x) {
   }
   return lo;
 }
+// first j in [0,n) with a[j * stride] >= x (n if none)
+__device__ __forceinline__ int lower_bound_ds(const double* a, long long stride, int n, double x) {
+  int lo = 0, len = n;
+  while (len > 0) {
+    int h = len >> 1;
+    if (a[(lo + h) * stride] < x) { lo += h + 1; len -= h + 1; } else { len = h; }
+  }
+  return lo;
+}
 // first j in [0,n) with a[j] > x (n if none)
 __device__ __forceinline__ int upper_bound_d(const double* a, int n, double x) {
   int lo = 0, len = n;
@@ -220,7 +229,8 @@ __device__ __forceinline__ void scan_advance_epoch(const DevBatch& B) {
 // (a) diagonal update, pass 1: count and store accepted candidates
 // ---------------------------------------------------------------------------
 template <int G>
-__global__ void __launch_bounds__(256) k_diag_count(DevBatch B) {
+// (at most 96 VGPRs, five waves per SIMD, where that needs no spills)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(G == 32 ? 4 : 5))) k_diag_count(DevBatch B) {
   if (blockIdx.x == 0 && threadIdx.x == 0) scan_advance_epoch(B);  // for k_scan, which follows
   if (overflowed(B)) return;
   const int groups_per_block = blockDim.x / G;
@@ -252,7 +262,10 @@ __global__ void __launch_bounds__(256) k_diag_count(DevBatch B) {
     // Poisson candidates on [0,1): exponential gaps on the 2^-52 grid
     int acc = 0;
     if (P.has_weight) {
-      const long long cb = P.cand_base + (long long)b * P.cand_cap;
+      // slot j of the bond at j * Nb (a chain holds fewer than 2^31 slots):
+      // the bonds of a wave write adjacent words
+      double* const ctb = B.cand_t + (P.cand_base + b);
+      uint8_t* const cgb = B.cand_g + (P.cand_base + b);
       const int base0 = B.soff_old[P.old_site_base + s0], m0 = B.soff_old[P.old_site_base + s0 + 1] - base0;
       const int base1 = B.soff_old[P.old_site_base + s1], m1 = B.soff_old[P.old_site_base + s1 + 1] - base1;
       int cur0 = 0, cur1 = 0;  // world-line cursors (candidate times ascend)
@@ -296,8 +309,8 @@ __global__ void __launch_bounds__(256) k_diag_count(DevBatch B) {
         if (accept) {
           int slot = acc + __popcll(m & below);
           if (slot < P.cand_cap) {
-            B.cand_t[cb + slot] = tau;
-            B.cand_g[cb + slot] = (uint8_t)g;
+            ctb[slot * P.Nb] = tau;
+            cgb[slot * P.Nb] = (uint8_t)g;
           } else {
             atomicOr(B.flags, FLAG_CAND_OVERFLOW);
           }
@@ -502,11 +515,13 @@ __global__ void __launch_bounds__(256) k_diag_fill(DevBatch B, int fill_cap) {
     const int nb = B.bo_new[gb];
     const int acc = B.acc_count[gb];
     const int kept = B.counts[gb] - acc;
-    const long long cb = P.cand_base + (long long)b * P.cand_cap;
+    const long long cb = P.cand_base + b;  // slot j at cb + j * Nb
+    const long long cs = P.Nb;
     const double* ct = B.cand_t + cb;
     const int info_b = gb << 3;
-    // tc: accepted candidates' times; to: the old operators' times
-    auto place = [&](const double* tc, const double* to) {
+    // tc: accepted candidates' times (slot j at tc[j * ts]); to: the old
+    // operators' times
+    auto place = [&](const double* tc, long long ts, const double* to) {
       // kept operators: regraph (graph_impl.h:323-327) and place
       for (int r = o0 + gl; r < o1; r += G) {
         if (!(B.oi_old[r] & 1)) continue;
@@ -514,15 +529,15 @@ __global__ void __launch_bounds__(256) k_diag_fill(DevBatch B, int fill_cap) {
         const double tau = B.ot_old[r];
         U4 x = philox(sweep, STAGE_REGRAPH, (uint32_t)b, (uint32_t)kr, P.key0, P.key1);
         int g = ((uint64_t)x.x < P.toff) ? 0 : 1;
-        int pos = nb + kr + lower_bound_d(tc, acc, tau);
+        int pos = nb + kr + lower_bound_ds(tc, ts, acc, tau);
         B.ot_new[pos] = tau;
         B.oi_new[pos] = info_b | (g << 1) | 1;
         B.kc_new[pos] = kr;
       }
       // accepted candidates: new diagonal operators
       for (int j = gl; j < acc; j += G) {
-        const double tau = tc[j];
-        const int g = B.cand_g[cb + j];
+        const double tau = tc[j * ts];
+        const int g = B.cand_g[cb + j * cs];
         int r = lower_bound_d(to, o1 - o0, tau) + o0;
         int kk = (r < o1) ? B.kpre[r] : kept;
         int pos = nb + j + kk;
@@ -539,16 +554,16 @@ __global__ void __launch_bounds__(256) k_diag_fill(DevBatch B, int fill_cap) {
         double* lc = fill_lists + (threadIdx.x >> 6) * 2 * fill_cap;
         double* lo = lc + fill_cap;
         __builtin_amdgcn_wave_barrier();  // the previous bond's searches are done
-        for (int q = gl; q < acc; q += 64) lc[q] = ct[q];
+        for (int q = gl; q < acc; q += 64) lc[q] = ct[q * cs];
         for (int q = gl; q < o1 - o0; q += 64) lo[q] = B.ot_old[o0 + q];
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        place(lc, lo);
+        place(lc, 1, lo);
         continue;
       }
     }
-    place(ct, B.ot_old + o0);
+    place(ct, cs, B.ot_old + o0);
   }
 }
 
