@@ -62,10 +62,10 @@ static void xxz_weights(double jxy_in, double jz, double a, double v[4], double*
 int check_params(const lg_params& p) {
   if (p.lattice != LG_LATTICE_CHAIN && p.lattice != LG_LATTICE_SQUARE)
     return set_error(LG_ERR_ARG, "lattice must be chain (0) or square (1)");
-  if (p.two_s < 0 || p.two_s > 2)
-    return set_error(LG_ERR_ARG, "two_s (2S) must be 1 (S=1/2; 0 means 1) or 2 (S=1)");
-  if (p.two_s == 2 && p.lattice != LG_LATTICE_CHAIN)
-    return set_error(LG_ERR_ARG, "S=1 (virtual sites) is built for the chain lattice only");
+  if (p.two_s < 0 || p.two_s > 4)
+    return set_error(LG_ERR_ARG, "two_s (2S) must be 1 (S=1/2; 0 means 1), 2, 3 or 4");
+  if (p.two_s > 2 && p.lattice != LG_LATTICE_CHAIN)
+    return set_error(LG_ERR_ARG, "S > 1 (virtual sites) is built for the chain lattice; the square lattice takes S <= 1");
   if (p.L < 2 || (p.L & 1))
     return set_error(LG_ERR_ARG, "L must be even and >= 2 (bipartite, sign-free lattice)");
   if (p.lattice == LG_LATTICE_SQUARE && (long long)p.L * p.L * 2 > (1ll << 27))
@@ -82,8 +82,8 @@ ChainDev make_chain_dev(const lg_params& p) {
   std::memset(&d, 0, sizeof(d));
   d.kind = p.lattice;
   d.L = p.L;
-  // S=1: two virtual sites per real site, four virtual bonds per real bond
-  d.nv = (p.two_s == 2) ? 2 : 1;
+  // S > 1/2: 2S virtual sites per real site, (2S)^2 virtual bonds per real bond
+  d.nv = (p.two_s >= 2) ? p.two_s : 1;
   d.N = d.nv * ((p.lattice == LG_LATTICE_CHAIN) ? p.L : p.L * p.L);
   d.Nb = d.nv * d.nv * ((p.lattice == LG_LATTICE_CHAIN) ? p.L : 2 * p.L * p.L);
   d.dim = (p.lattice == LG_LATTICE_CHAIN) ? 1 : 2;
@@ -384,6 +384,14 @@ int Batch::set_layout(std::vector<ChainDev>& ds, const std::vector<uint64_t>& sw
     d.site_base = (int)sites;
     d.bond_base = (int)bonds;
     d.cand_base = cand;
+    // candidate slots slot-major (slot j of bond b at j * Nb + b) when bonds
+    // take few candidates, so that the bonds of a wave write adjacent words
+    // (512-chain scan: diag_fill 0.124 -> 0.108 ms); bond-major when they
+    // take many, so that a bond's slots share lines (square 128^2, beta=128,
+    // about 180 slots per bond: slot-major cost diag_count 0.081 -> 0.084 ms)
+    const bool slot_major = d.cand_cap <= 128;
+    d.cand_bstride = slot_major ? 1 : d.cand_cap;
+    d.cand_sstride = slot_major ? d.Nb : 1;
     sites += d.N;
     bonds += d.Nb;
     cand += (long long)d.Nb * d.cand_cap;
@@ -454,14 +462,17 @@ int Batch::set_layout(std::vector<ChainDev>& ds, const std::vector<uint64_t>& sw
   // occupancy.
   if (nb_total > 0) ops_bond /= nb_total;
   c.fill_cap = 1.25 * ops_bond <= 256 ? 256 : (1.25 * ops_bond <= 512 ? 512 : 1024);
-  bool split = false;
-  for (const auto& d : chains) split = split || d.nv > 1;
+  bool split = false, many_lists = false;
+  for (const auto& d : chains) {
+    split = split || d.nv > 1;
+    many_lists = many_lists || site_lists(d) > 4;
+  }
   if (!tuned) {
     const double per_site = nsites > 0 ? ev / nsites : 0;
     c.link_cap = per_site < 48 ? 0 : (per_site < 96 ? 128 : (per_site <= 400 ? 512 : 1536));
     // virtual sites: only the per-operator and per-site kernels write their
-    // separate top segments
-    if (split) c.link_cap = per_site < 8 ? 0 : 1;
+    // separate top segments, and the per-site kernel merges four lists at most
+    if (split) c.link_cap = (per_site < 8 || many_lists) ? 0 : 1;
     // the estimator's LDS table: hot configurations (many one- and
     // two-segment clusters, which overflow the table) flush it every 1024
     // segments (512-chain scan: estimate 0.172 -> 0.166 ms); cold ones once
@@ -484,6 +495,11 @@ int Batch::set_layout(std::vector<ChainDev>& ds, const std::vector<uint64_t>& sw
     const double want = std::max(256.0, ev / 256.0);
     c.blocks_op = c.blocks_node = std::max(64, (int)std::min<double>(cus * mult, want));
   }
+  // tuned settings meet chains they were not tuned for when a server admits
+  // jobs: the LDS-staged linking kernels need wrapping segments, the
+  // per-site one at most four lists per site
+  if (split && c.link_cap > 1) c.link_cap = 0;
+  if (many_lists && c.link_cap == 1) c.link_cap = 0;
   return apply_cfg(c);
 }
 
@@ -729,6 +745,7 @@ int Batch::tune(uint32_t sweeps_per_trial) {
       else if (pass == 1) trial.blocks_op = trial.blocks_node = cus * mults[i];
       else if (pass == 2) {
         if (links[i] > 1 && has_virtual_sites()) continue;  // these kernels assume wrapping segments
+        if (links[i] == 1 && max_site_lists() > 4) continue;  // it merges four lists at most
         trial.link_cap = links[i];
       }
       else if (pass == 3) trial.compress = i;
@@ -772,7 +789,9 @@ int Batch::set_launch(const lg_launch& l) {
     if (l.link != 0 && l.link != 1 && l.link != 128 && l.link != 512 && l.link != 1536)
       return set_error(LG_ERR_ARG, "launch link must be 0, 1, 128, 512 or 1536");
     if (l.link > 1 && has_virtual_sites())
-      return set_error(LG_ERR_ARG, "chains with virtual sites (S=1) take linking kernel 0 or 1");
+      return set_error(LG_ERR_ARG, "chains with virtual sites (S > 1/2) take linking kernel 0 or 1");
+    if (l.link == 1 && max_site_lists() > 4)
+      return set_error(LG_ERR_ARG, "sites with more than four bond lists (S > 1 on the chain, S = 1 on the square lattice) take linking kernel 0");
     c.link_cap = l.link;
   }
   if (l.compress >= 0) {

@@ -52,7 +52,7 @@ struct ChainDev {
   int site_base, bond_base;          // layout of the configuration being built
   int old_site_base, old_bond_base;  // layout of the previous configuration
   int cand_cap;                      // candidate slots per bond
-  long long cand_base;               // the chain's candidate slots: slot j of bond b at cand_base + j * Nb + b
+  long long cand_base;               // the chain's candidate slots: slot j of bond b at cand_base + b * cand_bstride + j * cand_sstride
   double beta, scale;                // scale = 2^52 / (beta * sum_g v_g)
   double offset_total;               // sum_b offset_b (energy.h / weight_impl.h)
   uint64_t thr[3];                   // graph thresholds: g = first g with x < thr[g], else 3
@@ -62,7 +62,8 @@ struct ChainDev {
   double h;                          // longitudinal field (site term -h S^z)
   double wsum;                       // sum_g v_g of a bond (candidate rate beta * wsum)
   int sched_n, sched_off;            // annealing schedule: entries and offset in the batch's table (0: fixed beta)
-  int nv;                            // virtual sites per real site (2S; 1 for S=1/2, 2 for S=1 on the chain)
+  int nv;                            // virtual sites per real site (2S; 1 for S=1/2)
+  int cand_bstride, cand_sstride;    // candidate slot strides per bond and per slot
 };
 
 // Temperature of Monte Carlo step `step` on an annealing schedule of n
@@ -224,20 +225,22 @@ LG_HD int choose_graph(uint32_t x, const ChainDev& c) {
 // chain: bond b = (b, b+1 mod L); site s has bonds {s (source), s-1 (target)}
 // square: site s = x*L + y; bond 2s = (s, s+y^) [dir 1], 2s+1 = (s, s+x^) [dir 0];
 //         site s has bonds {2s, 2s+1 (source), 2(s-y^), 2(s-x^)+1 (target)}
-// S=1 chain (two virtual sites per real site, path_integral.C:589-615):
-//         virtual site v = 2s + a; real bond i = (i, i+1) carries the four
-//         virtual bonds 4i + 2a + c = (2i + a, 2(i+1) + c); virtual site v has
-//         bonds {4s+2a, 4s+2a+1 (source), 4(s-1)+a, 4(s-1)+2+a (target)}
-LG_HD void bond_sites(const ChainDev& c, int b, int& s0, int& s1) {
+// Spin S > 1/2 (path_integral.C:589-615, looper's virtual lattice,
+// lattice.h:104-175): real site r carries nv = 2S virtual sites v = r*nv + a
+// (a < nv); real bond rb = (rs, rt) carries the nv^2 virtual bonds
+// rb*nv^2 + a*nv + c = (rs*nv + a, rt*nv + c), source-major as looper's
+// virtual_graph orders them.  Virtual site v = r*nv + a has nv*z bond lists
+// (z = 2 on the chain, 4 on the square lattice): for the j-th incident real
+// bond rb of r (the real order above: source bonds first) and c < nv,
+// list j*nv + c is bond rb*nv^2 + a*nv + c (r the source) or
+// rb*nv^2 + c*nv + a (r the target).  Every virtual site carries its real
+// site's gauge.
+
+// sites of a real bond (real lattice)
+LG_HD void real_bond_sites(const ChainDev& c, int b, int& s0, int& s1) {
   if (c.kind == 0) {
-    // v = log2(virtual sites per site): bond 4i + 2a + d joins virtual site
-    // a of site i to virtual site d of site i+1 (v = 1); bond i joins sites
-    // i and i+1 (v = 0: a = d = 0).  One formula keeps the kernels' register
-    // count that of the plain chain.
-    const int v = (c.nv == 2) ? 1 : 0;
-    const int i = b >> (2 * v), a = (b >> 1) & v, d = b & v;
-    s0 = (i << v) + a;
-    s1 = (((i + 1 == c.L) ? 0 : i + 1) << v) + d;
+    s0 = b;
+    s1 = (b + 1 == c.L) ? 0 : b + 1;
   } else {
     int s = b >> 1, x = s / c.L, y = s - x * c.L;
     s0 = s;
@@ -246,27 +249,59 @@ LG_HD void bond_sites(const ChainDev& c, int b, int& s0, int& s1) {
   }
 }
 
-LG_HD int bond_dir(const ChainDev& c, int b) { return (c.kind == 0) ? 0 : ((b & 1) ? 0 : 1); }
+LG_HD void bond_sites(const ChainDev& c, int b, int& s0, int& s1) {
+  if (c.nv == 1) {
+    real_bond_sites(c, b, s0, s1);
+  } else {
+    const int nn = c.nv * c.nv;
+    const int rb = b / nn, q = b - rb * nn, a = q / c.nv, d = q - a * c.nv;
+    int r0, r1;
+    real_bond_sites(c, rb, r0, r1);
+    s0 = r0 * c.nv + a;
+    s1 = r1 * c.nv + d;
+  }
+}
 
-// bonds incident to site s; returns their count (2 or 4).  The four entries
-// are chosen into scalars and stored once: stores of different entries on
-// different paths get merged into an indexed store, which puts the caller's
-// array in scratch memory.
+LG_HD int bond_dir(const ChainDev& c, int b) {
+  return (c.kind == 0) ? 0 : (((b / (c.nv * c.nv)) & 1) ? 0 : 1);
+}
+
+// bond lists of a site: nv * (2 on the chain, 4 on the square lattice)
+LG_HD int site_lists(const ChainDev& c) { return c.nv * (c.kind == 0 ? 2 : 4); }
+
+// real bond j of real site r (source bonds first: chain {r, r-1}; square
+// {2r, 2r+1, 2(r - y^), 2(r - x^) + 1})
+LG_HD int real_site_bond(const ChainDev& c, int r, int j) {
+  if (c.kind == 0) return j == 0 ? r : ((r == 0) ? c.L - 1 : r - 1);
+  const int x = r / c.L, y = r - x * c.L;
+  if (j < 2) return 2 * r + j;
+  if (j == 2) return 2 * (x * c.L + ((y == 0) ? c.L - 1 : y - 1));
+  return 2 * (((x == 0) ? c.L - 1 : x - 1) * c.L + y) + 1;
+}
+
+// bond list i of site s (i < site_lists(c))
+LG_HD int site_bond(const ChainDev& c, int s, int i) {
+  if (c.nv == 1) return real_site_bond(c, s, i);
+  const int r = s / c.nv, a = s - r * c.nv;
+  const int j = i / c.nv, q = i - j * c.nv;
+  const int rb = real_site_bond(c, r, j);
+  const bool source = j < (c.kind == 0 ? 1 : 2);
+  return rb * c.nv * c.nv + (source ? a * c.nv + q : q * c.nv + a);
+}
+
+// bonds incident to site s when it has at most four lists (S=1/2, and S=1
+// on the chain); returns their count.  The four entries are chosen into
+// scalars and stored once: stores of different entries on different paths
+// get merged into an indexed store, which puts the caller's array in
+// scratch memory.
 LG_HD int site_bonds(const ChainDev& c, int s, int* bl) {
   int b0, b1, b2, b3, n;
-  if (c.kind == 0 && c.nv == 2) {
-    const int r = s >> 1, a = s & 1, rm = (r == 0) ? c.L - 1 : r - 1;
-    b0 = 4 * r + 2 * a;
-    b1 = b0 + 1;
-    b2 = 4 * rm + a;
-    b3 = b2 + 2;
-    n = 4;
-  } else if (c.kind == 0) {
+  if (c.nv == 1 && c.kind == 0) {
     b0 = s;
     b1 = (s == 0) ? c.L - 1 : s - 1;
     b2 = b3 = -1;  // unused
     n = 2;
-  } else {
+  } else if (c.nv == 1) {
     const int x = s / c.L, y = s - x * c.L;
     const int ym = (y == 0) ? c.L - 1 : y - 1;
     const int xm = (x == 0) ? c.L - 1 : x - 1;
@@ -275,6 +310,12 @@ LG_HD int site_bonds(const ChainDev& c, int s, int* bl) {
     b2 = 2 * (x * c.L + ym);
     b3 = 2 * (xm * c.L + y) + 1;
     n = 4;
+  } else {
+    n = site_lists(c);
+    b0 = site_bond(c, s, 0);
+    b1 = site_bond(c, s, 1);
+    b2 = n > 2 ? site_bond(c, s, 2) : -1;
+    b3 = n > 3 ? site_bond(c, s, 3) : -1;
   }
   bl[0] = b0;
   bl[1] = b1;
@@ -284,9 +325,9 @@ LG_HD int site_bonds(const ChainDev& c, int s, int* bl) {
 }
 
 LG_HD int gauge(const ChainDev& c, int s) {
-  if (c.kind == 0 && c.nv == 2) return ((s >> 1) & 1) ? -1 : 1;   // the real site's
-  if (c.kind == 0) return (s & 1) ? -1 : 1;
-  int x = s / c.L, y = s - x * c.L;
+  const int r = (c.nv == 1) ? s : s / c.nv;
+  if (c.kind == 0) return (r & 1) ? -1 : 1;
+  int x = r / c.L, y = r - x * c.L;
   return ((x + y) & 1) ? -1 : 1;
 }
 

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -81,6 +82,11 @@ class Batch {
   bool has_virtual_sites() const {
     for (const auto& d : chains) if (d.nv > 1) return true;
     return false;
+  }
+  int max_site_lists() const {
+    int n = 0;
+    for (const auto& d : chains) n = std::max(n, site_lists(d));
+    return n;
   }
   // annealing schedule of a single-chain batch: n (start step, temperature)
   // pairs (looper/temperature.h); n == 0 removes it.  Rebuilds the device

@@ -262,10 +262,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(G == 3
     // Poisson candidates on [0,1): exponential gaps on the 2^-52 grid
     int acc = 0;
     if (P.has_weight) {
-      // slot j of the bond at j * Nb (a chain holds fewer than 2^31 slots):
-      // the bonds of a wave write adjacent words
-      double* const ctb = B.cand_t + (P.cand_base + b);
-      uint8_t* const cgb = B.cand_g + (P.cand_base + b);
+      // slot j of the bond at j * cand_sstride (a chain holds fewer than 2^31 slots)
+      const long long cb = P.cand_base + (long long)b * P.cand_bstride;
+      double* const ctb = B.cand_t + cb;
+      uint8_t* const cgb = B.cand_g + cb;
       const int base0 = B.soff_old[P.old_site_base + s0], m0 = B.soff_old[P.old_site_base + s0 + 1] - base0;
       const int base1 = B.soff_old[P.old_site_base + s1], m1 = B.soff_old[P.old_site_base + s1 + 1] - base1;
       int cur0 = 0, cur1 = 0;  // world-line cursors (candidate times ascend)
@@ -309,8 +309,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(G == 3
         if (accept) {
           int slot = acc + __popcll(m & below);
           if (slot < P.cand_cap) {
-            ctb[slot * P.Nb] = tau;
-            cgb[slot * P.Nb] = (uint8_t)g;
+            ctb[slot * P.cand_sstride] = tau;
+            cgb[slot * P.cand_sstride] = (uint8_t)g;
           } else {
             atomicOr(B.flags, FLAG_CAND_OVERFLOW);
           }
@@ -344,10 +344,17 @@ constexpr int SCAN_CHUNK = SCAN_ITEMS * SCAN_THREADS;
 __device__ __forceinline__ int site_events(const DevBatch& B, int gs) {
   const ChainDev& P = B.chains[B.site_chain[gs]];
   int s = gs - P.site_base;
-  int bl[4];
-  int nb = site_bonds(P, s, bl);
   int m = 0;
-  for (int i = 0; i < nb; ++i) m += B.counts[P.bond_base + bl[i]];
+  if (P.nv == 1) {
+    int bl[4];
+    const int nb = site_bonds(P, s, bl);
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      if (i < nb) m += B.counts[P.bond_base + bl[i]];
+  } else {  // virtual sites: up to eight lists
+    const int nb = site_lists(P);
+    for (int i = 0; i < nb; ++i) m += B.counts[P.bond_base + site_bond(P, s, i)];
+  }
   return m;
 }
 
@@ -515,8 +522,8 @@ __global__ void __launch_bounds__(256) k_diag_fill(DevBatch B, int fill_cap) {
     const int nb = B.bo_new[gb];
     const int acc = B.acc_count[gb];
     const int kept = B.counts[gb] - acc;
-    const long long cb = P.cand_base + b;  // slot j at cb + j * Nb
-    const long long cs = P.Nb;
+    const long long cb = P.cand_base + (long long)b * P.cand_bstride;  // slot j at cb + j * cs
+    const long long cs = P.cand_sstride;
     const double* ct = B.cand_t + cb;
     const int info_b = gb << 3;
     // tc: accepted candidates' times (slot j at tc[j * ts]); to: the old
@@ -926,21 +933,30 @@ __global__ void __launch_bounds__(256) k_link_ops(DevBatch B) {
 #pragma unroll
     for (int side = 0; side < 2; ++side) {
       const int s = side ? s1 : s0;
-      int bl[4];
-      const int nbd = site_bonds(P, s, bl);
       int rank = r;
       int par = B.kc_new[k];
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        if (i >= nbd) break;
-        const int bp = bl[i];
-        if (bp == b) continue;
+      // the event's rank among the site's other lists (ties by bond index)
+      // and the off-diagonal operators below it there
+      auto other_list = [&](int bp) {
+        if (bp == b) return;
         const int gbp = P.bond_base + bp;
         const int lo = B.bo_new[gbp], hi = B.bo_new[gbp + 1];
         const int q = (bp < b) ? upper_bound_d(B.ot_new + lo, hi - lo, tau) : lower_bound_d(B.ot_new + lo, hi - lo, tau);
         rank += q;
         if (q < hi - lo) par += B.kc_new[lo + q];
         else if (hi > lo) par += B.kc_new[hi - 1] + (B.oi_new[hi - 1] & 1);
+      };
+      if (P.nv == 1) {
+        int bl[4];
+        const int nbd = site_bonds(P, s, bl);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          if (i >= nbd) break;
+          other_list(bl[i]);
+        }
+      } else {  // virtual sites: up to eight lists
+        const int nbd = site_lists(P);
+        for (int i = 0; i < nbd; ++i) other_list(site_bond(P, s, i));
       }
       const int gs = P.site_base + s;
       const int n0 = B.soff_new[gs], n1 = B.soff_new[gs + 1];
@@ -1097,33 +1113,89 @@ __global__ void __launch_bounds__(256) k_uf_union(DevBatch B) {
   }
 }
 
-// tau boundary of the virtual sites of an S=1 chain (path_integral.C:589-615,
-// the S > 1/2 branch of the symmetrization): per real site, the tau = 0 ends
-// (lowest segments) of its two virtual sites are joined to their tau = 1 ends
+// tau boundary of the virtual sites (S > 1/2; path_integral.C:589-615, the
+// S > 1/2 branch of the symmetrization): per real site, the tau = 0 ends
+// (lowest segments) of its nv virtual sites are joined to their tau = 1 ends
 // (top segments) by a permutation that maps equal spins onto each other,
 // uniform among those that do (looper::partitioned_random_shuffle,
-// looper/permutation.h:88-112).  With equal spins at tau = 0 it is the
-// identity or the exchange with probability 1/2 each, decided by bit 0 of
-// Philox(sweep, STAGE_BOUNDARY, real site, 0); otherwise the spins allow one
-// permutation.  (The spins at tau = 1 are those at tau = 0 in some order:
-// every update keeps each real site's magnetization periodic.)
+// looper/permutation.h:88-112).  For each spin value u = 0, 1 in turn, the
+// bottoms of spin u in virtual-site order are matched to the tops of spin u
+// in virtual-site order permuted by Fisher-Yates: for i = n_u - 1 down to 1,
+// j = i - (x mod (i + 1)) with x the next word of Philox(sweep,
+// STAGE_BOUNDARY, real site, 0), swap positions i and j.  (The spins at
+// tau = 1 are those at tau = 0 in some order: every update keeps each real
+// site's magnetization periodic.)  For nv = 2 with equal spins this is the
+// exchange iff bit 0 of the first word is set.  One thread per real site;
+// nv <= 4, so the lists live in registers as 4-bit fields.
+__device__ __forceinline__ int pick4(int i, int a0, int a1, int a2, int a3) {
+  return i == 0 ? a0 : i == 1 ? a1 : i == 2 ? a2 : a3;
+}
+
+__device__ __forceinline__ uint32_t pick_word(const U4& x, int k) {
+  return k == 0 ? x.x : k == 1 ? x.y : k == 2 ? x.z : x.w;
+}
+
 __global__ void __launch_bounds__(256) k_boundary(DevBatch B) {
   if (overflowed(B)) return;
   const int stride = gridDim.x * blockDim.x;
   for (int gs = blockIdx.x * blockDim.x + threadIdx.x; gs < B.nsites; gs += stride) {
     const int c = B.site_chain[gs];
     const ChainDev& P = B.chains[c];
-    if (P.nv != 2) continue;
+    const int nv = P.nv;
+    if (nv == 1) continue;
     const int v = gs - P.site_base;
-    if (v & 1) continue;  // one thread per real site: its virtual site 0
-    const int a0 = B.soff_new[gs], a1 = B.soff_new[gs + 1], a2 = B.soff_new[gs + 2];
-    const int bot0 = a0, top0 = a1 - 1, bot1 = a1, top1 = a2 - 1;
-    const int s0 = B.segspin_new[bot0], s1 = B.segspin_new[bot1], e0 = B.segspin_new[top0];
-    const bool swap = (s0 == s1)
-        ? (philox(B.sweep[c], STAGE_BOUNDARY, (uint32_t)(v >> 1), 0u, P.key0, P.key1).x & 1u) != 0
-        : (s0 != e0);
-    uf_unite(B.parent, bot0, swap ? top1 : top0);
-    uf_unite(B.parent, bot1, swap ? top0 : top1);
+    const int r = v / nv;
+    if (v != r * nv) continue;  // one thread per real site: its virtual site 0
+    int bot[4], top[4];
+    uint32_t bs = 0, es = 0;    // bit a: spin of virtual site a's bottom / top segment
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      bot[a] = top[a] = 0;
+      if (a < nv) {
+        bot[a] = B.soff_new[gs + a];
+        top[a] = B.soff_new[gs + a + 1] - 1;
+        bs |= (uint32_t)B.segspin_new[bot[a]] << a;
+        es |= (uint32_t)B.segspin_new[top[a]] << a;
+      }
+    }
+    const uint32_t all = (1u << nv) - 1;
+    const U4 x = philox(B.sweep[c], STAGE_BOUNDARY, (uint32_t)r, 0u, P.key0, P.key1);
+    int k = 0;
+    uint32_t perm[2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int n = __popc(u ? (bs & all) : (~bs & all));
+      uint32_t p = 0x3210u;
+#pragma unroll
+      for (int i = 3; i >= 1; --i) {
+        if (i < n) {
+          const int j = i - (int)(pick_word(x, k) % (uint32_t)(i + 1));
+          ++k;
+          const uint32_t vi = (p >> (4 * i)) & 15u, vj = (p >> (4 * j)) & 15u;
+          p &= ~((15u << (4 * i)) | (15u << (4 * j)));
+          p |= (vi << (4 * j)) | (vj << (4 * i));
+        }
+      }
+      perm[u] = p;
+    }
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      if (a >= nv) continue;
+      const uint32_t u = (bs >> a) & 1u;
+      const uint32_t cls_b = u ? bs : ~bs, cls_e = u ? es : ~es;
+      const int rank = __popc(cls_b & all & ((1u << a) - 1));
+      const int t = (int)(((u ? perm[1] : perm[0]) >> (4 * rank)) & 15u);
+      // the top of spin u with rank t among the tops of spin u
+      int target = 0, cnt = 0;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        if (q < nv && ((cls_e >> q) & 1u)) {
+          if (cnt == t) target = q;
+          ++cnt;
+        }
+      }
+      uf_unite(B.parent, bot[a], pick4(target, top[0], top[1], top[2], top[3]));
+    }
   }
 }
 

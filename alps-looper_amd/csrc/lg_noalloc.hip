@@ -299,7 +299,7 @@ int lg_cluster_config_noalloc(const lg_params* p, const int32_t* spins, const in
   const double per_site = 2.0 * nops / d.N;
   c.link_cap = per_site < 48 ? 0 : (per_site < 96 ? 128 : (per_site <= 400 ? 512 : 1536));
   if (d.nv > 1) {  // virtual sites: the kernels that write separate top segments, and the boundary
-    c.link_cap = per_site < 8 ? 0 : 1;
+    c.link_cap = (per_site < 8 || site_lists(d) > 4) ? 0 : 1;
     B.any_split = 1;
   }
   c.compress = 1;

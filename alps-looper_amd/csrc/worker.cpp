@@ -243,14 +243,17 @@ class loop_worker {
     if (lp_.lattice == LG_LATTICE_SQUARE && p.defined("W") && (int)std::llround(p.num("W")) != lp_.L)
       throw std::invalid_argument("square lattice with W != L is not supported");
     if (p.str("MODEL", "spin") != "spin") throw std::invalid_argument("only MODEL = spin is supported");
-    // spin size (model_parameter.h): S=1/2, or S=1 on the chain through two
-    // virtual sites per real site (path_integral.C:589-615)
+    // spin size (model_parameter.h): S > 1/2 through 2S virtual sites per
+    // real site (path_integral.C:589-615): S <= 2 on the chain, S <= 1 on
+    // the square lattice
     lp_.two_s = 1;
     if (p.defined("local_S")) {
       const double S = p.num("local_S");
-      if (std::fabs(S - 1.0) < 1e-12 && lp_.lattice == LG_LATTICE_CHAIN) lp_.two_s = 2;
-      else if (std::fabs(S - 0.5) > 1e-12)
-        throw std::invalid_argument("local_S must be 1/2, or 1 on the chain lattice");
+      const long long t = std::llround(2.0 * S);
+      const int smax = (lp_.lattice == LG_LATTICE_CHAIN) ? 4 : 2;
+      if (std::fabs(2.0 * S - (double)t) > 1e-12 || t < 1 || t > smax)
+        throw std::invalid_argument("local_S must be 1/2, 1, 3/2 or 2 on the chain lattice, 1/2 or 1 on the square lattice");
+      lp_.two_s = (int32_t)t;
     }
     for (const char* k : {"Gamma", "D"})
       if (p.defined(k) && p.num(k) != 0.0)
@@ -303,7 +306,7 @@ class loop_worker {
       }
     }
     if (mc || ms || !custom.empty()) {
-      if (lp_.two_s == 2) throw std::invalid_argument("correlation measurements are built for S=1/2 only");
+      if (lp_.two_s > 1) throw std::invalid_argument("correlation measurements are built for S=1/2 only");
       if (lp_.L < 2 || lp_.L % 2) throw std::invalid_argument("L must be even");
       const int origin = p.defined("INITIAL_SITE") ? (int)std::llround(p.num("INITIAL_SITE")) : -1;
       if (p.defined("INITIAL_SITE") && (origin < 0 || origin >= n_sites_))

@@ -154,8 +154,8 @@ def params(lattice, L, beta, jxy=1.0, jz=None, seed=23125, chain_id=0, force_sca
 
 
 def lattice_size(lattice, L, two_s=1):
-    """(sites, bonds) the device sweeps: virtual ones for S=1 (two per real site)"""
-    nv = 2 if two_s == 2 else 1
+    """(sites, bonds) the device sweeps: virtual ones for S > 1/2 (2S per real site)"""
+    nv = two_s if two_s >= 2 else 1
     return (nv * L, nv * nv * L) if lattice == CHAIN else (nv * L * L, nv * nv * 2 * L * L)
 
 

@@ -44,12 +44,13 @@ enum { LG_LATTICE_CHAIN = 0, LG_LATTICE_SQUARE = 1 };
  * SEED, FORCE_SCATTER, local_S (loop_config.h, lattice.h:304-307,
  * temperature.h:43-79, weight_impl.h:165-188); the spin model is the XXZ
  * model H = sum_b [Jz Sz Sz + Jxy/2 (S+S- + S-S+)] with spin S = two_s / 2.
- * S=1 (two_s = 2, chain lattice only) is swept as looper sweeps it: two
- * virtual S=1/2 sites per real site (virtual site 2s + a), four virtual bonds
- * per real bond (bond 4i + 2a + c joins 2i + a and 2(i+1) + c), and a random
- * spin-conserving permutation joining the tau = 1 and tau = 0 ends of each real
- * site's virtual world lines (path_integral.C:589-615); sites, bonds, spins
- * and operators at the ABI are then the virtual ones.
+ * S > 1/2 (two_s = 2, 3, 4 on the chain; two_s = 2 on the square lattice) is
+ * swept as looper sweeps it: nv = 2S virtual S=1/2 sites per real site
+ * (virtual site r*nv + a), nv^2 virtual bonds per real bond (real bond rb =
+ * (rs, rt) carries rb*nv^2 + a*nv + c = (rs*nv + a, rt*nv + c)), and a random
+ * spin-conserving permutation joining the tau = 1 and tau = 0 ends of each
+ * real site's virtual world lines (path_integral.C:589-615); sites, bonds,
+ * spins and operators at the ABI are then the virtual ones.
  * chain_id selects an independent random stream for the same seed. */
 typedef struct lg_params {
   int32_t lattice;       /* LG_LATTICE_CHAIN or LG_LATTICE_SQUARE */
@@ -61,7 +62,7 @@ typedef struct lg_params {
   uint32_t seed;         /* SEED */
   uint32_t chain_id;     /* independent stream index */
   double h;              /* longitudinal field h (site term -h S^z, looper's FIELD path) */
-  int32_t two_s;         /* 2S: 1 (S=1/2; 0 is taken as 1) or 2 (S=1, chain lattice) */
+  int32_t two_s;         /* 2S: 1 (S=1/2; 0 is taken as 1), 2, 3 or 4 (chain), 2 (square) */
 } lg_params;
 
 /* Per-sweep collector sums (all improved estimators) -- see the collector of

@@ -63,19 +63,19 @@ def _sector_hamiltonian(states, bonds, jxy, jz):
     return H
 
 
-def _spin1_sectors(N, bonds, jxy, jz):
-    """S=1: basis states as base-3 digits (digit q -> m = 1 - q), grouped by
-    total S^z; H = sum_b [Jz Sz Sz + Jxy/2 (S+S- + S-S+)] with
-    S+|m> = sqrt(2 - m(m+1)) |m+1>."""
-    ms = np.array(list(itertools.product((1, 0, -1), repeat=N)), dtype=int)[:, ::-1]  # site i = column i
-    tot = ms.sum(axis=1)
+def _spin_sectors(N, bonds, jxy, jz, two_s):
+    """Spin S = two_s/2: basis states as base-(2S+1) digits (digit q -> m = S - q),
+    grouped by total S^z; H = sum_b [Jz Sz Sz + Jxy/2 (S+S- + S-S+)] with
+    S+|m> = sqrt(S(S+1) - m(m+1)) |m+1>."""
+    S = 0.5 * two_s
+    levels = [S - q for q in range(two_s + 1)]
+    ms = np.array(list(itertools.product(levels, repeat=N)), dtype=float)[:, ::-1]  # site i = column i
+    tot2 = np.rint(2 * ms.sum(axis=1)).astype(int)
     out = []
-    for M in range(-N, N + 1):
-        idx = np.nonzero(tot == M)[0]
-        if len(idx) == 0:
-            continue
+    for M2 in sorted(set(tot2.tolist())):
+        idx = np.nonzero(tot2 == M2)[0]
         conf = ms[idx]
-        key = {tuple(c): n for n, c in enumerate(conf)}
+        key = {tuple(np.rint(2 * c).astype(int)): n for n, c in enumerate(conf)}
         dim = len(idx)
         H = np.zeros((dim, dim))
         for n, c in enumerate(conf):
@@ -83,25 +83,25 @@ def _spin1_sectors(N, bonds, jxy, jz):
                 H[n, n] += jz * c[a] * c[b]
                 for da, db in ((1, -1), (-1, 1)):
                     ma, mb = c[a] + da, c[b] + db
-                    if abs(ma) > 1 or abs(mb) > 1:
+                    if abs(ma) > S + 1e-9 or abs(mb) > S + 1e-9:
                         continue
-                    amp = np.sqrt(2 - c[a] * (c[a] + da)) * np.sqrt(2 - c[b] * (c[b] + db))
-                    d = list(c)
+                    amp = np.sqrt(S * (S + 1) - c[a] * (c[a] + da)) * np.sqrt(S * (S + 1) - c[b] * (c[b] + db))
+                    d = c.copy()
                     d[a], d[b] = ma, mb
-                    H[key[tuple(d)], n] += 0.5 * jxy * amp
-        out.append((conf.astype(float), H))
+                    H[key[tuple(np.rint(2 * d).astype(int))], n] += 0.5 * jxy * amp
+        out.append((conf, H))
     return out
 
 
-def diagonalize_spin1_chain(L, T, jxy=1.0, jz=1.0):
-    """Thermal averages of the S=1 XXZ chain (periodic, L sites), keyed by the
-    reference's observable names; staggered quantities with the gauge
-    (-1)^site (diag.C:159-293, 313-488 for S=1)"""
+def diagonalize_spin(kind, L, T, two_s, jxy=1.0, jz=1.0):
+    """Thermal averages of the spin-S XXZ model (S = two_s/2, periodic chain or
+    square lattice), keyed by the reference's observable names; staggered
+    quantities with the lattice's gauge (diag.C:159-293, 313-488 for spin S)"""
     beta = 1.0 / T
-    N, bonds, gauge = lattice_bonds("chain", L)
+    N, bonds, gauge = lattice_bonds(kind, L)
     V = float(N)
     sectors = []
-    for conf, H in _spin1_sectors(N, bonds, jxy, jz):
+    for conf, H in _spin_sectors(N, bonds, jxy, jz, two_s):
         E, U = np.linalg.eigh(H)
         sectors.append((E, U, conf.sum(axis=1), conf @ gauge))
     e0 = min(sec[0].min() for sec in sectors)
@@ -129,6 +129,11 @@ def diagonalize_spin1_chain(L, T, jxy=1.0, jz=1.0):
         res[name + "Susceptibility"] = kub / Z / V
     res["Staggered Structure Factor"] = res["Staggered Magnetization^2"] / V
     return res
+
+
+def diagonalize_spin1_chain(L, T, jxy=1.0, jz=1.0):
+    """Thermal averages of the S=1 XXZ chain (periodic, L sites)"""
+    return diagonalize_spin("chain", L, T, 2, jxy, jz)
 
 
 def diagonalize(kind, L, T, jxy=1.0, jz=1.0, h=0.0):

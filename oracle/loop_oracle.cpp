@@ -42,6 +42,7 @@
 #include <cstring>
 #include <deque>
 #include <random>
+#include <stdexcept>
 #include <vector>
 
 namespace {
@@ -153,24 +154,35 @@ struct Lattice {
   int nv = 1;  // virtual sites per real site (2S)
   std::vector<int> src, tgt, dir, gauge;
   std::vector<std::vector<int>> site_bonds;  // bonds incident to each site
-  // nv = 2 (S=1, chain only): looper's virtual lattice for S > 1/2
-  // (lattice.h:104-175, virtual_graph): real site s -> virtual sites 2s,
-  // 2s+1; real bond i = (i, i+1) -> the four virtual bonds 4i + 2a + c =
-  // (2i + a, 2(i+1) + c); every virtual site carries its real site's gauge
+  // nv > 1 (S = nv/2): looper's virtual lattice for S > 1/2
+  // (lattice.h:104-175, virtual_graph): real site r -> virtual sites
+  // r*nv + a; real bond rb = (rs, rt) -> the nv^2 virtual bonds
+  // rb*nv^2 + a*nv + c = (rs*nv + a, rt*nv + c), in real-bond order and
+  // source-major within; every virtual site carries its real site's gauge
+  // and every virtual bond its real bond's direction
   void init(int k, int l, int n_virtual = 1) {
-    kind = k; L = l; nv = n_virtual;
-    if (kind == 0 && nv == 2) {
-      N = 2 * L; Nb = 4 * L; dim = 1;
-      src.resize(Nb); tgt.resize(Nb); dir.assign(Nb, 0);
-      for (int i = 0; i < L; ++i)
-        for (int a = 0; a < 2; ++a)
-          for (int c = 0; c < 2; ++c) {
-            src[4 * i + 2 * a + c] = 2 * i + a;
-            tgt[4 * i + 2 * a + c] = 2 * ((i + 1) % L) + c;
+    if (n_virtual > 1) {
+      Lattice real;
+      real.init(k, l, 1);
+      kind = k; L = l; nv = n_virtual; dim = real.dim;
+      N = real.N * nv; Nb = real.Nb * nv * nv;
+      src.resize(Nb); tgt.resize(Nb); dir.resize(Nb);
+      for (int rb = 0; rb < real.Nb; ++rb)
+        for (int a = 0; a < nv; ++a)
+          for (int c = 0; c < nv; ++c) {
+            const int b = rb * nv * nv + a * nv + c;
+            src[b] = real.src[rb] * nv + a;
+            tgt[b] = real.tgt[rb] * nv + c;
+            dir[b] = real.dir[rb];
           }
       gauge.resize(N);
-      for (int v = 0; v < N; ++v) gauge[v] = ((v / 2) % 2 == 0) ? 1 : -1;
-    } else if (kind == 0) {
+      for (int v = 0; v < N; ++v) gauge[v] = real.gauge[v / nv];
+      site_bonds.assign(N, std::vector<int>());
+      for (int b = 0; b < Nb; ++b) { site_bonds[src[b]].push_back(b); site_bonds[tgt[b]].push_back(b); }
+      return;
+    }
+    kind = k; L = l; nv = 1;
+    if (kind == 0) {
       N = L; Nb = L; dim = 1;
       src.resize(Nb); tgt.resize(Nb); dir.assign(Nb, 0);
       for (int b = 0; b < Nb; ++b) { src[b] = b; tgt[b] = (b + 1) % L; }
@@ -1105,19 +1117,32 @@ struct PhiloxChain {
       else if (g == 0) { unite(l0, l1); unite(u0, u1); }
       else { unite(l0, u1); unite(l1, u0); }
     }
-    // virtual sites (S=1): each real site's tau = 0 ends (lowest segments)
-    // meet its tau = 1 ends (top segments) through a spin-conserving
-    // permutation; with equal tau = 0 spins the exchange is taken iff bit 0 of
-    // Philox(sweep, STAGE_BOUNDARY, real site, 0) is set
+    // virtual sites (S > 1/2): each real site's tau = 0 ends (lowest
+    // segments) meet its tau = 1 ends (top segments) through a
+    // spin-conserving permutation: for u = 0, 1 in turn, the bottoms of spin
+    // u in virtual-site order are matched to the tops of spin u in
+    // virtual-site order permuted by Fisher-Yates (i = n_u - 1 .. 1, j = i -
+    // (x mod (i + 1)), x the next word of Philox(sweep, STAGE_BOUNDARY, real
+    // site, 0), swap positions i and j)
     if (split()) {
-      for (int r = 0; r < N / 2; ++r) {
-        const int v0 = 2 * r, v1 = 2 * r + 1;
-        const int bot0 = soff[v0], top0 = soff[v0 + 1] - 1, bot1 = soff[v1], top1 = soff[v1 + 1] - 1;
-        const bool swap = (seg_spin[bot0] == seg_spin[bot1])
-            ? (philox4x32_10(sweep, STAGE_BOUNDARY, (uint32_t)r, 0, key0, key1).v[0] & 1) != 0
-            : seg_spin[bot0] != seg_spin[top0];
-        unite(bot0, swap ? top1 : top0);
-        unite(bot1, swap ? top0 : top1);
+      const int nv = lat.nv;
+      for (int r = 0; r < N / nv; ++r) {
+        const auto x = philox4x32_10(sweep, STAGE_BOUNDARY, (uint32_t)r, 0, key0, key1);
+        int k = 0;
+        for (int u = 0; u < 2; ++u) {
+          std::vector<int> bots, tops;
+          for (int a = 0; a < nv; ++a) {
+            const int v = r * nv + a;
+            if (seg_spin[soff[v]] == u) bots.push_back(soff[v]);
+            if (seg_spin[soff[v + 1] - 1] == u) tops.push_back(soff[v + 1] - 1);
+          }
+          if (bots.size() != tops.size()) throw std::logic_error("virtual site spins not conserved");
+          for (int i = (int)tops.size() - 1; i >= 1; --i) {
+            const int j = i - (int)(x.v[k++] % (uint32_t)(i + 1));
+            std::swap(tops[i], tops[j]);
+          }
+          for (size_t i = 0; i < bots.size(); ++i) unite(bots[i], tops[i]);
+        }
       }
     }
     for (int i = 0; i < M; ++i) parent[i] = find(i);
@@ -1239,7 +1264,7 @@ void or_thresholds(double jxy, double jz, double force_scatter, uint64_t* thr4, 
 }
 
 int or_lattice_info(int kind, int L, int nv, int* N, int* Nb, int* dim) {
-  if ((kind != 0 && kind != 1) || L < 2 || nv < 1 || nv > 2 || (nv == 2 && kind != 0)) return -1;
+  if ((kind != 0 && kind != 1) || L < 2 || nv < 1 || nv > (kind == 0 ? 4 : 2)) return -1;
   Lattice lat; lat.init(kind, L, nv);
   *N = lat.N; *Nb = lat.Nb; *dim = lat.dim;
   return 0;

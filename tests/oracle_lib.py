@@ -124,8 +124,8 @@ def schedule_temperature(steps, temps, step):
 
 
 def lattice(kind, L, nv=1):
-    """(sites, bonds, source, target, gauge); nv = 2: the virtual lattice of
-    the S=1 chain (two virtual sites per real site)"""
+    """(sites, bonds, source, target, gauge); nv > 1: the virtual lattice of
+    spin S = nv/2 (nv virtual sites per real site)"""
     N, Nb, dim = I(), I(), I()
     assert lib.or_lattice_info(kind, L, nv, ctypes.byref(N), ctypes.byref(Nb), ctypes.byref(dim)) == 0
     src = np.zeros(Nb.value, np.int32)
@@ -146,7 +146,7 @@ def xxz_weights(jxy, jz, fs=0.0):
 class LooperChain:
     def __init__(self, kind, L, beta, jxy=1.0, jz=None, seed=23125, fs=0.0, field=0.0, two_s=1):
         jz = jxy if jz is None else jz
-        self.h = lib.or_looper_create(kind, L, beta, jxy, jz, fs, seed, field, 2 if two_s == 2 else 1)
+        self.h = lib.or_looper_create(kind, L, beta, jxy, jz, fs, seed, field, two_s if two_s >= 2 else 1)
 
     def sweep(self, n):
         rec = np.zeros((n, R))
@@ -165,7 +165,7 @@ class PhiloxChain:
     def __init__(self, kind, L, beta, jxy=1.0, jz=None, seed=23125, chain_id=0, fs=0.0, field=0.0, two_s=1):
         jz = jxy if jz is None else jz
         self.kind, self.L = kind, L
-        nv = 2 if two_s == 2 else 1
+        nv = two_s if two_s >= 2 else 1
         self.N, self.Nb, _, _, _ = lattice(kind, L, nv)
         self.h = lib.or_philox_create(kind, L, beta, jxy, jz, fs, field, seed, chain_id, nv)
 

@@ -18,8 +18,8 @@ FX = json.load(open(os.path.join(HERE, "golden", "reference_fixtures.json")))
     ('LATTICE="chain lattice"; T=0.1', "L not defined"),
     ('LATTICE="triangular lattice"; L=4; T=1', "unsupported LATTICE"),
     ("L=4; T=1; Gamma=0.3", "not supported"),
-    ("L=4; local_S=3/2; T=1", "local_S"),
-    ('LATTICE="square lattice"; L=4; local_S=1; T=1', "local_S"),
+    ("L=4; local_S=5/2; T=1", "local_S"),
+    ('LATTICE="square lattice"; L=4; local_S=3/2; T=1', "local_S"),
     ("L=4; T=-1", "negative temperature"),
     ("L=4", "temperature is not initialized"),
     ("L=4; T=1; SWEEPS=[100:x]", "SWEEPS"),
