@@ -642,6 +642,7 @@ DevBatch Batch::dev(int from) const {
 }
 
 int Batch::capture(int from) {
+  const auto t0 = std::chrono::steady_clock::now();
   hipGraph_t g = nullptr;
   LG_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
   launch_sweep(dev(from), cfg, stream);
@@ -649,6 +650,8 @@ int Batch::capture(int from) {
   hipError_t e = hipGraphInstantiate(&graph_exec[from], g, nullptr, nullptr, 0);
   hipGraphDestroy(g);
   if (e != hipSuccess) return set_error(LG_ERR_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+  ++graph_captures;
+  graph_capture_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   return LG_OK;
 }
 
@@ -1089,7 +1092,11 @@ int Batch::reload(const std::vector<lg_params>& ps, const std::vector<HostConfig
 
 void Batch::adopt(Batch& o) {
   release();
+  const uint64_t captures = graph_captures;
+  const double capture_s = graph_capture_s;
   *this = o;  // member-wise: device pointers, host tables, settings
+  graph_captures += captures;  // the counters cover the handle's lifetime
+  graph_capture_s += capture_s;
   o.allocations.clear();
   o.graph_exec[0] = o.graph_exec[1] = nullptr;
   o.stream = nullptr;

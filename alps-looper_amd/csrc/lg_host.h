@@ -119,6 +119,8 @@ class Batch {
   int cur = 0;                 // st[cur] holds the latest configuration
   LaunchCfg cfg;
   bool use_graph = false;
+  uint64_t graph_captures = 0;     // sweep graphs captured and instantiated
+  double graph_capture_s = 0;      // host time spent in them
   hipStream_t stream = nullptr;
 
  private:

@@ -69,6 +69,8 @@ struct lg_server {
   bool stop = false;
   std::string fatal;
   uint64_t next_id = 1, submitted = 0, total_updates = 0, launches = 0, preempted = 0;
+  uint64_t graph_captures = 0;    // the batch's, copied under the lock after each chunk
+  double graph_capture_s = 0;
   double busy_s = 0;
   std::vector<double> latencies;
 
@@ -240,6 +242,8 @@ struct lg_server {
       }
       busy_s += std::chrono::duration<double>(t1 - t0).count();
       launches += chunk;
+      graph_captures = batch.graph_captures;
+      graph_capture_s = batch.graph_capture_s;
       for (size_t k = 0; k < next.size(); ++k) {
         Job& j = jobs[next[k]];
         if (j.status != LG_JOB_RUNNING) continue;  // cancelled meanwhile
@@ -406,6 +410,8 @@ int lg_server_stats_get(lg_server* s, lg_server_stats* st) {
   st->updates_per_s = s->busy_s > 0 ? s->total_updates / s->busy_s : 0.0;
   st->sweep_launches = s->launches;
   st->preemptions = s->preempted;
+  st->graph_captures = s->graph_captures;
+  st->graph_capture_s = s->graph_capture_s;
   return LG_OK;
 }
 

@@ -424,7 +424,8 @@ class ServerStats(ctypes.Structure):
                 ("queued", ctypes.c_uint64), ("p50_latency_s", ctypes.c_double), ("p99_latency_s", ctypes.c_double),
                 ("mean_latency_s", ctypes.c_double), ("updates_per_s", ctypes.c_double), ("busy_s", ctypes.c_double),
                 ("total_updates", ctypes.c_uint64), ("sweep_launches", ctypes.c_uint64),
-                ("preemptions", ctypes.c_uint64)]
+                ("preemptions", ctypes.c_uint64), ("graph_captures", ctypes.c_uint64),
+                ("graph_capture_s", ctypes.c_double)]
 
 
 def _server_bind():

@@ -1,8 +1,8 @@
 """Summaries of rocprofv3 output (rocpd SQLite databases) for the sweep kernels.
 
-    python alps-looper_amd/profiling.py trace  RESULTS.db [--last N]
+    python alps-looper_amd/profiling.py trace  RESULTS.db [--last N] [--only k1,k2,...]
     python alps-looper_amd/profiling.py pmc    FETCH.db WRITE.db COPY_FETCH.db COPY_WRITE.db \
-                                               --copy-mib 1024 [--last N] [--nop NOP] [--json OUT]
+                                               --copy-mib 1024 [--last N] [--nop NOP] [--json OUT] [--only ...]
     python alps-looper_amd/profiling.py counters RESULTS.db [RESULTS.db ...] [--last N]
 
 `trace` prints per-kernel call counts and average durations (the top_kernels
@@ -70,9 +70,13 @@ def main():
     ap.add_argument("--copy-mib", type=float, default=1024)
     ap.add_argument("--nop", type=float, default=None, help="operators per sweep (bytes per operator column)")
     ap.add_argument("--json", default=None)
+    ap.add_argument("--only", default=None, help="comma-separated kernels (short names) to list")
     a = ap.parse_args()
+    only = set(a.only.split(",")) if a.only else None
     if a.mode == "trace":
         t = trace(a.dbs[0], a.last)
+        if only:
+            t = {k: v for k, v in t.items() if k in only}
         tot = sum(x["avg_us"] for k, x in t.items() if k not in ("copy_stream",) and not k.startswith("_"))
         print("| kernel | calls | avg us | share of sweep |")
         print("|---|---|---|---|")
@@ -120,7 +124,7 @@ def main():
     print("|---|---|---|---|" + ("---|" if a.nop else ""))
     res = {}
     for k in sorted(F, key=lambda k: -(F[k] * sf + W.get(k, 0) * sw)):
-        if k.startswith("__amd"):
+        if k.startswith("__amd") or (only and k not in only):
             continue
         fb = F[k] * sf * 1024
         wb = W.get(k, 0) * sw * 1024

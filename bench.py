@@ -534,6 +534,8 @@ def run_serve(args, lg):
         "p50_queue_s": _percentile(qs, 50), "p99_queue_s": _percentile(qs, 99),
         "busy_fraction": st["busy_s"] / wall, "updates_per_busy_s": st["updates_per_s"],
         "batched_sweeps": int(st["sweep_launches"]), "ms_per_batched_sweep": 1e3 * st["busy_s"] / max(1, st["sweep_launches"]),
+        "graph_captures": int(st["graph_captures"]), "graph_capture_s": st["graph_capture_s"],
+        "ms_per_graph_capture": 1e3 * st["graph_capture_s"] / max(1, st["graph_captures"]),
         "saturated_value": sat_rate, "calibration_seconds": args.serve_calib, "wall_s": wall,
     }
     if not args.no_cpu:

@@ -69,6 +69,8 @@ typedef struct lg_server_stats {
   uint64_t total_updates;
   uint64_t sweep_launches; /* batched sweeps executed */
   uint64_t preemptions;    /* jobs saved and re-queued by lg_server_preempt */
+  uint64_t graph_captures; /* sweep graphs captured (one per admission that changes the layout, two buffers each) */
+  double graph_capture_s;  /* host time spent capturing and instantiating them (part of the job latencies) */
 } lg_server_stats;
 
 int lg_server_last_error(char* buf, size_t len);

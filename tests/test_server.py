@@ -73,6 +73,9 @@ def test_admission_by_footprint_queues_jobs(loopgpu):
     assert all(r["status"] == lg.JOB_DONE for r in res)
     assert max(r["queue_s"] for r in res) > 0
     assert st["done"] == 6 and st["total_updates"] > 0 and st["updates_per_s"] > 0
+    # every admission that changes the layout re-captures the sweep graph;
+    # the host time of those captures is counted
+    assert st["graph_captures"] >= 2 and st["graph_capture_s"] > 0
 
 
 @pytest.mark.gpu
