@@ -124,11 +124,23 @@ __device__ __forceinline__ bool overflowed(const DevBatch& B) {
   return __hip_atomic_load(B.flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
 }
 
-// Site of each segment node (node_site): stored for every segment in batches
-// of more than one chain, and for the first segment of each site (the only
-// one that needs it) when the batch is one chain.
+// chain of a segment node by binary search over the chains' first nodes
+// (a few cached loads; no per-node table read)
+__device__ __forceinline__ int chain_of_node(const DevBatch& B, int node) {
+  int lo = 0, n = B.nchains;
+  while (n > 1) {
+    const int h = n >> 1;
+    if (B.chain_node0[lo + h] <= node) { lo += h; n -= h; } else { n = h; }
+  }
+  return lo;
+}
+
+// Chain of a segment node.  The site of a segment (node_site) is stored for
+// the first segment of each site only, the one the estimator needs: storing
+// it for every segment of a batch cost the linking kernels a scattered 4-byte
+// store per segment.
 __device__ __forceinline__ int node_chain(const DevBatch& B, int node) {
-  return B.nchains == 1 ? 0 : B.site_chain[B.node_site[node]];
+  return B.nchains == 1 ? 0 : chain_of_node(B, node);
 }
 
 // site (global index) whose first segment is `node`
@@ -803,7 +815,7 @@ __global__ void __launch_bounds__(64 * WAVES) k_link(DevBatch B) {
           B.segspin_new[node] = sp;
           B.evflags[node] = fl;
           B.parent[node] = node;
-          if (B.nchains > 1 || rank == 0) B.node_site[node] = gs;
+          if (rank == 0) B.node_site[node] = gs;
         }
       }
     }
@@ -816,7 +828,7 @@ __global__ void __launch_bounds__(64 * WAVES) k_link(DevBatch B) {
         const uint8_t f = lfl[e];
         B.segspin_new[n0 + r] = f >> 7;
         B.evflags[n0 + r] = f & 0x7f;
-        if (B.nchains > 1 || r == 0) B.node_site[n0 + r] = gs;
+        if (r == 0) B.node_site[n0 + r] = gs;
         lpre[e] = (uint16_t)r;
       }
     }
@@ -969,12 +981,11 @@ __global__ void __launch_bounds__(256) k_link_ops(DevBatch B) {
       B.segspin_new[node] = sp;
       B.evflags[node] = (uint8_t)((info & 1) | (side ? 0 : EV_SRC) | dflag | (rank == 0 ? EV_FIRST : 0) | gflag |
                                   (split && rank == 0 ? EV_NOWRAP : 0));
-      if (B.nchains > 1 || rank == 0) B.node_site[node] = gs;
+      if (rank == 0) B.node_site[node] = gs;
       if (split && up == n1 - 1) {  // the last event: the site's top segment, up to tau = 1
         B.evt_new[up] = 1.0;
         B.segspin_new[up] = sp ^ (uint8_t)(info & 1);
         B.evflags[up] = gflag;
-        if (B.nchains > 1) B.node_site[up] = gs;
       }
       if (side) { B.l1[k] = node; B.u1[k] = up; } else { B.l0[k] = node; B.u0[k] = up; }
       lo_leg[side] = node;
@@ -1040,7 +1051,6 @@ __device__ __forceinline__ void link_site_events(const DevBatch& B, int gs) {
   double t2 = p2 < e2 ? B.ot_new[p2] : 3.0, t3 = p3 < e3 ? B.ot_new[p3] : 3.0;
   const int n0 = B.soff_new[gs], n1 = B.soff_new[gs + 1];
   const uint8_t gflag = gauge(P, s) < 0 ? EV_GAUGE : 0;
-  const bool all_sites = B.nchains > 1;
   const bool split = P.nv > 1;     // virtual site: m + 1 segments, no wrap
   const int ne = split ? n1 - 1 : n1;
   uint8_t spin = B.spins_old[P.old_site_base + s];
@@ -1062,7 +1072,7 @@ __device__ __forceinline__ void link_site_events(const DevBatch& B, int gs) {
     B.segspin_new[node] = spin;
     B.evflags[node] = (uint8_t)(off | fl | (node == n0 ? EV_FIRST | (split ? EV_NOWRAP : 0) : 0) | gflag);
     B.parent[node] = node;
-    if (all_sites || node == n0) B.node_site[node] = gs;
+    if (node == n0) B.node_site[node] = gs;
     if (fl & EV_SRC) { B.l0[k] = node; B.u0[k] = up; } else { B.l1[k] = node; B.u1[k] = up; }
     spin ^= (uint8_t)off;
     // advance the chosen list
@@ -1076,7 +1086,6 @@ __device__ __forceinline__ void link_site_events(const DevBatch& B, int gs) {
     B.segspin_new[ne] = spin;
     B.evflags[ne] = gflag;
     B.parent[ne] = ne;
-    if (all_sites) B.node_site[ne] = gs;
   }
 }
 
@@ -1224,16 +1233,6 @@ __global__ void __launch_bounds__(256) k_uf_compress(DevBatch B) {
   }
 }
 
-// chain of a segment node by binary search over the chains' first nodes
-// (a few cached loads; no per-node table read)
-__device__ __forceinline__ int chain_of_node(const DevBatch& B, int node) {
-  int lo = 0, n = B.nchains;
-  while (n > 1) {
-    const int h = n >> 1;
-    if (B.chain_node0[lo + h] <= node) { lo += h; n -= h; } else { n = h; }
-  }
-  return lo;
-}
 
 // ---------------------------------------------------------------------------
 // (e) improved estimators + (d) flip bits, per segment node.
@@ -1690,7 +1689,14 @@ __global__ void __launch_bounds__(COL_THREADS) k_flip_collect(DevBatch B, int cl
       for (int u = 0; u < COL_U; ++u) {
         if (!root[u]) continue;
         const int i = nd[u];
-        const int c = (i < c0_end) ? c0 : node_chain(B, i);
+        // chains are contiguous in node order: from the chunk's first chain
+        // forward over the few cached chain starts, a binary search past eight
+        int c = c0;
+        if (i >= c0_end) {
+          int t = 0;
+          while (t < 8 && c + 1 < B.nchains && B.chain_node0[c + 1] <= i) { ++c; ++t; }
+          if (t == 8) c = chain_of_node(B, i);
+        }
         if (c != cl) {
           if (cl >= 0) flush(cl);
 #pragma unroll
