@@ -131,6 +131,7 @@ void Batch::release() {
     if (graph_exec[i]) { hipGraphExecDestroy(graph_exec[i]); graph_exec[i] = nullptr; }
   for (void* p : allocations) hipFree(p);
   allocations.clear();
+  if (rec_host) { hipHostFree(rec_host); rec_host = nullptr; }
   if (stream) { hipStreamDestroy(stream); stream = nullptr; }
 }
 
@@ -361,6 +362,16 @@ int Batch::allocate() {
   if ((rc = alloc(rec_acc, (size_t)cap_chains * R_COUNT))) return rc;
   ring_size = (int)std::max<long long>(16, std::min<long long>(4096, (64ll << 20) / ((long long)cap_chains * R_COUNT * 8)));
   if ((rc = alloc(rec_ring, (size_t)ring_size * cap_chains * R_COUNT))) return rc;
+  // records reach the host through pinned memory: a copy to pageable memory
+  // of a size not seen before costs the runtime milliseconds of staging
+  if (rec_host) { hipHostFree(rec_host); rec_host = nullptr; }
+  {
+    const hipError_t e = hipHostMalloc((void**)&rec_host, (size_t)ring_size * cap_chains * R_COUNT * sizeof(double));
+    if (e != hipSuccess) {
+      rec_host = nullptr;
+      return set_error(LG_ERR_DEVICE, std::string("hipHostMalloc of the record mirror: ") + hipGetErrorString(e));
+    }
+  }
   if ((rc = alloc(ring_pos, 1))) return rc;
   if ((rc = alloc(end_count, 1))) return rc;
   LG_HIP(hipMemset(end_count, 0, sizeof(int)));
@@ -698,7 +709,6 @@ int Batch::run(uint32_t n, lg_sweep_record* out) {
   if (rc) return rc;
   const size_t nc = chains.size();
   if (nc == 0) return LG_OK;
-  std::vector<double> buf;
   uint32_t done = 0;
   while (done < n) {
     uint32_t chunk = std::min<uint32_t>(n - done, (uint32_t)ring_size);
@@ -707,16 +717,14 @@ int Batch::run(uint32_t n, lg_sweep_record* out) {
       rc = enqueue_sweep();
       if (rc) return rc;
     }
-    if (out) {
-      buf.resize((size_t)chunk * nc * R_COUNT);
-      LG_HIP(hipMemcpyAsync(buf.data(), rec_ring, buf.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
-    }
+    const size_t words = (size_t)chunk * nc * R_COUNT;
+    if (out) LG_HIP(hipMemcpyAsync(rec_host, rec_ring, words * sizeof(double), hipMemcpyDeviceToHost, stream));
     LG_HIP(hipStreamSynchronize(stream));
     rc = check_flags();
     if (rc) return rc;
     if (out) {
       static_assert(sizeof(lg_sweep_record) == R_COUNT * sizeof(double), "record layout");
-      std::memcpy(out + (size_t)done * nc, buf.data(), buf.size() * sizeof(double));
+      std::memcpy(out + (size_t)done * nc, rec_host, words * sizeof(double));
     }
     for (size_t c = 0; c < nc; ++c) host_sweep[c] += chunk;
     done += chunk;
@@ -1095,6 +1103,7 @@ void Batch::adopt(Batch& o) {
   const uint64_t captures = graph_captures;
   const double capture_s = graph_capture_s;
   *this = o;  // member-wise: device pointers, host tables, settings
+  o.rec_host = nullptr;
   graph_captures += captures;  // the counters cover the handle's lifetime
   graph_capture_s += capture_s;
   o.allocations.clear();

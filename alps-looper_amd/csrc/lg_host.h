@@ -173,6 +173,7 @@ class Batch {
   uint32_t* d_sweep = nullptr;
   double* rec_acc = nullptr;
   double* rec_ring = nullptr;
+  double* rec_host = nullptr;     // pinned host mirror of the record ring (hipHostMalloc)
   int* ring_pos = nullptr;
   int* end_count = nullptr;
   int ring_size = 0;
