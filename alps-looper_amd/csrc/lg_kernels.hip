@@ -135,6 +135,14 @@ __device__ __forceinline__ int chain_of_node(const DevBatch& B, int node) {
   return lo;
 }
 
+// chain of `node`, stepping forward from chain c (one at or before it) over
+// the few cached chain starts; a binary search after eight steps
+__device__ __forceinline__ int chain_from(const DevBatch& B, int c, int node) {
+  int t = 0;
+  while (t < 8 && c + 1 < B.nchains && B.chain_node0[c + 1] <= node) { ++c; ++t; }
+  return t == 8 ? chain_of_node(B, node) : c;
+}
+
 // Chain of a segment node.  The site of a segment (node_site) is stored for
 // the first segment of each site only, the one the estimator needs: storing
 // it for every segment of a batch cost the linking kernels a scattered 4-byte
@@ -1349,6 +1357,7 @@ __global__ void __launch_bounds__(EST_THREADS) __attribute__((amdgpu_waves_per_e
   int clo, chi;
   xcd_share(nchunks, clo, chi);
   for (int chunk = clo; chunk < chi; ++chunk) {
+    const int cb = chain_of_node(B, min(chunk * echunk, M - 1));  // the chunk's first chain
     // the thread's items in groups of EST_U: every memory step of a group
     // (parents, the walks to the roots, the segments' own data, the events
     // below them) is issued for all its items before any is waited for
@@ -1396,7 +1405,7 @@ __global__ void __launch_bounds__(EST_THREADS) __attribute__((amdgpu_waves_per_e
           }
 #pragma unroll
           for (int u = 0; u < EST_U; ++u) {
-            const int c = chain_of_node(B, nd[u]);
+            const int c = chain_from(B, cb, nd[u]);
             const ChainDev& P = B.chains[c];
             f[u] = P.h != 0.0 ? 0u
                  : (philox(B.sweep[c], STAGE_FLIP, (uint32_t)(r[u] - B.chain_node0[c]), 0u, P.key0, P.key1).x & 1u);
@@ -1690,13 +1699,8 @@ __global__ void __launch_bounds__(COL_THREADS) k_flip_collect(DevBatch B, int cl
         if (!root[u]) continue;
         const int i = nd[u];
         // chains are contiguous in node order: from the chunk's first chain
-        // forward over the few cached chain starts, a binary search past eight
-        int c = c0;
-        if (i >= c0_end) {
-          int t = 0;
-          while (t < 8 && c + 1 < B.nchains && B.chain_node0[c + 1] <= i) { ++c; ++t; }
-          if (t == 8) c = chain_of_node(B, i);
-        }
+        // forward over the few cached chain starts
+        const int c = (i < c0_end) ? c0 : chain_from(B, c0, i);
         if (c != cl) {
           if (cl >= 0) flush(cl);
 #pragma unroll
