@@ -602,7 +602,7 @@ def main():
     elif args.workload == "c2":
         out = run_single(args, lg, lg.SQUARE, 128, 128.0, "c2_square128_hbm_traffic")
     elif args.workload == "c5":
-        out = run_single(args, lg, lg.SQUARE, 512, 512.0)
+        out = run_single(args, lg, lg.SQUARE, 512, 512.0, "c5_square512_hbm_traffic")
     elif args.workload == "chain128":
         out = run_single(args, lg, lg.CHAIN, 128, 100.0, None,
                          "chain L=128 periodic S=1/2 Heisenberg AF, J=1, T=0.01, one chain, plain GPU path "

@@ -113,6 +113,23 @@ def test_noalloc_equilibrated_configuration_with_field(loopgpu):
             np.testing.assert_array_equal(a, b)
 
 
+def test_noalloc_virtual_sites(loopgpu):
+    """S > 1/2 (virtual sites, split world lines and the boundary
+    permutation): S = 1, 3/2 chains and the S = 1 square lattice"""
+    lg = loopgpu
+    for kind, L, two_s, beta in ((O.CHAIN, 16, 2, 4.0), (O.CHAIN, 8, 3, 3.0), (O.SQUARE, 4, 2, 2.0)):
+        p = lg.params(kind, L, beta, 1.0, 1.0, seed=5, chain_id=9, two_s=two_s)
+        g = lg.Chain(p)
+        g.sweep(30)
+        cfg = g.get_config()
+        print("noalloc virtual sites", kind, L, two_s, len(cfg[2]), flush=True)
+        for sweep in (30, 31):
+            ref = lg.cluster_config(p, *cfg, sweep)
+            got = noalloc(lg, p, *cfg, sweep)
+            for a, b, name in zip(got, ref, ("labels", "spins", "types", "clusters")):
+                np.testing.assert_array_equal(a, b, err_msg="%s %s" % (name, (kind, L, two_s, sweep)))
+
+
 def test_noalloc_rejects_oversized_and_invalid_input(loopgpu):
     lg = loopgpu
     rng = np.random.default_rng(9)
