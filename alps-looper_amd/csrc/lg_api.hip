@@ -72,8 +72,14 @@ int check_params(const lg_params& p) {
     return set_error(LG_ERR_ARG, "square lattice too large");
   if (!(p.beta > 0) || !std::isfinite(p.beta))
     return set_error(LG_ERR_ARG, "beta must be positive (negative temperature)");
-  if (!std::isfinite(p.jxy) || !std::isfinite(p.jz) || !std::isfinite(p.h))
-    return set_error(LG_ERR_ARG, "couplings and field must be finite");
+  if (!std::isfinite(p.jxy) || !std::isfinite(p.jz) || !std::isfinite(p.h) || !std::isfinite(p.hx))
+    return set_error(LG_ERR_ARG, "couplings and fields must be finite");
+  // an antiferromagnetic XY coupling with a transverse field has a sign
+  // problem that no gauge removes (looper/model_parameter.h:617-644 finds the
+  // lattice frustrated and the reference takes its SIGN path)
+  if (p.hx != 0 && p.jxy > 0)
+    return set_error(LG_ERR_ARG, "a transverse field with Jxy > 0 has a sign problem (the reference's SIGN path); "
+                                 "it is built for Jxy <= 0");
   return LG_OK;
 }
 
@@ -108,7 +114,17 @@ ChainDev make_chain_dev(const lg_params& p) {
   d.key0 = p.seed;
   d.key1 = p.chain_id;
   d.h = p.h;
-  double lam = p.beta * W;
+  // transverse field: one site graph of weight and offset |hx|/2 per site
+  // (weight_impl.h:63-88), its operators on a bond of the site's own after
+  // the lattice's bonds
+  d.nb_real = d.Nb;
+  if (p.hx != 0) {
+    d.site_w = std::fabs(p.hx) / 2;
+    d.site_scale = candidate_scale(p.beta, d.site_w);
+    d.Nb += d.N;
+    d.offset_total += d.N * d.site_w;
+  }
+  double lam = p.beta * std::max(W, d.site_w);
   d.cand_cap = (int)std::ceil(lam + 12.0 * std::sqrt(lam) + 16.0);
   return d;
 }
@@ -117,7 +133,8 @@ ChainDev make_chain_dev(const lg_params& p) {
 static long long op_capacity(const lg_params& p, const ChainDev& d) {
   double v[4], off;
   xxz_weights(p.jxy, p.jz, p.force_scatter, v, &off);
-  double bound = p.beta * d.Nb * (off + std::fabs(p.jz) / 4 + std::fabs(p.jxy) / 2);
+  double bound = p.beta * d.nb_real * (off + std::fabs(p.jz) / 4 + std::fabs(p.jxy) / 2);
+  bound += p.beta * d.N * std::fabs(p.hx);  // site operators: beta |hx| (1/2 + |<S^x>|) per site at most
   return (long long)std::ceil(bound + 10.0 * std::sqrt(bound) + 1024.0);
 }
 
@@ -211,6 +228,7 @@ int Batch::init(const lg_params* p, int n) {
 int Batch::init_exchange(const lg_params& p, const double* betas, int n, uint32_t seed) {
   if (n < 2) return set_error(LG_ERR_ARG, "exchange needs at least two temperatures");
   if (!betas) return set_error(LG_ERR_ARG, "null pointer");
+  if (p.hx != 0) return set_error(LG_ERR_UNSUPPORTED, "replica exchange with a transverse field is not built");
   std::vector<lg_params> ps(n, p);
   for (int t = 0; t < n; ++t) {
     ps[t].beta = betas[t];
@@ -448,7 +466,7 @@ int Batch::set_layout(std::vector<ChainDev>& ds, const std::vector<uint64_t>& sw
   }
   double lam_max = 0;
   for (auto& d : chains)
-    if (d.has_weight) lam_max = std::max(lam_max, std::ldexp(1.0, 52) / d.scale);
+    lam_max = std::max(lam_max, d.beta * std::max(d.has_weight ? d.wsum : 0.0, d.site_w));
   int G = 4;
   while (G < 64 && G < lam_max) G *= 2;
   if (!tuned) cfg.group = G;
@@ -456,11 +474,12 @@ int Batch::set_layout(std::vector<ChainDev>& ds, const std::vector<uint64_t>& sw
   // beta * (offset + half the largest diagonal energy)) picks the staging size
   double ev = 0, ops_bond = 0, nb_total = 0;
   for (size_t c = 0; c < chains.size(); ++c) {
-    const double per_bond = chains[c].offset_total / chains[c].Nb +
+    const ChainDev& d = chains[c];
+    const double per_bond = (d.offset_total - d.N * d.site_w) / d.nb_real +
                             0.5 * (std::fabs(params[c].jz) / 4 + std::fabs(params[c].jxy) / 2);
-    ev += 2.0 * chains[c].beta * chains[c].Nb * per_bond;
-    ops_bond += chains[c].Nb * chains[c].beta * per_bond;
-    nb_total += chains[c].Nb;
+    ev += 2.0 * d.beta * d.nb_real * per_bond + d.beta * d.N * 2 * d.site_w;
+    ops_bond += d.nb_real * d.beta * per_bond + d.beta * d.N * 2 * d.site_w;
+    nb_total += d.Nb;
   }
   LaunchCfg c = cfg;
   // the fill pass stages a bond's lists in LDS when they fit: room for the
@@ -511,6 +530,7 @@ int Batch::set_layout(std::vector<ChainDev>& ds, const std::vector<uint64_t>& sw
   // per-site one at most four lists per site
   if (split && c.link_cap > 1) c.link_cap = 0;
   if (many_lists && c.link_cap == 1) c.link_cap = 0;
+  if (has_site_bonds()) c.link_cap = 0;  // site operators: the per-operator kernel
   return apply_cfg(c);
 }
 
@@ -641,9 +661,11 @@ DevBatch Batch::dev(int from) const {
   B.ring_pos = ring_pos; B.end_count = end_count; B.ring_size = ring_size;
   B.any_field = 0;
   B.any_split = 0;
+  B.any_site = 0;
   for (const auto& d : chains) {
     if (d.h != 0.0) B.any_field = 1;
     if (d.nv > 1) B.any_split = 1;
+    if (d.Nb > d.nb_real) B.any_site = 1;
   }
   B.nslots = nslots; B.xslot = xslot; B.xbeta = xbeta; B.xscale = xscale; B.xlnr = xlnr;
   B.xacc = xacc; B.xstep = xstep; B.xkey0 = xkey0; B.xkey1 = xkey1;
@@ -757,6 +779,7 @@ int Batch::tune(uint32_t sweeps_per_trial) {
       else if (pass == 2) {
         if (links[i] > 1 && has_virtual_sites()) continue;  // these kernels assume wrapping segments
         if (links[i] == 1 && max_site_lists() > 4) continue;  // it merges four lists at most
+        if (links[i] != 0 && has_site_bonds()) continue;      // site operators: the per-operator kernel
         trial.link_cap = links[i];
       }
       else if (pass == 3) trial.compress = i;
@@ -803,6 +826,8 @@ int Batch::set_launch(const lg_launch& l) {
       return set_error(LG_ERR_ARG, "chains with virtual sites (S > 1/2) take linking kernel 0 or 1");
     if (l.link == 1 && max_site_lists() > 4)
       return set_error(LG_ERR_ARG, "sites with more than four bond lists (S > 1 on the chain, S = 1 on the square lattice) take linking kernel 0");
+    if (l.link != 0 && has_site_bonds())
+      return set_error(LG_ERR_ARG, "chains with a transverse field take linking kernel 0");
     c.link_cap = l.link;
   }
   if (l.compress >= 0) {
@@ -866,8 +891,11 @@ void host_site_arrays(const ChainDev& d, const std::vector<int>& spins, const st
   std::vector<std::vector<int>> ev(N);
   for (int b = 0; b < d.Nb; ++b) {
     int s0, s1;
-    bond_sites(d, b, s0, s1);
-    for (int k = bo[b]; k < bo[b + 1]; ++k) { ev[s0].push_back(k); ev[s1].push_back(k); }
+    op_sites(d, b, s0, s1);
+    for (int k = bo[b]; k < bo[b + 1]; ++k) {
+      ev[s0].push_back(k);
+      if (s1 != s0 || b < d.nb_real) ev[s1].push_back(k);  // a site operator is one event
+    }
   }
   std::vector<int> bond_of(ot.size());
   for (int b = 0; b < d.Nb; ++b) for (int k = bo[b]; k < bo[b + 1]; ++k) bond_of[k] = b;
@@ -912,6 +940,8 @@ int validate_config(const ChainDev& d, const int32_t* bond_offsets, const double
       if (k > bond_offsets[b] && times[k] < times[k - 1])
         return set_error(LG_ERR_ARG, "operator times not ascending within a bond");
       if (types[k] < 0 || (types[k] >> 2) > 3) return set_error(LG_ERR_ARG, "invalid operator type");
+      if (b >= d.nb_real && (types[k] >> 2) != 0)
+        return set_error(LG_ERR_ARG, "a site operator (transverse field) takes graph 0");
     }
   return LG_OK;
 }
@@ -1118,6 +1148,7 @@ int Batch::sync_schedule() {
     if (d.sched_n <= 0) continue;
     d.beta = 1.0 / schedule_temperature(sched_host.data() + 2 * d.sched_off, d.sched_n, (int)host_sweep[c]);
     d.scale = candidate_scale(d.beta, d.wsum);
+    d.site_scale = candidate_scale(d.beta, d.site_w);
     any = true;
   }
   if (!any) return LG_OK;
@@ -1169,6 +1200,7 @@ int Batch::set_schedule(int n, const int32_t* steps, const double* temps) {
   if (n == 0) {  // fixed temperature again
     chains[0].beta = p.beta;
     chains[0].scale = candidate_scale(p.beta, chains[0].wsum);
+    chains[0].site_scale = candidate_scale(p.beta, chains[0].site_w);
     LG_HIP(hipMemcpy(d_chains, chains.data(), sizeof(ChainDev), hipMemcpyHostToDevice));
     invalidate_graphs();
     return LG_OK;
@@ -1334,6 +1366,7 @@ int lg_cluster_config(const lg_params* p, const int32_t* spins, const int32_t* b
                       const int32_t* types, uint64_t sweep, int32_t* labels, int32_t* spins_out,
                       int32_t* types_out, int64_t* n_clusters) {
   if (!p || !spins || !bond_offsets) return set_error(LG_ERR_ARG, "null pointer");
+  if (p->hx != 0) return set_error(LG_ERR_UNSUPPORTED, "the cluster stage of a given configuration takes hx = 0");
   lg_params q = *p;
   // capacity from the configuration itself
   lg::ChainDev d = lg::make_chain_dev(q);

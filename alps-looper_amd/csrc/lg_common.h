@@ -11,6 +11,10 @@
 //    info = bond << 3 | graph << 1 | offdiagonal  (cf. operator.h:52-117).
 //  * Imaginary time tau lives on the grid k * 2^-52, k in [0, 2^52), and is
 //    stored as an exact FP64 value.
+//  * A transverse field adds one bond per site after the lattice's bonds
+//    (ChainDev::nb_real): its operators are site operators, one event on
+//    the site, the world line cut there (looper's site graph,
+//    graph_impl.h:66-85).
 //  * Union-find nodes are world-line segments: for site s with m events
 //    (operator legs touching s, ordered by (tau, bond, rank)), node soff[s]+j
 //    is the segment just below event j; node soff[s] is the wrapping segment
@@ -64,6 +68,11 @@ struct ChainDev {
   int sched_n, sched_off;            // annealing schedule: entries and offset in the batch's table (0: fixed beta)
   int nv;                            // virtual sites per real site (2S; 1 for S=1/2)
   int cand_bstride, cand_sstride;    // candidate slot strides per bond and per slot
+  // transverse field -hx S^x: bonds [nb_real, Nb) are the sites' own,
+  // bond nb_real + s holding site s's operators (Nb = nb_real + N); without
+  // one nb_real = Nb
+  int nb_real;
+  double site_w, site_scale;         // site graph weight |hx|/2; candidate scale 2^52 / (beta site_w)
 };
 
 // Temperature of Monte Carlo step `step` on an annealing schedule of n
@@ -260,6 +269,16 @@ LG_HD void bond_sites(const ChainDev& c, int b, int& s0, int& s1) {
     s0 = r0 * c.nv + a;
     s1 = r1 * c.nv + d;
   }
+}
+
+// sites of the operators of bond b: a site bond (transverse field) has its
+// one site on both sides
+LG_HD void op_sites(const ChainDev& c, int b, int& s0, int& s1) {
+  if (b >= c.nb_real) {
+    s0 = s1 = b - c.nb_real;
+    return;
+  }
+  bond_sites(c, b, s0, s1);
 }
 
 LG_HD int bond_dir(const ChainDev& c, int b) {

@@ -83,6 +83,12 @@ class Batch {
     for (const auto& d : chains) if (d.nv > 1) return true;
     return false;
   }
+  // some chain has a transverse field: site operators, which only the
+  // per-operator linking kernel places
+  bool has_site_bonds() const {
+    for (const auto& d : chains) if (d.Nb > d.nb_real) return true;
+    return false;
+  }
   int max_site_lists() const {
     int n = 0;
     for (const auto& d : chains) n = std::max(n, site_lists(d));

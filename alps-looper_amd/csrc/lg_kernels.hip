@@ -248,6 +248,106 @@ __device__ __forceinline__ void scan_advance_epoch(const DevBatch& B) {
 // ---------------------------------------------------------------------------
 // (a) diagonal update, pass 1: count and store accepted candidates
 // ---------------------------------------------------------------------------
+// One bond's candidates: kept off-diagonal operators counted, Poisson
+// candidates drawn and tested against the previous configuration's spins,
+// the accepted ones stored.  SITE: a site bond (transverse field), whose
+// candidates come at rate beta |hx|/2 with the one site graph (g = 0),
+// compatible with either spin (graph_impl.h:66-69).  The two cases are
+// separate kernels, so that the bond case keeps its registers.
+template <int G, bool SITE>
+__device__ __forceinline__ void diag_count_bond(const DevBatch& B, const ChainDev& P, int c, int gb, int gl,
+                                                uint64_t below) {
+  const int b = gb - P.bond_base;
+  const uint32_t sweep = B.sweep[c];
+  int s0, s1;
+  if constexpr (SITE) s0 = s1 = b - P.nb_real;
+  else bond_sites(P, b, s0, s1);
+  // kept (off-diagonal) operators of the previous configuration
+  const int ob = P.old_bond_base + b;
+  const int o0 = B.bo_old[ob], o1 = B.bo_old[ob + 1];
+  int kept = 0;
+  for (int r0 = o0; r0 < o1; r0 += G) {
+    int r = r0 + gl;
+    bool off = (r < o1) && (B.oi_old[r] & 1);
+    uint64_t m = gballot<G>(off);
+    if (r < o1) B.kpre[r] = kept + __popcll(m & below);
+    kept += __popcll(m);
+  }
+  // Poisson candidates on [0,1): exponential gaps on the 2^-52 grid
+  int acc = 0;
+  if (SITE ? P.site_w > 0 : P.has_weight != 0) {
+    const double scale = SITE ? P.site_scale : P.scale;
+    // slot j of the bond at j * cand_sstride (a chain holds fewer than 2^31 slots)
+    const long long cb = P.cand_base + (long long)b * P.cand_bstride;
+    double* const ctb = B.cand_t + cb;
+    uint8_t* const cgb = B.cand_g + cb;
+    const int base0 = B.soff_old[P.old_site_base + s0], m0 = B.soff_old[P.old_site_base + s0 + 1] - base0;
+    const int base1 = B.soff_old[P.old_site_base + s1], m1 = B.soff_old[P.old_site_base + s1 + 1] - base1;
+    int cur0 = 0, cur1 = 0;  // world-line cursors (candidate times ascend)
+    // a wave per bond: the two lines in LDS when they fit (a wave-uniform
+    // choice; the wave's own slice, so no block barrier)
+    double* lt0 = nullptr;
+    double* lt1 = nullptr;
+    if constexpr (G == 64) {
+      __shared__ double lines[256 / 64][2][DIAG_LDS];
+      if (m0 <= DIAG_LDS && m1 <= DIAG_LDS) {
+        lt0 = lines[threadIdx.x >> 6][0];
+        lt1 = lines[threadIdx.x >> 6][1];
+        __builtin_amdgcn_wave_barrier();  // the previous bond's searches are done
+        for (int q = gl; q < m0; q += 64) lt0[q] = B.evt_old[base0 + q];
+        for (int q = gl; q < m1; q += 64) lt1[q] = B.evt_old[base1 + q];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      }
+    }
+    uint64_t Tb = 0;
+    uint32_t kb = 0;
+    while (true) {
+      U4 x = philox(sweep, STAGE_DIAG, (uint32_t)b, kb + (uint32_t)gl, P.key0, P.key1);
+      uint64_t gap = exp_gap(x, scale);
+      uint64_t incl = gscan_incl_u64<G>(gap);
+      uint64_t T = Tb + incl;
+      const bool valid = T < TGRID;
+      // lanes past the end of [0,1) search for -1 (found at once)
+      const double tau = valid ? (double)T * TGRID_INV : -1.0;
+      const int g = SITE ? 0 : choose_graph(x.z, P);
+      int c0, c1;
+      if (lt0) {
+        lds_spins_prev(B, lt0, base0, m0, lt1, base1, m1, tau, c0, c1);
+      } else {
+        c0 = group_spin_prev<G>(B, base0, m0, tau, cur0);
+        c1 = group_spin_prev<G>(B, base1, m1, tau, cur1);
+      }
+      const bool accept = valid && (c0 >= 0) && (c1 >= 0) && (SITE || ((g & 1) ^ c0 ^ c1) != 0);
+      uint64_t m = gballot<G>(accept);
+      if (accept) {
+        int slot = acc + __popcll(m & below);
+        if (slot < P.cand_cap) {
+          ctb[slot * P.cand_sstride] = tau;
+          cgb[slot * P.cand_sstride] = (uint8_t)g;
+        } else {
+          atomicOr(B.flags, FLAG_CAND_OVERFLOW);
+        }
+      }
+      acc += __popcll(m);
+      Tb += gbcast_u64<G>(incl, G - 1);
+      kb += G;
+      if (Tb >= TGRID) break;
+      // draws beyond 4x the slot count: capacity error (keeps the loop bounded)
+      if (kb > 4u * (uint32_t)P.cand_cap + 64u) {
+        if (gl == 0) atomicOr(B.flags, FLAG_CAND_OVERFLOW);
+        break;
+      }
+    }
+    if (acc > P.cand_cap) acc = P.cand_cap;
+  }
+  if (gl == 0) {
+    B.counts[gb] = kept + acc;
+    B.acc_count[gb] = acc;
+  }
+}
+
 template <int G>
 // (at most 96 VGPRs, five waves per SIMD, where that needs no spills)
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(G == 32 ? 4 : 5))) k_diag_count(DevBatch B) {
@@ -264,93 +364,25 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(G == 3
        gb += gridDim.x * groups_per_block) {
     const int c = B.bond_chain[gb];
     const ChainDev P = B.chains[c];
-    const int b = gb - P.bond_base;
-    const uint32_t sweep = B.sweep[c];
-    int s0, s1;
-    bond_sites(P, b, s0, s1);
-    // kept (off-diagonal) operators of the previous configuration
-    const int ob = P.old_bond_base + b;
-    const int o0 = B.bo_old[ob], o1 = B.bo_old[ob + 1];
-    int kept = 0;
-    for (int r0 = o0; r0 < o1; r0 += G) {
-      int r = r0 + gl;
-      bool off = (r < o1) && (B.oi_old[r] & 1);
-      uint64_t m = gballot<G>(off);
-      if (r < o1) B.kpre[r] = kept + __popcll(m & below);
-      kept += __popcll(m);
-    }
-    // Poisson candidates on [0,1): exponential gaps on the 2^-52 grid
-    int acc = 0;
-    if (P.has_weight) {
-      // slot j of the bond at j * cand_sstride (a chain holds fewer than 2^31 slots)
-      const long long cb = P.cand_base + (long long)b * P.cand_bstride;
-      double* const ctb = B.cand_t + cb;
-      uint8_t* const cgb = B.cand_g + cb;
-      const int base0 = B.soff_old[P.old_site_base + s0], m0 = B.soff_old[P.old_site_base + s0 + 1] - base0;
-      const int base1 = B.soff_old[P.old_site_base + s1], m1 = B.soff_old[P.old_site_base + s1 + 1] - base1;
-      int cur0 = 0, cur1 = 0;  // world-line cursors (candidate times ascend)
-      // a wave per bond: the two lines in LDS when they fit (a wave-uniform
-      // choice; the wave's own slice, so no block barrier)
-      double* lt0 = nullptr;
-      double* lt1 = nullptr;
-      if constexpr (G == 64) {
-        __shared__ double lines[256 / 64][2][DIAG_LDS];
-        if (m0 <= DIAG_LDS && m1 <= DIAG_LDS) {
-          lt0 = lines[threadIdx.x >> 6][0];
-          lt1 = lines[threadIdx.x >> 6][1];
-          __builtin_amdgcn_wave_barrier();  // the previous bond's searches are done
-          for (int q = gl; q < m0; q += 64) lt0[q] = B.evt_old[base0 + q];
-          for (int q = gl; q < m1; q += 64) lt1[q] = B.evt_old[base1 + q];
-          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-          __builtin_amdgcn_wave_barrier();
-          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        }
-      }
-      uint64_t Tb = 0;
-      uint32_t kb = 0;
-      while (true) {
-        U4 x = philox(sweep, STAGE_DIAG, (uint32_t)b, kb + (uint32_t)gl, P.key0, P.key1);
-        uint64_t gap = exp_gap(x, P.scale);
-        uint64_t incl = gscan_incl_u64<G>(gap);
-        uint64_t T = Tb + incl;
-        const bool valid = T < TGRID;
-        // lanes past the end of [0,1) search for -1 (found at once)
-        const double tau = valid ? (double)T * TGRID_INV : -1.0;
-        const int g = choose_graph(x.z, P);
-        int c0, c1;
-        if (lt0) {
-          lds_spins_prev(B, lt0, base0, m0, lt1, base1, m1, tau, c0, c1);
-        } else {
-          c0 = group_spin_prev<G>(B, base0, m0, tau, cur0);
-          c1 = group_spin_prev<G>(B, base1, m1, tau, cur1);
-        }
-        const bool accept = valid && (c0 >= 0) && (c1 >= 0) && (((g & 1) ^ c0 ^ c1) != 0);
-        uint64_t m = gballot<G>(accept);
-        if (accept) {
-          int slot = acc + __popcll(m & below);
-          if (slot < P.cand_cap) {
-            ctb[slot * P.cand_sstride] = tau;
-            cgb[slot * P.cand_sstride] = (uint8_t)g;
-          } else {
-            atomicOr(B.flags, FLAG_CAND_OVERFLOW);
-          }
-        }
-        acc += __popcll(m);
-        Tb += gbcast_u64<G>(incl, G - 1);
-        kb += G;
-        if (Tb >= TGRID) break;
-        // draws beyond 4x the slot count: capacity error (keeps the loop bounded)
-        if (kb > 4u * (uint32_t)P.cand_cap + 64u) {
-          if (gl == 0) atomicOr(B.flags, FLAG_CAND_OVERFLOW);
-          break;
-        }
-      }
-      if (acc > P.cand_cap) acc = P.cand_cap;
-    }
-    if (gl == 0) {
-      B.counts[gb] = kept + acc;
-      B.acc_count[gb] = acc;
-    }
+    if (gb - P.bond_base >= P.nb_real) continue;  // site bonds: k_diag_count_site
+    diag_count_bond<G, false>(B, P, c, gb, gl, below);
+  }
+}
+
+// the site bonds (transverse field), a group of G lanes per site
+template <int G>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(G == 32 ? 4 : 5))) k_diag_count_site(DevBatch B) {
+  if (overflowed(B)) return;
+  const int groups_per_block = blockDim.x / G;
+  const int gl = glane<G>();
+  const uint64_t below = (gl == 0) ? 0ull : ((1ull << gl) - 1);
+  int gs_first = blockIdx.x * groups_per_block + threadIdx.x / G;
+  if (G == 64) gs_first = __builtin_amdgcn_readfirstlane(gs_first);
+  for (int gs = gs_first; gs < B.nsites; gs += gridDim.x * groups_per_block) {
+    const int c = B.site_chain[gs];
+    const ChainDev P = B.chains[c];
+    if (P.Nb == P.nb_real) continue;  // no transverse field
+    diag_count_bond<G, true>(B, P, c, P.bond_base + P.nb_real + (gs - P.site_base), gl, below);
   }
 }
 
@@ -375,6 +407,7 @@ __device__ __forceinline__ int site_events(const DevBatch& B, int gs) {
     const int nb = site_lists(P);
     for (int i = 0; i < nb; ++i) m += B.counts[P.bond_base + site_bond(P, s, i)];
   }
+  if (B.any_site && P.Nb > P.nb_real) m += B.counts[P.bond_base + P.nb_real + s];  // site operators
   return m;
 }
 
@@ -555,7 +588,7 @@ __global__ void __launch_bounds__(256) k_diag_fill(DevBatch B, int fill_cap) {
         const int kr = B.kpre[r];
         const double tau = B.ot_old[r];
         U4 x = philox(sweep, STAGE_REGRAPH, (uint32_t)b, (uint32_t)kr, P.key0, P.key1);
-        int g = ((uint64_t)x.x < P.toff) ? 0 : 1;
+        int g = ((B.any_site && b >= P.nb_real) || (uint64_t)x.x < P.toff) ? 0 : 1;  // a site operator keeps the site graph
         int pos = nb + kr + lower_bound_ds(tc, ts, acc, tau);
         B.ot_new[pos] = tau;
         B.oi_new[pos] = info_b | (g << 1) | 1;
@@ -934,6 +967,9 @@ __global__ void __launch_bounds__(256) k_node_init(DevBatch B) {
 // the linking itself writes no parent, so the unions of different threads
 // only meet in the lock-free union-find.  This saves the separate union
 // pass, which re-read the four legs and the type of every operator.
+// SITES: some chain has a transverse field (site operators); a separate
+// instance, so that the plain one carries none of its tests.
+template <bool SITES>
 __global__ void __launch_bounds__(256) k_link_ops(DevBatch B) {
   if (overflowed(B)) return;
   const int nops = B.totals[0];
@@ -946,12 +982,21 @@ __global__ void __launch_bounds__(256) k_link_ops(DevBatch B) {
     const int b = gb - P.bond_base;
     const double tau = B.ot_new[k];
     const int r = k - B.bo_new[gb];
-    const uint8_t dflag = bond_dir(P, b) ? EV_DIR1 : 0;
+    // a site operator (transverse field): one event, on its site, with no
+    // bond vector (no source flag), and no union: the world line is cut
+    const bool site = SITES && b >= P.nb_real;
+    const uint8_t dflag = (!site && bond_dir(P, b)) ? EV_DIR1 : 0;
     int s0, s1;
-    bond_sites(P, b, s0, s1);
+    if (SITES) op_sites(P, b, s0, s1);
+    else bond_sites(P, b, s0, s1);
     int lo_leg[2], up_leg[2];  // lower and upper segment of each side
 #pragma unroll
     for (int side = 0; side < 2; ++side) {
+      if (side && site) {
+        B.l1[k] = lo_leg[0];
+        B.u1[k] = up_leg[0];
+        break;
+      }
       const int s = side ? s1 : s0;
       int rank = r;
       int par = B.kc_new[k];
@@ -978,6 +1023,7 @@ __global__ void __launch_bounds__(256) k_link_ops(DevBatch B) {
         const int nbd = site_lists(P);
         for (int i = 0; i < nbd; ++i) other_list(site_bond(P, s, i));
       }
+      if (SITES && P.Nb > P.nb_real) other_list(P.nb_real + s);  // the site's own operators
       const int gs = P.site_base + s;
       const int n0 = B.soff_new[gs], n1 = B.soff_new[gs + 1];
       const int node = n0 + rank;
@@ -987,8 +1033,8 @@ __global__ void __launch_bounds__(256) k_link_ops(DevBatch B) {
       const uint8_t sp = B.spins_old[P.old_site_base + s] ^ (uint8_t)(par & 1);
       B.evt_new[node] = tau;
       B.segspin_new[node] = sp;
-      B.evflags[node] = (uint8_t)((info & 1) | (side ? 0 : EV_SRC) | dflag | (rank == 0 ? EV_FIRST : 0) | gflag |
-                                  (split && rank == 0 ? EV_NOWRAP : 0));
+      B.evflags[node] = (uint8_t)((info & 1) | ((side || site) ? 0 : EV_SRC) | dflag | (rank == 0 ? EV_FIRST : 0) |
+                                  gflag | (split && rank == 0 ? EV_NOWRAP : 0));
       if (rank == 0) B.node_site[node] = gs;
       if (split && up == n1 - 1) {  // the last event: the site's top segment, up to tau = 1
         B.evt_new[up] = 1.0;
@@ -999,6 +1045,7 @@ __global__ void __launch_bounds__(256) k_link_ops(DevBatch B) {
       lo_leg[side] = node;
       up_leg[side] = up;
     }
+    if (site) continue;
     const int g = (info >> 1) & 3;
     const int l0 = lo_leg[0], u0 = up_leg[0], l1 = lo_leg[1], u1 = up_leg[1];
     if (g & 2) {            // frozen: all four legs (graph_impl.h:282-283)
@@ -1807,6 +1854,7 @@ __global__ void __launch_bounds__(END_THREADS) k_end_sweep(DevBatch B) {
       const double beta = 1.0 / schedule_temperature(B.sched + 2 * P.sched_off, P.sched_n, (int)B.sweep[c]);
       W->beta = beta;
       W->scale = candidate_scale(beta, P.wsum);
+      W->site_scale = candidate_scale(beta, P.site_w);
     }
     __threadfence();
     is_last = (atomicAdd(B.end_count, 1) == (int)gridDim.x - 1);
@@ -1904,7 +1952,13 @@ static void launch_diag(const DevBatch& B, const LaunchCfg& L, hipStream_t st, b
     const size_t lds = G == 64 ? (size_t)(256 / 64) * 2 * L.fill_cap * sizeof(double) : 0;
     hipLaunchKernelGGL(k_diag_fill<G>, dim3(blocks), dim3(256), lds, st, B, L.fill_cap);
   }
-  else hipLaunchKernelGGL(k_diag_count<G>, dim3(blocks), dim3(256), 0, st, B);
+  else {
+    hipLaunchKernelGGL(k_diag_count<G>, dim3(blocks), dim3(256), 0, st, B);
+    if (B.any_site) {
+      const int sblocks = max(1, min(L.max_blocks_bond, (B.nsites + groups_per_block - 1) / groups_per_block));
+      hipLaunchKernelGGL(k_diag_count_site<G>, dim3(sblocks), dim3(256), 0, st, B);
+    }
+  }
 }
 
 static void diag_dispatch(const DevBatch& B, const LaunchCfg& L, hipStream_t st, bool fill) {
@@ -1957,7 +2011,8 @@ void launch_cluster_stage(const DevBatch& B, const LaunchCfg& L, hipStream_t st,
   // ms at 8 steps per block, 5.95 at one)
   if (L.link_cap == 0) {
     hipLaunchKernelGGL(k_node_init, dim3(L.blocks_node), dim3(256), 0, st, B);
-    hipLaunchKernelGGL(k_link_ops, dim3(L.blocks_op), dim3(256), 0, st, B);
+    if (B.any_site) hipLaunchKernelGGL(k_link_ops<true>, dim3(L.blocks_op), dim3(256), 0, st, B);
+    else hipLaunchKernelGGL(k_link_ops<false>, dim3(L.blocks_op), dim3(256), 0, st, B);
   } else if (L.link_cap == 1) {
     const int blocks = max(1, min(L.blocks_op, (B.nsites + 255) / 256));
     hipLaunchKernelGGL(k_link_site, dim3(blocks), dim3(256), 0, st, B);

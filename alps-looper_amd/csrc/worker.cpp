@@ -255,9 +255,11 @@ class loop_worker {
         throw std::invalid_argument("local_S must be 1/2, 1, 3/2 or 2 on the chain lattice, 1/2 or 1 on the square lattice");
       lp_.two_s = (int32_t)t;
     }
-    for (const char* k : {"Gamma", "D"})
-      if (p.defined(k) && p.num(k) != 0.0)
-        throw std::invalid_argument(std::string("parameter ") + k + " != 0 (transverse field / single-ion terms) is not supported");
+    if (p.defined("D") && p.num("D") != 0.0)
+      throw std::invalid_argument("parameter D != 0 (single-ion term) is not supported");
+    // transverse field -Gamma S^x (model_parameter.h:67, the ALPS spin
+    // model's Gamma): site operators; sign-free with Jxy <= 0
+    lp_.hx = p.num("Gamma", 0.0);
     lp_.h = p.num("h", 0.0);
     // improved estimators unless a field or DISABLE_IMPROVED_ESTIMATOR (path_integral.C:217)
     improved_ = lp_.h == 0.0 && !p.defined("DISABLE_IMPROVED_ESTIMATOR");
@@ -307,6 +309,7 @@ class loop_worker {
     }
     if (mc || ms || !custom.empty()) {
       if (lp_.two_s > 1) throw std::invalid_argument("correlation measurements are built for S=1/2 only");
+      if (lp_.hx != 0) throw std::invalid_argument("correlation measurements are built without a transverse field");
       if (lp_.L < 2 || lp_.L % 2) throw std::invalid_argument("L must be even");
       const int origin = p.defined("INITIAL_SITE") ? (int)std::llround(p.num("INITIAL_SITE")) : -1;
       if (p.defined("INITIAL_SITE") && (origin < 0 || origin >= n_sites_))

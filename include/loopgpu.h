@@ -50,7 +50,12 @@ enum { LG_LATTICE_CHAIN = 0, LG_LATTICE_SQUARE = 1 };
  * (rs, rt) carries rb*nv^2 + a*nv + c = (rs*nv + a, rt*nv + c)), and a random
  * spin-conserving permutation joining the tau = 1 and tau = 0 ends of each
  * real site's virtual world lines (path_integral.C:589-615); sites, bonds,
- * spins and operators at the ABI are then the virtual ones.
+ * spins and operators at the ABI are then the virtual ones.  A transverse
+ * field (hx != 0) adds one bond per (virtual) site after the lattice's bonds:
+ * bond Nb_lattice + s holds site s's operators (graph 0, one leg below and
+ * one above on site s; an off-diagonal one flips that spin), so the bond
+ * count at the ABI (bond_offsets of lg_chain_get_config / set_config) is
+ * Nb_lattice + N.
  * chain_id selects an independent random stream for the same seed. */
 typedef struct lg_params {
   int32_t lattice;       /* LG_LATTICE_CHAIN or LG_LATTICE_SQUARE */
@@ -63,6 +68,8 @@ typedef struct lg_params {
   uint32_t chain_id;     /* independent stream index */
   double h;              /* longitudinal field h (site term -h S^z, looper's FIELD path) */
   int32_t two_s;         /* 2S: 1 (S=1/2; 0 is taken as 1), 2, 3 or 4 (chain), 2 (square) */
+  double hx;             /* transverse field hx (site term -hx S^x, looper/model_parameter.h:67);
+                            sign-free only with jxy <= 0, so jxy > 0 with hx != 0 is rejected */
 } lg_params;
 
 /* Per-sweep collector sums (all improved estimators) -- see the collector of

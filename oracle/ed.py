@@ -33,6 +33,8 @@ def lattice_bonds(kind, L):
             bonds.append((s, x * L + (y + 1) % L))
             bonds.append((s, ((x + 1) % L) * L + y))
         gauge = [1 if (s // L + s % L) % 2 == 0 else -1 for s in range(N)]
+    elif kind == "site":  # one site, no bonds (the reference's "site" lattice)
+        N, bonds, gauge = 1, [], [1]
     else:
         raise ValueError(kind)
     return N, bonds, np.array(gauge, dtype=float)
@@ -127,6 +129,68 @@ def diagonalize_spin(kind, L, T, two_s, jxy=1.0, jz=1.0):
             kub += (A * A * wij).sum()
         res[name + "Magnetization^2"] = v2 / Z
         res[name + "Susceptibility"] = kub / Z / V
+    res["Staggered Structure Factor"] = res["Staggered Magnetization^2"] / V
+    return res
+
+
+def diagonalize_transverse(kind, L, T, two_s=1, jxy=1.0, jz=1.0, hx=0.0, h=0.0):
+    """Thermal averages of the spin-S XXZ model in a transverse field,
+    H = sum_b [Jz Sz Sz + Jxy/2 (S+S- + S-S+)] - hx sum_i S^x_i - h sum_i S^z_i
+    (looper/model_parameter.h:63-68, S^x = (S+ + S-)/2), on the whole product
+    space: S^z is no longer conserved, so the Hamiltonian is diagonalised
+    densely in one block and every static average and Kubo susceptibility
+    (diag.C:252-293) is taken in its eigenbasis.  Sizes: (2S+1)^N states."""
+    beta = 1.0 / T
+    N, bonds, gauge = lattice_bonds(kind, L)
+    V = float(N)
+    S = 0.5 * two_s
+    levels = [S - q for q in range(two_s + 1)]
+    ms = np.array(list(itertools.product(levels, repeat=N)), dtype=float)[:, ::-1]
+    dim = len(ms)
+    key = {tuple(np.rint(2 * c).astype(int)): n for n, c in enumerate(ms)}
+    H = np.zeros((dim, dim))
+    for n, c in enumerate(ms):
+        H[n, n] -= h * c.sum()
+        for (a, b) in bonds:
+            H[n, n] += jz * c[a] * c[b]
+            for da, db in ((1, -1), (-1, 1)):
+                ma, mb = c[a] + da, c[b] + db
+                if abs(ma) > S + 1e-9 or abs(mb) > S + 1e-9:
+                    continue
+                amp = np.sqrt(S * (S + 1) - c[a] * (c[a] + da)) * np.sqrt(S * (S + 1) - c[b] * (c[b] + db))
+                d = c.copy()
+                d[a], d[b] = ma, mb
+                H[key[tuple(np.rint(2 * d).astype(int))], n] += 0.5 * jxy * amp
+        for a in range(N):  # -hx S^x_a = -hx/2 (S+_a + S-_a)
+            for da in (1, -1):
+                ma = c[a] + da
+                if abs(ma) > S + 1e-9:
+                    continue
+                d = c.copy()
+                d[a] = ma
+                H[key[tuple(np.rint(2 * d).astype(int))], n] -= 0.5 * hx * np.sqrt(S * (S + 1) - c[a] * (c[a] + da))
+    E, U = np.linalg.eigh(H)
+    e0 = E.min()
+    w = np.exp(-beta * (E - e0))
+    Z = w.sum()
+    res = {}
+    ene = (w * E).sum() / Z
+    ene2 = (w * E * E).sum() / Z
+    res["Energy"] = ene
+    res["Energy Density"] = ene / V
+    res["Specific Heat"] = beta * beta * (ene2 - ene * ene) / V
+    res["Ground State Energy"] = e0
+    dE = E[:, None] - E[None, :]
+    with np.errstate(divide="ignore", invalid="ignore"):
+        wij = -(w[:, None] - w[None, :]) / dE
+    wij = np.where(np.abs(dE) <= 1e-12, beta * w[:, None] * np.ones_like(dE), wij)
+    for name, d in (("", ms.sum(axis=1)), ("Staggered ", ms @ gauge)):
+        A = U.T @ (d[:, None] * U)        # <i|O|j> in the eigenbasis
+        A2 = U.T @ ((d ** 2)[:, None] * U)
+        A4 = U.T @ ((d ** 4)[:, None] * U)
+        res[name + "Magnetization^2"] = (w * np.diag(A2)).sum() / Z
+        res[name + "Magnetization^4"] = (w * np.diag(A4)).sum() / Z
+        res[name + "Susceptibility"] = (A * A * wij).sum() / Z / V
     res["Staggered Structure Factor"] = res["Staggered Magnetization^2"] / V
     return res
 

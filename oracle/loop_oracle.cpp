@@ -152,6 +152,7 @@ inline uint64_t exp_gap(const u32x4& x, double scale) {
 struct Lattice {
   int kind = 0, L = 0, N = 0, Nb = 0, dim = 1;
   int nv = 1;  // virtual sites per real site (2S)
+  int nb_real = 0;  // bonds between two sites; bonds nb_real + s are site s's own (transverse field)
   std::vector<int> src, tgt, dir, gauge;
   std::vector<std::vector<int>> site_bonds;  // bonds incident to each site
   // nv > 1 (S = nv/2): looper's virtual lattice for S > 1/2
@@ -179,6 +180,7 @@ struct Lattice {
       for (int v = 0; v < N; ++v) gauge[v] = real.gauge[v / nv];
       site_bonds.assign(N, std::vector<int>());
       for (int b = 0; b < Nb; ++b) { site_bonds[src[b]].push_back(b); site_bonds[tgt[b]].push_back(b); }
+      nb_real = Nb;
       return;
     }
     kind = k; L = l; nv = 1;
@@ -201,9 +203,21 @@ struct Lattice {
     }
     site_bonds.assign(N, std::vector<int>());
     for (int b = 0; b < Nb; ++b) { site_bonds[src[b]].push_back(b); site_bonds[tgt[b]].push_back(b); }
+    nb_real = Nb;
   }
+  // Transverse field: site s's operators live on bond nb_real + s, whose
+  // source and target are s (a site operator has one leg below and one above
+  // on its own site, and no bond vector)
+  void add_site_bonds() {
+    for (int s = 0; s < N; ++s) {
+      src.push_back(s); tgt.push_back(s); dir.push_back(-1);
+      site_bonds[s].push_back(Nb + s);
+    }
+    Nb += N;
+  }
+  bool is_site_bond(int b) const { return b >= nb_real; }
   // relative bond vector component d (ALPS bond_vector_relative, unit lattice constant)
-  int vr(int b, int d) const { return (kind == 0) ? 1 : (dir[b] == d ? 1 : 0); }
+  int vr(int b, int d) const { return is_site_bond(b) ? 0 : (kind == 0) ? 1 : (dir[b] == d ? 1 : 0); }
 };
 
 // ---------------------------------------------------------------------------
@@ -545,7 +559,7 @@ struct LooperChain {
   Schedule sched; // annealing: beta = 1 / T(mcs) before each sweep (path_integral.C:325)
 
   void init(int kind, int L, double b, double jxy, double jz, double fs, uint32_t seed, double field = 0,
-            int nv = 1) {
+            int nv = 1, double hx = 0) {
     lat.init(kind, L, nv);
     w = xxz_weights(jxy, jz, fs);
     beta = b;
@@ -555,9 +569,19 @@ struct LooperChain {
     for (int bd = 0; bd < lat.Nb; ++bd)
       for (int g = 0; g < 4; ++g)
         if (w.v[g] > 1e-10) { wl.push_back(w.v[g]); graph_of.push_back({bd, g}); graph_weight += w.v[g]; }
+    offset_total = lat.Nb * w.offset;
+    // transverse field -hx S^x: the site graph (weight and offset |hx|/2,
+    // weight_impl.h:63-88) at every site, drawn with the bond graphs in one
+    // table by weight (the model's choose_graph over all locations)
+    if (hx != 0) {
+      const int nb = lat.Nb;
+      lat.add_site_bonds();
+      const double vs = std::fabs(hx) / 2;
+      for (int s = 0; s < lat.N; ++s) { wl.push_back(vs); graph_of.push_back({nb + s, 0}); graph_weight += vs; }
+      offset_total += lat.N * vs;
+    }
     if (!wl.empty()) alias.init(wl);
     if (graph_weight < 1e-20) graph_weight = 1e-20;
-    offset_total = lat.Nb * w.offset;
     p_off = (w.v[0] + w.v[1] > 1e-10) ? w.v[0] / (w.v[0] + w.v[1]) : 1;
     spins.assign(lat.N, 0);
     spins_c.assign(lat.N, 0);
@@ -587,7 +611,8 @@ struct LooperChain {
       if (times[tmi] < ops_p[opi].time) {
         auto bg = graph_of[alias(rnd)];
         int b = bg.first, g = bg.second;
-        if (is_compatible(g, spins_c[lat.src[b]], spins_c[lat.tgt[b]])) {
+        // a site graph is compatible with either spin (graph_impl.h:66-69)
+        if (lat.is_site_bond(b) || is_compatible(g, spins_c[lat.src[b]], spins_c[lat.tgt[b]])) {
           ops.push_back(LooperOp{b, g << 2, times[tmi], 0, 0});
           ++tmi;
         } else {
@@ -601,6 +626,16 @@ struct LooperChain {
       }
       LooperOp& oi = ops.back();
       int b = oi.bond, s0 = lat.src[b], s1 = lat.tgt[b];
+      if (lat.is_site_bond(b)) {
+        // site operator (path_integral.C:555-563): the world line is cut,
+        // a new fragment starts above (graph_impl.h:78-84)
+        if (oi.type & 1) spins_c[s0] ^= 1;
+        frag[fid] = UFNode();
+        oi.loop0 = current[s0];
+        oi.loop1 = current[s0] = fid;
+        ++fid;
+        continue;
+      }
       if (oi.type & 1) {
         int g = (rnd() < p_off) ? 0 : 1;    // choose_offdiagonal, graph_impl.h:323-327
         oi.type = (oi.type & 1) | (g << 2);
@@ -633,6 +668,15 @@ struct LooperChain {
     for (int s = 0; s < N; ++s) est_start_bottom(est[frag[s].id_], lat.gauge[s], spins_c[s]);
     for (auto& op : ops) {
       int g = op.type >> 2;
+      if (lat.is_site_bond(op.bond)) {
+        // end_s / begin_s (path_integral.C:714-723): the segment below ends
+        // in loop0, the one above starts in loop1; no bond vector
+        const int s = lat.src[op.bond];
+        est_end(est[frag[op.loop0].id_], op.time, lat.gauge[s], spins_c[s]);
+        if (op.type & 1) spins_c[s] ^= 1;
+        est_begin(est[frag[op.loop1].id_], op.time, lat.gauge[s], spins_c[s]);
+        continue;
+      }
       if (is_frozen(g)) continue;
       int b = op.bond, s0 = lat.src[b], s1 = lat.tgt[b];
       double tt = op.time;
@@ -801,7 +845,7 @@ SiteEvents build_site_events(const Lattice& lat, int nops, const int* bond, cons
   std::vector<std::vector<std::pair<int, int>>> per(lat.N);  // (op, side)
   for (int k = 0; k < nops; ++k) {
     per[lat.src[bond[k]]].push_back({k, 0});
-    per[lat.tgt[bond[k]]].push_back({k, 1});
+    if (!lat.is_site_bond(bond[k])) per[lat.tgt[bond[k]]].push_back({k, 1});
   }
   se.off.assign(lat.N + 1, 0);
   se.pos0.assign(nops, -1);
@@ -820,6 +864,8 @@ SiteEvents build_site_events(const Lattice& lat, int nops, const int* bond, cons
       if (v[j].second == 0) se.pos0[v[j].first] = (int)j; else se.pos1[v[j].first] = (int)j;
     }
   }
+  for (int k = 0; k < nops; ++k)
+    if (lat.is_site_bond(bond[k])) se.pos1[k] = se.pos0[k];  // both legs on the one site
   return se;
 }
 
@@ -939,6 +985,7 @@ struct PhiloxChain {
   Thresholds th;
   double beta = 1, scale = 0, offset_total = 0, W = 0;
   double h = 0;   // longitudinal field
+  double vs = 0, site_scale = 0;  // transverse field: site graph weight |hx|/2 and its candidate scale
   Schedule sched; // annealing: beta and candidate scale from T(sweep) before each sweep
   uint32_t key0 = 0, key1 = 0;
   uint32_t sweep = 0;
@@ -958,7 +1005,7 @@ struct PhiloxChain {
   int last_nodes = 0;
 
   void init(int kind, int L, double b, double jxy, double jz, double fs, uint32_t seed, uint32_t chain,
-            double field = 0, int nv = 1) {
+            double field = 0, int nv = 1, double hx = 0) {
     h = field;
     lat.init(kind, L, nv);
     w = xxz_weights(jxy, jz, fs);
@@ -967,6 +1014,14 @@ struct PhiloxChain {
     W = w.v[0] + w.v[1] + w.v[2] + w.v[3];
     scale = (W > 0) ? std::ldexp(1.0, 52) / (beta * W) : 0;
     offset_total = lat.Nb * w.offset;
+    // transverse field -hx S^x: one site graph of weight |hx|/2 and offset
+    // |hx|/2 per site (weight_impl.h:63-88), its operators on the site bonds
+    if (hx != 0) {
+      lat.add_site_bonds();
+      vs = std::fabs(hx) / 2;
+      site_scale = std::ldexp(1.0, 52) / (beta * vs);
+      offset_total += lat.N * vs;
+    }
     key0 = seed; key1 = chain;
     spins.assign(lat.N, 0);
     boff.assign(lat.Nb + 1, 0);
@@ -1023,6 +1078,7 @@ struct PhiloxChain {
     if (sched.on()) {
       beta = 1.0 / sched.T((int)sweep);
       scale = (W > 0) ? std::ldexp(1.0, 52) / (beta * W) : 0;
+      if (vs > 0) site_scale = std::ldexp(1.0, 52) / (beta * vs);
     }
     // (a) diagonal update, per bond
     std::vector<int> nboff(Nb + 1, 0);
@@ -1034,18 +1090,21 @@ struct PhiloxChain {
     std::vector<int> acc_g;
     for (int b = 0; b < Nb; ++b) {
       int s0 = lat.src[b], s1 = lat.tgt[b];
+      // a site bond's candidates: rate beta |hx|/2, the one site graph
+      // (g = 0), compatible with either spin (graph_impl.h:66-69)
+      const bool site = lat.is_site_bond(b);
       acc_t.clear(); acc_g.clear();
-      if (W > 0) {
+      if (site ? vs > 0 : W > 0) {
         uint64_t T = 0;
         for (uint32_t k = 0;; ++k) {
           u32x4 x = philox4x32_10(sweep, STAGE_DIAG, (uint32_t)b, k, key0, key1);
-          T += exp_gap(x, scale);
+          T += exp_gap(x, site ? site_scale : scale);
           if (T >= TGRID) break;
           double tau = (double)T * (1.0 / 4503599627370496.0);
-          int g = choose_graph(x.v[2], th);
+          int g = site ? 0 : choose_graph(x.v[2], th);
           int c0 = spin_at(s0, tau), c1 = spin_at(s1, tau);
           if (c0 < 0 || c1 < 0) continue;
-          if (is_compatible(g, c0, c1)) { acc_t.push_back(tau); acc_g.push_back(g); }
+          if (site || is_compatible(g, c0, c1)) { acc_t.push_back(tau); acc_g.push_back(g); }
         }
       }
       // merge kept (regraphed) off-diagonal ops with accepted candidates
@@ -1054,7 +1113,7 @@ struct PhiloxChain {
       for (int r = boff[b]; r < boff[b + 1]; ++r) {
         if ((otype[r] & 1) == 0) continue;
         u32x4 x = philox4x32_10(sweep, STAGE_REGRAPH, (uint32_t)b, (uint32_t)kept_idx, key0, key1);
-        int g = ((uint64_t)x.v[0] < th.toff) ? 0 : 1;
+        int g = (site || (uint64_t)x.v[0] < th.toff) ? 0 : 1;
         ++kept_idx;
         while (ai < acc_t.size() && acc_t[ai] < otime[r]) {
           ntime.push_back(acc_t[ai]); ntype.push_back(acc_g[ai] << 2); ++ai;
@@ -1293,9 +1352,9 @@ void or_uf_pairs(int n, int npairs, const int* pairs, int variant, int* parent, 
 
 // ---- looper mode ---------------------------------------------------------
 void* or_looper_create(int kind, int L, double beta, double jxy, double jz, double fs, uint32_t seed, double h,
-                       int nv) {
+                       int nv, double hx) {
   LooperChain* c = new LooperChain();
-  c->init(kind, L, beta, jxy, jz, fs, seed, h, nv);
+  c->init(kind, L, beta, jxy, jz, fs, seed, h, nv, hx);
   return c;
 }
 void or_looper_sweep(void* h, int n, double* rec) {
@@ -1343,9 +1402,9 @@ int or_segment_of_elements(int kind, int L, int nops, const int* bond, const dou
 
 // ---- philox mode -------------------------------------------------------------
 void* or_philox_create(int kind, int L, double beta, double jxy, double jz, double fs, double h, uint32_t seed,
-                       uint32_t chain, int nv) {
+                       uint32_t chain, int nv, double hx) {
   PhiloxChain* c = new PhiloxChain();
-  c->init(kind, L, beta, jxy, jz, fs, seed, chain, h, nv);
+  c->init(kind, L, beta, jxy, jz, fs, seed, chain, h, nv, hx);
   return c;
 }
 void or_philox_sweep(void* h, int n, double* rec) {
@@ -1358,6 +1417,7 @@ void or_philox_set_beta(void* h, double beta) {
   PhiloxChain* c = (PhiloxChain*)h;
   c->beta = beta;
   c->scale = (c->W > 0) ? std::ldexp(1.0, 52) / (beta * c->W) : 0;
+  if (c->vs > 0) c->site_scale = std::ldexp(1.0, 52) / (beta * c->vs);
 }
 int or_philox_num_ops(void* h) { return (int)((PhiloxChain*)h)->otime.size(); }
 int or_philox_num_nodes(void* h) { return ((PhiloxChain*)h)->last_nodes; }

@@ -17,7 +17,9 @@ Sources (data files of the reference's tests):
   loop.op                       end-to-end run: ED and path-integral QMC, chain L=4, T=0.25;
                                 and the path-integral run with MEASURE[Correlations] and
                                 MEASURE[Structure Factor] (chain L=4, S=1/2, T=0.5): its
-                                scalar results and vector entries ("Name[label]")
+                                scalar results and vector entries ("Name[label]"); the ED and
+                                path-integral blocks of its other models (S=1 chain L=4, one S=1
+                                site in a transverse field Gamma=1 or a field h=0.3, Ising chain)
 """
 
 import json
@@ -119,7 +121,8 @@ def standalone():
     return res
 
 
-def loop_op():
+def loop_op_blocks():
+    """every block of loop.op: its input parameters and its results"""
     lines = read("loop.op")
     blocks = []
     cur = None
@@ -136,11 +139,40 @@ def loop_op():
         if cur is not None and m:
             err = m.group(3).rstrip(";")
             cur["results"][m.group(1)] = [float(m.group(2)), float("inf") if err == "inf" else float(err)]
-    keep = []
-    for b in blocks[:2]:  # L=4 chain, T=0.25: diagonalization and "loop; path integral"
-        keep.append({"algorithm": b["params"].get("ALGORITHM"), "params": b["params"],
-                     "results": {k: v for k, v in b["results"].items() if not k.startswith("Local")}})
-    return keep
+    return blocks
+
+
+def keep_blocks(blocks):
+    return [{"algorithm": b["params"].get("ALGORITHM"), "params": b["params"],
+             "results": {k: v for k, v in b["results"].items() if not k.startswith("Local")}} for b in blocks]
+
+
+def loop_op():
+    return keep_blocks(loop_op_blocks()[:2])  # L=4 chain, T=0.25: diagonalization and "loop; path integral"
+
+
+def loop_op_models():
+    """the diagonalization and "loop; path integral" blocks of the other
+    models of loop.ip, keyed by model: S=1 chain L=4 (T=1/L), one S=1 site
+    in a transverse field Gamma=1 (T=1/L), one S=1 site in a longitudinal
+    field h=0.3 (T=1/L), S=1/2 Ising chain L=4 (Jxy=0, Jz=1, T=0.5)"""
+    out = {}
+    for b in loop_op_blocks():
+        p = b["params"]
+        if p.get("ALGORITHM") not in ("diagonalization", "loop; path integral") or "MEASURE[Correlations]" in p:
+            continue
+        if p.get("LATTICE") == "site":
+            key = "site_spin1_gamma1" if p.get("Gamma") == "1" else "site_spin1_h03"
+        elif p.get("local_S") == "1" and "D" not in p:
+            key = "chain4_spin1"
+        elif p.get("Jxy") == "0" and p.get("local_S") == "1/2":
+            key = "chain4_ising"
+        else:
+            continue
+        out.setdefault(key, []).extend(keep_blocks([b]))
+    assert sorted(out) == ["chain4_ising", "chain4_spin1", "site_spin1_gamma1", "site_spin1_h03"]
+    assert all(len(v) == 2 for v in out.values())
+    return out
 
 
 def loop_op_correlations():
@@ -279,6 +311,7 @@ def main():
         "standalone_loop_op": standalone(),
         "loop_op_chain4": loop_op(),
         "loop_op_chain4_correlations": loop_op_correlations(),
+        "loop_op_models": loop_op_models(),
     }
     with open(os.path.join(OUT, "reference_fixtures.json"), "w") as f:
         json.dump(fx, f, indent=1, sort_keys=True)
