@@ -8,8 +8,8 @@
 //    annealing T_START_i / T_DURATION_i and their TEMPERATURE_, BETA_,
 //    INVERSE_TEMPERATURE_ forms (temperature.h:56-81),
 //    SWEEPS (default "[65536:]") and THERMALIZATION (default SWEEPS.min/8,
-//    montecarlo.h:41-45), SEED, FORCE_SCATTER (weight_impl.h:169), h,
-//    DISABLE_IMPROVED_ESTIMATOR, MEASURE[Correlations] / MEASURE[Structure
+//    montecarlo.h:41-45), SEED, FORCE_SCATTER (weight_impl.h:169), h, Gamma
+//    (transverse field, model_parameter.h:67), DISABLE_IMPROVED_ESTIMATOR, MEASURE[Correlations] / MEASURE[Structure
 //    Factor] / INITIAL_SITE (correlation.h:74-108);
 //  * init_observables / run / is_thermalized / progress / save / load with the
 //    worker's observable names (path_integral.C:307-321,831-862, energy.h,
