@@ -72,7 +72,8 @@ int check_params(const lg_params& p) {
     return set_error(LG_ERR_ARG, "square lattice too large");
   if (!(p.beta > 0) || !std::isfinite(p.beta))
     return set_error(LG_ERR_ARG, "beta must be positive (negative temperature)");
-  if (!std::isfinite(p.jxy) || !std::isfinite(p.jz) || !std::isfinite(p.h) || !std::isfinite(p.hx))
+  if (!std::isfinite(p.jxy) || !std::isfinite(p.jz) || !std::isfinite(p.h) || !std::isfinite(p.hx) ||
+      !std::isfinite(p.d))
     return set_error(LG_ERR_ARG, "couplings and fields must be finite");
   // an antiferromagnetic XY coupling with a transverse field has a sign
   // problem that no gauge removes (looper/model_parameter.h:617-644 finds the
@@ -81,6 +82,23 @@ int check_params(const lg_params& p) {
     return set_error(LG_ERR_ARG, "a transverse field with Jxy > 0 has a sign problem (the reference's SIGN path); "
                                  "it is built for Jxy <= 0");
   return LG_OK;
+}
+
+// graph thresholds of weights v: g = first g with x < thr[g] (2^32 from the
+// last graph of nonzero weight on); regraph threshold of an off-diagonal
+// operator v0 / (v0 + v1)
+static void graph_thresholds(const double* v, uint64_t* thr, uint64_t* toff) {
+  const double W = v[0] + v[1] + v[2] + v[3];
+  int last = -1;
+  for (int g = 0; g < 4; ++g) if (v[g] > 0) last = g;
+  double cum = 0;
+  for (int g = 0; g < 3; ++g) {
+    cum += v[g];
+    if (last < 0 || g >= last) thr[g] = 1ull << 32;
+    else thr[g] = (uint64_t)std::floor(std::ldexp(cum / W, 32));
+  }
+  const double s01 = v[0] + v[1];
+  *toff = (s01 > 0) ? (uint64_t)std::floor(std::ldexp(v[0] / s01, 32)) : (1ull << 32);
 }
 
 ChainDev make_chain_dev(const lg_params& p) {
@@ -101,22 +119,31 @@ ChainDev make_chain_dev(const lg_params& p) {
   d.wsum = W;
   d.scale = candidate_scale(p.beta, W);
   d.offset_total = d.Nb * off;
-  int last = -1;
-  for (int g = 0; g < 4; ++g) if (v[g] > 0) last = g;
-  double cum = 0;
-  for (int g = 0; g < 3; ++g) {
-    cum += v[g];
-    if (last < 0 || g >= last) d.thr[g] = 1ull << 32;
-    else d.thr[g] = (uint64_t)std::floor(std::ldexp(cum / W, 32));
-  }
-  double s01 = v[0] + v[1];
-  d.toff = (s01 > 0) ? (uint64_t)std::floor(std::ldexp(v[0] / s01, 32)) : (1ull << 32);
+  graph_thresholds(v, d.thr, &d.toff);
   d.key0 = p.seed;
   d.key1 = p.chain_id;
   d.h = p.h;
+  // single-ion term D (S^z)^2 = D nv/4 + sum_{a<c} 2D s_a s_c per real site
+  // (model_parameter.h:592-595, weight_impl.h:194-197, 400-404): in-site
+  // bonds after the lattice's, the pairs of each real site's virtual sites,
+  // with the weights of an Ising coupling 2D; the constant in the offset
+  d.nb_lat = d.Nb;
+  double dw = 0;
+  if (p.d != 0) {
+    double vd[4], offd;
+    xxz_weights(0.0, 2 * p.d, p.force_scatter, vd, &offd);
+    dw = vd[0] + vd[1] + vd[2] + vd[3];
+    d.npair = d.nv * (d.nv - 1) / 2;
+    const int nr = d.N / d.nv;
+    d.Nb += nr * d.npair;
+    d.dwsum = dw;
+    d.dscale = candidate_scale(p.beta, dw);
+    graph_thresholds(vd, d.dthr, &d.dtoff);
+    d.offset_total += nr * d.npair * offd + nr * 0.25 * d.nv * p.d;
+  }
   // transverse field: one site graph of weight and offset |hx|/2 per site
   // (weight_impl.h:63-88), its operators on a bond of the site's own after
-  // the lattice's bonds
+  // the lattice's (and in-site) bonds
   d.nb_real = d.Nb;
   if (p.hx != 0) {
     d.site_w = std::fabs(p.hx) / 2;
@@ -124,7 +151,7 @@ ChainDev make_chain_dev(const lg_params& p) {
     d.Nb += d.N;
     d.offset_total += d.N * d.site_w;
   }
-  double lam = p.beta * std::max(W, d.site_w);
+  double lam = p.beta * std::max(std::max(W, dw), d.site_w);
   d.cand_cap = (int)std::ceil(lam + 12.0 * std::sqrt(lam) + 16.0);
   return d;
 }
@@ -133,7 +160,12 @@ ChainDev make_chain_dev(const lg_params& p) {
 static long long op_capacity(const lg_params& p, const ChainDev& d) {
   double v[4], off;
   xxz_weights(p.jxy, p.jz, p.force_scatter, v, &off);
-  double bound = p.beta * d.nb_real * (off + std::fabs(p.jz) / 4 + std::fabs(p.jxy) / 2);
+  double bound = p.beta * d.nb_lat * (off + std::fabs(p.jz) / 4 + std::fabs(p.jxy) / 2);
+  if (d.npair > 0) {  // in-site bonds: an Ising coupling 2D
+    double vd[4], offd;
+    xxz_weights(0.0, 2 * p.d, p.force_scatter, vd, &offd);
+    bound += p.beta * (d.nb_real - d.nb_lat) * (offd + std::fabs(p.d) / 2);
+  }
   bound += p.beta * d.N * std::fabs(p.hx);  // site operators: beta |hx| (1/2 + |<S^x>|) per site at most
   return (long long)std::ceil(bound + 10.0 * std::sqrt(bound) + 1024.0);
 }
@@ -228,7 +260,8 @@ int Batch::init(const lg_params* p, int n) {
 int Batch::init_exchange(const lg_params& p, const double* betas, int n, uint32_t seed) {
   if (n < 2) return set_error(LG_ERR_ARG, "exchange needs at least two temperatures");
   if (!betas) return set_error(LG_ERR_ARG, "null pointer");
-  if (p.hx != 0) return set_error(LG_ERR_UNSUPPORTED, "replica exchange with a transverse field is not built");
+  if (p.hx != 0 || p.d != 0)
+    return set_error(LG_ERR_UNSUPPORTED, "replica exchange with a transverse field or a single-ion term is not built");
   std::vector<lg_params> ps(n, p);
   for (int t = 0; t < n; ++t) {
     ps[t].beta = betas[t];
@@ -466,7 +499,7 @@ int Batch::set_layout(std::vector<ChainDev>& ds, const std::vector<uint64_t>& sw
   }
   double lam_max = 0;
   for (auto& d : chains)
-    lam_max = std::max(lam_max, d.beta * std::max(d.has_weight ? d.wsum : 0.0, d.site_w));
+    lam_max = std::max(lam_max, d.beta * std::max(std::max(d.has_weight ? d.wsum : 0.0, d.dwsum), d.site_w));
   int G = 4;
   while (G < 64 && G < lam_max) G *= 2;
   if (!tuned) cfg.group = G;
@@ -475,6 +508,7 @@ int Batch::set_layout(std::vector<ChainDev>& ds, const std::vector<uint64_t>& sw
   double ev = 0, ops_bond = 0, nb_total = 0;
   for (size_t c = 0; c < chains.size(); ++c) {
     const ChainDev& d = chains[c];
+    // (in-site bonds: counted as lattice bonds; site operators: beta |hx| per site)
     const double per_bond = (d.offset_total - d.N * d.site_w) / d.nb_real +
                             0.5 * (std::fabs(params[c].jz) / 4 + std::fabs(params[c].jxy) / 2);
     ev += 2.0 * d.beta * d.nb_real * per_bond + d.beta * d.N * 2 * d.site_w;
@@ -530,7 +564,7 @@ int Batch::set_layout(std::vector<ChainDev>& ds, const std::vector<uint64_t>& sw
   // per-site one at most four lists per site
   if (split && c.link_cap > 1) c.link_cap = 0;
   if (many_lists && c.link_cap == 1) c.link_cap = 0;
-  if (has_site_bonds()) c.link_cap = 0;  // site operators: the per-operator kernel
+  if (has_extra_bonds()) c.link_cap = 0;  // in-site bonds, site operators: the per-operator kernel
   return apply_cfg(c);
 }
 
@@ -661,11 +695,12 @@ DevBatch Batch::dev(int from) const {
   B.ring_pos = ring_pos; B.end_count = end_count; B.ring_size = ring_size;
   B.any_field = 0;
   B.any_split = 0;
-  B.any_site = 0;
+  B.any_extra = 0;
   for (const auto& d : chains) {
     if (d.h != 0.0) B.any_field = 1;
     if (d.nv > 1) B.any_split = 1;
-    if (d.Nb > d.nb_real) B.any_site = 1;
+    if (d.nb_real > d.nb_lat) B.any_extra |= 1;
+    if (d.Nb > d.nb_real) B.any_extra |= 2;
   }
   B.nslots = nslots; B.xslot = xslot; B.xbeta = xbeta; B.xscale = xscale; B.xlnr = xlnr;
   B.xacc = xacc; B.xstep = xstep; B.xkey0 = xkey0; B.xkey1 = xkey1;
@@ -779,7 +814,7 @@ int Batch::tune(uint32_t sweeps_per_trial) {
       else if (pass == 2) {
         if (links[i] > 1 && has_virtual_sites()) continue;  // these kernels assume wrapping segments
         if (links[i] == 1 && max_site_lists() > 4) continue;  // it merges four lists at most
-        if (links[i] != 0 && has_site_bonds()) continue;      // site operators: the per-operator kernel
+        if (links[i] != 0 && has_extra_bonds()) continue;     // in-site bonds, site operators: the per-operator kernel
         trial.link_cap = links[i];
       }
       else if (pass == 3) trial.compress = i;
@@ -826,8 +861,8 @@ int Batch::set_launch(const lg_launch& l) {
       return set_error(LG_ERR_ARG, "chains with virtual sites (S > 1/2) take linking kernel 0 or 1");
     if (l.link == 1 && max_site_lists() > 4)
       return set_error(LG_ERR_ARG, "sites with more than four bond lists (S > 1 on the chain, S = 1 on the square lattice) take linking kernel 0");
-    if (l.link != 0 && has_site_bonds())
-      return set_error(LG_ERR_ARG, "chains with a transverse field take linking kernel 0");
+    if (l.link != 0 && has_extra_bonds())
+      return set_error(LG_ERR_ARG, "chains with a transverse field or a single-ion term take linking kernel 0");
     c.link_cap = l.link;
   }
   if (l.compress >= 0) {
@@ -1149,6 +1184,7 @@ int Batch::sync_schedule() {
     d.beta = 1.0 / schedule_temperature(sched_host.data() + 2 * d.sched_off, d.sched_n, (int)host_sweep[c]);
     d.scale = candidate_scale(d.beta, d.wsum);
     d.site_scale = candidate_scale(d.beta, d.site_w);
+    d.dscale = candidate_scale(d.beta, d.dwsum);
     any = true;
   }
   if (!any) return LG_OK;
@@ -1201,6 +1237,7 @@ int Batch::set_schedule(int n, const int32_t* steps, const double* temps) {
     chains[0].beta = p.beta;
     chains[0].scale = candidate_scale(p.beta, chains[0].wsum);
     chains[0].site_scale = candidate_scale(p.beta, chains[0].site_w);
+    chains[0].dscale = candidate_scale(p.beta, chains[0].dwsum);
     LG_HIP(hipMemcpy(d_chains, chains.data(), sizeof(ChainDev), hipMemcpyHostToDevice));
     invalidate_graphs();
     return LG_OK;
@@ -1366,7 +1403,8 @@ int lg_cluster_config(const lg_params* p, const int32_t* spins, const int32_t* b
                       const int32_t* types, uint64_t sweep, int32_t* labels, int32_t* spins_out,
                       int32_t* types_out, int64_t* n_clusters) {
   if (!p || !spins || !bond_offsets) return set_error(LG_ERR_ARG, "null pointer");
-  if (p->hx != 0) return set_error(LG_ERR_UNSUPPORTED, "the cluster stage of a given configuration takes hx = 0");
+  if (p->hx != 0 || p->d != 0)
+    return set_error(LG_ERR_UNSUPPORTED, "the cluster stage of a given configuration takes hx = 0 and d = 0");
   lg_params q = *p;
   // capacity from the configuration itself
   lg::ChainDev d = lg::make_chain_dev(q);

@@ -11,10 +11,11 @@
 //    info = bond << 3 | graph << 1 | offdiagonal  (cf. operator.h:52-117).
 //  * Imaginary time tau lives on the grid k * 2^-52, k in [0, 2^52), and is
 //    stored as an exact FP64 value.
-//  * A transverse field adds one bond per site after the lattice's bonds
-//    (ChainDev::nb_real): its operators are site operators, one event on
-//    the site, the world line cut there (looper's site graph,
-//    graph_impl.h:66-85).
+//  * A single-ion term adds in-site bonds between the virtual sites of each
+//    real site after the lattice's bonds (ChainDev::nb_lat); a transverse
+//    field then adds one bond per site (ChainDev::nb_real): its operators
+//    are site operators, one event on the site, the world line cut there
+//    (looper's site graph, graph_impl.h:66-85).
 //  * Union-find nodes are world-line segments: for site s with m events
 //    (operator legs touching s, ordered by (tau, bond, rank)), node soff[s]+j
 //    is the segment just below event j; node soff[s] is the wrapping segment
@@ -73,6 +74,13 @@ struct ChainDev {
   // one nb_real = Nb
   int nb_real;
   double site_w, site_scale;         // site graph weight |hx|/2; candidate scale 2^52 / (beta site_w)
+  // single-ion term D (S^z)^2 (S > 1/2): bonds [nb_lat, nb_real) are
+  // in-site bonds, npair = nv (nv - 1) / 2 per real site, an Ising coupling
+  // 2D between two virtual sites of one real site with graphs of their own;
+  // without it nb_lat = nb_real
+  int nb_lat, npair;
+  double dwsum, dscale;              // in-site bonds: sum_g v_g and candidate scale 2^52 / (beta dwsum)
+  uint64_t dthr[3], dtoff;           // in-site bonds: graph and regraph thresholds
 };
 
 // Temperature of Monte Carlo step `step` on an annealing schedule of n
@@ -230,6 +238,13 @@ LG_HD int choose_graph(uint32_t x, const ChainDev& c) {
   return g;
 }
 
+// the same for the in-site bonds of the single-ion term
+LG_HD int choose_graph_d(uint32_t x, const ChainDev& c) {
+  int g = 0;
+  while (g < 3 && (uint64_t)x >= c.dthr[g]) ++g;
+  return g;
+}
+
 // ---- lattice geometry (ALPS conventions) ----------------------------------
 // chain: bond b = (b, b+1 mod L); site s has bonds {s (source), s-1 (target)}
 // square: site s = x*L + y; bond 2s = (s, s+y^) [dir 1], 2s+1 = (s, s+x^) [dir 0];
@@ -271,11 +286,36 @@ LG_HD void bond_sites(const ChainDev& c, int b, int& s0, int& s1) {
   }
 }
 
+// In-site bonds of the single-ion term (looper's in-real-site bonds,
+// lattice.h:632-646): real site r's pairs a < c of virtual sites, a outer,
+// pair p(a, c) = a nv - a (a + 1) / 2 + (c - a - 1), bond nb_lat + r npair + p.
+LG_HD int pair_index(int nv, int a, int c) { return a * nv - a * (a + 1) / 2 + (c - a - 1); }
+
+LG_HD void dpair_sites(const ChainDev& c, int b, int& s0, int& s1) {
+  const int q = b - c.nb_lat, r = q / c.npair;
+  int rem = q - r * c.npair, a = 0;
+  while (rem >= c.nv - 1 - a) { rem -= c.nv - 1 - a; ++a; }
+  s0 = r * c.nv + a;
+  s1 = r * c.nv + a + 1 + rem;
+}
+
+// in-site bond j (j < nv - 1) of virtual site s: with its real site's j-th
+// other virtual site
+LG_HD int dpair_bond(const ChainDev& c, int s, int j) {
+  const int r = s / c.nv, a = s - r * c.nv;
+  const int o = (j < a) ? j : j + 1;
+  return c.nb_lat + r * c.npair + (a < o ? pair_index(c.nv, a, o) : pair_index(c.nv, o, a));
+}
+
 // sites of the operators of bond b: a site bond (transverse field) has its
 // one site on both sides
 LG_HD void op_sites(const ChainDev& c, int b, int& s0, int& s1) {
   if (b >= c.nb_real) {
     s0 = s1 = b - c.nb_real;
+    return;
+  }
+  if (b >= c.nb_lat) {
+    dpair_sites(c, b, s0, s1);
     return;
   }
   bond_sites(c, b, s0, s1);

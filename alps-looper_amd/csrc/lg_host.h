@@ -83,10 +83,11 @@ class Batch {
     for (const auto& d : chains) if (d.nv > 1) return true;
     return false;
   }
-  // some chain has a transverse field: site operators, which only the
+  // some chain has bonds beyond the lattice's (in-site bonds of a
+  // single-ion term, site operators of a transverse field), which only the
   // per-operator linking kernel places
-  bool has_site_bonds() const {
-    for (const auto& d : chains) if (d.Nb > d.nb_real) return true;
+  bool has_extra_bonds() const {
+    for (const auto& d : chains) if (d.Nb > d.nb_lat) return true;
     return false;
   }
   int max_site_lists() const {

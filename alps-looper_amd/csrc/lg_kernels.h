@@ -83,7 +83,7 @@ struct DevBatch {
   unsigned long long* fmask;   // [nodes / 64 + 1] flip bit of every node's cluster, 64 nodes per word
   unsigned long long* rmask;   // [nodes / 64 + 1] 1 for the roots, 64 nodes per word
   int any_split;               // some chain has virtual sites (S=1): segments without wrap, boundary permutation
-  int any_site;                // some chain has a transverse field: site bonds, k_diag_count_site
+  int any_extra;               // bonds beyond the lattice's in some chain: 1 single-ion in-site bonds, 2 transverse-field site bonds (bits)
 };
 
 struct LaunchCfg {

@@ -250,17 +250,23 @@ __device__ __forceinline__ void scan_advance_epoch(const DevBatch& B) {
 // ---------------------------------------------------------------------------
 // One bond's candidates: kept off-diagonal operators counted, Poisson
 // candidates drawn and tested against the previous configuration's spins,
-// the accepted ones stored.  SITE: a site bond (transverse field), whose
-// candidates come at rate beta |hx|/2 with the one site graph (g = 0),
-// compatible with either spin (graph_impl.h:66-69).  The two cases are
-// separate kernels, so that the bond case keeps its registers.
-template <int G, bool SITE>
+// the accepted ones stored.  CLS: BOND_LATTICE, BOND_INSITE (the
+// single-ion term's in-site bonds: their own rate and graphs) or BOND_SITE
+// (a site bond of the transverse field, whose candidates come at rate
+// beta |hx|/2 with the one site graph g = 0, compatible with either spin,
+// graph_impl.h:66-69).  The lattice bonds and the others are separate
+// kernels, so that the lattice case keeps its registers.
+enum : int { BOND_LATTICE = 0, BOND_INSITE = 1, BOND_SITE = 2 };
+
+template <int G, int CLS>
 __device__ __forceinline__ void diag_count_bond(const DevBatch& B, const ChainDev& P, int c, int gb, int gl,
                                                 uint64_t below) {
+  constexpr bool SITE = CLS == BOND_SITE;
   const int b = gb - P.bond_base;
   const uint32_t sweep = B.sweep[c];
   int s0, s1;
   if constexpr (SITE) s0 = s1 = b - P.nb_real;
+  else if constexpr (CLS == BOND_INSITE) dpair_sites(P, b, s0, s1);
   else bond_sites(P, b, s0, s1);
   // kept (off-diagonal) operators of the previous configuration
   const int ob = P.old_bond_base + b;
@@ -275,8 +281,8 @@ __device__ __forceinline__ void diag_count_bond(const DevBatch& B, const ChainDe
   }
   // Poisson candidates on [0,1): exponential gaps on the 2^-52 grid
   int acc = 0;
-  if (SITE ? P.site_w > 0 : P.has_weight != 0) {
-    const double scale = SITE ? P.site_scale : P.scale;
+  if (SITE ? P.site_w > 0 : (CLS == BOND_INSITE ? P.dwsum > 0 : P.has_weight != 0)) {
+    const double scale = SITE ? P.site_scale : (CLS == BOND_INSITE ? P.dscale : P.scale);
     // slot j of the bond at j * cand_sstride (a chain holds fewer than 2^31 slots)
     const long long cb = P.cand_base + (long long)b * P.cand_bstride;
     double* const ctb = B.cand_t + cb;
@@ -311,7 +317,7 @@ __device__ __forceinline__ void diag_count_bond(const DevBatch& B, const ChainDe
       const bool valid = T < TGRID;
       // lanes past the end of [0,1) search for -1 (found at once)
       const double tau = valid ? (double)T * TGRID_INV : -1.0;
-      const int g = SITE ? 0 : choose_graph(x.z, P);
+      const int g = SITE ? 0 : (CLS == BOND_INSITE ? choose_graph_d(x.z, P) : choose_graph(x.z, P));
       int c0, c1;
       if (lt0) {
         lds_spins_prev(B, lt0, base0, m0, lt1, base1, m1, tau, c0, c1);
@@ -364,14 +370,18 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(G == 3
        gb += gridDim.x * groups_per_block) {
     const int c = B.bond_chain[gb];
     const ChainDev P = B.chains[c];
-    if (gb - P.bond_base >= P.nb_real) continue;  // site bonds: k_diag_count_site
-    diag_count_bond<G, false>(B, P, c, gb, gl, below);
+    if (gb - P.bond_base >= P.nb_lat) continue;  // in-site and site bonds: k_diag_count_extra
+    diag_count_bond<G, BOND_LATTICE>(B, P, c, gb, gl, below);
   }
 }
 
-// the site bonds (transverse field), a group of G lanes per site
-template <int G>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(G == 32 ? 4 : 5))) k_diag_count_site(DevBatch B) {
+// the bonds beyond the lattice's, a group of G lanes per (virtual) site:
+// CLS BOND_INSITE, the in-site bonds to its real site's higher virtual sites
+// (single-ion term); BOND_SITE, its site bond (transverse field).  One kernel
+// per class, each with the registers of one instance.
+template <int G, int CLS>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CLS == BOND_INSITE || G == 32 ? 4 : 5)))
+k_diag_count_extra(DevBatch B) {
   if (overflowed(B)) return;
   const int groups_per_block = blockDim.x / G;
   const int gl = glane<G>();
@@ -381,8 +391,17 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(G == 3
   for (int gs = gs_first; gs < B.nsites; gs += gridDim.x * groups_per_block) {
     const int c = B.site_chain[gs];
     const ChainDev P = B.chains[c];
-    if (P.Nb == P.nb_real) continue;  // no transverse field
-    diag_count_bond<G, true>(B, P, c, P.bond_base + P.nb_real + (gs - P.site_base), gl, below);
+    const int s = gs - P.site_base;
+    if constexpr (CLS == BOND_INSITE) {
+      if (P.npair > 0) {
+        const int r = s / P.nv, a = s - r * P.nv;
+        for (int o = a + 1; o < P.nv; ++o)
+          diag_count_bond<G, BOND_INSITE>(B, P, c, P.bond_base + P.nb_lat + r * P.npair + pair_index(P.nv, a, o), gl,
+                                          below);
+      }
+    } else {
+      if (P.Nb > P.nb_real) diag_count_bond<G, BOND_SITE>(B, P, c, P.bond_base + P.nb_real + s, gl, below);
+    }
   }
 }
 
@@ -407,7 +426,10 @@ __device__ __forceinline__ int site_events(const DevBatch& B, int gs) {
     const int nb = site_lists(P);
     for (int i = 0; i < nb; ++i) m += B.counts[P.bond_base + site_bond(P, s, i)];
   }
-  if (B.any_site && P.Nb > P.nb_real) m += B.counts[P.bond_base + P.nb_real + s];  // site operators
+  if (B.any_extra) {
+    for (int j = 0, nd = P.npair > 0 ? P.nv - 1 : 0; j < nd; ++j) m += B.counts[P.bond_base + dpair_bond(P, s, j)];  // nv - 1 in-site lists
+    if (P.Nb > P.nb_real) m += B.counts[P.bond_base + P.nb_real + s];  // site operators
+  }
   return m;
 }
 
@@ -588,7 +610,10 @@ __global__ void __launch_bounds__(256) k_diag_fill(DevBatch B, int fill_cap) {
         const int kr = B.kpre[r];
         const double tau = B.ot_old[r];
         U4 x = philox(sweep, STAGE_REGRAPH, (uint32_t)b, (uint32_t)kr, P.key0, P.key1);
-        int g = ((B.any_site && b >= P.nb_real) || (uint64_t)x.x < P.toff) ? 0 : 1;  // a site operator keeps the site graph
+        // an in-site bond regraphs by its own threshold; a site operator
+        // keeps the site graph
+        const uint64_t toff = (B.any_extra && b >= P.nb_lat) ? (b >= P.nb_real ? (1ull << 32) : P.dtoff) : P.toff;
+        int g = ((uint64_t)x.x < toff) ? 0 : 1;
         int pos = nb + kr + lower_bound_ds(tc, ts, acc, tau);
         B.ot_new[pos] = tau;
         B.oi_new[pos] = info_b | (g << 1) | 1;
@@ -967,9 +992,11 @@ __global__ void __launch_bounds__(256) k_node_init(DevBatch B) {
 // the linking itself writes no parent, so the unions of different threads
 // only meet in the lock-free union-find.  This saves the separate union
 // pass, which re-read the four legs and the type of every operator.
-// SITES: some chain has a transverse field (site operators); a separate
-// instance, so that the plain one carries none of its tests.
-template <bool SITES>
+// EXTRA: some chain has bonds beyond the lattice's (in-site bonds of the
+// single-ion term: two legs, no bond vector; site operators of the
+// transverse field); a separate instance, so that the plain one carries
+// none of their tests.
+template <bool EXTRA>
 __global__ void __launch_bounds__(256) k_link_ops(DevBatch B) {
   if (overflowed(B)) return;
   const int nops = B.totals[0];
@@ -984,10 +1011,11 @@ __global__ void __launch_bounds__(256) k_link_ops(DevBatch B) {
     const int r = k - B.bo_new[gb];
     // a site operator (transverse field): one event, on its site, with no
     // bond vector (no source flag), and no union: the world line is cut
-    const bool site = SITES && b >= P.nb_real;
-    const uint8_t dflag = (!site && bond_dir(P, b)) ? EV_DIR1 : 0;
+    const bool site = EXTRA && b >= P.nb_real;
+    const bool lattice = !EXTRA || b < P.nb_lat;  // bonds with a bond vector
+    const uint8_t dflag = (lattice && bond_dir(P, b)) ? EV_DIR1 : 0;
     int s0, s1;
-    if (SITES) op_sites(P, b, s0, s1);
+    if (EXTRA) op_sites(P, b, s0, s1);
     else bond_sites(P, b, s0, s1);
     int lo_leg[2], up_leg[2];  // lower and upper segment of each side
 #pragma unroll
@@ -1023,7 +1051,10 @@ __global__ void __launch_bounds__(256) k_link_ops(DevBatch B) {
         const int nbd = site_lists(P);
         for (int i = 0; i < nbd; ++i) other_list(site_bond(P, s, i));
       }
-      if (SITES && P.Nb > P.nb_real) other_list(P.nb_real + s);  // the site's own operators
+      if (EXTRA) {
+        for (int j = 0, nd = P.npair > 0 ? P.nv - 1 : 0; j < nd; ++j) other_list(dpair_bond(P, s, j));  // nv - 1 in-site lists
+        if (P.Nb > P.nb_real) other_list(P.nb_real + s);                            // the site's own operators
+      }
       const int gs = P.site_base + s;
       const int n0 = B.soff_new[gs], n1 = B.soff_new[gs + 1];
       const int node = n0 + rank;
@@ -1033,7 +1064,7 @@ __global__ void __launch_bounds__(256) k_link_ops(DevBatch B) {
       const uint8_t sp = B.spins_old[P.old_site_base + s] ^ (uint8_t)(par & 1);
       B.evt_new[node] = tau;
       B.segspin_new[node] = sp;
-      B.evflags[node] = (uint8_t)((info & 1) | ((side || site) ? 0 : EV_SRC) | dflag | (rank == 0 ? EV_FIRST : 0) |
+      B.evflags[node] = (uint8_t)((info & 1) | ((side || !lattice) ? 0 : EV_SRC) | dflag | (rank == 0 ? EV_FIRST : 0) |
                                   gflag | (split && rank == 0 ? EV_NOWRAP : 0));
       if (rank == 0) B.node_site[node] = gs;
       if (split && up == n1 - 1) {  // the last event: the site's top segment, up to tau = 1
@@ -1855,6 +1886,7 @@ __global__ void __launch_bounds__(END_THREADS) k_end_sweep(DevBatch B) {
       W->beta = beta;
       W->scale = candidate_scale(beta, P.wsum);
       W->site_scale = candidate_scale(beta, P.site_w);
+      W->dscale = candidate_scale(beta, P.dwsum);
     }
     __threadfence();
     is_last = (atomicAdd(B.end_count, 1) == (int)gridDim.x - 1);
@@ -1954,9 +1986,10 @@ static void launch_diag(const DevBatch& B, const LaunchCfg& L, hipStream_t st, b
   }
   else {
     hipLaunchKernelGGL(k_diag_count<G>, dim3(blocks), dim3(256), 0, st, B);
-    if (B.any_site) {
+    if (B.any_extra) {
       const int sblocks = max(1, min(L.max_blocks_bond, (B.nsites + groups_per_block - 1) / groups_per_block));
-      hipLaunchKernelGGL(k_diag_count_site<G>, dim3(sblocks), dim3(256), 0, st, B);
+      if (B.any_extra & 1) hipLaunchKernelGGL((k_diag_count_extra<G, BOND_INSITE>), dim3(sblocks), dim3(256), 0, st, B);
+      if (B.any_extra & 2) hipLaunchKernelGGL((k_diag_count_extra<G, BOND_SITE>), dim3(sblocks), dim3(256), 0, st, B);
     }
   }
 }
@@ -2011,7 +2044,7 @@ void launch_cluster_stage(const DevBatch& B, const LaunchCfg& L, hipStream_t st,
   // ms at 8 steps per block, 5.95 at one)
   if (L.link_cap == 0) {
     hipLaunchKernelGGL(k_node_init, dim3(L.blocks_node), dim3(256), 0, st, B);
-    if (B.any_site) hipLaunchKernelGGL(k_link_ops<true>, dim3(L.blocks_op), dim3(256), 0, st, B);
+    if (B.any_extra) hipLaunchKernelGGL(k_link_ops<true>, dim3(L.blocks_op), dim3(256), 0, st, B);
     else hipLaunchKernelGGL(k_link_ops<false>, dim3(L.blocks_op), dim3(256), 0, st, B);
   } else if (L.link_cap == 1) {
     const int blocks = max(1, min(L.blocks_op, (B.nsites + 255) / 256));

@@ -170,7 +170,7 @@ DevBatch na_dev(unsigned char* base, int N, int Nb, bool field) {
   B.ring_size = NA_RING;
   B.any_field = field ? 1 : 0;
   B.any_split = 0;
-  B.any_site = 0;
+  B.any_extra = 0;
   B.nslots = 0;
   B.op_cap = (int)NA_OPS;
   B.node_cap = (int)NA_NODES;
@@ -256,7 +256,8 @@ int lg_cluster_config_noalloc(const lg_params* p, const int32_t* spins, const in
     return set_error(LG_ERR_ARG, "null pointer");
   int rc = check_params(*p);
   if (rc) return rc;
-  if (p->hx != 0) return set_error(LG_ERR_UNSUPPORTED, "the cluster stage of a given configuration takes hx = 0");
+  if (p->hx != 0 || p->d != 0)
+    return set_error(LG_ERR_UNSUPPORTED, "the cluster stage of a given configuration takes hx = 0 and d = 0");
   ChainDev d = make_chain_dev(*p);
   if (d.N > NA_SITES || n_ops < 0 || n_ops > NA_OPS)
     return set_error(LG_ERR_CAPACITY, "configuration beyond the no-allocation maximum (" +

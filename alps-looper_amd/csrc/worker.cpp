@@ -9,7 +9,8 @@
 //    INVERSE_TEMPERATURE_ forms (temperature.h:56-81),
 //    SWEEPS (default "[65536:]") and THERMALIZATION (default SWEEPS.min/8,
 //    montecarlo.h:41-45), SEED, FORCE_SCATTER (weight_impl.h:169), h, Gamma
-//    (transverse field, model_parameter.h:67), DISABLE_IMPROVED_ESTIMATOR, MEASURE[Correlations] / MEASURE[Structure
+//    (transverse field, model_parameter.h:67), D (single-ion term),
+//    DISABLE_IMPROVED_ESTIMATOR, MEASURE[Correlations] / MEASURE[Structure
 //    Factor] / INITIAL_SITE (correlation.h:74-108);
 //  * init_observables / run / is_thermalized / progress / save / load with the
 //    worker's observable names (path_integral.C:307-321,831-862, energy.h,
@@ -255,8 +256,9 @@ class loop_worker {
         throw std::invalid_argument("local_S must be 1/2, 1, 3/2 or 2 on the chain lattice, 1/2 or 1 on the square lattice");
       lp_.two_s = (int32_t)t;
     }
-    if (p.defined("D") && p.num("D") != 0.0)
-      throw std::invalid_argument("parameter D != 0 (single-ion term) is not supported");
+    // single-ion term D (S^z)^2 (model_parameter.h:69): in-site bonds of the
+    // virtual sites
+    lp_.d = p.num("D", 0.0);
     // transverse field -Gamma S^x (model_parameter.h:67, the ALPS spin
     // model's Gamma): site operators; sign-free with Jxy <= 0
     lp_.hx = p.num("Gamma", 0.0);
@@ -269,7 +271,11 @@ class loop_worker {
     double J = p.num("J", 1.0);
     lp_.jxy = p.num("Jxy", J);
     lp_.jz = p.num("Jz", J);
-    lp_.force_scatter = p.num("FORCE_SCATTER", 0.0);
+    // looper scatters by default where it finds the model frustrated, which a
+    // positive single-ion term of S > 1/2 makes it (model_parameter.h:646-661,
+    // model_impl.h:51)
+    const bool frustrated = lp_.d > 0 && lp_.two_s > 1;
+    lp_.force_scatter = p.num("FORCE_SCATTER", frustrated ? 0.1 : 0.0);
     lp_.seed = (uint32_t)(int64_t)std::llround(p.num("SEED", 0.0));
     lp_.chain_id = (uint32_t)std::llround(p.num("CHAIN_ID", 0.0));
     if (parse_sweeps(p.str("SWEEPS", "[65536:]"), sweeps_min_, sweeps_max_))

@@ -58,7 +58,7 @@ class LGParams(ctypes.Structure):
         ("lattice", ctypes.c_int32), ("L", ctypes.c_int32), ("beta", ctypes.c_double),
         ("jxy", ctypes.c_double), ("jz", ctypes.c_double), ("force_scatter", ctypes.c_double),
         ("seed", ctypes.c_uint32), ("chain_id", ctypes.c_uint32), ("h", ctypes.c_double),
-        ("two_s", ctypes.c_int32), ("hx", ctypes.c_double),
+        ("two_s", ctypes.c_int32), ("hx", ctypes.c_double), ("d", ctypes.c_double),
     ]
 
 
@@ -147,18 +147,21 @@ def kernel_names():
 
 
 def params(lattice, L, beta, jxy=1.0, jz=None, seed=23125, chain_id=0, force_scatter=0.0, h=0.0, two_s=1,
-           hx=0.0):
+           hx=0.0, d=0.0):
     if jz is None:
         jz = jxy
     return LGParams(int(lattice), int(L), float(beta), float(jxy), float(jz), float(force_scatter),
-                    int(seed), int(chain_id), float(h), int(two_s), float(hx))
+                    int(seed), int(chain_id), float(h), int(two_s), float(hx), float(d))
 
 
-def lattice_size(lattice, L, two_s=1, hx=0.0):
+def lattice_size(lattice, L, two_s=1, hx=0.0, d=0.0):
     """(sites, bonds) the device sweeps: virtual ones for S > 1/2 (2S per real site); a
-    transverse field adds one bond per site, after the lattice's, for the site operators"""
+    single-ion term adds the in-site bonds of each real site's virtual sites, and a
+    transverse field one bond per site, after the lattice's, for the site operators"""
     nv = two_s if two_s >= 2 else 1
     n, nb = (nv * L, nv * nv * L) if lattice == CHAIN else (nv * L * L, nv * nv * 2 * L * L)
+    if d != 0:
+        nb += (n // nv) * nv * (nv - 1) // 2
     return n, nb + (n if hx != 0 else 0)
 
 
@@ -173,7 +176,7 @@ class Chain:
         self.p = p
         self.h = ctypes.c_void_p()
         _check(lib.lg_chain_create(ctypes.byref(p), ctypes.byref(self.h)))
-        self.n_sites, self.n_bonds = lattice_size(p.lattice, p.L, p.two_s, p.hx)
+        self.n_sites, self.n_bonds = lattice_size(p.lattice, p.L, p.two_s, p.hx, p.d)
 
     def sweep(self, n, records=True):
         rec = np.zeros((n, REC_SIZE)) if records else None

@@ -55,7 +55,10 @@ enum { LG_LATTICE_CHAIN = 0, LG_LATTICE_SQUARE = 1 };
  * bond Nb_lattice + s holds site s's operators (graph 0, one leg below and
  * one above on site s; an off-diagonal one flips that spin), so the bond
  * count at the ABI (bond_offsets of lg_chain_get_config / set_config) is
- * Nb_lattice + N.
+ * Nb_lattice + N.  A single-ion term (d != 0, S > 1/2) adds, after the
+ * lattice's bonds and before those site bonds, the in-site bonds of each real
+ * site r: for its virtual sites a < c (a outer), bond (r*nv + a, r*nv + c),
+ * nv (nv - 1) / 2 per real site (looper's in-real-site bonds).
  * chain_id selects an independent random stream for the same seed. */
 typedef struct lg_params {
   int32_t lattice;       /* LG_LATTICE_CHAIN or LG_LATTICE_SQUARE */
@@ -70,6 +73,8 @@ typedef struct lg_params {
   int32_t two_s;         /* 2S: 1 (S=1/2; 0 is taken as 1), 2, 3 or 4 (chain), 2 (square) */
   double hx;             /* transverse field hx (site term -hx S^x, looper/model_parameter.h:67);
                             sign-free only with jxy <= 0, so jxy > 0 with hx != 0 is rejected */
+  double d;              /* single-ion term D (S^z)^2 (model_parameter.h:69); for S > 1/2 it adds
+                            in-site bonds; looper's worker scatters (FORCE_SCATTER 0.1) when D > 0 */
 } lg_params;
 
 /* Per-sweep collector sums (all improved estimators) -- see the collector of

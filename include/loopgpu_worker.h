@@ -21,7 +21,8 @@
  * temperature.h:56-113, no longer than THERMALIZATION, path_integral.C:
  * 212-213), J, Jxy, Jz, MODEL (= spin), local_S (1/2, 1, 3/2, 2 on the chain; 1/2, 1 on the square lattice), SWEEPS
  * (default "[65536:]"), THERMALIZATION (default SWEEPS.min/8), SEED,
- * FORCE_SCATTER, CHAIN_ID, h, Gamma (transverse field, sign-free with Jxy <= 0),
+ * FORCE_SCATTER (default 0.1 where D > 0 and S > 1/2, as looper), CHAIN_ID, h,
+ * Gamma (transverse field, sign-free with Jxy <= 0), D (single-ion term),
  * DISABLE_IMPROVED_ESTIMATOR,
  * MEASURE[Correlations], MEASURE[Structure Factor], INITIAL_SITE
  * (correlation.h:59-109).  Invalid or unsupported parameters give

@@ -133,10 +133,11 @@ def diagonalize_spin(kind, L, T, two_s, jxy=1.0, jz=1.0):
     return res
 
 
-def diagonalize_transverse(kind, L, T, two_s=1, jxy=1.0, jz=1.0, hx=0.0, h=0.0):
+def diagonalize_transverse(kind, L, T, two_s=1, jxy=1.0, jz=1.0, hx=0.0, h=0.0, d=0.0):
     """Thermal averages of the spin-S XXZ model in a transverse field,
     H = sum_b [Jz Sz Sz + Jxy/2 (S+S- + S-S+)] - hx sum_i S^x_i - h sum_i S^z_i
-    (looper/model_parameter.h:63-68, S^x = (S+ + S-)/2), on the whole product
+    + D sum_i (S^z_i)^2 (looper/model_parameter.h:63-68, S^x = (S+ + S-)/2;
+    d the single-ion D), on the whole product
     space: S^z is no longer conserved, so the Hamiltonian is diagonalised
     densely in one block and every static average and Kubo susceptibility
     (diag.C:252-293) is taken in its eigenbasis.  Sizes: (2S+1)^N states."""
@@ -149,8 +150,9 @@ def diagonalize_transverse(kind, L, T, two_s=1, jxy=1.0, jz=1.0, hx=0.0, h=0.0):
     dim = len(ms)
     key = {tuple(np.rint(2 * c).astype(int)): n for n, c in enumerate(ms)}
     H = np.zeros((dim, dim))
+    dion = d  # (d names a basis state below)
     for n, c in enumerate(ms):
-        H[n, n] -= h * c.sum()
+        H[n, n] += dion * (c * c).sum() - h * c.sum()
         for (a, b) in bonds:
             H[n, n] += jz * c[a] * c[b]
             for da, db in ((1, -1), (-1, 1)):

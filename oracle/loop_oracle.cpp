@@ -152,6 +152,7 @@ inline uint64_t exp_gap(const u32x4& x, double scale) {
 struct Lattice {
   int kind = 0, L = 0, N = 0, Nb = 0, dim = 1;
   int nv = 1;  // virtual sites per real site (2S)
+  int nb_lat = 0;   // the lattice's (virtual) bonds; [nb_lat, nb_real): in-site bonds of the single-ion term
   int nb_real = 0;  // bonds between two sites; bonds nb_real + s are site s's own (transverse field)
   std::vector<int> src, tgt, dir, gauge;
   std::vector<std::vector<int>> site_bonds;  // bonds incident to each site
@@ -180,7 +181,7 @@ struct Lattice {
       for (int v = 0; v < N; ++v) gauge[v] = real.gauge[v / nv];
       site_bonds.assign(N, std::vector<int>());
       for (int b = 0; b < Nb; ++b) { site_bonds[src[b]].push_back(b); site_bonds[tgt[b]].push_back(b); }
-      nb_real = Nb;
+      nb_lat = nb_real = Nb;
       return;
     }
     kind = k; L = l; nv = 1;
@@ -203,6 +204,23 @@ struct Lattice {
     }
     site_bonds.assign(N, std::vector<int>());
     for (int b = 0; b < Nb; ++b) { site_bonds[src[b]].push_back(b); site_bonds[tgt[b]].push_back(b); }
+    nb_lat = nb_real = Nb;
+  }
+  // Single-ion term D (S^z)^2 of S > 1/2: looper's in-real-site bonds
+  // (lattice.h:632-646), after the lattice's bonds, for each real site r the
+  // pairs a < c of its virtual sites, a outer: bond (r*nv + a, r*nv + c), an
+  // Ising coupling 2D (weight_impl.h:194-197); no bond vector
+  void add_d_bonds() {
+    const int nr = N / nv;
+    for (int r = 0; r < nr; ++r)
+      for (int a = 0; a < nv; ++a)
+        for (int c = a + 1; c < nv; ++c) {
+          const int b = (int)src.size();
+          src.push_back(r * nv + a); tgt.push_back(r * nv + c); dir.push_back(-1);
+          site_bonds[r * nv + a].push_back(b);
+          site_bonds[r * nv + c].push_back(b);
+        }
+    Nb = (int)src.size();
     nb_real = Nb;
   }
   // Transverse field: site s's operators live on bond nb_real + s, whose
@@ -216,8 +234,9 @@ struct Lattice {
     Nb += N;
   }
   bool is_site_bond(int b) const { return b >= nb_real; }
+  bool is_d_bond(int b) const { return b >= nb_lat && b < nb_real; }
   // relative bond vector component d (ALPS bond_vector_relative, unit lattice constant)
-  int vr(int b, int d) const { return is_site_bond(b) ? 0 : (kind == 0) ? 1 : (dir[b] == d ? 1 : 0); }
+  int vr(int b, int d) const { return b >= nb_lat ? 0 : (kind == 0) ? 1 : (dir[b] == d ? 1 : 0); }
 };
 
 // ---------------------------------------------------------------------------
@@ -541,8 +560,8 @@ struct LooperOp {
 
 struct LooperChain {
   Lattice lat;
-  Weights w;
-  double beta = 1, offset_total = 0, graph_weight = 0, p_off = 1;
+  Weights w, wd;  // lattice bonds; in-site bonds of the single-ion term
+  double beta = 1, offset_total = 0, graph_weight = 0, p_off = 1, p_off_d = 1;
   std::mt19937 eng;
   std::uniform_real_distribution<double> u01{0.0, 1.0};
   Alias alias;
@@ -559,7 +578,7 @@ struct LooperChain {
   Schedule sched; // annealing: beta = 1 / T(mcs) before each sweep (path_integral.C:325)
 
   void init(int kind, int L, double b, double jxy, double jz, double fs, uint32_t seed, double field = 0,
-            int nv = 1, double hx = 0) {
+            int nv = 1, double hx = 0, double d = 0) {
     lat.init(kind, L, nv);
     w = xxz_weights(jxy, jz, fs);
     beta = b;
@@ -570,6 +589,20 @@ struct LooperChain {
       for (int g = 0; g < 4; ++g)
         if (w.v[g] > 1e-10) { wl.push_back(w.v[g]); graph_of.push_back({bd, g}); graph_weight += w.v[g]; }
     offset_total = lat.Nb * w.offset;
+    // single-ion term D (S^z)^2 = D nv/4 + sum_{a<c} 2D s_a s_c per real
+    // site: the in-site bonds with the weights of an Ising coupling 2D
+    // (weight_impl.h:194-197) and the constant (model_parameter.h:592-595,
+    // weight_impl.h:400-404)
+    if (d != 0) {
+      const int nb = lat.Nb;
+      if (nv > 1) lat.add_d_bonds();
+      wd = xxz_weights(0.0, 2 * d, fs);
+      for (int bd = nb; bd < lat.Nb; ++bd)
+        for (int g = 0; g < 4; ++g)
+          if (wd.v[g] > 1e-10) { wl.push_back(wd.v[g]); graph_of.push_back({bd, g}); graph_weight += wd.v[g]; }
+      offset_total += (lat.Nb - nb) * wd.offset + (lat.N / nv) * 0.25 * nv * d;
+      p_off_d = (wd.v[0] + wd.v[1] > 1e-10) ? wd.v[0] / (wd.v[0] + wd.v[1]) : 1;
+    }
     // transverse field -hx S^x: the site graph (weight and offset |hx|/2,
     // weight_impl.h:63-88) at every site, drawn with the bond graphs in one
     // table by weight (the model's choose_graph over all locations)
@@ -637,7 +670,7 @@ struct LooperChain {
         continue;
       }
       if (oi.type & 1) {
-        int g = (rnd() < p_off) ? 0 : 1;    // choose_offdiagonal, graph_impl.h:323-327
+        int g = (rnd() < (lat.is_d_bond(b) ? p_off_d : p_off)) ? 0 : 1;    // choose_offdiagonal, graph_impl.h:323-327
         oi.type = (oi.type & 1) | (g << 2);
         spins_c[s0] ^= 1;
         spins_c[s1] ^= 1;
@@ -981,8 +1014,9 @@ void looper_labels(const Lattice& lat, int nops, const int* bond, const double* 
 // ---------------------------------------------------------------------------
 struct PhiloxChain {
   Lattice lat;
-  Weights w;
-  Thresholds th;
+  Weights w, wd;     // lattice bonds; in-site bonds of the single-ion term
+  Thresholds th, thd;
+  double Wd = 0, scale_d = 0;
   double beta = 1, scale = 0, offset_total = 0, W = 0;
   double h = 0;   // longitudinal field
   double vs = 0, site_scale = 0;  // transverse field: site graph weight |hx|/2 and its candidate scale
@@ -1005,7 +1039,7 @@ struct PhiloxChain {
   int last_nodes = 0;
 
   void init(int kind, int L, double b, double jxy, double jz, double fs, uint32_t seed, uint32_t chain,
-            double field = 0, int nv = 1, double hx = 0) {
+            double field = 0, int nv = 1, double hx = 0, double d = 0) {
     h = field;
     lat.init(kind, L, nv);
     w = xxz_weights(jxy, jz, fs);
@@ -1014,6 +1048,18 @@ struct PhiloxChain {
     W = w.v[0] + w.v[1] + w.v[2] + w.v[3];
     scale = (W > 0) ? std::ldexp(1.0, 52) / (beta * W) : 0;
     offset_total = lat.Nb * w.offset;
+    // single-ion term: in-site bonds with the weights of an Ising coupling
+    // 2D, their own candidate rate and thresholds, and the constant D nv/4
+    // per real site
+    if (d != 0) {
+      const int nb = lat.Nb;
+      if (nv > 1) lat.add_d_bonds();
+      wd = xxz_weights(0.0, 2 * d, fs);
+      thd = make_thresholds(wd);
+      Wd = wd.v[0] + wd.v[1] + wd.v[2] + wd.v[3];
+      scale_d = (Wd > 0) ? std::ldexp(1.0, 52) / (beta * Wd) : 0;
+      offset_total += (lat.Nb - nb) * wd.offset + (lat.N / nv) * 0.25 * nv * d;
+    }
     // transverse field -hx S^x: one site graph of weight |hx|/2 and offset
     // |hx|/2 per site (weight_impl.h:63-88), its operators on the site bonds
     if (hx != 0) {
@@ -1079,6 +1125,7 @@ struct PhiloxChain {
       beta = 1.0 / sched.T((int)sweep);
       scale = (W > 0) ? std::ldexp(1.0, 52) / (beta * W) : 0;
       if (vs > 0) site_scale = std::ldexp(1.0, 52) / (beta * vs);
+      if (Wd > 0) scale_d = std::ldexp(1.0, 52) / (beta * Wd);
     }
     // (a) diagonal update, per bond
     std::vector<int> nboff(Nb + 1, 0);
@@ -1093,15 +1140,18 @@ struct PhiloxChain {
       // a site bond's candidates: rate beta |hx|/2, the one site graph
       // (g = 0), compatible with either spin (graph_impl.h:66-69)
       const bool site = lat.is_site_bond(b);
+      // an in-site bond of the single-ion term: its own rate and graphs
+      const bool dbond = lat.is_d_bond(b);
+      const Thresholds& tb = dbond ? thd : th;
       acc_t.clear(); acc_g.clear();
-      if (site ? vs > 0 : W > 0) {
+      if (site ? vs > 0 : (dbond ? Wd > 0 : W > 0)) {
         uint64_t T = 0;
         for (uint32_t k = 0;; ++k) {
           u32x4 x = philox4x32_10(sweep, STAGE_DIAG, (uint32_t)b, k, key0, key1);
-          T += exp_gap(x, site ? site_scale : scale);
+          T += exp_gap(x, site ? site_scale : (dbond ? scale_d : scale));
           if (T >= TGRID) break;
           double tau = (double)T * (1.0 / 4503599627370496.0);
-          int g = site ? 0 : choose_graph(x.v[2], th);
+          int g = site ? 0 : choose_graph(x.v[2], tb);
           int c0 = spin_at(s0, tau), c1 = spin_at(s1, tau);
           if (c0 < 0 || c1 < 0) continue;
           if (site || is_compatible(g, c0, c1)) { acc_t.push_back(tau); acc_g.push_back(g); }
@@ -1113,7 +1163,7 @@ struct PhiloxChain {
       for (int r = boff[b]; r < boff[b + 1]; ++r) {
         if ((otype[r] & 1) == 0) continue;
         u32x4 x = philox4x32_10(sweep, STAGE_REGRAPH, (uint32_t)b, (uint32_t)kept_idx, key0, key1);
-        int g = (site || (uint64_t)x.v[0] < th.toff) ? 0 : 1;
+        int g = (site || (uint64_t)x.v[0] < tb.toff) ? 0 : 1;
         ++kept_idx;
         while (ai < acc_t.size() && acc_t[ai] < otime[r]) {
           ntime.push_back(acc_t[ai]); ntype.push_back(acc_g[ai] << 2); ++ai;
@@ -1352,9 +1402,9 @@ void or_uf_pairs(int n, int npairs, const int* pairs, int variant, int* parent, 
 
 // ---- looper mode ---------------------------------------------------------
 void* or_looper_create(int kind, int L, double beta, double jxy, double jz, double fs, uint32_t seed, double h,
-                       int nv, double hx) {
+                       int nv, double hx, double d) {
   LooperChain* c = new LooperChain();
-  c->init(kind, L, beta, jxy, jz, fs, seed, h, nv, hx);
+  c->init(kind, L, beta, jxy, jz, fs, seed, h, nv, hx, d);
   return c;
 }
 void or_looper_sweep(void* h, int n, double* rec) {
@@ -1402,9 +1452,9 @@ int or_segment_of_elements(int kind, int L, int nops, const int* bond, const dou
 
 // ---- philox mode -------------------------------------------------------------
 void* or_philox_create(int kind, int L, double beta, double jxy, double jz, double fs, double h, uint32_t seed,
-                       uint32_t chain, int nv, double hx) {
+                       uint32_t chain, int nv, double hx, double d) {
   PhiloxChain* c = new PhiloxChain();
-  c->init(kind, L, beta, jxy, jz, fs, seed, chain, h, nv, hx);
+  c->init(kind, L, beta, jxy, jz, fs, seed, chain, h, nv, hx, d);
   return c;
 }
 void or_philox_sweep(void* h, int n, double* rec) {
@@ -1418,6 +1468,7 @@ void or_philox_set_beta(void* h, double beta) {
   c->beta = beta;
   c->scale = (c->W > 0) ? std::ldexp(1.0, 52) / (beta * c->W) : 0;
   if (c->vs > 0) c->site_scale = std::ldexp(1.0, 52) / (beta * c->vs);
+  if (c->Wd > 0) c->scale_d = std::ldexp(1.0, 52) / (beta * c->Wd);
 }
 int or_philox_num_ops(void* h) { return (int)((PhiloxChain*)h)->otime.size(); }
 int or_philox_num_nodes(void* h) { return ((PhiloxChain*)h)->last_nodes; }

@@ -19,7 +19,8 @@ Sources (data files of the reference's tests):
                                 MEASURE[Structure Factor] (chain L=4, S=1/2, T=0.5): its
                                 scalar results and vector entries ("Name[label]"); the ED and
                                 path-integral blocks of its other models (S=1 chain L=4, one S=1
-                                site in a transverse field Gamma=1 or a field h=0.3, Ising chain)
+                                site in a transverse field Gamma=1 or a field h=0.3, Ising chain,
+                                S=1 Ising chain with a single-ion term D=-0.2)
 """
 
 import json
@@ -155,7 +156,8 @@ def loop_op_models():
     """the diagonalization and "loop; path integral" blocks of the other
     models of loop.ip, keyed by model: S=1 chain L=4 (T=1/L), one S=1 site
     in a transverse field Gamma=1 (T=1/L), one S=1 site in a longitudinal
-    field h=0.3 (T=1/L), S=1/2 Ising chain L=4 (Jxy=0, Jz=1, T=0.5)"""
+    field h=0.3 (T=1/L), S=1/2 Ising chain L=4 (Jxy=0, Jz=1, T=0.5), S=1
+    Ising chain L=4 with a single-ion term (Jxy=0, Jz=-1, D=-0.2, T=2)"""
     out = {}
     for b in loop_op_blocks():
         p = b["params"]
@@ -165,12 +167,14 @@ def loop_op_models():
             key = "site_spin1_gamma1" if p.get("Gamma") == "1" else "site_spin1_h03"
         elif p.get("local_S") == "1" and "D" not in p:
             key = "chain4_spin1"
+        elif p.get("local_S") == "1" and p.get("D") == "-0.2":
+            key = "chain4_spin1_d"
         elif p.get("Jxy") == "0" and p.get("local_S") == "1/2":
             key = "chain4_ising"
         else:
             continue
         out.setdefault(key, []).extend(keep_blocks([b]))
-    assert sorted(out) == ["chain4_ising", "chain4_spin1", "site_spin1_gamma1", "site_spin1_h03"]
+    assert sorted(out) == ["chain4_ising", "chain4_spin1", "chain4_spin1_d", "site_spin1_gamma1", "site_spin1_h03"]
     assert all(len(v) == 2 for v in out.values())
     return out
 

@@ -35,7 +35,7 @@ lib.or_lattice_info.argtypes = [I, I, I, vp, vp, vp]
 lib.or_lattice_bonds.argtypes = [I, I, I, vp, vp, vp]
 lib.or_uf_pairs.argtypes = [I, I, vp, I, vp, vp, vp]
 lib.or_looper_create.restype = vp
-lib.or_looper_create.argtypes = [I, I, D, D, D, D, U, D, I, D]
+lib.or_looper_create.argtypes = [I, I, D, D, D, D, U, D, I, D, D]
 lib.or_looper_sweep.argtypes = [vp, I, vp]
 lib.or_looper_destroy.argtypes = [vp]
 lib.or_standalone_run.argtypes = [U, D, U, U, vp, vp]
@@ -44,7 +44,7 @@ lib.or_looper_labels.argtypes = [I, I, I, vp, vp, vp, I, vp]
 lib.or_segment_of_elements.argtypes = [I, I, I, vp, vp, vp]
 lib.or_segment_of_elements.restype = I
 lib.or_philox_create.restype = vp
-lib.or_philox_create.argtypes = [I, I, D, D, D, D, D, U, U, I, D]
+lib.or_philox_create.argtypes = [I, I, D, D, D, D, D, U, U, I, D, D]
 lib.or_philox_sweep.argtypes = [vp, I, vp]
 lib.or_philox_set_beta.argtypes = [vp, D]
 lib.or_philox_num_ops.argtypes = [vp]
@@ -144,9 +144,9 @@ def xxz_weights(jxy, jz, fs=0.0):
 
 
 class LooperChain:
-    def __init__(self, kind, L, beta, jxy=1.0, jz=None, seed=23125, fs=0.0, field=0.0, two_s=1, hx=0.0):
+    def __init__(self, kind, L, beta, jxy=1.0, jz=None, seed=23125, fs=0.0, field=0.0, two_s=1, hx=0.0, d=0.0):
         jz = jxy if jz is None else jz
-        self.h = lib.or_looper_create(kind, L, beta, jxy, jz, fs, seed, field, two_s if two_s >= 2 else 1, hx)
+        self.h = lib.or_looper_create(kind, L, beta, jxy, jz, fs, seed, field, two_s if two_s >= 2 else 1, hx, d)
 
     def sweep(self, n):
         rec = np.zeros((n, R))
@@ -163,14 +163,16 @@ class LooperChain:
 
 class PhiloxChain:
     def __init__(self, kind, L, beta, jxy=1.0, jz=None, seed=23125, chain_id=0, fs=0.0, field=0.0, two_s=1,
-                 hx=0.0):
+                 hx=0.0, d=0.0):
         jz = jxy if jz is None else jz
         self.kind, self.L = kind, L
         nv = two_s if two_s >= 2 else 1
         self.N, self.Nb, _, _, _ = lattice(kind, L, nv)
+        if d != 0:
+            self.Nb += (self.N // nv) * nv * (nv - 1) // 2  # in-site bonds of the single-ion term
         if hx != 0:
-            self.Nb += self.N  # bonds Nb + s: site s's operators (transverse field)
-        self.h = lib.or_philox_create(kind, L, beta, jxy, jz, fs, field, seed, chain_id, nv, hx)
+            self.Nb += self.N  # then one bond per site: its operators (transverse field)
+        self.h = lib.or_philox_create(kind, L, beta, jxy, jz, fs, field, seed, chain_id, nv, hx, d)
 
     def sweep(self, n):
         rec = np.zeros((n, R))
