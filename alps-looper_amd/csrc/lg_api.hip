@@ -75,6 +75,18 @@ int check_params(const lg_params& p) {
   if (!std::isfinite(p.jxy) || !std::isfinite(p.jz) || !std::isfinite(p.h) || !std::isfinite(p.hx) ||
       !std::isfinite(p.d))
     return set_error(LG_ERR_ARG, "couplings and fields must be finite");
+  if (p.algorithm != LG_ALGORITHM_PATH_INTEGRAL && p.algorithm != LG_ALGORITHM_SSE)
+    return set_error(LG_ERR_ARG, "algorithm must be 0 (loop; path integral) or 1 (loop; sse)");
+  if (p.algorithm == LG_ALGORITHM_SSE) {
+    if (p.hx != 0 || p.d != 0 || p.h != 0)
+      return set_error(LG_ERR_ARG, "the SSE worker is built for lattice bonds only (h = hx = d = 0)");
+    const long long nv = p.two_s >= 2 ? p.two_s : 1;
+    const long long nr = p.lattice == LG_LATTICE_CHAIN ? p.L : (long long)p.L * p.L;
+    const long long nb = nv * nv * (p.lattice == LG_LATTICE_CHAIN ? p.L : 2ll * p.L * p.L);
+    if (nv * nr > SSE_MAX_SITES || nb > SSE_MAX_BONDS)
+      return set_error(LG_ERR_ARG, "the SSE worker takes at most " + std::to_string(SSE_MAX_SITES) +
+                                       " (virtual) sites and " + std::to_string(SSE_MAX_BONDS) + " bonds");
+  }
   // an antiferromagnetic XY coupling with a transverse field has a sign
   // problem that no gauge removes (looper/model_parameter.h:617-644 finds the
   // lattice frustrated and the reference takes its SIGN path)
@@ -120,6 +132,13 @@ ChainDev make_chain_dev(const lg_params& p) {
   d.scale = candidate_scale(p.beta, W);
   d.offset_total = d.Nb * off;
   graph_thresholds(v, d.thr, &d.toff);
+  // SSE: choose_diagonal (graph_impl.h:297-312): g = 0 w.p. v0 / (v0 + v2)
+  // for an antiparallel pair, g = 1 w.p. v1 / (v1 + v3) for a parallel one
+  d.sse = p.algorithm == LG_ALGORITHM_SSE ? 1 : 0;
+  for (int c = 0; c < 2; ++c) {
+    const double den = v[c] + v[c + 2];
+    d.pdiag[c] = (den > 0) ? (uint64_t)std::floor(std::ldexp(v[c] / den, 32)) : (1ull << 32);
+  }
   d.key0 = p.seed;
   d.key1 = p.chain_id;
   d.h = p.h;
@@ -227,6 +246,7 @@ int Batch::init(const lg_params* p, int n) {
   for (int c = 0; c < n; ++c) {
     int rc = check_params(p[c]);
     if (rc) return rc;
+    if (p[c].algorithm != p[0].algorithm) return set_error(LG_ERR_ARG, "a batch holds chains of one algorithm");
   }
   int rc = check_device();
   if (rc) return rc;
@@ -242,6 +262,9 @@ int Batch::init(const lg_params* p, int n) {
   cap.ops = std::max(cap.ops, min_op_cap);
   rc = reserve(cap, false);
   if (rc) return rc;
+  if (p[0].algorithm == LG_ALGORITHM_SSE)
+    for (int i = 0; i < 2; ++i)
+      if ((rc = alloc(sse_str[i], op_cap))) return rc;
   std::vector<ChainDev> ds(n);
   for (int c = 0; c < n; ++c) ds[c] = make_chain_dev(p[c]);
   params.assign(p, p + n);
@@ -260,8 +283,9 @@ int Batch::init(const lg_params* p, int n) {
 int Batch::init_exchange(const lg_params& p, const double* betas, int n, uint32_t seed) {
   if (n < 2) return set_error(LG_ERR_ARG, "exchange needs at least two temperatures");
   if (!betas) return set_error(LG_ERR_ARG, "null pointer");
-  if (p.hx != 0 || p.d != 0)
-    return set_error(LG_ERR_UNSUPPORTED, "replica exchange with a transverse field or a single-ion term is not built");
+  if (p.hx != 0 || p.d != 0 || p.algorithm != LG_ALGORITHM_PATH_INTEGRAL)
+    return set_error(LG_ERR_UNSUPPORTED,
+                     "replica exchange with a transverse field, a single-ion term or SSE chains is not built");
   std::vector<lg_params> ps(n, p);
   for (int t = 0; t < n; ++t) {
     ps[t].beta = betas[t];
@@ -441,11 +465,17 @@ int Batch::allocate() {
 
 // New bases (contiguous in the given order), chain tables, sweep counters.
 int Batch::set_layout(std::vector<ChainDev>& ds, const std::vector<uint64_t>& sweeps, bool check_old) {
-  long long sites = 0, bonds = 0, cand = 0;
-  for (auto& d : ds) {
+  long long sites = 0, bonds = 0, cand = 0, strs = 0;
+  for (size_t c = 0; c < ds.size(); ++c) {
+    ChainDev& d = ds[c];
     d.site_base = (int)sites;
     d.bond_base = (int)bonds;
     d.cand_base = cand;
+    if (d.sse) {  // the chain's operator string: its share of the operator capacity
+      d.sse_base = (int)strs;
+      d.sse_cap = (int)std::min<long long>(footprint(params[c]).ops, (long long)op_cap - strs);
+      strs += d.sse_cap;
+    }
     // candidate slots slot-major (slot j of bond b at j * Nb + b) when bonds
     // take few candidates, so that the bonds of a wave write adjacent words
     // (512-chain scan: diag_fill 0.124 -> 0.108 ms); bond-major when they
@@ -609,6 +639,9 @@ int Batch::reset_state() {
 // after its current extent.  Nothing is copied on the device; the next sweep
 // reads each chain at its old bases and writes the new layout.
 int Batch::relayout(const std::vector<int>& keep, const std::vector<lg_params>& add) {
+  if (is_sse()) return set_error(LG_ERR_UNSUPPORTED, "relayout of SSE chains is not built");
+  for (const auto& p : add)
+    if (p.algorithm != LG_ALGORITHM_PATH_INTEGRAL) return set_error(LG_ERR_UNSUPPORTED, "SSE chains are not served");
   std::vector<ChainDev> ds;
   std::vector<uint64_t> sw;
   std::vector<lg_params> ps;
@@ -705,6 +738,10 @@ DevBatch Batch::dev(int from) const {
   B.nslots = nslots; B.xslot = xslot; B.xbeta = xbeta; B.xscale = xscale; B.xlnr = xlnr;
   B.xacc = xacc; B.xstep = xstep; B.xkey0 = xkey0; B.xkey1 = xkey1;
   B.op_cap = op_cap; B.node_cap = node_cap;
+  B.any_sse = is_sse() ? 1 : 0;
+  B.sse_old = sse_str[from];
+  B.sse_new = sse_str[1 - from];
+  B.sse_perm = kpre;  // (the path-integral diagonal update's, unused by SSE batches)
   B.sched = d_sched;
   return B;
 }
@@ -1014,6 +1051,22 @@ int Batch::set_config(int c, const int32_t* spins_in, const int32_t* bond_offset
   LG_HIP(hipMemcpy(s.soff, soff.data(), (d.N + 1) * sizeof(int), hipMemcpyHostToDevice));
   LG_HIP(hipMemcpy(s.evt, evt.data(), evt.size() * sizeof(double), hipMemcpyHostToDevice));
   LG_HIP(hipMemcpy(s.segspin, segspin.data(), segspin.size(), hipMemcpyHostToDevice));
+  if (d.sse) {  // SSE: the string in series order, which the times keep (ties by bond)
+    std::vector<int> bond_of(nops), order(nops), str(nops);
+    for (int b = 0; b < d.Nb; ++b)
+      for (int k = bo[b]; k < bo[b + 1]; ++k) bond_of[k] = b;
+    for (long long k = 0; k < nops; ++k) order[k] = (int)k;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+      if (ot[a] != ot[b]) return ot[a] < ot[b];
+      return bond_of[a] < bond_of[b];
+    });
+    if (nops > d.sse_cap) return set_error(LG_ERR_CAPACITY, "configuration exceeds the operator string capacity");
+    for (long long p = 0; p < nops; ++p) {
+      const int k = order[p];
+      str[p] = (bond_of[k] << 3) | ((types[k] >> 2) << 1) | (types[k] & 1);
+    }
+    if (nops) LG_HIP(hipMemcpy(sse_str[cur] + d.sse_base, str.data(), nops * sizeof(int), hipMemcpyHostToDevice));
+  }
   uint32_t sw = (uint32_t)sweep;
   LG_HIP(hipMemcpy(d_sweep, &sw, sizeof(uint32_t), hipMemcpyHostToDevice));
   host_sweep[c] = sweep;
@@ -1195,6 +1248,7 @@ int Batch::sync_schedule() {
 }
 
 int Batch::set_schedule(int n, const int32_t* steps, const double* temps) {
+  if (is_sse()) return set_error(LG_ERR_UNSUPPORTED, "annealing of SSE chains is not built");
   if (chains.size() != 1) return set_error(LG_ERR_UNSUPPORTED, "an annealing schedule needs a single-chain handle");
   if (n < 0 || (n > 0 && (!steps || !temps))) return set_error(LG_ERR_ARG, "invalid schedule");
   const lg_params p = params[0];
@@ -1403,8 +1457,9 @@ int lg_cluster_config(const lg_params* p, const int32_t* spins, const int32_t* b
                       const int32_t* types, uint64_t sweep, int32_t* labels, int32_t* spins_out,
                       int32_t* types_out, int64_t* n_clusters) {
   if (!p || !spins || !bond_offsets) return set_error(LG_ERR_ARG, "null pointer");
-  if (p->hx != 0 || p->d != 0)
-    return set_error(LG_ERR_UNSUPPORTED, "the cluster stage of a given configuration takes hx = 0 and d = 0");
+  if (p->hx != 0 || p->d != 0 || p->algorithm != LG_ALGORITHM_PATH_INTEGRAL)
+    return set_error(LG_ERR_UNSUPPORTED,
+                     "the cluster stage of a given configuration takes hx = 0, d = 0 and the path integral");
   lg_params q = *p;
   // capacity from the configuration itself
   lg::ChainDev d = lg::make_chain_dev(q);

@@ -39,7 +39,7 @@ constexpr uint64_t TGRID = 1ull << 52;
 constexpr double TGRID_INV = 1.0 / 4503599627370496.0;
 constexpr double EMPTY_TIME = 2.0;  // ev_time of the single node of an event-free site
 
-enum : uint32_t { STAGE_DIAG = 1, STAGE_REGRAPH = 2, STAGE_FLIP = 3, STAGE_EXCHANGE = 4, STAGE_BOUNDARY = 5 };
+enum : uint32_t { STAGE_DIAG = 1, STAGE_REGRAPH = 2, STAGE_FLIP = 3, STAGE_EXCHANGE = 4, STAGE_BOUNDARY = 5, STAGE_SSE = 6 };
 
 // ev_flags bits
 // EV_FIRST: lowest segment of its site (wraps through tau = 1, unless
@@ -81,7 +81,14 @@ struct ChainDev {
   int nb_lat, npair;
   double dwsum, dscale;              // in-site bonds: sum_g v_g and candidate scale 2^52 / (beta dwsum)
   uint64_t dthr[3], dtoff;           // in-site bonds: graph and regraph thresholds
+  // "loop; sse" (sse.C): the operator string in series order at
+  // [sse_base, sse_base + sse_cap) of the batch's string arrays, one int
+  // per operator (chain-local bond << 3 | graph << 1 | off-diagonal)
+  int sse, sse_base, sse_cap;
+  uint64_t pdiag[2];                 // choose_diagonal thresholds (antiparallel, parallel pair)
 };
+// SSE chains: their sites' spins and bonds' counters are staged in LDS
+constexpr int SSE_MAX_SITES = 4096, SSE_MAX_BONDS = 4096;
 
 // Temperature of Monte Carlo step `step` on an annealing schedule of n
 // (start step, temperature) pairs stored as doubles, start steps ascending,

@@ -86,6 +86,7 @@ class Batch {
   // some chain has bonds beyond the lattice's (in-site bonds of a
   // single-ion term, site operators of a transverse field), which only the
   // per-operator linking kernel places
+  bool is_sse() const { return !chains.empty() && chains[0].sse; }
   bool has_extra_bonds() const {
     for (const auto& d : chains) if (d.Nb > d.nb_lat) return true;
     return false;
@@ -124,6 +125,7 @@ class Batch {
   bool tuned = false;
   State st[2];
   int cur = 0;                 // st[cur] holds the latest configuration
+  int* sse_str[2] = {nullptr, nullptr};  // SSE batches: operator strings, sse_str[cur] the latest
   LaunchCfg cfg;
   bool use_graph = false;
   uint64_t graph_captures = 0;     // sweep graphs captured and instantiated

@@ -84,6 +84,10 @@ struct DevBatch {
   unsigned long long* rmask;   // [nodes / 64 + 1] 1 for the roots, 64 nodes per word
   int any_split;               // some chain has virtual sites (S=1): segments without wrap, boundary permutation
   int any_extra;               // bonds beyond the lattice's in some chain: 1 single-ion in-site bonds, 2 transverse-field site bonds (bits)
+  int any_sse;                 // the chains are SSE chains ("loop; sse"; a batch holds one kind)
+  const int* sse_old;          // operator strings of the previous configuration (ChainDev::sse_base)
+  int* sse_new;                // operator strings being built
+  int* sse_perm;               // [ops] string position -> operator index of the new configuration
 };
 
 struct LaunchCfg {

@@ -1972,6 +1972,167 @@ __global__ void k_canon_label(DevBatch B, int chain, int nelem, const int* canon
 }
 
 // ---------------------------------------------------------------------------
+// "loop; sse" (sse.C:174-400): the same loop clusters on the stochastic
+// series expansion.  A chain's operator string (series order) is walked by
+// one wave, all lanes carrying the same state: before each kept operator,
+// and again after each insertion, a gap tries an insertion when
+// (n + 1) u < beta W_tot (W_tot = Nb sum_g v_g; bond uniform from word 2,
+// graph from word 3 by the thresholds, kept if compatible); a diagonal
+// operator is removed when beta W_tot u < n (the next one follows with no gap
+// between) or else regraphed by choose_diagonal (graph_impl.h:297-312); an
+// off-diagonal one by choose_offdiagonal.  Step k of the walk takes the words
+// Philox(sweep, 6, 0, k), computed 64 steps at a time, one per lane, and
+// handed round by lane shuffles; the spins and the per-bond counters live in
+// LDS.  The new string then becomes bond-major operators at times p / n, on
+// which the path-integral cluster stage runs unchanged (lengths in units of
+// the string), and the flipped types return to the string.
+// ---------------------------------------------------------------------------
+constexpr int SSE_THREADS = 64;
+
+__global__ void __launch_bounds__(SSE_THREADS) k_sse_diag(DevBatch B) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) scan_advance_epoch(B);  // for k_scan, which follows
+  if (overflowed(B)) return;
+  const int c = blockIdx.x;
+  const ChainDev& P = B.chains[c];
+  __shared__ uint8_t spc[SSE_MAX_SITES];
+  __shared__ int cnt[SSE_MAX_BONDS];
+  const int lane = threadIdx.x;
+  const int Nb = P.Nb;
+  for (int s = lane; s < P.N; s += SSE_THREADS) spc[s] = B.spins_old[P.old_site_base + s];
+  for (int b = lane; b < Nb; b += SSE_THREADS) cnt[b] = 0;
+  __syncthreads();
+  const uint32_t sweep = B.sweep[c];
+  const int n_old = B.bo_old[P.old_bond_base + Nb] - B.bo_old[P.old_bond_base];
+  const int* so = B.sse_old + P.sse_base;
+  int* sn = B.sse_new + P.sse_base;
+  const double bw = P.beta * ((double)Nb * P.wsum);
+  int nop = n_old, out = 0, i = 0, step = 0;
+  int xbase = -SSE_THREADS, obase = -SSE_THREADS;  // first step / old operator held by the lanes
+  U4 xr = {0, 0, 0, 0};
+  int ol = 0;
+  bool try_gap = true, full = false;
+  while (try_gap || i < n_old) {
+    if (step - xbase >= SSE_THREADS) {
+      xbase = step;
+      xr = philox(sweep, STAGE_SSE, 0u, (uint32_t)(step + lane), P.key0, P.key1);
+    }
+    const int j = step - xbase;
+    U4 x;
+    x.x = __shfl(xr.x, j);
+    x.y = __shfl(xr.y, j);
+    x.z = __shfl(xr.z, j);
+    x.w = __shfl(xr.w, j);
+    ++step;
+    const double u = uniform53(x);
+    int ne, b;
+    if (try_gap) {
+      try_gap = false;
+      if (!((double)(nop + 1) * u < bw)) continue;
+      b = (int)(((uint64_t)x.z * (uint64_t)Nb) >> 32);
+      const int g = choose_graph(x.w, P);
+      int s0, s1;
+      bond_sites(P, b, s0, s1);
+      if (((g & 1) ^ spc[s0] ^ spc[s1]) == 0) continue;
+      ne = (b << 3) | (g << 1);
+      ++nop;
+    } else {
+      if (i - obase >= SSE_THREADS) {
+        obase = i;
+        ol = (i + lane < n_old) ? so[i + lane] : 0;
+      }
+      const int e = __shfl(ol, i - obase);
+      ++i;
+      b = e >> 3;
+      int s0, s1;
+      bond_sites(P, b, s0, s1);
+      const int c0 = spc[s0], c1 = spc[s1];
+      if ((e & 1) == 0) {
+        if (bw * u < (double)nop) {  // removed
+          --nop;
+          continue;
+        }
+        const int cc = 1 ^ c0 ^ c1;  // 0 antiparallel, 1 parallel
+        ne = (b << 3) | (((((uint64_t)x.z < P.pdiag[cc]) ? 0 : 2) ^ cc) << 1);
+      } else {
+        ne = (b << 3) | ((((uint64_t)x.z < P.toff) ? 0 : 1) << 1) | 1;
+        if (lane == 0) {
+          spc[s0] = (uint8_t)(c0 ^ 1);
+          spc[s1] = (uint8_t)(c1 ^ 1);
+        }
+      }
+    }
+    if (out >= P.sse_cap) {
+      full = true;
+      break;
+    }
+    if (lane == 0) {
+      sn[out] = ne;
+      cnt[b] += 1;
+    }
+    ++out;
+    try_gap = true;
+  }
+  if (full && lane == 0) atomicOr(B.flags, FLAG_OP_OVERFLOW);
+  __syncthreads();
+  for (int b = lane; b < Nb; b += SSE_THREADS) B.counts[P.bond_base + b] = cnt[b];
+}
+
+// the new string into bond-major order (series order within a bond), times
+// p / n; perm[p] = the operator index of string position p
+__global__ void __launch_bounds__(SSE_THREADS) k_sse_scatter(DevBatch B) {
+  if (overflowed(B)) return;
+  const int c = blockIdx.x;
+  const ChainDev& P = B.chains[c];
+  __shared__ int cur[SSE_MAX_BONDS], koff[SSE_MAX_BONDS];
+  const int lane = threadIdx.x, Nb = P.Nb;
+  for (int b = lane; b < Nb; b += SSE_THREADS) {
+    cur[b] = B.bo_new[P.bond_base + b];
+    koff[b] = 0;
+  }
+  __syncthreads();
+  const int n = B.bo_new[P.bond_base + Nb] - B.bo_new[P.bond_base];
+  const int* sn = B.sse_new + P.sse_base;
+  int* perm = B.sse_perm + P.sse_base;
+  const double dn = (double)n;
+  for (int p0 = 0; p0 < n; p0 += SSE_THREADS) {
+    const int el = (p0 + lane < n) ? sn[p0 + lane] : 0;
+    const int m = min(SSE_THREADS, n - p0);
+    int pos = 0, kc = 0;
+    for (int j = 0; j < m; ++j) {  // in series order: each bond's next slot and off-diagonal count
+      const int e = __shfl(el, j);
+      const int b = e >> 3;
+      const int k = cur[b], q = koff[b];
+      if (lane == 0) {
+        cur[b] = k + 1;
+        koff[b] = q + (e & 1);
+      }
+      if (lane == j) {
+        pos = k;
+        kc = q;
+      }
+    }
+    if (p0 + lane < n) {
+      B.ot_new[pos] = (double)(p0 + lane) / dn;
+      B.oi_new[pos] = ((P.bond_base + (el >> 3)) << 3) | (el & 7);
+      B.kc_new[pos] = kc;
+      perm[p0 + lane] = pos;
+    }
+  }
+}
+
+// the flipped operator types back into the string
+__global__ void __launch_bounds__(256) k_sse_gather(DevBatch B) {
+  if (overflowed(B)) return;
+  const int c = blockIdx.x;
+  const ChainDev& P = B.chains[c];
+  // (k_end_sweep has moved the chain's old bases to the configuration just built)
+  const int o0 = B.bo_new[P.bond_base], n = B.bo_new[P.bond_base + P.Nb] - o0;
+  int* sn = B.sse_new + P.sse_base;
+  const int* perm = B.sse_perm + P.sse_base;
+  for (int p = threadIdx.x; p < n; p += blockDim.x) sn[p] = (sn[p] & ~1) | (B.oi_new[perm[p]] & 1);
+}
+
+// ---------------------------------------------------------------------------
 // host-side launchers
 // ---------------------------------------------------------------------------
 template <int G>
@@ -2021,7 +2182,23 @@ static inline void mark(hipEvent_t* ev, int i, hipStream_t st) {
   if (ev) (void)hipEventRecord(ev[i], st);
 }
 
+static void launch_sweep_sse(const DevBatch& B, const LaunchCfg& L, hipStream_t st, hipEvent_t* ev) {
+  mark(ev, 0, st);
+  hipLaunchKernelGGL(k_sse_diag, dim3(B.nchains), dim3(SSE_THREADS), 0, st, B);  // also the scan's epoch
+  mark(ev, 1, st);
+  hipLaunchKernelGGL(k_scan, dim3(scan_blocks(B)), dim3(SCAN_THREADS), 0, st, B);
+  mark(ev, 2, st);
+  hipLaunchKernelGGL(k_sse_scatter, dim3(B.nchains), dim3(SSE_THREADS), 0, st, B);
+  mark(ev, 3, st);
+  launch_cluster_stage(B, L, st, ev ? ev + 3 : nullptr);
+  hipLaunchKernelGGL(k_sse_gather, dim3(B.nchains), dim3(256), 0, st, B);
+}
+
 void launch_sweep(const DevBatch& B, const LaunchCfg& L, hipStream_t st, hipEvent_t* ev) {
+  if (B.any_sse) {
+    launch_sweep_sse(B, L, st, ev);
+    return;
+  }
   const int sb = scan_blocks(B);
   mark(ev, 0, st);
   diag_dispatch(B, L, st, false);  // also clears the scan's ticket, next epoch

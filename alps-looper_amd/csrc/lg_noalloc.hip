@@ -256,8 +256,9 @@ int lg_cluster_config_noalloc(const lg_params* p, const int32_t* spins, const in
     return set_error(LG_ERR_ARG, "null pointer");
   int rc = check_params(*p);
   if (rc) return rc;
-  if (p->hx != 0 || p->d != 0)
-    return set_error(LG_ERR_UNSUPPORTED, "the cluster stage of a given configuration takes hx = 0 and d = 0");
+  if (p->hx != 0 || p->d != 0 || p->algorithm != LG_ALGORITHM_PATH_INTEGRAL)
+    return set_error(LG_ERR_UNSUPPORTED,
+                     "the cluster stage of a given configuration takes hx = 0, d = 0 and the path integral");
   ChainDev d = make_chain_dev(*p);
   if (d.N > NA_SITES || n_ops < 0 || n_ops > NA_OPS)
     return set_error(LG_ERR_CAPACITY, "configuration beyond the no-allocation maximum (" +

@@ -314,6 +314,8 @@ int lg_server_submit(lg_server* s, const lg_job* job, uint64_t* id) {
   int rc = lg::check_params(job->params);
   if (rc) return serr(rc, lg::g_last_error_copy());
   if (job->sweeps == 0) return serr(LG_ERR_ARG, "job needs at least one measurement sweep");
+  if (job->params.algorithm != LG_ALGORITHM_PATH_INTEGRAL)
+    return serr(LG_ERR_UNSUPPORTED, "the server runs path-integral jobs");
   lg::Footprint f = lg::footprint(job->params);
   std::lock_guard<std::mutex> lk(s->mu);
   if (!s->fatal.empty()) return serr(LG_ERR_DEVICE, s->fatal);

@@ -244,6 +244,11 @@ class loop_worker {
     if (lp_.lattice == LG_LATTICE_SQUARE && p.defined("W") && (int)std::llround(p.num("W")) != lp_.L)
       throw std::invalid_argument("square lattice with W != L is not supported");
     if (p.str("MODEL", "spin") != "spin") throw std::invalid_argument("only MODEL = spin is supported");
+    // the worker: "loop; path integral" (path_integral.C) or "loop; sse" (sse.C)
+    const std::string alg = p.str("ALGORITHM", "loop; path integral");
+    if (alg == "loop; sse") lp_.algorithm = LG_ALGORITHM_SSE;
+    else if (alg != "loop; path integral" && alg != "loop")
+      throw std::invalid_argument("unsupported ALGORITHM \"" + alg + "\" (loop; path integral, loop; sse)");
     // spin size (model_parameter.h): S > 1/2 through 2S virtual sites per
     // real site (path_integral.C:589-615): S <= 2 on the chain, S <= 1 on
     // the square lattice
@@ -518,6 +523,10 @@ class loop_worker {
     m["Energy"] << r.energy;
     m["Energy Density"] << r.energy / vol;
     m["Energy^2"] << p2(r.energy) - nop / p2(beta);
+    // SSE: the tau-integrated sums come in units of the string;
+    // beta (n S + S0) / (n + 1) (susceptibility.h:213-251, 326-333)
+    const bool sse = lp_.algorithm == LG_ALGORITHM_SSE;
+    auto chi = [&](double s, double s0) { return sse ? beta * (nop * s + s0) / (nop + 1) / vol : beta * s / vol; };
     if (!improved_) {
       // normal estimator (susceptibility.h:304-337), the configuration before the flip
       const double mags[2][2] = {{r.umag0, r.umag_a}, {r.smag0, r.smag_a}};
@@ -532,7 +541,7 @@ class loop_worker {
         m[P + "Magnetization Density^2"] << p2(mg / vol);
         m[P + "Magnetization^4"] << p4(mg);
         m[P + "Magnetization Density^4"] << p4(mg / vol);
-        m[P + "Susceptibility"] << beta * p2(ma) / vol;
+        m[P + "Susceptibility"] << chi(p2(ma), p2(mg));
       }
       return;
     }
@@ -553,12 +562,12 @@ class loop_worker {
       m[P + "Magnetization Density^2"] << s.mag2 / p2(vol);
       m[P + "Magnetization^4"] << 3 * p2(s.mag2) - 2 * s.mag4;
       m[P + "Magnetization Density^4"] << (3 * p2(s.mag2) - 2 * s.mag4) / p4(vol);
-      m[P + "Susceptibility"] << beta * s.sus / vol;
+      m[P + "Susceptibility"] << chi(s.sus, s.mag2);
       m[gen + "Magnetization^2"] << s.size2;
       m[gen + "Magnetization Density^2"] << s.size2 / p2(vol);
       m[gen + "Magnetization^4"] << 3 * p2(s.size2) - 2 * s.size4;
       m[gen + "Magnetization Density^4"] << (3 * p2(s.size2) - 2 * s.size4) / p4(vol);
-      m[gen + "Susceptibility"] << beta * s.gsus / vol;
+      m[gen + "Susceptibility"] << chi(s.gsus, s.size2);
     }
   }
 

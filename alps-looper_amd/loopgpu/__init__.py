@@ -59,6 +59,7 @@ class LGParams(ctypes.Structure):
         ("jxy", ctypes.c_double), ("jz", ctypes.c_double), ("force_scatter", ctypes.c_double),
         ("seed", ctypes.c_uint32), ("chain_id", ctypes.c_uint32), ("h", ctypes.c_double),
         ("two_s", ctypes.c_int32), ("hx", ctypes.c_double), ("d", ctypes.c_double),
+        ("algorithm", ctypes.c_int32),
     ]
 
 
@@ -146,12 +147,15 @@ def kernel_names():
     return [lib.lg_profile_kernel_name(i).decode() for i in range(lib.lg_profile_num_kernels())]
 
 
+PATH_INTEGRAL, SSE = 0, 1   # lg_params.algorithm: "loop; path integral", "loop; sse"
+
+
 def params(lattice, L, beta, jxy=1.0, jz=None, seed=23125, chain_id=0, force_scatter=0.0, h=0.0, two_s=1,
-           hx=0.0, d=0.0):
+           hx=0.0, d=0.0, algorithm=PATH_INTEGRAL):
     if jz is None:
         jz = jxy
     return LGParams(int(lattice), int(L), float(beta), float(jxy), float(jz), float(force_scatter),
-                    int(seed), int(chain_id), float(h), int(two_s), float(hx), float(d))
+                    int(seed), int(chain_id), float(h), int(two_s), float(hx), float(d), int(algorithm))
 
 
 def lattice_size(lattice, L, two_s=1, hx=0.0, d=0.0):

@@ -75,7 +75,13 @@ typedef struct lg_params {
                             sign-free only with jxy <= 0, so jxy > 0 with hx != 0 is rejected */
   double d;              /* single-ion term D (S^z)^2 (model_parameter.h:69); for S > 1/2 it adds
                             in-site bonds; looper's worker scatters (FORCE_SCATTER 0.1) when D > 0 */
+  int32_t algorithm;     /* LG_ALGORITHM_PATH_INTEGRAL ("loop; path integral") or LG_ALGORITHM_SSE
+                            ("loop; sse", sse.C): the stochastic series expansion, lattice bonds only
+                            (h = hx = d = 0), at most 4096 (virtual) sites and bonds; a batch holds
+                            chains of one algorithm */
 } lg_params;
+
+enum { LG_ALGORITHM_PATH_INTEGRAL = 0, LG_ALGORITHM_SSE = 1 };
 
 /* Per-sweep collector sums (all improved estimators) -- see the collector of
  * susceptibility.h:177-198 and stiffness.h:117-121. */

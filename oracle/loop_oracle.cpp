@@ -74,7 +74,7 @@ inline u32x4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
 
 // Counter layout shared with the device: (sweep, stage, element, sub-index),
 // key = (seed, chain id).
-enum { STAGE_DIAG = 1, STAGE_REGRAPH = 2, STAGE_FLIP = 3, STAGE_BOUNDARY = 5 };
+enum { STAGE_DIAG = 1, STAGE_REGRAPH = 2, STAGE_FLIP = 3, STAGE_BOUNDARY = 5, STAGE_SSE = 6 };
 
 // ---------------------------------------------------------------------------
 // Deterministic natural logarithm built only from IEEE-754 +,-,*,/ (no fused
@@ -750,6 +750,131 @@ struct LooperChain {
     for (int s = 0; s < N; ++s) if (to_flip[frag[s].id_]) spins[s] ^= 1;
     ++mcs;
   }
+
+  // "loop; sse" (sse.C:174-400): the same loop clusters on the stochastic
+  // series.  The operator string is walked once; before each kept operator
+  // (and again after each insertion) a gap tries an insertion with
+  // probability beta W / (n + 1), the graph drawn by weight and kept if
+  // compatible; a diagonal operator is removed with probability
+  // n / (beta W), else its graph is re-chosen (choose_diagonal,
+  // graph_impl.h:297-312); an off-diagonal one is regraphed
+  // (choose_offdiagonal).  Times are positions in the new string (start at
+  // 0, stop at n).  Lattice bonds only (no site operators, no single-ion
+  // term, no field).  The record's tau-integrated sums (usize, umag, ssize,
+  // smag) are stored divided by n^2, i.e. with lengths in units of the
+  // string (n > 0), so that the susceptibility is
+  // beta (n umag + umag2) / (n + 1) / V as in susceptibility.h:213-251.
+  void sweep_sse(double* rec) {
+    const int N = lat.N;
+    if (sched.on()) beta = 1.0 / sched.T((int)mcs);
+    auto rnd = [&]() { return u01(eng); };
+    int nop = (int)ops.size();
+    std::swap(ops, ops_p);
+    ops.clear();
+    spins_c = spins;
+    frag.assign(N, UFNode());
+    for (int s = 0; s < N; ++s) current[s] = s;
+    int fid = N;
+    const double bw = beta * graph_weight;
+    const double pd[2] = {(w.v[0] + w.v[2] > 1e-10) ? w.v[0] / (w.v[0] + w.v[2]) : 1,
+                          (w.v[1] + w.v[3] > 1e-10) ? w.v[1] / (w.v[1] + w.v[3]) : 1};
+    bool try_gap = true;
+    size_t opi = 0;
+    while (try_gap || opi < ops_p.size()) {
+      if (try_gap) {
+        if ((nop + 1) * rnd() < bw) {
+          auto bg = graph_of[alias(rnd)];
+          const int b = bg.first, g = bg.second;
+          if (is_compatible(g, spins_c[lat.src[b]], spins_c[lat.tgt[b]])) {
+            ops.push_back(LooperOp{b, g << 2, 0.0, 0, 0});
+            ++nop;
+          } else {
+            try_gap = false;
+            continue;
+          }
+        } else {
+          try_gap = false;
+          continue;
+        }
+      } else {
+        LooperOp o = ops_p[opi];
+        const int c0 = spins_c[lat.src[o.bond]], c1 = spins_c[lat.tgt[o.bond]];
+        if ((o.type & 1) == 0) {
+          if (bw * rnd() < nop) {
+            --nop;
+            ++opi;
+            continue;
+          }
+          const int c = 1 ^ c0 ^ c1;  // 0 antiparallel, 1 parallel
+          o.type = ((rnd() < pd[c] ? 0 : 2) ^ c) << 2;
+        } else {
+          o.type = 1 | ((rnd() < p_off ? 0 : 1) << 2);
+        }
+        ops.push_back(o);
+        ++opi;
+        try_gap = true;
+      }
+      LooperOp& oi = ops.back();
+      const int b = oi.bond, s0 = lat.src[b], s1 = lat.tgt[b];
+      if (oi.type & 1) { spins_c[s0] ^= 1; spins_c[s1] ^= 1; }
+      if ((int)frag.size() < fid + 1) frag.resize(fid + 1);
+      Reconnect r = reconnect(frag, fid, oi.type >> 2, current[s0], current[s1]);
+      fid = r.fid; current[s0] = r.curr0; current[s1] = r.curr1;
+      oi.loop0 = r.loop0; oi.loop1 = r.loop1;
+    }
+    const int n = (int)ops.size();
+    for (int k = 0; k < n; ++k) ops[k].time = k;
+    if (lat.nv == 1) {
+      for (int s = 0; s < N; ++s) uf_unify(frag, s, current[s]);
+    } else {
+      std::vector<int> perm(lat.nv);
+      for (int off = 0; off < N; off += lat.nv) {
+        for (int i = 0; i < lat.nv; ++i) perm[i] = i;
+        partitioned_shuffle(perm.data(), lat.nv, &spins[off], &spins_c[off], rnd);
+        for (int i = 0; i < lat.nv; ++i) uf_unify(frag, off + i, current[off + perm[i]]);
+      }
+    }
+    int nc = uf_set_id(frag, 0, fid, 0);
+    uf_copy_id(frag, 0, fid);
+    est.assign(nc, Estimate());
+    spins_c = spins;
+    for (int s = 0; s < N; ++s) est_start_bottom(est[frag[s].id_], lat.gauge[s], spins_c[s]);
+    for (auto& op : ops) {
+      const int g = op.type >> 2;
+      if (is_frozen(g)) continue;
+      const int b = op.bond, s0 = lat.src[b], s1 = lat.tgt[b];
+      const double tt = op.time;
+      Estimate& el0 = est[frag[loop_l0(g, op.loop0, op.loop1)].id_];
+      Estimate& el1 = est[frag[loop_l1(g, op.loop0, op.loop1)].id_];
+      est_end(el0, tt, lat.gauge[s0], spins_c[s0]);
+      for (int d = 0; d < lat.dim; ++d) el0.winding[d] += (1 - 2 * spins_c[s0]) * lat.vr(b, d);
+      est_end(el1, tt, lat.gauge[s1], spins_c[s1]);
+      if (op.type & 1) { spins_c[s0] ^= 1; spins_c[s1] ^= 1; }
+      Estimate& eu0 = est[frag[loop_u0(g, op.loop0, op.loop1)].id_];
+      Estimate& eu1 = est[frag[loop_u1(g, op.loop0, op.loop1)].id_];
+      est_begin(eu0, tt, lat.gauge[s0], spins_c[s0]);
+      for (int d = 0; d < lat.dim; ++d) eu0.winding[d] -= (1 - 2 * spins_c[s0]) * lat.vr(b, d);
+      est_begin(eu1, tt, lat.gauge[s1], spins_c[s1]);
+    }
+    for (int s = 0; s < N; ++s) est_end(est[frag[current[s]].id_], (double)n, lat.gauge[s], spins_c[s]);
+    for (int i = 0; i < R_COUNT; ++i) rec[i] = 0;
+    for (int c = 0; c < nc; ++c) collect(est[c], rec);
+    if (n > 0) {
+      const double n2 = (double)n * n;
+      rec[R_USIZE] /= n2; rec[R_UMAG] /= n2; rec[R_SSIZE] /= n2; rec[R_SMAG] /= n2;
+    }
+    rec[R_NOP] = n;
+    rec[R_NC] = nc;
+    rec[R_ENERGY] = offset_total - n / beta;
+    rec[R_BETA] = beta;
+    rec[R_SWEEP] = (double)mcs;
+    to_flip.assign(nc, 0);
+    for (int c = 0; c < nc; ++c) to_flip[c] = (2 * rnd() - 1) < 0;
+    for (auto& op : ops)
+      if ((to_flip[frag[op.loop0].id_] ^ to_flip[frag[op.loop1].id_]) & 1) op.type ^= 1;
+    for (int s = 0; s < N; ++s) if (to_flip[frag[s].id_]) spins[s] ^= 1;
+    ++mcs;
+  }
 };
 
 // ---------------------------------------------------------------------------
@@ -1044,6 +1169,12 @@ struct PhiloxChain {
     lat.init(kind, L, nv);
     w = xxz_weights(jxy, jz, fs);
     th = make_thresholds(w);
+    // choose_diagonal (graph_impl.h:297-312): g = 0 w.p. v0 / (v0 + v2) for
+    // an antiparallel pair, g = 1 w.p. v1 / (v1 + v3) for a parallel one
+    for (int c = 0; c < 2; ++c) {
+      const double den = w.v[c] + w.v[c + 2];
+      pdiag[c] = (den > 0) ? (uint64_t)std::floor(std::ldexp(w.v[c] / den, 32)) : (1ull << 32);
+    }
     beta = b;
     W = w.v[0] + w.v[1] + w.v[2] + w.v[3];
     scale = (W > 0) ? std::ldexp(1.0, 52) / (beta * W) : 0;
@@ -1176,6 +1307,86 @@ struct PhiloxChain {
     boff.swap(nboff);
     otime.swap(ntime);
     otype.swap(ntype);
+    cluster_stage(rec);
+  }
+
+  // "loop; sse" in the device's formulation: the operator string in series
+  // order (bond, type), walked once by one thread per chain with one Philox
+  // word set per step, Philox(sweep, 6, 0, step): a gap tries an insertion
+  // when (n + 1) u < beta W_tot (u from the first 53 bits; the bond
+  // uniform from word 2, the graph from word 3 by the thresholds, kept if
+  // compatible, and the gap tried again), a kept diagonal operator is
+  // removed when beta W_tot u < n (the next operator follows with no gap
+  // between) or else regraphed by word 2 against the diagonal threshold of
+  // its spin pair, an off-diagonal one by word 2 against the off-diagonal
+  // threshold.  The new string becomes bond-major operators with times
+  // p / n, the cluster stage runs on them, and the flipped types return
+  // to the string.  Lattice bonds only.
+  std::vector<int> sbond, stype;          // SSE operator string
+  uint64_t pdiag[2] = {0, 0};             // choose_diagonal thresholds: antiparallel, parallel
+
+  void sweep_sse(double* rec) {
+    const int N = lat.N, Nb = lat.Nb;
+    if (sched.on()) beta = 1.0 / sched.T((int)sweep);
+    const double bw = beta * ((double)Nb * W);
+    std::vector<int> spc(spins), nb, nt;
+    int nop = (int)sbond.size();
+    uint32_t step = 0;
+    bool try_gap = true;
+    size_t i = 0;
+    while (try_gap || i < sbond.size()) {
+      const u32x4 x = philox4x32_10(sweep, STAGE_SSE, 0, step++, key0, key1);
+      const uint64_t u53 = ((uint64_t)x.v[0] << 21) | (uint64_t)(x.v[1] >> 11);
+      const double u = (double)(u53 + 1) * (1.0 / 9007199254740992.0);
+      if (try_gap) {
+        try_gap = false;
+        if ((double)(nop + 1) * u < bw) {
+          const int b = (int)(((uint64_t)x.v[2] * (uint64_t)Nb) >> 32);
+          const int g = choose_graph(x.v[3], th);
+          if (is_compatible(g, spc[lat.src[b]], spc[lat.tgt[b]])) {
+            nb.push_back(b); nt.push_back(g << 2);
+            ++nop;
+            try_gap = true;
+          }
+        }
+        continue;
+      }
+      const int b = sbond[i], t = stype[i], s0 = lat.src[b], s1 = lat.tgt[b];
+      ++i;
+      if ((t & 1) == 0) {
+        if (bw * u < (double)nop) { --nop; continue; }
+        const int c = 1 ^ spc[s0] ^ spc[s1];
+        nb.push_back(b); nt.push_back(((((uint64_t)x.v[2] < pdiag[c]) ? 0 : 2) ^ c) << 2);
+      } else {
+        nb.push_back(b); nt.push_back(1 | ((((uint64_t)x.v[2] < th.toff) ? 0 : 1) << 2));
+        spc[s0] ^= 1; spc[s1] ^= 1;
+      }
+      try_gap = true;
+    }
+    const int n = (int)nb.size();
+    std::vector<int> cnt(Nb + 1, 0), perm(n);
+    for (int p = 0; p < n; ++p) ++cnt[nb[p] + 1];
+    boff.assign(Nb + 1, 0);
+    for (int b = 0; b < Nb; ++b) boff[b + 1] = boff[b] + cnt[b + 1];
+    otime.assign(n, 0);
+    otype.assign(n, 0);
+    std::vector<int> cur(boff.begin(), boff.end() - 1);
+    for (int p = 0; p < n; ++p) {
+      const int k = cur[nb[p]]++;
+      otime[k] = (double)p / (double)n;
+      otype[k] = nt[p];
+      perm[p] = k;
+    }
+    cluster_stage(rec);
+    for (int p = 0; p < n; ++p) nt[p] = otype[perm[p]];
+    sbond.swap(nb);
+    stype.swap(nt);
+  }
+
+  // the cluster stage on the operators in boff / otime / otype (linking,
+  // union-find, estimators, flips, the record); advances the sweep counter
+  void cluster_stage(double* rec) {
+    const int N = lat.N, Nb = lat.Nb;
     int nops = (int)otime.size();
     std::vector<int> bond(nops);
     for (int b = 0; b < Nb; ++b) for (int k = boff[b]; k < boff[b + 1]; ++k) bond[k] = b;
@@ -1407,6 +1618,10 @@ void* or_looper_create(int kind, int L, double beta, double jxy, double jz, doub
   c->init(kind, L, beta, jxy, jz, fs, seed, h, nv, hx, d);
   return c;
 }
+void or_looper_sweep_sse(void* h, int n, double* rec) {
+  LooperChain* c = (LooperChain*)h;
+  for (int i = 0; i < n; ++i) c->sweep_sse(rec + (size_t)i * R_COUNT);
+}
 void or_looper_sweep(void* h, int n, double* rec) {
   LooperChain* c = (LooperChain*)h;
   for (int i = 0; i < n; ++i) c->sweep(rec + (size_t)i * R_COUNT);
@@ -1456,6 +1671,10 @@ void* or_philox_create(int kind, int L, double beta, double jxy, double jz, doub
   PhiloxChain* c = new PhiloxChain();
   c->init(kind, L, beta, jxy, jz, fs, seed, chain, h, nv, hx, d);
   return c;
+}
+void or_philox_sweep_sse(void* h, int n, double* rec) {
+  PhiloxChain* c = (PhiloxChain*)h;
+  for (int i = 0; i < n; ++i) c->sweep_sse(rec + (size_t)i * R_COUNT);
 }
 void or_philox_sweep(void* h, int n, double* rec) {
   PhiloxChain* c = (PhiloxChain*)h;

@@ -20,7 +20,8 @@ Sources (data files of the reference's tests):
                                 scalar results and vector entries ("Name[label]"); the ED and
                                 path-integral blocks of its other models (S=1 chain L=4, one S=1
                                 site in a transverse field Gamma=1 or a field h=0.3, Ising chain,
-                                S=1 Ising chain with a single-ion term D=-0.2)
+                                S=1 Ising chain with a single-ion term D=-0.2); its "loop; sse"
+                                blocks of the S=1/2, S=1 and Ising chains
 """
 
 import json
@@ -179,6 +180,22 @@ def loop_op_models():
     return out
 
 
+def loop_op_sse():
+    """the "loop; sse" blocks of loop.op for the models the SSE worker here
+    takes (lattice bonds only), keyed by model: S=1/2 chain L=4 (T=0.25),
+    S=1 chain L=4 (T=1/L), S=1/2 Ising chain L=4 (Jxy=0, Jz=1, T=0.5)"""
+    out = {}
+    for b in loop_op_blocks():
+        p = b["params"]
+        if p.get("ALGORITHM") != "loop; sse" or p.get("LATTICE") != "chain lattice" or "D" in p:
+            continue
+        key = ("chain4_spin1" if p.get("local_S") == "1" else
+               "chain4_ising" if p.get("Jxy") == "0" else "chain4")
+        out[key] = keep_blocks([b])[0]
+    assert sorted(out) == ["chain4", "chain4_ising", "chain4_spin1"]
+    return out
+
+
 def loop_op_correlations():
     """the "loop; path integral" block of loop.op with MEASURE[Correlations]
     and local_S = 1/2: scalar results and vector entries keyed Name[label]"""
@@ -316,6 +333,7 @@ def main():
         "loop_op_chain4": loop_op(),
         "loop_op_chain4_correlations": loop_op_correlations(),
         "loop_op_models": loop_op_models(),
+        "loop_op_sse": loop_op_sse(),
     }
     with open(os.path.join(OUT, "reference_fixtures.json"), "w") as f:
         json.dump(fx, f, indent=1, sort_keys=True)

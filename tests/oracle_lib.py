@@ -37,6 +37,7 @@ lib.or_uf_pairs.argtypes = [I, I, vp, I, vp, vp, vp]
 lib.or_looper_create.restype = vp
 lib.or_looper_create.argtypes = [I, I, D, D, D, D, U, D, I, D, D]
 lib.or_looper_sweep.argtypes = [vp, I, vp]
+lib.or_looper_sweep_sse.argtypes = [vp, I, vp]
 lib.or_looper_destroy.argtypes = [vp]
 lib.or_standalone_run.argtypes = [U, D, U, U, vp, vp]
 lib.or_bfs_labels.argtypes = [I, I, I, vp, vp, vp, vp]
@@ -46,6 +47,7 @@ lib.or_segment_of_elements.restype = I
 lib.or_philox_create.restype = vp
 lib.or_philox_create.argtypes = [I, I, D, D, D, D, D, U, U, I, D, D]
 lib.or_philox_sweep.argtypes = [vp, I, vp]
+lib.or_philox_sweep_sse.argtypes = [vp, I, vp]
 lib.or_philox_set_beta.argtypes = [vp, D]
 lib.or_philox_num_ops.argtypes = [vp]
 lib.or_philox_num_nodes.argtypes = [vp]
@@ -153,6 +155,12 @@ class LooperChain:
         lib.or_looper_sweep(self.h, n, _p(rec))
         return rec
 
+    def sweep_sse(self, n):
+        """n sweeps of the SSE worker (sse.C); tau-integrated sums in units of the string"""
+        rec = np.zeros((n, R))
+        lib.or_looper_sweep_sse(self.h, n, _p(rec))
+        return rec
+
     def set_schedule(self, steps, temps):
         st, tp = _schedule_arrays(steps, temps)
         lib.or_looper_set_schedule(self.h, len(st), _p(st), _p(tp))
@@ -177,6 +185,12 @@ class PhiloxChain:
     def sweep(self, n):
         rec = np.zeros((n, R))
         lib.or_philox_sweep(self.h, n, _p(rec))
+        return rec
+
+    def sweep_sse(self, n):
+        """n sweeps of the SSE worker in the device's formulation (times p / n)"""
+        rec = np.zeros((n, R))
+        lib.or_philox_sweep_sse(self.h, n, _p(rec))
         return rec
 
     def set_beta(self, beta):

@@ -59,3 +59,15 @@ def observables(rec, beta, V, dim):
         "Number of Clusters": rec[:, 1],
         "Stiffness": rec[:, 17] / (beta * dim),
     }
+
+
+# SSE (sse.C): the same record with the tau-integrated sums in units of the
+# string; the susceptibilities are beta (n S + S0) / (n + 1) / V with n the
+# operator count (susceptibility.h:213-251)
+def observables_sse(rec, beta, V, dim):
+    rec = np.asarray(rec)
+    out = observables(rec, beta, V, dim)
+    n = rec[:, 0]
+    out["Susceptibility"] = beta * (n * rec[:, 9] + rec[:, 5]) / (n + 1) / V
+    out["Staggered Susceptibility"] = beta * (n * rec[:, 16] + rec[:, 12]) / (n + 1) / V
+    return out
