@@ -1982,8 +1982,8 @@ __global__ void k_canon_label(DevBatch B, int chain, int nelem, const int* canon
 // between) or else regraphed by choose_diagonal (graph_impl.h:297-312); an
 // off-diagonal one by choose_offdiagonal.  Step k of the walk takes the words
 // Philox(sweep, 6, 0, k), computed 64 steps at a time, one per lane, and
-// handed round by lane shuffles; the spins and the per-bond counters live in
-// LDS.  The new string then becomes bond-major operators at times p / n, on
+// staged in LDS with the spins, the per-bond counters and sites and the
+// operators in flight, so that a step waits on no global memory.  The new string then becomes bond-major operators at times p / n, on
 // which the path-integral cluster stage runs unchanged (lengths in units of
 // the string), and the flipped types return to the string.
 // ---------------------------------------------------------------------------
@@ -1993,13 +1993,26 @@ __global__ void __launch_bounds__(SSE_THREADS) k_sse_diag(DevBatch B) {
   if (blockIdx.x == 0 && threadIdx.x == 0) scan_advance_epoch(B);  // for k_scan, which follows
   if (overflowed(B)) return;
   const int c = blockIdx.x;
-  const ChainDev& P = B.chains[c];
+  // (a copy: the string stores below could alias the descriptor, which
+  // would make every step reload its fields)
+  const ChainDev P = B.chains[c];
+  // The walk's working set in LDS, so that a step waits on no global memory:
+  // spins, per-bond counters, each bond's two sites, the next 64 old
+  // operators and the last (up to 64) emitted ones, stored 64 at a time
   __shared__ uint8_t spc[SSE_MAX_SITES];
   __shared__ int cnt[SSE_MAX_BONDS];
+  __shared__ uint32_t ends[SSE_MAX_BONDS];  // source | target << 16
+  __shared__ int ochunk[SSE_THREADS], obuf[SSE_THREADS];
+  __shared__ U4 xbuf[SSE_THREADS];          // the Philox words of the next 64 steps
   const int lane = threadIdx.x;
   const int Nb = P.Nb;
   for (int s = lane; s < P.N; s += SSE_THREADS) spc[s] = B.spins_old[P.old_site_base + s];
-  for (int b = lane; b < Nb; b += SSE_THREADS) cnt[b] = 0;
+  for (int b = lane; b < Nb; b += SSE_THREADS) {
+    int s0, s1;
+    bond_sites(P, b, s0, s1);
+    cnt[b] = 0;
+    ends[b] = (uint32_t)s0 | ((uint32_t)s1 << 16);
+  }
   __syncthreads();
   const uint32_t sweep = B.sweep[c];
   const int n_old = B.bo_old[P.old_bond_base + Nb] - B.bo_old[P.old_bond_base];
@@ -2008,20 +2021,13 @@ __global__ void __launch_bounds__(SSE_THREADS) k_sse_diag(DevBatch B) {
   const double bw = P.beta * ((double)Nb * P.wsum);
   int nop = n_old, out = 0, i = 0, step = 0;
   int xbase = -SSE_THREADS, obase = -SSE_THREADS;  // first step / old operator held by the lanes
-  U4 xr = {0, 0, 0, 0};
-  int ol = 0;
   bool try_gap = true, full = false;
   while (try_gap || i < n_old) {
     if (step - xbase >= SSE_THREADS) {
       xbase = step;
-      xr = philox(sweep, STAGE_SSE, 0u, (uint32_t)(step + lane), P.key0, P.key1);
+      xbuf[lane] = philox(sweep, STAGE_SSE, 0u, (uint32_t)(step + lane), P.key0, P.key1);
     }
-    const int j = step - xbase;
-    U4 x;
-    x.x = __shfl(xr.x, j);
-    x.y = __shfl(xr.y, j);
-    x.z = __shfl(xr.z, j);
-    x.w = __shfl(xr.w, j);
+    const U4 x = xbuf[step - xbase];  // (one wave: its LDS accesses complete in order)
     ++step;
     const double u = uniform53(x);
     int ne, b;
@@ -2030,21 +2036,20 @@ __global__ void __launch_bounds__(SSE_THREADS) k_sse_diag(DevBatch B) {
       if (!((double)(nop + 1) * u < bw)) continue;
       b = (int)(((uint64_t)x.z * (uint64_t)Nb) >> 32);
       const int g = choose_graph(x.w, P);
-      int s0, s1;
-      bond_sites(P, b, s0, s1);
-      if (((g & 1) ^ spc[s0] ^ spc[s1]) == 0) continue;
+      const uint32_t e2 = ends[b];
+      if (((g & 1) ^ spc[e2 & 0xffff] ^ spc[e2 >> 16]) == 0) continue;
       ne = (b << 3) | (g << 1);
       ++nop;
     } else {
-      if (i - obase >= SSE_THREADS) {
+      if (i - obase >= SSE_THREADS) {  // the next 64 old operators
         obase = i;
-        ol = (i + lane < n_old) ? so[i + lane] : 0;
+        ochunk[lane] = (i + lane < n_old) ? so[i + lane] : 0;
       }
-      const int e = __shfl(ol, i - obase);
+      const int e = ochunk[i - obase];
       ++i;
       b = e >> 3;
-      int s0, s1;
-      bond_sites(P, b, s0, s1);
+      const uint32_t e2 = ends[b];
+      const int s0 = (int)(e2 & 0xffff), s1 = (int)(e2 >> 16);
       const int c0 = spc[s0], c1 = spc[s1];
       if ((e & 1) == 0) {
         if (bw * u < (double)nop) {  // removed
@@ -2066,12 +2071,15 @@ __global__ void __launch_bounds__(SSE_THREADS) k_sse_diag(DevBatch B) {
       break;
     }
     if (lane == 0) {
-      sn[out] = ne;
-      cnt[b] += 1;
+      obuf[out & (SSE_THREADS - 1)] = ne;
+      atomicAdd(&cnt[b], 1);
     }
     ++out;
+    if ((out & (SSE_THREADS - 1)) == 0) sn[out - SSE_THREADS + lane] = obuf[lane];  // 64 emitted
     try_gap = true;
   }
+  const int rest = out & (SSE_THREADS - 1);
+  if (lane < rest) sn[out - rest + lane] = obuf[lane];
   if (full && lane == 0) atomicOr(B.flags, FLAG_OP_OVERFLOW);
   __syncthreads();
   for (int b = lane; b < Nb; b += SSE_THREADS) B.counts[P.bond_base + b] = cnt[b];
@@ -2082,7 +2090,7 @@ __global__ void __launch_bounds__(SSE_THREADS) k_sse_diag(DevBatch B) {
 __global__ void __launch_bounds__(SSE_THREADS) k_sse_scatter(DevBatch B) {
   if (overflowed(B)) return;
   const int c = blockIdx.x;
-  const ChainDev& P = B.chains[c];
+  const ChainDev P = B.chains[c];  // (a copy, as in k_sse_diag)
   __shared__ int cur[SSE_MAX_BONDS], koff[SSE_MAX_BONDS];
   const int lane = threadIdx.x, Nb = P.Nb;
   for (int b = lane; b < Nb; b += SSE_THREADS) {

@@ -421,6 +421,39 @@ def run_scan(args, lg):
     return out
 
 
+def run_sse_scan(args, lg):
+    """the 512-chain scan with the SSE worker ("loop; sse"): the same chains,
+    batched, HIP-graph replay, autotuned; updates/s over the timed sweeps"""
+    plist = [lg.params(lg.SQUARE, 32, 1.0 / T, 1.0, 1.0, seed=SEED, chain_id=i, algorithm=lg.SSE)
+             for i, T in enumerate(scan_temps())]
+    equil = args.equilibrate if args.equilibrate is not None else 400
+    b = lg.Batch(plist)
+    log("sse scan: equilibrate %d batched sweeps" % equil)
+    run_chunked(lambda k: b.run(k, records=False), equil, 200)
+    log("sse scan: tune")
+    b.tune(8)
+    b.set_graph(True)
+    if args.warmup:
+        b.run(args.warmup)
+    log("sse scan: timed %d batched sweeps" % args.steps)
+    t0 = time.perf_counter()
+    rec = b.run(args.steps)
+    dt = time.perf_counter() - t0
+    nop_total = float(rec[:, :, 0].sum())
+    prof = b.profile(args.profile_sweeps)
+    cfg = {"workload": "512-chain temperature scan with the SSE worker (loop; sse): S=1/2 Heisenberg AF on "
+                       "the 32x32 periodic square lattice, T_i=0.05*40^(i/511), chain i keyed (23125, i); batched, "
+                       "HIP-graph sweep, autotuned launch settings",
+           "lattice": "square lattice", "L": 32, "chains": len(plist), "algorithm": "loop; sse", "seed": SEED}
+    out = base_line(args, nop_total / dt, 1e3 * dt / args.steps, cfg)
+    E = rec[:, :, 2].mean(axis=0) / 1024.0
+    out.update({"equilibration_sweeps": equil, "n_op_per_batched_sweep": nop_total / args.steps,
+                "energy_density_coldest_hottest": [float(E[0]), float(E[-1])], "launch": b.launch(),
+                "kernel_ms": prof, "roofline": None, "cpu_baseline": None})
+    b.close()
+    return out
+
+
 # ---------------------------------------------------------------------------
 # serving mix
 # ---------------------------------------------------------------------------
@@ -563,7 +596,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true", help="default workload: skip the serving part")
-    ap.add_argument("--workload", choices=["c3", "scan", "serve", "c2", "c5", "chain128", "chain1024"], default="c3")
+    ap.add_argument("--workload", choices=["c3", "scan", "serve", "c2", "c5", "chain128", "chain1024", "sse"],
+                    default="c3")
     ap.add_argument("--serve-seconds", type=float, default=90.0, help="serve: seconds of Poisson arrivals")
     ap.add_argument("--serve-calib", type=float, default=15.0, help="serve: seconds of the calibration burst")
     ap.add_argument("--serve-burst", type=int, default=64, help="serve: jobs of the calibration burst")
@@ -603,6 +637,8 @@ def main():
         out = run_single(args, lg, lg.SQUARE, 128, 128.0, "c2_square128_hbm_traffic")
     elif args.workload == "c5":
         out = run_single(args, lg, lg.SQUARE, 512, 512.0, "c5_square512_hbm_traffic")
+    elif args.workload == "sse":
+        out = run_sse_scan(args, lg)
     elif args.workload == "chain128":
         out = run_single(args, lg, lg.CHAIN, 128, 100.0, None,
                          "chain L=128 periodic S=1/2 Heisenberg AF, J=1, T=0.01, one chain, plain GPU path "
