@@ -13,7 +13,8 @@
  * susceptibility.h:59-94, stiffness.h:78-80).
  *
  * Parameters are text in the reference's .ip syntax ("KEY = value;" or one
- * per line; expressions over other parameters allowed), keys: LATTICE
+ * per line; expressions over other parameters allowed), keys: ALGORITHM
+ * ("loop; path integral" (default) | "loop; sse"), LATTICE
  * ("chain lattice" | "square lattice"), L, W, T | TEMPERATURE | BETA |
  * INVERSE_TEMPERATURE, the annealing schedule T_START_i / T_DURATION_i (or
  * TEMPERATURE_START_i / TEMPERATURE_DURATION_i, BETA_START_i /
