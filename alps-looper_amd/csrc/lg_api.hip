@@ -1615,7 +1615,10 @@ int lg_chain_save(lg_chain* h, void* buf, size_t len) {
   put(&no, 8);
   for (long long i = 0; i < nops; ++i) {
     const int k = order[i];
-    const int32_t type = types[k], loc = (bond[k] << 1) | 1;
+    // looper's location (location_impl.h:37): pos << 1 | is_bond; a site
+    // operator (transverse field) sits at its site
+    const int32_t type = types[k];
+    const int32_t loc = bond[k] >= d.nb_real ? ((bond[k] - d.nb_real) << 1) : ((bond[k] << 1) | 1);
     put(&type, 4);
     put(&loc, 4);
     put(&times[k], 8);
@@ -1650,14 +1653,17 @@ int lg_chain_load(lg_chain* h, const void* buf, size_t len) {
     get(&type[i], 4);
     get(&loc, 4);
     get(&time[i], 8);
-    bond[i] = loc >> 1;
-    if ((loc & 1) != 1 || bond[i] < 0 || bond[i] >= d.Nb) return set_error(LG_ERR_ARG, "checkpoint operator location invalid");
+    const int pos = loc >> 1;
+    const bool ok = (loc & 1) ? (pos >= 0 && pos < d.nb_real) : (d.Nb > d.nb_real && pos >= 0 && pos < d.N);
+    if (!ok) return set_error(LG_ERR_ARG, "checkpoint operator location invalid");
+    bond[i] = (loc & 1) ? pos : d.nb_real + pos;
   }
   lg_params q;
   if (!get(&q, sizeof(lg_params))) return set_error(LG_ERR_ARG, "checkpoint truncated");
   const lg_params& p = h->b.params[0];
   if (q.lattice != p.lattice || q.L != p.L || q.beta != p.beta || q.jxy != p.jxy || q.jz != p.jz ||
-      q.force_scatter != p.force_scatter || q.seed != p.seed || q.chain_id != p.chain_id || q.h != p.h)
+      q.force_scatter != p.force_scatter || q.seed != p.seed || q.chain_id != p.chain_id || q.h != p.h ||
+      q.two_s != p.two_s || q.hx != p.hx || q.d != p.d || q.algorithm != p.algorithm)
     return set_error(LG_ERR_ARG, "checkpoint of a different chain (parameters, seed or chain id)");
   // bond-major order for the device: a stable sort by bond keeps each bond's
   // operators in their imaginary-time order

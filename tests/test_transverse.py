@@ -231,3 +231,34 @@ def test_worker_transverse_ising_matches_ed(loopgpu):
                  "Staggered Susceptibility"):
         m, e, _ = w.result(name)
         assert stats.sigmas(m, e, ref[name]) < 4.0, (name, m, e, ref[name])
+
+
+@pytest.mark.gpu
+def test_transverse_checkpoint_locations_and_resume(loopgpu):
+    """a chain with site operators and in-site bonds saved mid-run: site
+    operators sit at their site in the blob (looper's location, bit 0
+    clear), bonds at bond << 1 | 1, and the loaded chain continues bit for
+    bit"""
+    p = loopgpu.params(O.CHAIN, 8, 3.0, -1.0, 1.0, seed=12, chain_id=3, two_s=2, hx=0.6, d=0.4,
+                       force_scatter=0.1)
+    a = loopgpu.Chain(p)
+    a.sweep(20)
+    spins, bo, times, types = a.get_config()
+    n_sites, n_bonds = loopgpu.lattice_size(O.CHAIN, 8, 2, 0.6, 0.4)
+    nb_real = n_bonds - n_sites
+    n_site_ops = int(bo[-1] - bo[nb_real])
+    assert n_site_ops > 0 and bo[nb_real] > 0
+    blob = a.save()
+    print("transverse checkpoint", flush=True)
+    off = 8 + 8 + 8 + 4 * n_sites
+    nops = int(np.frombuffer(blob[off:off + 8], np.uint64)[0])
+    rec = np.frombuffer(blob[off + 8:off + 8 + 16 * nops], dtype=[("type", "<i4"), ("loc", "<i4"), ("time", "<f8")])
+    assert nops == int(bo[-1])
+    site = (rec["loc"] & 1) == 0
+    assert int(site.sum()) == n_site_ops
+    assert (rec["loc"][site] >> 1).max() < n_sites and (rec["loc"][~site] >> 1).max() < nb_real
+    assert (np.diff(rec["time"]) >= 0).all()
+    ra = a.sweep(15)
+    b = loopgpu.Chain(p)
+    b.load(blob)
+    _cmp(b.sweep(15), ra, "resumed")
