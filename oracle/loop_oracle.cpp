@@ -760,10 +760,11 @@ struct LooperChain {
   // graph_impl.h:297-312); an off-diagonal one is regraphed
   // (choose_offdiagonal).  Times are positions in the new string (start at
   // 0, stop at n).  Lattice bonds only (no site operators, no single-ion
-  // term, no field).  The record's tau-integrated sums (usize, umag, ssize,
-  // smag) are stored divided by n^2, i.e. with lengths in units of the
-  // string (n > 0), so that the susceptibility is
-  // beta (n umag + umag2) / (n + 1) / V as in susceptibility.h:213-251.
+  // term, no field).  The record's tau-integrated sums are stored with
+  // lengths in units of the string (n > 0): the squared ones (usize, umag,
+  // ssize, smag) divided by n^2, the normal estimator's (umag_a, smag_a) by
+  // n, so that the susceptibility is beta (n umag + umag2) / (n + 1) / V as
+  // in susceptibility.h:213-251.
   void sweep_sse(double* rec) {
     const int N = lat.N;
     if (sched.on()) beta = 1.0 / sched.T((int)mcs);
@@ -862,6 +863,7 @@ struct LooperChain {
     if (n > 0) {
       const double n2 = (double)n * n;
       rec[R_USIZE] /= n2; rec[R_UMAG] /= n2; rec[R_SSIZE] /= n2; rec[R_SMAG] /= n2;
+      rec[R_UMAGA] /= n; rec[R_SMAGA] /= n;  // the normal estimator's sums
     }
     rec[R_NOP] = n;
     rec[R_NC] = nc;
@@ -1251,7 +1253,7 @@ struct PhiloxChain {
   }
 
   void sweep_once(double* rec) {
-    const int N = lat.N, Nb = lat.Nb;
+    const int Nb = lat.Nb;
     if (sched.on()) {
       beta = 1.0 / sched.T((int)sweep);
       scale = (W > 0) ? std::ldexp(1.0, 52) / (beta * W) : 0;
@@ -1326,7 +1328,7 @@ struct PhiloxChain {
   uint64_t pdiag[2] = {0, 0};             // choose_diagonal thresholds: antiparallel, parallel
 
   void sweep_sse(double* rec) {
-    const int N = lat.N, Nb = lat.Nb;
+    const int Nb = lat.Nb;
     if (sched.on()) beta = 1.0 / sched.T((int)sweep);
     const double bw = beta * ((double)Nb * W);
     std::vector<int> spc(spins), nb, nt;

@@ -66,6 +66,19 @@ def test_oracle_sse_matches_reference_sse_blocks(key, two_s, T, jxy, jz):
         assert stats.sigmas(m, e, qmc[k][0], qmc[k][1]) < 4.0, (k, m, e, qmc[k])
 
 
+def test_oracle_sse_normal_estimator_matches_ed():
+    """the normal estimator's SSE form beta (n M_a^2 + M_0^2) / (n + 1) / V
+    (susceptibility.h:326-333) from both oracle modes"""
+    ref = ed.diagonalize_transverse("chain", 6, 0.5, 1)
+    for ch in (O.LooperChain(O.CHAIN, 6, 2.0, 1.0, 1.0, seed=4), O.PhiloxChain(O.CHAIN, 6, 2.0, 1.0, 1.0, seed=4,
+                                                                               chain_id=1)):
+        ch.sweep_sse(2000)
+        r = ch.sweep_sse(40000)
+        n = r[:, 0]
+        m, e = stats.mean_err(2.0 * (n * r[:, 20] ** 2 + r[:, 3] ** 2) / (n + 1) / 6)
+        assert stats.sigmas(m, e, ref["Susceptibility"]) < 4.0, (m, e, ref["Susceptibility"])
+
+
 def test_sse_parameter_checks():
     import loopgpu
     for kw, frag in ((dict(h=0.2), "lattice bonds only"), (dict(hx=0.3, jxy=-1.0), "lattice bonds only"),
@@ -179,3 +192,16 @@ def test_worker_sse_matches_reference_run(loopgpu, key, local_s, T, jxy):
         assert stats.sigmas(m, e, ref[name]) < 4.0, (name, m, e, ref[name])
         if name in qmc:
             assert stats.sigmas(m, e, qmc[name][0], qmc[name][1]) < 4.0, (name, m, e, qmc[name])
+
+
+@pytest.mark.gpu
+def test_worker_sse_normal_estimator_matches_ed(loopgpu):
+    ref = ed.diagonalize_transverse("chain", 6, 0.5, 1)
+    w = loopgpu.Worker('ALGORITHM = "loop; sse"\nL = 6\nT = 0.5\nDISABLE_IMPROVED_ESTIMATOR = 1\n'
+                       'SWEEPS = 65536\nSEED = 5')
+    print("worker sse normal estimator", flush=True)
+    w.run()
+    for name in ("Energy", "Magnetization^2", "Susceptibility", "Staggered Magnetization^2",
+                 "Staggered Susceptibility"):
+        m, e, _ = w.result(name)
+        assert stats.sigmas(m, e, ref[name]) < 4.0, (name, m, e, ref[name])
