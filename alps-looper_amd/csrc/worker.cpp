@@ -321,6 +321,8 @@ class loop_worker {
     if (mc || ms || !custom.empty()) {
       if (lp_.two_s > 1) throw std::invalid_argument("correlation measurements are built for S=1/2 only");
       if (lp_.hx != 0) throw std::invalid_argument("correlation measurements are built without a transverse field");
+      if (lp_.algorithm != LG_ALGORITHM_PATH_INTEGRAL)
+        throw std::invalid_argument("correlation measurements are built for the path-integral worker");
       if (lp_.L < 2 || lp_.L % 2) throw std::invalid_argument("L must be even");
       const int origin = p.defined("INITIAL_SITE") ? (int)std::llround(p.num("INITIAL_SITE")) : -1;
       if (p.defined("INITIAL_SITE") && (origin < 0 || origin >= n_sites_))

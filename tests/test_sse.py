@@ -98,7 +98,8 @@ def test_sse_parameter_checks():
         loopgpu.Chain(loopgpu.params(loopgpu.CHAIN, 8, 1.0, algorithm=2))
     assert e.value.code == -1
     for txt, frag in (('ALGORITHM = "loop; sse"\nL = 8\nT = 1\nh = 0.1', "lattice bonds only"),
-                      ('ALGORITHM = "loop; worm"\nL = 8\nT = 1', "unsupported ALGORITHM")):
+                      ('ALGORITHM = "loop; worm"\nL = 8\nT = 1', "unsupported ALGORITHM"),
+                      ('ALGORITHM = "loop; sse"\nL = 8\nT = 1\nMEASURE[Correlations] = true', "path-integral worker")):
         with pytest.raises(loopgpu.LoopGPUError) as e:
             loopgpu.Worker(txt)
         assert frag in str(e.value)
