@@ -78,11 +78,13 @@ int check_params(const lg_params& p) {
   if (p.algorithm != LG_ALGORITHM_PATH_INTEGRAL && p.algorithm != LG_ALGORITHM_SSE)
     return set_error(LG_ERR_ARG, "algorithm must be 0 (loop; path integral) or 1 (loop; sse)");
   if (p.algorithm == LG_ALGORITHM_SSE) {
-    if (p.hx != 0 || p.d != 0 || p.h != 0)
-      return set_error(LG_ERR_ARG, "the SSE worker is built for lattice bonds only (h = hx = d = 0)");
+    if (p.h != 0)
+      return set_error(LG_ERR_ARG, "the SSE worker is built without a longitudinal field (h = 0)");
     const long long nv = p.two_s >= 2 ? p.two_s : 1;
     const long long nr = p.lattice == LG_LATTICE_CHAIN ? p.L : (long long)p.L * p.L;
-    const long long nb = nv * nv * (p.lattice == LG_LATTICE_CHAIN ? p.L : 2ll * p.L * p.L);
+    long long nb = nv * nv * (p.lattice == LG_LATTICE_CHAIN ? p.L : 2ll * p.L * p.L);
+    if (p.d != 0) nb += nr * nv * (nv - 1) / 2;  // in-site bonds
+    if (p.hx != 0) nb += nv * nr;                // site bonds
     if (nv * nr > SSE_MAX_SITES || nb > SSE_MAX_BONDS)
       return set_error(LG_ERR_ARG, "the SSE worker takes at most " + std::to_string(SSE_MAX_SITES) +
                                        " (virtual) sites and " + std::to_string(SSE_MAX_BONDS) + " bonds");
@@ -111,6 +113,42 @@ static void graph_thresholds(const double* v, uint64_t* thr, uint64_t* toff) {
   }
   const double s01 = v[0] + v[1];
   *toff = (s01 > 0) ? (uint64_t)std::floor(std::ldexp(v[0] / s01, 32)) : (1ull << 32);
+}
+
+// SSE insertion thresholds (ChainDev::sse_cls, sse_tl, sse_td) and the
+// in-site bonds' choose_diagonal thresholds; the oracle's philox mode
+// evaluates the same expressions
+static void sse_selection(ChainDev& d, const double* v, double dw, double dion, double fs) {
+  const double wl = d.nb_lat * (v[0] + v[1] + v[2] + v[3]);
+  const double wd = (d.nb_real - d.nb_lat) * dw, ws = (d.Nb - d.nb_real) * d.site_w;
+  const double wt = wl + wd + ws;
+  d.sse_wtot = wt;
+  auto t32 = [&](double x) -> uint64_t {
+    const double y = std::floor(std::ldexp(x / wt, 32));
+    return y >= 4294967296.0 ? (1ull << 32) : (uint64_t)y;
+  };
+  d.sse_cls[0] = (wd + ws > 0) ? t32(wl) : (1ull << 32);
+  d.sse_cls[1] = (ws > 0) ? t32(wl + wd) : (1ull << 32);
+  int last = -1;
+  for (int g = 0; g < 4; ++g) if (v[g] > 0) last = g;
+  double cum = 0;
+  for (int g = 0; g < 3; ++g) {
+    cum += v[g];
+    d.sse_tl[g] = (last < 0 || g >= last) ? d.sse_cls[0] : t32(d.nb_lat * cum);
+  }
+  double vd[4] = {0, 0, 0, 0}, offd;
+  if (dion != 0) xxz_weights(0.0, 2 * dion, fs, vd, &offd);
+  last = -1;
+  for (int g = 0; g < 4; ++g) if (vd[g] > 0) last = g;
+  cum = 0;
+  for (int g = 0; g < 3; ++g) {
+    cum += vd[g];
+    d.sse_td[g] = (last < 0 || g >= last) ? d.sse_cls[1] : t32(wl + (d.nb_real - d.nb_lat) * cum);
+  }
+  for (int c = 0; c < 2; ++c) {
+    const double den = vd[c] + vd[c + 2];
+    d.dpdiag[c] = (den > 0) ? (uint64_t)std::floor(std::ldexp(vd[c] / den, 32)) : (1ull << 32);
+  }
 }
 
 ChainDev make_chain_dev(const lg_params& p) {
@@ -172,6 +210,7 @@ ChainDev make_chain_dev(const lg_params& p) {
   }
   double lam = p.beta * std::max(std::max(W, dw), d.site_w);
   d.cand_cap = (int)std::ceil(lam + 12.0 * std::sqrt(lam) + 16.0);
+  sse_selection(d, v, dw, d.npair > 0 ? p.d : 0.0, p.force_scatter);
   return d;
 }
 

@@ -86,6 +86,13 @@ struct ChainDev {
   // per operator (chain-local bond << 3 | graph << 1 | off-diagonal)
   int sse, sse_base, sse_cap;
   uint64_t pdiag[2];                 // choose_diagonal thresholds (antiparallel, parallel pair)
+  uint64_t dpdiag[2];                // the same for the in-site bonds
+  // SSE insertions: the location's class from word 2 of the step, lattice
+  // bonds below sse_cls[0], in-site bonds below sse_cls[1], site bonds
+  // above; the graph by word 2 against sse_tl / sse_td (thresholds scaled
+  // into the class's range), the location within the class by word 3
+  uint64_t sse_cls[2], sse_tl[3], sse_td[3];
+  double sse_wtot;                   // sum of the weights of every (location, graph)
 };
 // SSE chains: their sites' spins and bonds' counters are staged in LDS
 constexpr int SSE_MAX_SITES = 4096, SSE_MAX_BONDS = 4096;

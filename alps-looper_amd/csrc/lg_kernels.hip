@@ -1980,7 +1980,10 @@ __global__ void k_canon_label(DevBatch B, int chain, int nelem, const int* canon
 // graph from word 3 by the thresholds, kept if compatible); a diagonal
 // operator is removed when beta W_tot u < n (the next one follows with no gap
 // between) or else regraphed by choose_diagonal (graph_impl.h:297-312); an
-// off-diagonal one by choose_offdiagonal.  Step k of the walk takes the words
+// off-diagonal one by choose_offdiagonal.  W_tot sums every location's
+// graphs: lattice bonds, the single-ion term's in-site bonds and the
+// transverse field's site operators (the site graph, always compatible; an
+// off-diagonal site operator flips its one spin).  Step k of the walk takes the words
 // Philox(sweep, 6, 0, k), computed 64 steps at a time, one per lane, and
 // staged in LDS with the spins, the per-bond counters and sites and the
 // operators in flight, so that a step waits on no global memory.  The new string then becomes bond-major operators at times p / n, on
@@ -2009,7 +2012,7 @@ __global__ void __launch_bounds__(SSE_THREADS) k_sse_diag(DevBatch B) {
   for (int s = lane; s < P.N; s += SSE_THREADS) spc[s] = B.spins_old[P.old_site_base + s];
   for (int b = lane; b < Nb; b += SSE_THREADS) {
     int s0, s1;
-    bond_sites(P, b, s0, s1);
+    op_sites(P, b, s0, s1);
     cnt[b] = 0;
     ends[b] = (uint32_t)s0 | ((uint32_t)s1 << 16);
   }
@@ -2018,7 +2021,7 @@ __global__ void __launch_bounds__(SSE_THREADS) k_sse_diag(DevBatch B) {
   const int n_old = B.bo_old[P.old_bond_base + Nb] - B.bo_old[P.old_bond_base];
   const int* so = B.sse_old + P.sse_base;
   int* sn = B.sse_new + P.sse_base;
-  const double bw = P.beta * ((double)Nb * P.wsum);
+  const double bw = P.beta * P.sse_wtot;
   int nop = n_old, out = 0, i = 0, step = 0;
   int xbase = -SSE_THREADS, obase = -SSE_THREADS;  // first step / old operator held by the lanes
   bool try_gap = true, full = false;
@@ -2034,10 +2037,20 @@ __global__ void __launch_bounds__(SSE_THREADS) k_sse_diag(DevBatch B) {
     if (try_gap) {
       try_gap = false;
       if (!((double)(nop + 1) * u < bw)) continue;
-      b = (int)(((uint64_t)x.z * (uint64_t)Nb) >> 32);
-      const int g = choose_graph(x.w, P);
+      // the location's class and graph by word 2, the location by word 3
+      const uint64_t z = x.z;
+      int g = 0;
+      if (z < P.sse_cls[0]) {
+        b = (int)(((uint64_t)x.w * (uint64_t)P.nb_lat) >> 32);
+        while (g < 3 && z >= P.sse_tl[g]) ++g;
+      } else if (z < P.sse_cls[1]) {
+        b = P.nb_lat + (int)(((uint64_t)x.w * (uint64_t)(P.nb_real - P.nb_lat)) >> 32);
+        while (g < 3 && z >= P.sse_td[g]) ++g;
+      } else {  // a site operator: the site graph, compatible with either spin
+        b = P.nb_real + (int)(((uint64_t)x.w * (uint64_t)P.N) >> 32);
+      }
       const uint32_t e2 = ends[b];
-      if (((g & 1) ^ spc[e2 & 0xffff] ^ spc[e2 >> 16]) == 0) continue;
+      if (b < P.nb_real && ((g & 1) ^ spc[e2 & 0xffff] ^ spc[e2 >> 16]) == 0) continue;
       ne = (b << 3) | (g << 1);
       ++nop;
     } else {
@@ -2051,18 +2064,21 @@ __global__ void __launch_bounds__(SSE_THREADS) k_sse_diag(DevBatch B) {
       const uint32_t e2 = ends[b];
       const int s0 = (int)(e2 & 0xffff), s1 = (int)(e2 >> 16);
       const int c0 = spc[s0], c1 = spc[s1];
+      const bool site = b >= P.nb_real, insite = b >= P.nb_lat && !site;
       if ((e & 1) == 0) {
         if (bw * u < (double)nop) {  // removed
           --nop;
           continue;
         }
         const int cc = 1 ^ c0 ^ c1;  // 0 antiparallel, 1 parallel
-        ne = (b << 3) | (((((uint64_t)x.z < P.pdiag[cc]) ? 0 : 2) ^ cc) << 1);
+        const uint64_t pd = insite ? P.dpdiag[cc] : P.pdiag[cc];
+        ne = (b << 3) | (site ? 0 : (((((uint64_t)x.z < pd) ? 0 : 2) ^ cc) << 1));
       } else {
-        ne = (b << 3) | ((((uint64_t)x.z < P.toff) ? 0 : 1) << 1) | 1;
+        const uint64_t to = insite ? P.dtoff : P.toff;
+        ne = (b << 3) | (site ? 0 : ((((uint64_t)x.z < to) ? 0 : 1) << 1)) | 1;
         if (lane == 0) {
           spc[s0] = (uint8_t)(c0 ^ 1);
-          spc[s1] = (uint8_t)(c1 ^ 1);
+          if (!site) spc[s1] = (uint8_t)(c1 ^ 1);  // a site operator flips its one spin
         }
       }
     }

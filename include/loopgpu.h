@@ -76,9 +76,10 @@ typedef struct lg_params {
   double d;              /* single-ion term D (S^z)^2 (model_parameter.h:69); for S > 1/2 it adds
                             in-site bonds; looper's worker scatters (FORCE_SCATTER 0.1) when D > 0 */
   int32_t algorithm;     /* LG_ALGORITHM_PATH_INTEGRAL ("loop; path integral") or LG_ALGORITHM_SSE
-                            ("loop; sse", sse.C): the stochastic series expansion, lattice bonds only
-                            (h = hx = d = 0), at most 4096 (virtual) sites and bonds; a batch holds
-                            chains of one algorithm */
+                            ("loop; sse", sse.C): the stochastic series expansion, without a
+                            longitudinal field (h = 0), at most 4096 (virtual) sites and 4096 bonds
+                            (in-site and site bonds included); a batch holds chains of one
+                            algorithm */
 } lg_params;
 
 enum { LG_ALGORITHM_PATH_INTEGRAL = 0, LG_ALGORITHM_SSE = 1 };

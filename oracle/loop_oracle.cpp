@@ -779,6 +779,8 @@ struct LooperChain {
     const double bw = beta * graph_weight;
     const double pd[2] = {(w.v[0] + w.v[2] > 1e-10) ? w.v[0] / (w.v[0] + w.v[2]) : 1,
                           (w.v[1] + w.v[3] > 1e-10) ? w.v[1] / (w.v[1] + w.v[3]) : 1};
+    const double pdd[2] = {(wd.v[0] + wd.v[2] > 1e-10) ? wd.v[0] / (wd.v[0] + wd.v[2]) : 1,
+                           (wd.v[1] + wd.v[3] > 1e-10) ? wd.v[1] / (wd.v[1] + wd.v[3]) : 1};
     bool try_gap = true;
     size_t opi = 0;
     while (try_gap || opi < ops_p.size()) {
@@ -786,7 +788,7 @@ struct LooperChain {
         if ((nop + 1) * rnd() < bw) {
           auto bg = graph_of[alias(rnd)];
           const int b = bg.first, g = bg.second;
-          if (is_compatible(g, spins_c[lat.src[b]], spins_c[lat.tgt[b]])) {
+          if (lat.is_site_bond(b) || is_compatible(g, spins_c[lat.src[b]], spins_c[lat.tgt[b]])) {
             ops.push_back(LooperOp{b, g << 2, 0.0, 0, 0});
             ++nop;
           } else {
@@ -800,6 +802,7 @@ struct LooperChain {
       } else {
         LooperOp o = ops_p[opi];
         const int c0 = spins_c[lat.src[o.bond]], c1 = spins_c[lat.tgt[o.bond]];
+        const bool site = lat.is_site_bond(o.bond), insite = lat.is_d_bond(o.bond);
         if ((o.type & 1) == 0) {
           if (bw * rnd() < nop) {
             --nop;
@@ -807,9 +810,10 @@ struct LooperChain {
             continue;
           }
           const int c = 1 ^ c0 ^ c1;  // 0 antiparallel, 1 parallel
-          o.type = ((rnd() < pd[c] ? 0 : 2) ^ c) << 2;
-        } else {
-          o.type = 1 | ((rnd() < p_off ? 0 : 1) << 2);
+          const double p = insite ? pdd[c] : pd[c];
+          o.type = site ? 0 : (((rnd() < p ? 0 : 2) ^ c) << 2);  // the site graph stays
+        } else if (!site) {
+          o.type = 1 | ((rnd() < (insite ? p_off_d : p_off) ? 0 : 1) << 2);
         }
         ops.push_back(o);
         ++opi;
@@ -817,8 +821,17 @@ struct LooperChain {
       }
       LooperOp& oi = ops.back();
       const int b = oi.bond, s0 = lat.src[b], s1 = lat.tgt[b];
-      if (oi.type & 1) { spins_c[s0] ^= 1; spins_c[s1] ^= 1; }
       if ((int)frag.size() < fid + 1) frag.resize(fid + 1);
+      if (lat.is_site_bond(b)) {  // the site graph: the world line is cut (graph_impl.h:78-84)
+        if (oi.type & 1) spins_c[s0] ^= 1;
+        frag[fid] = UFNode();
+        oi.loop0 = current[s0];
+        oi.loop1 = current[s0] = fid;
+        ++fid;
+        try_gap = true;
+        continue;
+      }
+      if (oi.type & 1) { spins_c[s0] ^= 1; spins_c[s1] ^= 1; }
       Reconnect r = reconnect(frag, fid, oi.type >> 2, current[s0], current[s1]);
       fid = r.fid; current[s0] = r.curr0; current[s1] = r.curr1;
       oi.loop0 = r.loop0; oi.loop1 = r.loop1;
@@ -842,6 +855,13 @@ struct LooperChain {
     for (int s = 0; s < N; ++s) est_start_bottom(est[frag[s].id_], lat.gauge[s], spins_c[s]);
     for (auto& op : ops) {
       const int g = op.type >> 2;
+      if (lat.is_site_bond(op.bond)) {  // end_s / begin_s, no bond vector
+        const int s = lat.src[op.bond];
+        est_end(est[frag[op.loop0].id_], op.time, lat.gauge[s], spins_c[s]);
+        if (op.type & 1) spins_c[s] ^= 1;
+        est_begin(est[frag[op.loop1].id_], op.time, lat.gauge[s], spins_c[s]);
+        continue;
+      }
       if (is_frozen(g)) continue;
       const int b = op.bond, s0 = lat.src[b], s1 = lat.tgt[b];
       const double tt = op.time;
@@ -1201,6 +1221,7 @@ struct PhiloxChain {
       site_scale = std::ldexp(1.0, 52) / (beta * vs);
       offset_total += lat.N * vs;
     }
+    sse_selection(lat.nb_real > lat.nb_lat ? d : 0.0, fs);
     key0 = seed; key1 = chain;
     spins.assign(lat.N, 0);
     boff.assign(lat.Nb + 1, 0);
@@ -1326,11 +1347,51 @@ struct PhiloxChain {
   // to the string.  Lattice bonds only.
   std::vector<int> sbond, stype;          // SSE operator string
   uint64_t pdiag[2] = {0, 0};             // choose_diagonal thresholds: antiparallel, parallel
+  uint64_t dpdiag[2] = {0, 0};            // the same for the in-site bonds
+  // insertions: the location's class and graph from word 2 (lattice bonds
+  // below sse_cls[0], in-site bonds below sse_cls[1], site bonds above; the
+  // graph against thresholds scaled into the class's range), the location
+  // within the class from word 3
+  uint64_t sse_cls[2] = {0, 0}, sse_tl[3] = {0, 0, 0}, sse_td[3] = {0, 0, 0};
+  double sse_wtot = 0;
+
+  void sse_selection(double dion, double fs) {
+    const double wl = lat.nb_lat * W;
+    const double wd = (lat.nb_real - lat.nb_lat) * Wd, ws = (lat.Nb - lat.nb_real) * vs;
+    const double wt = wl + wd + ws;
+    sse_wtot = wt;
+    auto t32 = [&](double x) -> uint64_t {
+      const double y = std::floor(std::ldexp(x / wt, 32));
+      return y >= 4294967296.0 ? (1ull << 32) : (uint64_t)y;
+    };
+    sse_cls[0] = (wd + ws > 0) ? t32(wl) : (1ull << 32);
+    sse_cls[1] = (ws > 0) ? t32(wl + wd) : (1ull << 32);
+    int last = -1;
+    for (int g = 0; g < 4; ++g) if (w.v[g] > 0) last = g;
+    double cum = 0;
+    for (int g = 0; g < 3; ++g) {
+      cum += w.v[g];
+      sse_tl[g] = (last < 0 || g >= last) ? sse_cls[0] : t32(lat.nb_lat * cum);
+    }
+    Weights vd = {{0, 0, 0, 0}, 0, 1};
+    if (dion != 0) vd = xxz_weights(0.0, 2 * dion, fs);
+    last = -1;
+    for (int g = 0; g < 4; ++g) if (vd.v[g] > 0) last = g;
+    cum = 0;
+    for (int g = 0; g < 3; ++g) {
+      cum += vd.v[g];
+      sse_td[g] = (last < 0 || g >= last) ? sse_cls[1] : t32(wl + (lat.nb_real - lat.nb_lat) * cum);
+    }
+    for (int c = 0; c < 2; ++c) {
+      const double den = vd.v[c] + vd.v[c + 2];
+      dpdiag[c] = (den > 0) ? (uint64_t)std::floor(std::ldexp(vd.v[c] / den, 32)) : (1ull << 32);
+    }
+  }
 
   void sweep_sse(double* rec) {
     const int Nb = lat.Nb;
     if (sched.on()) beta = 1.0 / sched.T((int)sweep);
-    const double bw = beta * ((double)Nb * W);
+    const double bw = beta * sse_wtot;
     std::vector<int> spc(spins), nb, nt;
     int nop = (int)sbond.size();
     uint32_t step = 0;
@@ -1343,9 +1404,18 @@ struct PhiloxChain {
       if (try_gap) {
         try_gap = false;
         if ((double)(nop + 1) * u < bw) {
-          const int b = (int)(((uint64_t)x.v[2] * (uint64_t)Nb) >> 32);
-          const int g = choose_graph(x.v[3], th);
-          if (is_compatible(g, spc[lat.src[b]], spc[lat.tgt[b]])) {
+          const uint64_t z = x.v[2];
+          int b, g = 0;
+          if (z < sse_cls[0]) {
+            b = (int)(((uint64_t)x.v[3] * (uint64_t)lat.nb_lat) >> 32);
+            while (g < 3 && z >= sse_tl[g]) ++g;
+          } else if (z < sse_cls[1]) {
+            b = lat.nb_lat + (int)(((uint64_t)x.v[3] * (uint64_t)(lat.nb_real - lat.nb_lat)) >> 32);
+            while (g < 3 && z >= sse_td[g]) ++g;
+          } else {
+            b = lat.nb_real + (int)(((uint64_t)x.v[3] * (uint64_t)lat.N) >> 32);
+          }
+          if (lat.is_site_bond(b) || is_compatible(g, spc[lat.src[b]], spc[lat.tgt[b]])) {
             nb.push_back(b); nt.push_back(g << 2);
             ++nop;
             try_gap = true;
@@ -1354,14 +1424,18 @@ struct PhiloxChain {
         continue;
       }
       const int b = sbond[i], t = stype[i], s0 = lat.src[b], s1 = lat.tgt[b];
+      const bool site = lat.is_site_bond(b), insite = lat.is_d_bond(b);
       ++i;
       if ((t & 1) == 0) {
         if (bw * u < (double)nop) { --nop; continue; }
         const int c = 1 ^ spc[s0] ^ spc[s1];
-        nb.push_back(b); nt.push_back(((((uint64_t)x.v[2] < pdiag[c]) ? 0 : 2) ^ c) << 2);
+        const uint64_t pd = insite ? dpdiag[c] : pdiag[c];
+        nb.push_back(b); nt.push_back(site ? 0 : (((((uint64_t)x.v[2] < pd) ? 0 : 2) ^ c) << 2));
       } else {
-        nb.push_back(b); nt.push_back(1 | ((((uint64_t)x.v[2] < th.toff) ? 0 : 1) << 2));
-        spc[s0] ^= 1; spc[s1] ^= 1;
+        const uint64_t to = insite ? thd.toff : th.toff;
+        nb.push_back(b); nt.push_back(1 | (site ? 0 : ((((uint64_t)x.v[2] < to) ? 0 : 1) << 2)));
+        spc[s0] ^= 1;
+        if (!site) spc[s1] ^= 1;  // a site operator flips its one spin
       }
       try_gap = true;
     }

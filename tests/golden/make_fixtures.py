@@ -21,7 +21,7 @@ Sources (data files of the reference's tests):
                                 path-integral blocks of its other models (S=1 chain L=4, one S=1
                                 site in a transverse field Gamma=1 or a field h=0.3, Ising chain,
                                 S=1 Ising chain with a single-ion term D=-0.2); its "loop; sse"
-                                blocks of the S=1/2, S=1 and Ising chains
+                                blocks of the same models
 """
 
 import json
@@ -181,18 +181,24 @@ def loop_op_models():
 
 
 def loop_op_sse():
-    """the "loop; sse" blocks of loop.op for the models the SSE worker here
-    takes (lattice bonds only), keyed by model: S=1/2 chain L=4 (T=0.25),
-    S=1 chain L=4 (T=1/L), S=1/2 Ising chain L=4 (Jxy=0, Jz=1, T=0.5)"""
+    """the "loop; sse" blocks of loop.op, keyed by model: S=1/2 chain L=4
+    (T=0.25), S=1 chain L=4 (T=1/L), S=1/2 Ising chain L=4 (Jxy=0, Jz=1,
+    T=0.5), one S=1 site in Gamma=1 (T=1/L), S=1 Ising chain with D=-0.2
+    (Jz=-1, T=2)"""
     out = {}
     for b in loop_op_blocks():
         p = b["params"]
-        if p.get("ALGORITHM") != "loop; sse" or p.get("LATTICE") != "chain lattice" or "D" in p:
+        if p.get("ALGORITHM") != "loop; sse":
             continue
-        key = ("chain4_spin1" if p.get("local_S") == "1" else
-               "chain4_ising" if p.get("Jxy") == "0" else "chain4")
+        if p.get("LATTICE") == "site":
+            key = "site_spin1_gamma1"
+        elif "D" in p:
+            key = "chain4_spin1_d"
+        else:
+            key = ("chain4_spin1" if p.get("local_S") == "1" else
+                   "chain4_ising" if p.get("Jxy") == "0" else "chain4")
         out[key] = keep_blocks([b])[0]
-    assert sorted(out) == ["chain4", "chain4_ising", "chain4_spin1"]
+    assert sorted(out) == ["chain4", "chain4_ising", "chain4_spin1", "chain4_spin1_d", "site_spin1_gamma1"]
     return out
 
 

@@ -66,6 +66,48 @@ def test_oracle_sse_matches_reference_sse_blocks(key, two_s, T, jxy, jz):
         assert stats.sigmas(m, e, qmc[k][0], qmc[k][1]) < 4.0, (k, m, e, qmc[k])
 
 
+# L, 2S, T, Jxy, Jz, hx, D, FORCE_SCATTER
+EXTRAS = [(4, 1, 0.5, 0.0, 1.0, 0.8, 0.0, 0.0), (4, 1, 0.5, -1.0, 1.0, 0.8, 0.0, 0.0),
+          (4, 2, 2.0, 0.0, -1.0, 0.0, -0.2, 0.0), (4, 2, 0.6, -1.0, 1.0, 0.7, 0.5, 0.1),
+          (4, 3, 0.8, 1.0, 1.0, 0.0, -0.4, 0.0)]
+
+
+@pytest.mark.parametrize("L,two_s,T,jxy,jz,hx,d,fs", EXTRAS)
+def test_oracle_sse_site_and_insite_operators_match_ed(L, two_s, T, jxy, jz, hx, d, fs):
+    """the SSE walk with the transverse field's site operators and the
+    single-ion term's in-site bonds"""
+    ref = ed.diagonalize_transverse("chain", L, T, two_s, jxy, jz, hx, 0.0, d)
+    beta = 1.0 / T
+    for name, ch in (("looper", O.LooperChain(O.CHAIN, L, beta, jxy, jz, seed=5, fs=fs, two_s=two_s, hx=hx, d=d)),
+                     ("philox", O.PhiloxChain(O.CHAIN, L, beta, jxy, jz, seed=5, chain_id=2, fs=fs, two_s=two_s,
+                                              hx=hx, d=d))):
+        ch.sweep_sse(2000)
+        obs = stats.observables_sse(ch.sweep_sse(40000), beta, L, 1)
+        for k in ("Energy", "Susceptibility", "Magnetization^2", "Staggered Magnetization^2"):
+            m, e = stats.mean_err(obs[k])
+            assert stats.sigmas(m, e, ref[k]) < 4.0, (name, k, m, e, ref[k])
+
+
+def test_oracle_sse_matches_reference_gamma_and_d_blocks():
+    """loop.op's "loop; sse" runs of one S=1 site in Gamma=1 (two uncoupled
+    sites here: per-site values, half the magnetization squares) and of the
+    S=1 Ising chain with D=-0.2"""
+    site = FX["site_spin1_gamma1"]["results"]
+    ch = O.LooperChain(O.CHAIN, 2, 4.0, 0.0, 0.0, seed=23125, two_s=2, hx=1.0)
+    ch.sweep_sse(2000)
+    obs = stats.observables_sse(ch.sweep_sse(40000), 4.0, 2, 1)
+    for k, scale in (("Energy Density", 1.0), ("Susceptibility", 1.0), ("Magnetization^2", 0.5)):
+        m, e = stats.mean_err(obs[k] * scale)
+        assert stats.sigmas(m, e, site[k][0], site[k][1]) < 4.0, (k, m, e, site[k])
+    dch = FX["chain4_spin1_d"]["results"]
+    ch = O.LooperChain(O.CHAIN, 4, 0.5, 0.0, -1.0, seed=23125, two_s=2, d=-0.2)
+    ch.sweep_sse(2000)
+    obs = stats.observables_sse(ch.sweep_sse(40000), 0.5, 4, 1)
+    for k in ("Energy", "Susceptibility", "Magnetization^2", "Staggered Magnetization^2"):
+        m, e = stats.mean_err(obs[k])
+        assert stats.sigmas(m, e, dch[k][0], dch[k][1]) < 4.0, (k, m, e, dch[k])
+
+
 def test_oracle_sse_normal_estimator_matches_ed():
     """the normal estimator's SSE form beta (n M_a^2 + M_0^2) / (n + 1) / V
     (susceptibility.h:326-333) from both oracle modes"""
@@ -81,23 +123,21 @@ def test_oracle_sse_normal_estimator_matches_ed():
 
 def test_sse_parameter_checks():
     import loopgpu
-    for kw, frag in ((dict(h=0.2), "lattice bonds only"), (dict(hx=0.3, jxy=-1.0), "lattice bonds only"),
-                     (dict(two_s=2, d=0.2), "lattice bonds only")):
-        args = dict(jxy=1.0, jz=1.0)
-        args.update(kw)
-        with pytest.raises(loopgpu.LoopGPUError) as e:
-            loopgpu.Chain(loopgpu.params(loopgpu.CHAIN, 8, 1.0, algorithm=loopgpu.SSE, **args))
-        assert e.value.code == -1 and frag in str(e.value)
     with pytest.raises(loopgpu.LoopGPUError) as e:
-        loopgpu.Chain(loopgpu.params(loopgpu.SQUARE, 64, 1.0, algorithm=loopgpu.SSE))
-    assert "at most 4096" in str(e.value)
+        loopgpu.Chain(loopgpu.params(loopgpu.CHAIN, 8, 1.0, h=0.2, algorithm=loopgpu.SSE))
+    assert e.value.code == -1 and "longitudinal field" in str(e.value)
+    for p in (loopgpu.params(loopgpu.SQUARE, 64, 1.0, algorithm=loopgpu.SSE),
+              loopgpu.params(loopgpu.CHAIN, 2050, 1.0, -1.0, 1.0, hx=0.5, algorithm=loopgpu.SSE)):
+        with pytest.raises(loopgpu.LoopGPUError) as e:   # (site bonds count against the bond limit)
+            loopgpu.Chain(p)
+        assert "at most 4096" in str(e.value)
     with pytest.raises(loopgpu.LoopGPUError) as e:
         loopgpu.Batch([loopgpu.params(loopgpu.CHAIN, 8, 1.0), loopgpu.params(loopgpu.CHAIN, 8, 1.0, algorithm=1)])
     assert "one algorithm" in str(e.value)
     with pytest.raises(loopgpu.LoopGPUError) as e:
         loopgpu.Chain(loopgpu.params(loopgpu.CHAIN, 8, 1.0, algorithm=2))
     assert e.value.code == -1
-    for txt, frag in (('ALGORITHM = "loop; sse"\nL = 8\nT = 1\nh = 0.1', "lattice bonds only"),
+    for txt, frag in (('ALGORITHM = "loop; sse"\nL = 8\nT = 1\nh = 0.1', "longitudinal field"),
                       ('ALGORITHM = "loop; worm"\nL = 8\nT = 1', "unsupported ALGORITHM"),
                       ('ALGORITHM = "loop; sse"\nL = 8\nT = 1\nMEASURE[Correlations] = true', "path-integral worker")):
         with pytest.raises(loopgpu.LoopGPUError) as e:
@@ -114,10 +154,13 @@ def _cmp(g, c, where):
     np.testing.assert_allclose(g[:, FLOAT], c[:, FLOAT], rtol=1e-9, atol=1e-9, err_msg=where)
 
 
-# kind, L, 2S, beta, Jxy, Jz
+# kind, L, 2S, beta, Jxy, Jz (, hx, D, FORCE_SCATTER)
 TRAJ = [(O.CHAIN, 4, 1, 2.0, 1.0, 1.0), (O.CHAIN, 64, 1, 10.0, 1.0, 1.0), (O.CHAIN, 16, 1, 4.0, 0.0, 1.0),
         (O.CHAIN, 32, 1, 6.0, -1.0, 0.5), (O.SQUARE, 4, 1, 3.0, 1.0, 1.0), (O.SQUARE, 16, 1, 4.0, 1.0, 1.0),
         (O.CHAIN, 8, 2, 4.0, 1.0, 1.0), (O.CHAIN, 8, 3, 3.0, 1.0, 0.8), (O.SQUARE, 4, 2, 2.0, 1.0, 1.0)]
+TRAJ_EXTRA = [(O.CHAIN, 16, 1, 4.0, 0.0, 1.0, 0.9, 0.0, 0.0), (O.SQUARE, 4, 1, 3.0, -1.0, 1.0, 0.6, 0.0, 0.0),
+              (O.CHAIN, 8, 2, 3.0, 1.0, 1.0, 0.0, -0.4, 0.0), (O.CHAIN, 8, 3, 2.0, -1.0, 0.8, 0.7, 0.5, 0.1),
+              (O.CHAIN, 2, 2, 4.0, 0.0, 0.0, 1.0, 0.0, 0.0)]
 LAUNCHES = [dict(), dict(link=0), dict(link=0, compress=1), dict(link=1), dict(link=128), dict(link=512),
             dict(link=1536, compress=1)]
 
@@ -149,6 +192,46 @@ def test_sse_bit_exact_with_oracle(loopgpu, kind, L, two_s, beta, jxy, jz):
         _cmp(g.sweep(40), rc, "%s %s" % ((kind, L, two_s, beta, jxy, jz), lc))
         for a, b in zip(g.get_config(), cs):
             np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind,L,two_s,beta,jxy,jz,hx,d,fs", TRAJ_EXTRA)
+def test_sse_site_and_insite_bit_exact_with_oracle(loopgpu, kind, L, two_s, beta, jxy, jz, hx, d, fs):
+    c = O.PhiloxChain(kind, L, beta, jxy, jz, seed=6, chain_id=2, fs=fs, two_s=two_s, hx=hx, d=d)
+    rc = c.sweep_sse(40)
+    cs = c.state()
+    for lc in (dict(), dict(compress=1)):
+        print("sse extra", kind, L, two_s, beta, jxy, jz, hx, d, fs, lc, flush=True)
+        g = loopgpu.Chain(loopgpu.params(kind, L, beta, jxy, jz, seed=6, chain_id=2, force_scatter=fs, two_s=two_s,
+                                         hx=hx, d=d, algorithm=loopgpu.SSE))
+        if lc:
+            g.set_launch(**lc)
+        assert g.launch()["link"] == 0
+        _cmp(g.sweep(40), rc, "%s %s" % ((kind, L, two_s, beta, jxy, jz, hx, d), lc))
+        for a, b in zip(g.get_config(), cs):
+            np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.gpu
+def test_worker_sse_gamma_and_d_match_reference_runs(loopgpu):
+    site = FX["site_spin1_gamma1"]["results"]
+    w = loopgpu.Worker('ALGORITHM = "loop; sse"\nL = 2\nJ = 0\nlocal_S = 1\nGamma = 1\nT = 0.25\n'
+                       'SWEEPS = 65536\nSEED = 23125')
+    print("worker sse gamma", flush=True)
+    w.run()
+    for name, scale in (("Energy Density", 1.0), ("Susceptibility", 1.0), ("Magnetization^2", 0.5)):
+        m, e, _ = w.result(name)
+        assert stats.sigmas(m * scale, e * scale, site[name][0], site[name][1]) < 4.0, (name, m, e, site[name])
+    dch = FX["chain4_spin1_d"]["results"]
+    ref = ed.diagonalize_transverse("chain", 4, 2.0, 2, 0.0, -1.0, 0.0, 0.0, -0.2)
+    w = loopgpu.Worker('ALGORITHM = "loop; sse"\nL = 4\nlocal_S = 1\nJxy = 0\nJz = -1\nD = -0.2\nT = 2\n'
+                       'SWEEPS = 65536\nSEED = 23125')
+    print("worker sse D", flush=True)
+    w.run()
+    for name in ("Energy", "Susceptibility", "Magnetization^2", "Staggered Magnetization^2"):
+        m, e, _ = w.result(name)
+        assert stats.sigmas(m, e, ref[name]) < 4.0, (name, m, e, ref[name])
+        assert stats.sigmas(m, e, dch[name][0], dch[name][1]) < 4.0, (name, m, e, dch[name])
 
 
 @pytest.mark.gpu
