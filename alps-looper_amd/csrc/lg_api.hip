@@ -79,7 +79,7 @@ int check_params(const lg_params& p) {
     return set_error(LG_ERR_ARG, "algorithm must be 0 (loop; path integral) or 1 (loop; sse)");
   if (p.algorithm == LG_ALGORITHM_SSE) {
     if (p.h != 0)
-      return set_error(LG_ERR_ARG, "the SSE worker is built without a longitudinal field (h = 0)");
+      return set_error(LG_ERR_ARG, "the SSE worker takes no longitudinal field (h = 0; sse.C:123 rejects it too)");
     const long long nv = p.two_s >= 2 ? p.two_s : 1;
     const long long nr = p.lattice == LG_LATTICE_CHAIN ? p.L : (long long)p.L * p.L;
     long long nb = nv * nv * (p.lattice == LG_LATTICE_CHAIN ? p.L : 2ll * p.L * p.L);
