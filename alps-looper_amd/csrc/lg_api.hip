@@ -89,12 +89,6 @@ int check_params(const lg_params& p) {
       return set_error(LG_ERR_ARG, "the SSE worker takes at most " + std::to_string(SSE_MAX_SITES) +
                                        " (virtual) sites and " + std::to_string(SSE_MAX_BONDS) + " bonds");
   }
-  // an antiferromagnetic XY coupling with a transverse field has a sign
-  // problem that no gauge removes (looper/model_parameter.h:617-644 finds the
-  // lattice frustrated and the reference takes its SIGN path)
-  if (p.hx != 0 && p.jxy > 0)
-    return set_error(LG_ERR_ARG, "a transverse field with Jxy > 0 has a sign problem (the reference's SIGN path); "
-                                 "it is built for Jxy <= 0");
   return LG_OK;
 }
 
@@ -211,6 +205,14 @@ ChainDev make_chain_dev(const lg_params& p) {
   double lam = p.beta * std::max(std::max(W, dw), d.site_w);
   d.cand_cap = (int)std::ceil(lam + 12.0 * std::sqrt(lam) + 16.0);
   sse_selection(d, v, dw, d.npair > 0 ? p.d : 0.0, p.force_scatter);
+  // an antiferromagnetic XY coupling with a transverse field has a sign
+  // problem that no gauge removes (model_parameter.h:617-644 finds the
+  // lattice frustrated; the reference takes its SIGN path): every sweep
+  // then records the configuration's sign
+  if (p.hx != 0 && p.jxy > 0) {
+    d.sgn_lattice = 1;
+    d.sgn_site = p.hx < 0 ? 1 : 0;
+  }
   return d;
 }
 

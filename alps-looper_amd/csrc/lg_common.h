@@ -93,6 +93,11 @@ struct ChainDev {
   // into the class's range), the location within the class by word 3
   uint64_t sse_cls[2], sse_tl[3], sse_td[3];
   double sse_wtot;                   // sum of the weights of every (location, graph)
+  // sign problem (Jxy > 0 with a transverse field, model_parameter.h:617-644):
+  // off-diagonal operators of negative weight on the lattice bonds
+  // (antiferromagnetic XY) and on the site bonds (hx < 0); the
+  // configuration's sign is (-1)^(their number)
+  int sgn_lattice, sgn_site;
 };
 // SSE chains: their sites' spins and bonds' counters are staged in LDS
 constexpr int SSE_MAX_SITES = 4096, SSE_MAX_BONDS = 4096;
@@ -411,6 +416,7 @@ enum {
   R_SMAG0, R_SSIZE2, R_SMAG2, R_SSIZE4, R_SMAG4, R_SSIZE, R_SMAG,
   R_W2, R_BETA, R_SWEEP,
   R_UMAGA, R_SMAGA, R_EFIELD,  // sum_c umag_c, sum_c smag_c (normal estimator), Zeeman energy
+  R_SIGN,                      // sign of the configuration's weight (+-1; 1 for sign-free models)
   R_COUNT
 };
 

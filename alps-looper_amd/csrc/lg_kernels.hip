@@ -1871,9 +1871,27 @@ __global__ void __launch_bounds__(END_THREADS) k_end_sweep(DevBatch B) {
     out[tid] = t;
   }
   __syncthreads();
+  // sign problem: the off-diagonal operators of negative weight before the
+  // flip (per bond: all operators less the diagonal ones the diagonal update
+  // placed), path_integral.C:551-576
+  __shared__ int negw[END_THREADS / 64];
+  int neg = 0;
+  if (P.sgn_lattice | P.sgn_site) {
+    for (int b = tid; b < P.Nb; b += END_THREADS) {
+      const bool negative = (b < P.nb_lat) ? P.sgn_lattice != 0 : (b >= P.nb_real && P.sgn_site != 0);
+      if (negative) neg += B.counts[P.bond_base + b] - B.acc_count[P.bond_base + b];
+    }
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) neg += __shfl_xor(neg, d, 64);
+    if (lane_id() == 0) negw[tid >> 6] = neg;
+    __syncthreads();
+    neg = 0;
+    for (int w = 0; w < END_THREADS / 64; ++w) neg += negw[w];
+  }
   if (tid == 0) {
     const double nop = bad ? 0.0 : (double)(B.bo_new[P.bond_base + P.Nb] - B.bo_new[P.bond_base]);
     out[R_NOP] = nop;
+    out[R_SIGN] = (neg & 1) ? -1.0 : 1.0;
     out[R_ENERGY] = P.offset_total - nop / P.beta + out[R_EFIELD];
     out[R_BETA] = P.beta;
     out[R_SWEEP] = (double)B.sweep[c];
@@ -2142,6 +2160,11 @@ __global__ void __launch_bounds__(SSE_THREADS) k_sse_scatter(DevBatch B) {
       perm[p0 + lane] = pos;
     }
   }
+  // each bond's diagonal operators, as the path-integral diagonal update
+  // leaves them (k_end_sweep counts the off-diagonal ones for the sign)
+  __syncthreads();
+  for (int b = lane; b < Nb; b += SSE_THREADS)
+    B.acc_count[P.bond_base + b] = (cur[b] - B.bo_new[P.bond_base + b]) - koff[b];
 }
 
 // the flipped operator types back into the string

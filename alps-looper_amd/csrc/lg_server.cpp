@@ -316,6 +316,8 @@ int lg_server_submit(lg_server* s, const lg_job* job, uint64_t* id) {
   if (job->sweeps == 0) return serr(LG_ERR_ARG, "job needs at least one measurement sweep");
   if (job->params.algorithm != LG_ALGORITHM_PATH_INTEGRAL)
     return serr(LG_ERR_UNSUPPORTED, "the server runs path-integral jobs");
+  if (job->params.hx != 0 && job->params.jxy > 0)
+    return serr(LG_ERR_UNSUPPORTED, "the server runs sign-free jobs (its results are not sign-weighted)");
   lg::Footprint f = lg::footprint(job->params);
   std::lock_guard<std::mutex> lk(s->mu);
   if (!s->fatal.empty()) return serr(LG_ERR_DEVICE, s->fatal);

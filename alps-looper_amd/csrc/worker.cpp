@@ -276,6 +276,17 @@ class loop_worker {
     double J = p.num("J", 1.0);
     lp_.jxy = p.num("Jxy", J);
     lp_.jz = p.num("Jz", J);
+    // an antiferromagnetic XY coupling with a transverse field: the SIGN
+    // path (model_parameter.h:617-644), observables weighted by the sign and
+    // measured with the normal estimators.  (The reference keeps its improved
+    // estimators there and multiplies them by an "improved" sign that is the
+    // parity of the sum of every cluster's negative-operator count,
+    // path_integral.C:782-802; each negative operator adds to two cluster
+    // ends, so the sum is even, the improved sign is the plain sign, and the
+    // improved magnetization estimators, which average over cluster flips
+    // that change the sign, are biased.)
+    signed_ = lp_.hx != 0 && lp_.jxy > 0;
+    if (signed_) improved_ = false;
     // looper scatters by default where it finds the model frustrated, which a
     // positive single-ion term of S > 1/2 makes it (model_parameter.h:646-661,
     // model_impl.h:51)
@@ -349,6 +360,8 @@ class loop_worker {
   // path_integral.C:307-321 + estimator init_observables
   void init_observables(std::map<std::string, Observable>& obs) const {
     for (const auto& n : names(improved_)) obs[n];
+    if (signed_)
+      for (const char* n : {"Sign", "Weight of Zero-Meron Sector", "Sign in Zero-Meron Sector"}) obs[n];
     if (corr_)
       for (const auto& n : corr_->names()) obs[n];
   }
@@ -522,9 +535,19 @@ class loop_worker {
     m["Volume"] << vol;
     m["Number of Sites"] << (double)n_sites_;
     m["Number of Clusters"] << r.n_clusters;
-    m["Energy"] << r.energy;
-    m["Energy Density"] << r.energy / vol;
-    m["Energy^2"] << p2(r.energy) - nop / p2(beta);
+    // sign problem (path_integral.C:836-861): the sign's observables, and
+    // every observable below weighted by the configuration's sign (the
+    // evaluator divides by the mean sign)
+    const double sg = signed_ ? r.sign : 1.0;
+    if (signed_) {
+      m["Sign"] << r.sign;
+      m["Weight of Zero-Meron Sector"] << 1.0;  // (the cluster signs always sum to an even number)
+      m["Sign in Zero-Meron Sector"] << r.sign;
+    }
+    auto put = [&](const std::string& k, double v) { m[k] << sg * v; };
+    put("Energy", r.energy);
+    put("Energy Density", r.energy / vol);
+    put("Energy^2", p2(r.energy) - nop / p2(beta));
     // SSE: the tau-integrated sums come in units of the string;
     // beta (n S + S0) / (n + 1) (susceptibility.h:213-251, 326-333)
     const bool sse = lp_.algorithm == LG_ALGORITHM_SSE;
@@ -535,19 +558,19 @@ class loop_worker {
       for (int k = 0; k < 2; ++k) {
         const std::string P = k ? "Staggered " : "", bar = "|" + P;
         const double mg = mags[k][0], ma = mags[k][1];
-        m[P + "Magnetization"] << mg;
-        m[P + "Magnetization Density"] << mg / vol;
-        m[bar + "Magnetization|"] << std::fabs(mg);
-        m[bar + "Magnetization Density|"] << std::fabs(mg) / vol;
-        m[P + "Magnetization^2"] << p2(mg);
-        m[P + "Magnetization Density^2"] << p2(mg / vol);
-        m[P + "Magnetization^4"] << p4(mg);
-        m[P + "Magnetization Density^4"] << p4(mg / vol);
-        m[P + "Susceptibility"] << chi(p2(ma), p2(mg));
+        put(P + "Magnetization", mg);
+        put(P + "Magnetization Density", mg / vol);
+        put(bar + "Magnetization|", std::fabs(mg));
+        put(bar + "Magnetization Density|", std::fabs(mg) / vol);
+        put(P + "Magnetization^2", p2(mg));
+        put(P + "Magnetization Density^2", p2(mg / vol));
+        put(P + "Magnetization^4", p4(mg));
+        put(P + "Magnetization Density^4", p4(mg / vol));
+        put(P + "Susceptibility", chi(p2(ma), p2(mg)));
       }
       return;
     }
-    m["Stiffness"] << r.w2 / (beta * dim_);
+    put("Stiffness", r.w2 / (beta * dim_));
     struct Set { const char* pre; double mag0, size2, mag2, size4, mag4, sus, gsus; };
     const Set sets[2] = {
         {"", r.umag0, r.usize2, r.umag2, r.usize4, r.umag4, r.umag, r.usize},
@@ -556,20 +579,20 @@ class loop_worker {
       std::string P(s.pre);
       std::string bar = "|" + P;
       std::string gen = "Generalized " + P;
-      m[P + "Magnetization"] << 0.0;
-      m[P + "Magnetization Density"] << 0.0;
-      m[bar + "Magnetization|"] << std::fabs(s.mag0);
-      m[bar + "Magnetization Density|"] << std::fabs(s.mag0) / vol;
-      m[P + "Magnetization^2"] << s.mag2;
-      m[P + "Magnetization Density^2"] << s.mag2 / p2(vol);
-      m[P + "Magnetization^4"] << 3 * p2(s.mag2) - 2 * s.mag4;
-      m[P + "Magnetization Density^4"] << (3 * p2(s.mag2) - 2 * s.mag4) / p4(vol);
-      m[P + "Susceptibility"] << chi(s.sus, s.mag2);
-      m[gen + "Magnetization^2"] << s.size2;
-      m[gen + "Magnetization Density^2"] << s.size2 / p2(vol);
-      m[gen + "Magnetization^4"] << 3 * p2(s.size2) - 2 * s.size4;
-      m[gen + "Magnetization Density^4"] << (3 * p2(s.size2) - 2 * s.size4) / p4(vol);
-      m[gen + "Susceptibility"] << chi(s.gsus, s.size2);
+      put(P + "Magnetization", 0.0);
+      put(P + "Magnetization Density", 0.0);
+      put(bar + "Magnetization|", std::fabs(s.mag0));
+      put(bar + "Magnetization Density|", std::fabs(s.mag0) / vol);
+      put(P + "Magnetization^2", s.mag2);
+      put(P + "Magnetization Density^2", s.mag2 / p2(vol));
+      put(P + "Magnetization^4", 3 * p2(s.mag2) - 2 * s.mag4);
+      put(P + "Magnetization Density^4", (3 * p2(s.mag2) - 2 * s.mag4) / p4(vol));
+      put(P + "Susceptibility", chi(s.sus, s.mag2));
+      put(gen + "Magnetization^2", s.size2);
+      put(gen + "Magnetization Density^2", s.size2 / p2(vol));
+      put(gen + "Magnetization^4", 3 * p2(s.size2) - 2 * s.size4);
+      put(gen + "Magnetization Density^4", (3 * p2(s.size2) - 2 * s.size4) / p4(vol));
+      put(gen + "Susceptibility", chi(s.gsus, s.size2));
     }
   }
 
@@ -577,6 +600,7 @@ class loop_worker {
   std::vector<int32_t> sched_steps_;   // annealing schedule (start step, temperature); last = final
   std::vector<double> sched_temps_;
   bool improved_ = true;
+  bool signed_ = false;          // sign problem: Jxy > 0 with a transverse field
   lg_chain* h_ = nullptr;
   uint64_t mcs_ = 0, therm_ = 0, sweeps_min_ = 0, sweeps_max_ = 0;
   int n_sites_ = 0, dim_ = 1;
@@ -589,12 +613,53 @@ class loop_worker {
 };
 
 // evaluator (energy.h:89-102, susceptibility.h:340-374)
+// jackknife over the common bins of three observables for f(<a>, <b>, <c>)
+template <class F>
+Result jackknife3(const Observable& a, const Observable& b, const Observable& c, F f) {
+  size_t nb = std::min(std::min(a.bins().size(), b.bins().size()), c.bins().size());
+  Result r{f(a.mean(), b.mean(), c.mean()), 0.0, 0.0, true};
+  if (nb < 2 || a.bin_size() != b.bin_size() || a.bin_size() != c.bin_size()) return r;
+  double sa = 0, sb = 0, sc = 0;
+  for (size_t i = 0; i < nb; ++i) { sa += a.bins()[i]; sb += b.bins()[i]; sc += c.bins()[i]; }
+  std::vector<double> jk(nb);
+  double m = 0;
+  for (size_t i = 0; i < nb; ++i) {
+    jk[i] = f((sa - a.bins()[i]) / (nb - 1), (sb - b.bins()[i]) / (nb - 1), (sc - c.bins()[i]) / (nb - 1));
+    m += jk[i];
+  }
+  m /= nb;
+  double v = 0;
+  for (double x : jk) v += (x - m) * (x - m);
+  r.error = std::sqrt(v * (nb - 1) / nb);
+  return r;
+}
+
 std::map<std::string, Result> evaluate(const std::map<std::string, Observable>& obs) {
   std::map<std::string, Result> out;
   for (const auto& kv : obs)
     if (kv.second.count()) out[kv.first] = Result{kv.second.mean(), kv.second.error(), kv.second.tau(), false};
   auto has = [&](const char* k) { return obs.count(k) && obs.at(k).count() > 0; };
-  if (has("Energy") && has("Energy^2") && has("Inverse Temperature") && has("Volume")) {
+  if (has("Sign")) {
+    // sign problem: every sign-weighted observable as <s O> / <s> (ALPS's
+    // signed observables); the unweighted ones and the sign's own stay
+    const Observable& sg = obs.at("Sign");
+    static const char* plain[] = {"Temperature", "Inverse Temperature", "Volume", "Number of Sites",
+                                  "Number of Clusters", "Sign", "Weight of Zero-Meron Sector",
+                                  "Sign in Zero-Meron Sector"};
+    for (const auto& kv : obs) {
+      if (!kv.second.count() || std::find(std::begin(plain), std::end(plain), kv.first) != std::end(plain)) continue;
+      Result r = jackknife(kv.second, sg, [](double x, double s) { return x / s; });
+      r.tau = kv.second.tau();
+      r.evaluated = false;
+      out[kv.first] = r;
+    }
+    if (has("Energy") && has("Energy^2") && has("Inverse Temperature") && has("Volume")) {
+      double b = obs.at("Inverse Temperature").mean(), v = obs.at("Volume").mean();
+      out["Specific Heat"] = jackknife3(obs.at("Energy^2"), obs.at("Energy"), sg, [&](double e2, double e, double s) {
+        return b * b * (e2 / s - (e / s) * (e / s)) / v;
+      });
+    }
+  } else if (has("Energy") && has("Energy^2") && has("Inverse Temperature") && has("Volume")) {
     double b = obs.at("Inverse Temperature").mean(), v = obs.at("Volume").mean();
     out["Specific Heat"] = jackknife(obs.at("Energy^2"), obs.at("Energy"),
                                      [&](double e2, double e) { return b * b * (e2 - e * e) / v; });
@@ -606,8 +671,13 @@ std::map<std::string, Result> evaluate(const std::map<std::string, Observable>& 
       {"Generalized Staggered Magnetization^2", "Generalized Staggered Magnetization^4",
        "Binder Ratio of Generalized Staggered Magnetization"}};
   for (auto& b : binder)
-    if (has(b[0]) && has(b[1]))
-      out[b[2]] = jackknife(obs.at(b[0]), obs.at(b[1]), [](double m2, double m4) { return m2 * m2 / m4; });
+    if (has(b[0]) && has(b[1])) {
+      if (has("Sign"))  // (m2 / s)^2 / (m4 / s)
+        out[b[2]] = jackknife3(obs.at(b[0]), obs.at(b[1]), obs.at("Sign"),
+                               [](double m2, double m4, double s) { return m2 * m2 / (m4 * s); });
+      else
+        out[b[2]] = jackknife(obs.at(b[0]), obs.at(b[1]), [](double m2, double m4) { return m2 * m2 / m4; });
+    }
   return out;
 }
 

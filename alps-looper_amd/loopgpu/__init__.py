@@ -19,7 +19,7 @@ SQUARE = 1
 REC_FIELDS = [
     "n_op", "n_clusters", "energy", "umag0", "usize2", "umag2", "usize4", "umag4", "usize", "umag",
     "smag0", "ssize2", "smag2", "ssize4", "smag4", "ssize", "smag", "w2", "beta", "sweep",
-    "umag_a", "smag_a", "e_field",
+    "umag_a", "smag_a", "e_field", "sign",
 ]
 REC_SIZE = len(REC_FIELDS)
 

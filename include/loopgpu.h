@@ -104,6 +104,8 @@ typedef struct lg_sweep_record {
    * staggered form (umag0 / smag0 above are the tau = 0 ones) */
   double umag_a, smag_a;
   double e_field;       /* Zeeman energy after the flip, sum_c (flip ? -w_c : w_c), included in energy */
+  double sign;          /* sign of the configuration's weight, (-1)^(off-diagonal operators of negative
+                           weight) (path_integral.C:551-576); 1 for sign-free models */
 } lg_sweep_record;
 
 typedef struct lg_chain lg_chain;   /* one chain, plain one-chain-per-launch path */

@@ -400,7 +400,20 @@ enum {
   R_SMAG0, R_SSIZE2, R_SMAG2, R_SSIZE4, R_SMAG4, R_SSIZE, R_SMAG,
   R_W2, R_BETA, R_SWEEP,
   R_UMAGA, R_SMAGA, R_EFIELD,  // normal estimator sums, Zeeman energy (FIELD path)
+  R_SIGN,                      // (-1)^(off-diagonal operators of negative weight); 1 if sign-free
   R_COUNT
+};
+
+// Sign problem (model_parameter.h:617-644): with Jxy > 0 and a transverse
+// field no gauge removes the negative weights; the off-diagonal operators on
+// the lattice bonds (antiferromagnetic XY) and, for hx < 0, on the site
+// bonds then count against the sign (path_integral.C:551-576)
+struct SignRule {
+  bool lattice = false, site = false;
+  void init(double jxy, double hx) { lattice = hx != 0 && jxy > 0; site = lattice && hx < 0; }
+  bool negative(const Lattice& lat, int b) const {
+    return b < lat.nb_lat ? lattice : (lat.is_site_bond(b) ? site : false);
+  }
 };
 
 struct Estimate {   // susceptibility + stiffness improved estimate (per cluster)
@@ -560,6 +573,7 @@ struct LooperOp {
 
 struct LooperChain {
   Lattice lat;
+  SignRule sgn;
   Weights w, wd;  // lattice bonds; in-site bonds of the single-ion term
   double beta = 1, offset_total = 0, graph_weight = 0, p_off = 1, p_off_d = 1;
   std::mt19937 eng;
@@ -580,6 +594,7 @@ struct LooperChain {
   void init(int kind, int L, double b, double jxy, double jz, double fs, uint32_t seed, double field = 0,
             int nv = 1, double hx = 0, double d = 0) {
     lat.init(kind, L, nv);
+    sgn.init(jxy, hx);
     w = xxz_weights(jxy, jz, fs);
     beta = b;
     h = field;
@@ -734,6 +749,9 @@ struct LooperChain {
     rec[R_ENERGY] = offset_total - ops.size() / beta;
     rec[R_BETA] = beta;
     rec[R_SWEEP] = (double)mcs;
+    int neg = 0;  // the sign (before the flip)
+    for (const auto& op : ops) if ((op.type & 1) && sgn.negative(lat, op.bond)) ++neg;
+    rec[R_SIGN] = (neg & 1) ? -1.0 : 1.0;
     // determine flip (781-803): (2r-1) < (FIELD ? tanh(beta w_c) : 0), with
     // the field weight w_c = -h umag_c (cluster.h:50-63); Zeeman energy of
     // the flipped cluster added to the energy (852-858)
@@ -890,6 +908,9 @@ struct LooperChain {
     rec[R_ENERGY] = offset_total - n / beta;
     rec[R_BETA] = beta;
     rec[R_SWEEP] = (double)mcs;
+    int neg = 0;  // the sign (before the flip)
+    for (const auto& op : ops) if ((op.type & 1) && sgn.negative(lat, op.bond)) ++neg;
+    rec[R_SIGN] = (neg & 1) ? -1.0 : 1.0;
     to_flip.assign(nc, 0);
     for (int c = 0; c < nc; ++c) to_flip[c] = (2 * rnd() - 1) < 0;
     for (auto& op : ops)
@@ -1161,6 +1182,7 @@ void looper_labels(const Lattice& lat, int nops, const int* bond, const double* 
 // ---------------------------------------------------------------------------
 struct PhiloxChain {
   Lattice lat;
+  SignRule sgn;
   Weights w, wd;     // lattice bonds; in-site bonds of the single-ion term
   Thresholds th, thd;
   double Wd = 0, scale_d = 0;
@@ -1189,6 +1211,7 @@ struct PhiloxChain {
             double field = 0, int nv = 1, double hx = 0, double d = 0) {
     h = field;
     lat.init(kind, L, nv);
+    sgn.init(jxy, hx);
     w = xxz_weights(jxy, jz, fs);
     th = make_thresholds(w);
     // choose_diagonal (graph_impl.h:297-312): g = 0 w.p. v0 / (v0 + v2) for
@@ -1593,6 +1616,9 @@ struct PhiloxChain {
     rec[R_ENERGY] = offset_total - nops / beta;
     rec[R_BETA] = beta;
     rec[R_SWEEP] = (double)sweep;
+    int neg = 0;  // the sign (before the flip)
+    for (int k = 0; k < nops; ++k) if ((otype[k] & 1) && sgn.negative(lat, bond[k])) ++neg;
+    rec[R_SIGN] = (neg & 1) ? -1.0 : 1.0;
     // (d) flip: one bit per root; with a field the cluster flips iff
     // 2u - 1 < tanh(beta w), w = -h umag, u from the first 53 bits
     std::vector<char> fb(M, 0);

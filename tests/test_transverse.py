@@ -130,10 +130,6 @@ def test_oracle_matches_reference_site_qmc():
 
 def test_transverse_field_parameter_checks():
     import loopgpu
-    # an antiferromagnetic XY coupling with hx: the reference's SIGN path
-    with pytest.raises(loopgpu.LoopGPUError) as e:
-        loopgpu.Chain(loopgpu.params(loopgpu.CHAIN, 4, 1.0, 1.0, 1.0, hx=0.5))
-    assert e.value.code == -1 and "sign problem" in str(e.value)
     with pytest.raises(loopgpu.LoopGPUError) as e:
         loopgpu.Exchange(loopgpu.params(loopgpu.CHAIN, 4, 1.0, -1.0, 1.0, hx=0.5), [1.0, 2.0])
     assert "transverse" in str(e.value)

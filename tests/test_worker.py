@@ -17,7 +17,6 @@ FX = json.load(open(os.path.join(HERE, "golden", "reference_fixtures.json")))
     ("L=7; T=1", "even"),
     ('LATTICE="chain lattice"; T=0.1', "L not defined"),
     ('LATTICE="triangular lattice"; L=4; T=1', "unsupported LATTICE"),
-    ("L=4; T=1; Gamma=0.3", "sign problem"),
     ('L=4; T=1; Jxy=-1; Jz=1; Gamma=0.3; MEASURE[Correlations]=true', "transverse field"),
     ("L=4; local_S=5/2; T=1", "local_S"),
     ('LATTICE="square lattice"; L=4; local_S=3/2; T=1', "local_S"),
