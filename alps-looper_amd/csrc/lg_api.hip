@@ -59,17 +59,26 @@ static void xxz_weights(double jxy_in, double jz, double a, double v[4], double*
   *offset = (v[0] + v[1] + v[2] + v[3]) / 2;
 }
 
+static_assert(LG_MAX_TWO_S == MAX_NV, "the largest 2S is the boundary kernel's virtual-site count");
+
 int check_params(const lg_params& p) {
   if (p.lattice != LG_LATTICE_CHAIN && p.lattice != LG_LATTICE_SQUARE)
     return set_error(LG_ERR_ARG, "lattice must be chain (0) or square (1)");
-  if (p.two_s < 0 || p.two_s > 4)
-    return set_error(LG_ERR_ARG, "two_s (2S) must be 1 (S=1/2; 0 means 1), 2, 3 or 4");
-  if (p.two_s > 2 && p.lattice != LG_LATTICE_CHAIN)
-    return set_error(LG_ERR_ARG, "S > 1 (virtual sites) is built for the chain lattice; the square lattice takes S <= 1");
+  if (p.two_s < 0 || p.two_s > LG_MAX_TWO_S)
+    return set_error(LG_ERR_ARG, "two_s (2S) must be 1 .. " + std::to_string(LG_MAX_TWO_S) + " (S=1/2 .. " +
+                                     std::to_string(LG_MAX_TWO_S / 2) + "; 0 means 1)");
   if (p.L < 2 || (p.L & 1))
     return set_error(LG_ERR_ARG, "L must be even and >= 2 (bipartite, sign-free lattice)");
-  if (p.lattice == LG_LATTICE_SQUARE && (long long)p.L * p.L * 2 > (1ll << 27))
-    return set_error(LG_ERR_ARG, "square lattice too large");
+  {
+    // every index of a chain (virtual bonds, in-site and site bonds, sites)
+    // stays far inside int32
+    const long long nv = p.two_s >= 2 ? p.two_s : 1;
+    const long long nr = p.lattice == LG_LATTICE_CHAIN ? p.L : (long long)p.L * p.L;
+    long long nb = nv * nv * (p.lattice == LG_LATTICE_CHAIN ? nr : 2 * nr);
+    if (p.d != 0) nb += nr * nv * (nv - 1) / 2;  // in-site bonds
+    if (p.hx != 0) nb += nr * nv;                // site bonds
+    if (nb > (1ll << 27)) return set_error(LG_ERR_ARG, "lattice too large (more than 2^27 bonds)");
+  }
   if (!(p.beta > 0) || !std::isfinite(p.beta))
     return set_error(LG_ERR_ARG, "beta must be positive (negative temperature)");
   if (!std::isfinite(p.jxy) || !std::isfinite(p.jz) || !std::isfinite(p.h) || !std::isfinite(p.hx) ||
@@ -938,7 +947,7 @@ int Batch::set_launch(const lg_launch& l) {
     if (l.link > 1 && has_virtual_sites())
       return set_error(LG_ERR_ARG, "chains with virtual sites (S > 1/2) take linking kernel 0 or 1");
     if (l.link == 1 && max_site_lists() > 4)
-      return set_error(LG_ERR_ARG, "sites with more than four bond lists (S > 1 on the chain, S = 1 on the square lattice) take linking kernel 0");
+      return set_error(LG_ERR_ARG, "sites with more than four bond lists (S > 1 on the chain, S > 1/2 on the square lattice) take linking kernel 0");
     if (l.link != 0 && has_extra_bonds())
       return set_error(LG_ERR_ARG, "chains with a transverse field or a single-ion term take linking kernel 0");
     c.link_cap = l.link;

@@ -101,6 +101,9 @@ struct ChainDev {
 };
 // SSE chains: their sites' spins and bonds' counters are staged in LDS
 constexpr int SSE_MAX_SITES = 4096, SSE_MAX_BONDS = 4096;
+// virtual sites per real site (2S) at most: the boundary permutation of a
+// real site's virtual world lines is held as eight 4-bit fields
+constexpr int MAX_NV = 8;
 
 // Temperature of Monte Carlo step `step` on an annealing schedule of n
 // (start step, temperature) pairs stored as doubles, start steps ascending,

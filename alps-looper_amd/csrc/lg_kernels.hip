@@ -1216,22 +1216,21 @@ __global__ void __launch_bounds__(256) k_uf_union(DevBatch B) {
 // looper/permutation.h:88-112).  For each spin value u = 0, 1 in turn, the
 // bottoms of spin u in virtual-site order are matched to the tops of spin u
 // in virtual-site order permuted by Fisher-Yates: for i = n_u - 1 down to 1,
-// j = i - (x mod (i + 1)) with x the next word of Philox(sweep,
-// STAGE_BOUNDARY, real site, 0), swap positions i and j.  (The spins at
-// tau = 1 are those at tau = 0 in some order: every update keeps each real
-// site's magnetization periodic.)  For nv = 2 with equal spins this is the
-// exchange iff bit 0 of the first word is set.  One thread per real site;
-// nv <= 4, so the lists live in registers as 4-bit fields.
-__device__ __forceinline__ int pick4(int i, int a0, int a1, int a2, int a3) {
-  return i == 0 ? a0 : i == 1 ? a1 : i == 2 ? a2 : a3;
-}
-
-__device__ __forceinline__ uint32_t pick_word(const U4& x, int k) {
-  return k == 0 ? x.x : k == 1 ? x.y : k == 2 ? x.z : x.w;
+// j = i - (x mod (i + 1)) with x the next word of the stream (word k is word
+// k mod 4 of Philox(sweep, STAGE_BOUNDARY, real site, k / 4)), swap
+// positions i and j.  (The spins at tau = 1 are those at tau = 0 in some
+// order: every update keeps each real site's magnetization periodic.)  For
+// nv = 2 with equal spins this is the exchange iff bit 0 of the first word
+// is set.  One thread per real site; nv <= 8, so a permutation lives in one
+// register as eight 4-bit fields.
+template <typename T>
+__device__ __forceinline__ T pick8(int i, T a0, T a1, T a2, T a3, T a4, T a5, T a6, T a7) {
+  return i < 4 ? (i == 0 ? a0 : i == 1 ? a1 : i == 2 ? a2 : a3) : (i == 4 ? a4 : i == 5 ? a5 : i == 6 ? a6 : a7);
 }
 
 __global__ void __launch_bounds__(256) k_boundary(DevBatch B) {
   if (overflowed(B)) return;
+  constexpr int NV = MAX_NV;
   const int stride = gridDim.x * blockDim.x;
   for (int gs = blockIdx.x * blockDim.x + threadIdx.x; gs < B.nsites; gs += stride) {
     const int c = B.site_chain[gs];
@@ -1241,10 +1240,10 @@ __global__ void __launch_bounds__(256) k_boundary(DevBatch B) {
     const int v = gs - P.site_base;
     const int r = v / nv;
     if (v != r * nv) continue;  // one thread per real site: its virtual site 0
-    int bot[4], top[4];
+    int bot[NV], top[NV];
     uint32_t bs = 0, es = 0;    // bit a: spin of virtual site a's bottom / top segment
 #pragma unroll
-    for (int a = 0; a < 4; ++a) {
+    for (int a = 0; a < NV; ++a) {
       bot[a] = top[a] = 0;
       if (a < nv) {
         bot[a] = B.soff_new[gs + a];
@@ -1254,17 +1253,22 @@ __global__ void __launch_bounds__(256) k_boundary(DevBatch B) {
       }
     }
     const uint32_t all = (1u << nv) - 1;
-    const U4 x = philox(B.sweep[c], STAGE_BOUNDARY, (uint32_t)r, 0u, P.key0, P.key1);
+    // words 0..3, and 4..7 when the two shuffles can take more than four
+    // (n_0 - 1 + n_1 - 1 <= nv - 1 words in all)
+    const U4 x0 = philox(B.sweep[c], STAGE_BOUNDARY, (uint32_t)r, 0u, P.key0, P.key1);
+    U4 x1 = {0u, 0u, 0u, 0u};
+    if (nv > 5) x1 = philox(B.sweep[c], STAGE_BOUNDARY, (uint32_t)r, 1u, P.key0, P.key1);
     int k = 0;
     uint32_t perm[2];
 #pragma unroll
     for (int u = 0; u < 2; ++u) {
       const int n = __popc(u ? (bs & all) : (~bs & all));
-      uint32_t p = 0x3210u;
+      uint32_t p = 0x76543210u;
 #pragma unroll
-      for (int i = 3; i >= 1; --i) {
+      for (int i = NV - 1; i >= 1; --i) {
         if (i < n) {
-          const int j = i - (int)(pick_word(x, k) % (uint32_t)(i + 1));
+          const uint32_t w = pick8(k, x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w);
+          const int j = i - (int)(w % (uint32_t)(i + 1));
           ++k;
           const uint32_t vi = (p >> (4 * i)) & 15u, vj = (p >> (4 * j)) & 15u;
           p &= ~((15u << (4 * i)) | (15u << (4 * j)));
@@ -1274,7 +1278,7 @@ __global__ void __launch_bounds__(256) k_boundary(DevBatch B) {
       perm[u] = p;
     }
 #pragma unroll
-    for (int a = 0; a < 4; ++a) {
+    for (int a = 0; a < NV; ++a) {
       if (a >= nv) continue;
       const uint32_t u = (bs >> a) & 1u;
       const uint32_t cls_b = u ? bs : ~bs, cls_e = u ? es : ~es;
@@ -1283,13 +1287,13 @@ __global__ void __launch_bounds__(256) k_boundary(DevBatch B) {
       // the top of spin u with rank t among the tops of spin u
       int target = 0, cnt = 0;
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
+      for (int q = 0; q < NV; ++q) {
         if (q < nv && ((cls_e >> q) & 1u)) {
           if (cnt == t) target = q;
           ++cnt;
         }
       }
-      uf_unite(B.parent, bot[a], pick4(target, top[0], top[1], top[2], top[3]));
+      uf_unite(B.parent, bot[a], pick8(target, top[0], top[1], top[2], top[3], top[4], top[5], top[6], top[7]));
     }
   }
 }

@@ -250,15 +250,14 @@ class loop_worker {
     else if (alg != "loop; path integral" && alg != "loop")
       throw std::invalid_argument("unsupported ALGORITHM \"" + alg + "\" (loop; path integral, loop; sse)");
     // spin size (model_parameter.h): S > 1/2 through 2S virtual sites per
-    // real site (path_integral.C:589-615): S <= 2 on the chain, S <= 1 on
-    // the square lattice
+    // real site (path_integral.C:589-615), up to S = LG_MAX_TWO_S / 2
     lp_.two_s = 1;
     if (p.defined("local_S")) {
       const double S = p.num("local_S");
       const long long t = std::llround(2.0 * S);
-      const int smax = (lp_.lattice == LG_LATTICE_CHAIN) ? 4 : 2;
-      if (std::fabs(2.0 * S - (double)t) > 1e-12 || t < 1 || t > smax)
-        throw std::invalid_argument("local_S must be 1/2, 1, 3/2 or 2 on the chain lattice, 1/2 or 1 on the square lattice");
+      if (std::fabs(2.0 * S - (double)t) > 1e-12 || t < 1 || t > LG_MAX_TWO_S)
+        throw std::invalid_argument("local_S must be a multiple of 1/2 from 1/2 to " +
+                                    std::to_string(LG_MAX_TWO_S / 2));
       lp_.two_s = (int32_t)t;
     }
     // single-ion term D (S^z)^2 (model_parameter.h:69): in-site bonds of the

@@ -39,13 +39,16 @@ enum {
 
 enum { LG_LATTICE_CHAIN = 0, LG_LATTICE_SQUARE = 1 };
 
+/* largest 2S taken (S = 4): nv = 2S virtual sites per real site */
+enum { LG_MAX_TWO_S = 8 };
+
 /* Parameters of one Markov chain.  Mirrors the worker's parameters:
  * LATTICE ("chain lattice" / "square lattice"), L, T or BETA, J or Jxy/Jz,
  * SEED, FORCE_SCATTER, local_S (loop_config.h, lattice.h:304-307,
  * temperature.h:43-79, weight_impl.h:165-188); the spin model is the XXZ
  * model H = sum_b [Jz Sz Sz + Jxy/2 (S+S- + S-S+)] with spin S = two_s / 2.
- * S > 1/2 (two_s = 2, 3, 4 on the chain; two_s = 2 on the square lattice) is
- * swept as looper sweeps it: nv = 2S virtual S=1/2 sites per real site
+ * S > 1/2 (two_s = 2 .. LG_MAX_TWO_S, both lattices) is swept as looper
+ * sweeps it: nv = 2S virtual S=1/2 sites per real site
  * (virtual site r*nv + a), nv^2 virtual bonds per real bond (real bond rb =
  * (rs, rt) carries rb*nv^2 + a*nv + c = (rs*nv + a, rt*nv + c)), and a random
  * spin-conserving permutation joining the tau = 1 and tau = 0 ends of each
@@ -70,9 +73,10 @@ typedef struct lg_params {
   uint32_t seed;         /* SEED */
   uint32_t chain_id;     /* independent stream index */
   double h;              /* longitudinal field h (site term -h S^z, looper's FIELD path) */
-  int32_t two_s;         /* 2S: 1 (S=1/2; 0 is taken as 1), 2, 3 or 4 (chain), 2 (square) */
+  int32_t two_s;         /* 2S: 1 (S=1/2; 0 is taken as 1) .. LG_MAX_TWO_S */
   double hx;             /* transverse field hx (site term -hx S^x, looper/model_parameter.h:67);
-                            sign-free only with jxy <= 0, so jxy > 0 with hx != 0 is rejected */
+                            sign-free only with jxy <= 0: with jxy > 0 every sweep records the
+                            configuration's sign (lg_sweep_record.sign) */
   double d;              /* single-ion term D (S^z)^2 (model_parameter.h:69); for S > 1/2 it adds
                             in-site bonds; looper's worker scatters (FORCE_SCATTER 0.1) when D > 0 */
   int32_t algorithm;     /* LG_ALGORITHM_PATH_INTEGRAL ("loop; path integral") or LG_ALGORITHM_SSE

@@ -20,7 +20,7 @@
  * TEMPERATURE_START_i / TEMPERATURE_DURATION_i, BETA_START_i /
  * BETA_DURATION_i, INVERSE_TEMPERATURE_START_i / INVERSE_TEMPERATURE_DURATION_i;
  * temperature.h:56-113, no longer than THERMALIZATION, path_integral.C:
- * 212-213), J, Jxy, Jz, MODEL (= spin), local_S (1/2, 1, 3/2, 2 on the chain; 1/2, 1 on the square lattice), SWEEPS
+ * 212-213), J, Jxy, Jz, MODEL (= spin), local_S (1/2, 1, 3/2, ... 4), SWEEPS
  * (default "[65536:]"), THERMALIZATION (default SWEEPS.min/8), SEED,
  * FORCE_SCATTER (default 0.1 where D > 0 and S > 1/2, as looper), CHAIN_ID, h,
  * Gamma (transverse field, sign-free with Jxy <= 0), D (single-ion term),

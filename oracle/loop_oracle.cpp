@@ -1541,12 +1541,15 @@ struct PhiloxChain {
     // spin-conserving permutation: for u = 0, 1 in turn, the bottoms of spin
     // u in virtual-site order are matched to the tops of spin u in
     // virtual-site order permuted by Fisher-Yates (i = n_u - 1 .. 1, j = i -
-    // (x mod (i + 1)), x the next word of Philox(sweep, STAGE_BOUNDARY, real
-    // site, 0), swap positions i and j)
+    // (x mod (i + 1)), x the next word of the real site's stream: word k is
+    // word k mod 4 of Philox(sweep, STAGE_BOUNDARY, real site, k / 4); swap
+    // positions i and j)
     if (split()) {
       const int nv = lat.nv;
       for (int r = 0; r < N / nv; ++r) {
-        const auto x = philox4x32_10(sweep, STAGE_BOUNDARY, (uint32_t)r, 0, key0, key1);
+        auto word = [&](int k) {
+          return philox4x32_10(sweep, STAGE_BOUNDARY, (uint32_t)r, (uint32_t)(k / 4), key0, key1).v[k % 4];
+        };
         int k = 0;
         for (int u = 0; u < 2; ++u) {
           std::vector<int> bots, tops;
@@ -1557,7 +1560,7 @@ struct PhiloxChain {
           }
           if (bots.size() != tops.size()) throw std::logic_error("virtual site spins not conserved");
           for (int i = (int)tops.size() - 1; i >= 1; --i) {
-            const int j = i - (int)(x.v[k++] % (uint32_t)(i + 1));
+            const int j = i - (int)(word(k++) % (uint32_t)(i + 1));
             std::swap(tops[i], tops[j]);
           }
           for (size_t i = 0; i < bots.size(); ++i) unite(bots[i], tops[i]);
@@ -1686,7 +1689,7 @@ void or_thresholds(double jxy, double jz, double force_scatter, uint64_t* thr4, 
 }
 
 int or_lattice_info(int kind, int L, int nv, int* N, int* Nb, int* dim) {
-  if ((kind != 0 && kind != 1) || L < 2 || nv < 1 || nv > (kind == 0 ? 4 : 2)) return -1;
+  if ((kind != 0 && kind != 1) || L < 2 || nv < 1 || nv > 8) return -1;
   Lattice lat; lat.init(kind, L, nv);
   *N = lat.N; *Nb = lat.Nb; *dim = lat.dim;
   return 0;

@@ -8,7 +8,7 @@ model_impl.h:51).
 
 CPU: the exact diagonalisation with D against the reference's own loop.op
 block (S=1 Ising chain, Jz=-1, D=-0.2, T=2) and against closed forms of one
-site; both oracle modes against it (S = 1, 3/2, 2; with a transverse field;
+site; both oracle modes against it (S = 1, 3/2, 2, 5/2; with a transverse field;
 D > 0 scattered) and against the reference's path-integral block.  GPU:
 bit-exact with the oracle's philox mode under the launches that take in-site
 bonds, batched beside plain chains, and the worker's D against loop.op and
@@ -60,7 +60,8 @@ MODELS = [(4, 2, 2.0, 0.0, -1.0, -0.2, 0.0, 0.0),   # loop.op's block
           (4, 2, 0.5, 1.0, 1.0, 0.3, 0.0, 0.1),     # D > 0, scattered as looper does
           (4, 3, 0.8, 1.0, 1.0, -0.4, 0.0, 0.0),
           (4, 2, 0.6, -1.0, 1.0, 0.5, 0.7, 0.1),    # with a transverse field
-          (2, 4, 1.0, 1.0, 0.5, -0.3, 0.0, 0.0)]
+          (2, 4, 1.0, 1.0, 0.5, -0.3, 0.0, 0.0),
+          (4, 5, 2.0, 1.0, 1.0, -0.4, 0.0, 0.0)]
 
 
 @pytest.mark.parametrize("L,two_s,T,jxy,jz,d,hx,fs", MODELS)
@@ -111,7 +112,8 @@ def _cmp(g, c, where):
 TRAJ = [(O.CHAIN, 4, 2, 0.5, 0.0, -1.0, -0.2, 0.0, 0.0), (O.CHAIN, 16, 2, 4.0, 1.0, 1.0, 0.4, 0.0, 0.1),
         (O.CHAIN, 8, 3, 3.0, 1.0, 0.8, -0.5, 0.0, 0.0), (O.CHAIN, 8, 4, 2.0, 1.0, 1.0, 0.3, 0.0, 0.1),
         (O.CHAIN, 8, 2, 3.0, -1.0, 1.0, 0.5, 0.6, 0.1), (O.SQUARE, 4, 2, 2.0, 1.0, 1.0, -0.6, 0.0, 0.0),
-        (O.CHAIN, 8, 1, 2.0, 1.0, 1.0, 0.7, 0.0, 0.0)]
+        (O.CHAIN, 8, 1, 2.0, 1.0, 1.0, 0.7, 0.0, 0.0), (O.SQUARE, 2, 4, 1.5, 1.0, 1.0, 0.4, 0.0, 0.1),
+        (O.CHAIN, 4, 6, 1.0, 1.0, 1.0, -0.3, 0.0, 0.0)]
 LAUNCHES = [dict(), dict(group=4), dict(group=64, compress=1), dict(group=16, blocks=64)]
 
 

@@ -1,6 +1,6 @@
 """Spin S > 1/2 through virtual sites (path_integral.C:589-615,
 looper/permutation.h, the virtual lattice of looper/lattice.h:104-175):
-S = 1, 3/2 and 2 on the chain, S = 1 on the square lattice.
+S = 1, 3/2, ... 4 on the chain and the square lattice.
 
 CPU: the oracle's restatement of random_shuffle / partitioned_random_shuffle
 against the reference's own test/permutation.op (mt19937 seed 23094, digit
@@ -92,7 +92,7 @@ def _two_site_levels(S):
     return out
 
 
-@pytest.mark.parametrize("two_s", [2, 3, 4])
+@pytest.mark.parametrize("two_s", [2, 3, 4, 5, 6, 8])
 def test_spin_s_ed_closed_forms(two_s):
     S = 0.5 * two_s
     for T in (0.5, 1.0, 3.0):
@@ -118,15 +118,19 @@ def test_spin_s_ed_closed_forms(two_s):
             assert a[k] == pytest.approx(c[k], rel=1e-10, abs=1e-12)
 
 
-def test_spin1_square_ed_closed_form():
-    """2x2 square: every neighbour pair carries two bonds, H = 2 (S_A + S_B)^2/2 - ...:
-    E0 = -2 * 2S(2S+1) = -12 for S=1"""
-    assert ed.diagonalize_spin("square", 2, 0.01, 2)["Ground State Energy"] == pytest.approx(-12.0, abs=1e-9)
+@pytest.mark.parametrize("two_s", [2, 3, 4])
+def test_spin_s_square_ed_closed_form(two_s):
+    """2x2 square: every neighbour pair carries two bonds, H = 2 S_A.S_B with the
+    sublattice spins S_A = S_0 + S_3, S_B = S_1 + S_2, so E0 = -2 * 2S(2S+1)
+    (-12 for S=1, -24 for S=3/2, -40 for S=2)"""
+    assert ed.diagonalize_spin("square", 2, 0.01, two_s)["Ground State Energy"] == pytest.approx(
+        -2.0 * two_s * (two_s + 1), abs=1e-9)
 
 
 # kind, L, 2S, T, Jz
 CASES = [(O.CHAIN, 4, 2, 0.5, 1.0), (O.CHAIN, 6, 2, 0.3, 1.0), (O.CHAIN, 4, 2, 0.6, 0.6),
-         (O.CHAIN, 4, 3, 0.8, 1.0), (O.CHAIN, 4, 4, 1.2, 0.8), (O.SQUARE, 2, 2, 0.9, 1.0)]
+         (O.CHAIN, 4, 3, 0.8, 1.0), (O.CHAIN, 4, 4, 1.2, 0.8), (O.SQUARE, 2, 2, 0.9, 1.0),
+         (O.CHAIN, 4, 5, 2.0, 1.0), (O.CHAIN, 4, 8, 4.0, 1.0), (O.SQUARE, 2, 3, 1.5, 1.0), (O.SQUARE, 2, 4, 2.0, 0.9)]
 NAMES = ["Energy", "Susceptibility", "Staggered Magnetization^2", "Staggered Susceptibility", "Magnetization^2"]
 
 
@@ -150,12 +154,16 @@ def test_oracle_modes_match_spin_s_ed(kind, L, two_s, T, jz):
 
 def test_spin_s_rejected_beyond_the_built_range():
     import loopgpu
-    for kind, two_s in ((loopgpu.SQUARE, 3), (loopgpu.CHAIN, 5)):
+    for kind, two_s in ((loopgpu.SQUARE, 9), (loopgpu.CHAIN, 9), (loopgpu.CHAIN, -1)):
         with pytest.raises(loopgpu.LoopGPUError) as e:
             loopgpu.Chain(loopgpu.params(kind, 4, 1.0, two_s=two_s))
         assert e.value.code == -1
-    for par in ('LATTICE = "square lattice"; L = 4; T = 1; local_S = 3/2', 'L = 4; T = 1; local_S = 5/2',
-                'L = 4; T = 1; local_S = 0.7'):
+    # more than 2^27 virtual bonds: S=4 on a 1026^2 square lattice
+    with pytest.raises(loopgpu.LoopGPUError) as e:
+        loopgpu.Chain(loopgpu.params(loopgpu.SQUARE, 1026, 1.0, two_s=8))
+    assert e.value.code == -1 and "too large" in str(e.value)
+    for par in ('LATTICE = "square lattice"; L = 4; T = 1; local_S = 9/2', 'L = 4; T = 1; local_S = 5',
+                'L = 4; T = 1; local_S = 0.7', 'L = 4; T = 1; local_S = 0'):
         with pytest.raises(loopgpu.LoopGPUError) as e:
             loopgpu.Worker(par)
         assert e.value.code == -1
@@ -173,7 +181,9 @@ def _cmp(g, c, where):
 # kind, L, 2S, beta, Jz
 TRAJ = [(O.CHAIN, 4, 2, 2.0, 1.0), (O.CHAIN, 8, 2, 6.0, 0.7), (O.CHAIN, 32, 2, 12.0, 1.0), (O.CHAIN, 2, 2, 3.0, 1.0),
         (O.CHAIN, 4, 3, 2.0, 1.0), (O.CHAIN, 16, 3, 8.0, 0.8), (O.CHAIN, 8, 4, 4.0, 1.0),
-        (O.SQUARE, 2, 2, 1.5, 1.0), (O.SQUARE, 4, 2, 3.0, 1.0), (O.SQUARE, 8, 2, 5.0, 0.9)]
+        (O.SQUARE, 2, 2, 1.5, 1.0), (O.SQUARE, 4, 2, 3.0, 1.0), (O.SQUARE, 8, 2, 5.0, 0.9),
+        (O.CHAIN, 4, 5, 2.0, 1.0), (O.CHAIN, 8, 6, 3.0, 0.9), (O.CHAIN, 6, 7, 2.0, 1.0), (O.CHAIN, 4, 8, 2.0, 1.0),
+        (O.SQUARE, 2, 3, 1.5, 1.0), (O.SQUARE, 4, 4, 2.0, 1.0), (O.SQUARE, 4, 8, 0.7, 1.0)]
 
 
 def _launches(kind, two_s):
@@ -234,7 +244,8 @@ def test_spin_s_chains_in_a_mixed_batch(loopgpu):
 
 WORKER = [("chain lattice", 6, "1", 0.3, 1.0), ("chain lattice", 4, "1", 0.6, 0.6),
           ("chain lattice", 4, "3/2", 0.8, 1.0), ("chain lattice", 4, "2", 1.2, 1.0),
-          ("square lattice", 2, "1", 0.9, 1.0)]
+          ("square lattice", 2, "1", 0.9, 1.0), ("chain lattice", 4, "5/2", 2.0, 1.0), ("chain lattice", 4, "4", 4.0, 1.0),
+          ("square lattice", 2, "3/2", 1.5, 1.0), ("square lattice", 2, "2", 2.0, 0.9)]
 
 
 @pytest.mark.gpu

@@ -28,7 +28,8 @@ NAMES = ["Energy", "Susceptibility", "Magnetization^2", "Staggered Magnetization
 
 # kind, L, 2S, T, Jxy, Jz
 MODELS = [(O.CHAIN, 8, 1, 0.5, 1.0, 1.0), (O.CHAIN, 4, 1, 0.3, 0.6, 1.2), (O.CHAIN, 4, 1, 0.5, 0.0, 1.0),
-          (O.SQUARE, 2, 1, 0.7, 1.0, 1.0), (O.CHAIN, 4, 2, 0.5, 1.0, 1.0), (O.CHAIN, 4, 3, 0.8, 1.0, 1.0)]
+          (O.SQUARE, 2, 1, 0.7, 1.0, 1.0), (O.CHAIN, 4, 2, 0.5, 1.0, 1.0), (O.CHAIN, 4, 3, 0.8, 1.0, 1.0),
+          (O.SQUARE, 2, 3, 1.5, 1.0, 1.0), (O.CHAIN, 4, 6, 2.5, 1.0, 0.8)]
 
 
 def _ed(kind, L, two_s, T, jxy, jz):
@@ -157,7 +158,8 @@ def _cmp(g, c, where):
 # kind, L, 2S, beta, Jxy, Jz (, hx, D, FORCE_SCATTER)
 TRAJ = [(O.CHAIN, 4, 1, 2.0, 1.0, 1.0), (O.CHAIN, 64, 1, 10.0, 1.0, 1.0), (O.CHAIN, 16, 1, 4.0, 0.0, 1.0),
         (O.CHAIN, 32, 1, 6.0, -1.0, 0.5), (O.SQUARE, 4, 1, 3.0, 1.0, 1.0), (O.SQUARE, 16, 1, 4.0, 1.0, 1.0),
-        (O.CHAIN, 8, 2, 4.0, 1.0, 1.0), (O.CHAIN, 8, 3, 3.0, 1.0, 0.8), (O.SQUARE, 4, 2, 2.0, 1.0, 1.0)]
+        (O.CHAIN, 8, 2, 4.0, 1.0, 1.0), (O.CHAIN, 8, 3, 3.0, 1.0, 0.8), (O.SQUARE, 4, 2, 2.0, 1.0, 1.0),
+        (O.CHAIN, 8, 7, 1.5, 1.0, 1.0), (O.SQUARE, 4, 4, 1.0, 1.0, 1.0)]
 TRAJ_EXTRA = [(O.CHAIN, 16, 1, 4.0, 0.0, 1.0, 0.9, 0.0, 0.0), (O.SQUARE, 4, 1, 3.0, -1.0, 1.0, 0.6, 0.0, 0.0),
               (O.CHAIN, 8, 2, 3.0, 1.0, 1.0, 0.0, -0.4, 0.0), (O.CHAIN, 8, 3, 2.0, -1.0, 0.8, 0.7, 0.5, 0.1),
               (O.CHAIN, 2, 2, 4.0, 0.0, 0.0, 1.0, 0.0, 0.0)]

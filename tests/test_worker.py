@@ -18,8 +18,8 @@ FX = json.load(open(os.path.join(HERE, "golden", "reference_fixtures.json")))
     ('LATTICE="chain lattice"; T=0.1', "L not defined"),
     ('LATTICE="triangular lattice"; L=4; T=1', "unsupported LATTICE"),
     ('L=4; T=1; Jxy=-1; Jz=1; Gamma=0.3; MEASURE[Correlations]=true', "transverse field"),
-    ("L=4; local_S=5/2; T=1", "local_S"),
-    ('LATTICE="square lattice"; L=4; local_S=3/2; T=1', "local_S"),
+    ("L=4; local_S=9/2; T=1", "local_S"),
+    ('LATTICE="square lattice"; L=4; local_S=0; T=1', "local_S"),
     ("L=4; T=-1", "negative temperature"),
     ("L=4", "temperature is not initialized"),
     ("L=4; T=1; SWEEPS=[100:x]", "SWEEPS"),
