@@ -264,7 +264,8 @@ class loop_worker {
     // virtual sites
     lp_.d = p.num("D", 0.0);
     // transverse field -Gamma S^x (model_parameter.h:67, the ALPS spin
-    // model's Gamma): site operators; sign-free with Jxy <= 0
+    // model's Gamma): site operators; sign-free with Jxy <= 0, signed (the
+    // sign measured, normal estimators) with Jxy > 0
     lp_.hx = p.num("Gamma", 0.0);
     lp_.h = p.num("h", 0.0);
     // improved estimators unless a field or DISABLE_IMPROVED_ESTIMATOR (path_integral.C:217)

@@ -23,7 +23,8 @@
  * 212-213), J, Jxy, Jz, MODEL (= spin), local_S (1/2, 1, 3/2, ... 4), SWEEPS
  * (default "[65536:]"), THERMALIZATION (default SWEEPS.min/8), SEED,
  * FORCE_SCATTER (default 0.1 where D > 0 and S > 1/2, as looper), CHAIN_ID, h,
- * Gamma (transverse field, sign-free with Jxy <= 0), D (single-ion term),
+ * Gamma (transverse field; sign-free with Jxy <= 0, with Jxy > 0 the worker
+ * measures "Sign" and reports sign-weighted normal estimators), D (single-ion term),
  * DISABLE_IMPROVED_ESTIMATOR,
  * MEASURE[Correlations], MEASURE[Structure Factor], INITIAL_SITE
  * (correlation.h:59-109).  Invalid or unsupported parameters give
