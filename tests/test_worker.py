@@ -89,3 +89,35 @@ def test_worker_checkpoint_at_any_step_resumes_exactly(loopgpu):
     with pytest.raises(loopgpu.LoopGPUError) as e:
         b.load(blob[:100])
     assert e.value.code == -1
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("par", [
+    # sign problem: the sign-weighted normal estimators and "Sign"
+    'L = 4\nJ = 1\nGamma = 0.8\nT = 0.5',
+    # the SSE worker with S=1, a single-ion term and a (sign-free) transverse field
+    'ALGORITHM = "loop; sse"\nL = 4\nlocal_S = 1\nJxy = -1\nJz = 1\nD = 0.3\nGamma = 0.4\nT = 0.5'])
+def test_worker_checkpoint_resumes_signed_and_sse_runs(loopgpu, par):
+    """the same resume as above for the signed and SSE workers, whose results
+    come from other accumulators"""
+    p = par + "\nSWEEPS = 2000\nTHERMALIZATION = 200\nSEED = 3"
+    c = loopgpu.Worker(p)
+    c.set_chunk(100)
+    c.run()
+    print("worker checkpoint", par.replace("\n", "; "), flush=True)
+    a = loopgpu.Worker(p)
+    a.set_chunk(100)
+    assert a.run(1237) == 1237
+    b = loopgpu.Worker(p)
+    b.load(a.save())
+    b.set_chunk(64)
+    b.run()
+    assert b.status()["mcs"] == c.status()["mcs"] == 2200
+    names = [l.split(":")[0] for l in c.report().splitlines()]
+    assert [l.split(":")[0] for l in b.report().splitlines()] == names
+    for name in ("Energy", "Sign", "Number of Clusters") if "Gamma = 0.8" in par else ("Energy", "Number of Clusters"):
+        assert b.result(name) == c.result(name), name
+    for name in ("Magnetization^2", "Susceptibility", "Staggered Magnetization^2", "Staggered Susceptibility",
+                 "Specific Heat"):
+        for x, y in zip(b.result(name), c.result(name)):
+            assert x == pytest.approx(y, rel=1e-9, abs=1e-12), name
