@@ -2047,12 +2047,20 @@ __global__ void __launch_bounds__(SSE_THREADS) k_sse_diag(DevBatch B) {
   int nop = n_old, out = 0, i = 0, step = 0;
   int xbase = -SSE_THREADS, obase = -SSE_THREADS;  // first step / old operator held by the lanes
   bool try_gap = true, full = false;
+  // Every lane runs the walk on the same values; reading each value loaded
+  // from LDS through readfirstlane tells the compiler so, which keeps the
+  // walk's state in scalar registers and its branches scalar.
+  auto uni = [](int v) { return __builtin_amdgcn_readfirstlane(v); };
   while (try_gap || i < n_old) {
     if (step - xbase >= SSE_THREADS) {
       xbase = step;
       xbuf[lane] = philox(sweep, STAGE_SSE, 0u, (uint32_t)(step + lane), P.key0, P.key1);
     }
-    const U4 x = xbuf[step - xbase];  // (one wave: its LDS accesses complete in order)
+    U4 x = xbuf[step - xbase];  // (one wave: its LDS accesses complete in order)
+    x.x = (uint32_t)uni((int)x.x);
+    x.y = (uint32_t)uni((int)x.y);
+    x.z = (uint32_t)uni((int)x.z);
+    x.w = (uint32_t)uni((int)x.w);
     ++step;
     const double u = uniform53(x);
     int ne, b;
@@ -2060,19 +2068,24 @@ __global__ void __launch_bounds__(SSE_THREADS) k_sse_diag(DevBatch B) {
       try_gap = false;
       if (!((double)(nop + 1) * u < bw)) continue;
       // the location's class and graph by word 2, the location by word 3
+      // (the thresholds ascend, so g counts those at or below z; no array
+      // of the descriptor is indexed at run time, which would put the
+      // whole copy in scratch memory)
       const uint64_t z = x.z;
       int g = 0;
       if (z < P.sse_cls[0]) {
         b = (int)(((uint64_t)x.w * (uint64_t)P.nb_lat) >> 32);
-        while (g < 3 && z >= P.sse_tl[g]) ++g;
+        g = (int)(z >= P.sse_tl[0]) + (int)(z >= P.sse_tl[1]) + (int)(z >= P.sse_tl[2]);
       } else if (z < P.sse_cls[1]) {
         b = P.nb_lat + (int)(((uint64_t)x.w * (uint64_t)(P.nb_real - P.nb_lat)) >> 32);
-        while (g < 3 && z >= P.sse_td[g]) ++g;
+        g = (int)(z >= P.sse_td[0]) + (int)(z >= P.sse_td[1]) + (int)(z >= P.sse_td[2]);
       } else {  // a site operator: the site graph, compatible with either spin
         b = P.nb_real + (int)(((uint64_t)x.w * (uint64_t)P.N) >> 32);
       }
-      const uint32_t e2 = ends[b];
-      if (b < P.nb_real && ((g & 1) ^ spc[e2 & 0xffff] ^ spc[e2 >> 16]) == 0) continue;
+      if (b < P.nb_real) {
+        const uint32_t e2 = (uint32_t)uni((int)ends[b]);
+        if (((g & 1) ^ uni(spc[e2 & 0xffff]) ^ uni(spc[e2 >> 16])) == 0) continue;
+      }
       ne = (b << 3) | (g << 1);
       ++nop;
     } else {
@@ -2080,12 +2093,12 @@ __global__ void __launch_bounds__(SSE_THREADS) k_sse_diag(DevBatch B) {
         obase = i;
         ochunk[lane] = (i + lane < n_old) ? so[i + lane] : 0;
       }
-      const int e = ochunk[i - obase];
+      const int e = uni(ochunk[i - obase]);
       ++i;
       b = e >> 3;
-      const uint32_t e2 = ends[b];
+      const uint32_t e2 = (uint32_t)uni((int)ends[b]);
       const int s0 = (int)(e2 & 0xffff), s1 = (int)(e2 >> 16);
-      const int c0 = spc[s0], c1 = spc[s1];
+      const int c0 = uni(spc[s0]), c1 = uni(spc[s1]);
       const bool site = b >= P.nb_real, insite = b >= P.nb_lat && !site;
       if ((e & 1) == 0) {
         if (bw * u < (double)nop) {  // removed
@@ -2093,7 +2106,7 @@ __global__ void __launch_bounds__(SSE_THREADS) k_sse_diag(DevBatch B) {
           continue;
         }
         const int cc = 1 ^ c0 ^ c1;  // 0 antiparallel, 1 parallel
-        const uint64_t pd = insite ? P.dpdiag[cc] : P.pdiag[cc];
+        const uint64_t pd = insite ? (cc ? P.dpdiag[1] : P.dpdiag[0]) : (cc ? P.pdiag[1] : P.pdiag[0]);
         ne = (b << 3) | (site ? 0 : (((((uint64_t)x.z < pd) ? 0 : 2) ^ cc) << 1));
       } else {
         const uint64_t to = insite ? P.dtoff : P.toff;
