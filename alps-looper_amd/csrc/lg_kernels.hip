@@ -2154,25 +2154,35 @@ __global__ void __launch_bounds__(SSE_THREADS) k_sse_scatter(DevBatch B) {
   int* perm = B.sse_perm + P.sse_base;
   const double dn = (double)n;
   for (int p0 = 0; p0 < n; p0 += SSE_THREADS) {
-    const int el = (p0 + lane < n) ? sn[p0 + lane] : 0;
+    // lanes past the end hold -1 (bond -1, which no element has)
+    const int el = (p0 + lane < n) ? sn[p0 + lane] : -1;
     const int m = min(SSE_THREADS, n - p0);
-    int pos = 0, kc = 0;
-    for (int j = 0; j < m; ++j) {  // in series order: each bond's next slot and off-diagonal count
+    // in series order, each element takes its bond's next slot and
+    // off-diagonal count: the elements of the same bond before it in these
+    // 64 add to the bond's counts so far, and the last of them carries the
+    // counts on to the next 64
+    int rank = 0, roff = 0;
+    bool last = true;
+    for (int j = 0; j < m; ++j) {
       const int e = __shfl(el, j);
-      const int b = e >> 3;
-      const int k = cur[b], q = koff[b];
-      if (lane == 0) {
-        cur[b] = k + 1;
-        koff[b] = q + (e & 1);
-      }
-      if (lane == j) {
-        pos = k;
-        kc = q;
+      if ((e >> 3) == (el >> 3)) {
+        if (j < lane) {
+          ++rank;
+          roff += e & 1;
+        } else if (j > lane) {
+          last = false;
+        }
       }
     }
     if (p0 + lane < n) {
+      const int b = el >> 3;
+      const int pos = cur[b] + rank, kc = koff[b] + roff;
+      if (last) {  // (one wave: these stores follow every lane's loads above)
+        cur[b] = pos + 1;
+        koff[b] = kc + (el & 1);
+      }
       B.ot_new[pos] = (double)(p0 + lane) / dn;
-      B.oi_new[pos] = ((P.bond_base + (el >> 3)) << 3) | (el & 7);
+      B.oi_new[pos] = ((P.bond_base + b) << 3) | (el & 7);
       B.kc_new[pos] = kc;
       perm[p0 + lane] = pos;
     }
