@@ -103,6 +103,7 @@ def _pool_size():
 
 PORT = ("the oracle's looper-mode restatement of path_integral.C (g++ -O3 -march=x86-64-v3; "
         "0.88x the reference's own standalone/loop.C on its published chain L=128, T=0.01 shape)")
+PORT_SSE = "the oracle's looper-mode restatement of sse.C (g++ -O3 -march=x86-64-v3)"
 
 
 def cpu_baseline(kind, L, beta, seconds):
@@ -144,12 +145,13 @@ def cpu_baseline(kind, L, beta, seconds):
     }
 
 
-def cpu_pool_baseline(chains, seconds, what, untimed=8, max_sweeps=64):
+def cpu_pool_baseline(chains, seconds, what, untimed=8, max_sweeps=64, sse=False):
     """Oracle looper-mode chains on a thread pool of the host cores (the
     oracle releases the GIL inside each sweep): each worker takes the next
     chain of the list, runs `untimed` sweeps from the empty operator string,
     then timed sweeps until `max_sweeps` or the shared deadline.  The rate is
-    the sum over workers of their timed operator updates / timed seconds."""
+    the sum over workers of their timed operator updates / timed seconds.
+    sse: the oracle's restatement of the SSE worker (sse.C) instead."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
     cores = _pool_size()
@@ -165,10 +167,11 @@ def cpu_pool_baseline(chains, seconds, what, untimed=8, max_sweeps=64):
                 state["next"] += 1
             kind, L, beta, jz, seed = chains[i % len(chains)]
             ch = O.LooperChain(kind, L, beta, 1.0, jz, seed=seed)
+            step = ch.sweep_sse if sse else ch.sweep
             n = 0
             while time.perf_counter() < t_end and n < untimed + max_sweeps:
                 t0 = time.perf_counter()
-                r = float(ch.sweep(1)[0, 0])
+                r = float(step(1)[0, 0])
                 dt = time.perf_counter() - t0
                 if n >= untimed:
                     ops += r
@@ -193,7 +196,7 @@ def cpu_pool_baseline(chains, seconds, what, untimed=8, max_sweeps=64):
         "value": state["rate"], "unit": "spin-operator updates/s", "cores": cores, "kind": "port",
         "sample": ("%s on a %d-thread pool for %.1f s: %s; %d chains, each %d untimed sweeps from the empty "
                    "operator string then up to %d timed sweeps, %d timed sweeps in total; CPU: %s"
-                   % (PORT, cores, wall, what, state["chains"], untimed, max_sweeps, state["sweeps"], cpu_model())),
+                   % (PORT_SSE if sse else PORT, cores, wall, what, state["chains"], untimed, max_sweeps, state["sweeps"], cpu_model())),
     }
 
 
@@ -451,6 +454,13 @@ def run_sse_scan(args, lg):
                 "energy_density_coldest_hottest": [float(E[0]), float(E[-1])], "launch": b.launch(),
                 "kernel_ms": prof, "roofline": None, "cpu_baseline": None})
     b.close()
+    if not args.no_cpu:
+        log("sse scan: cpu baseline %.0f s" % args.cpu_seconds)
+        chains = [(1, 32, 1.0 / T, 1.0, SEED) for T in scan_temps()[::8]]
+        out["cpu_baseline"] = cpu_pool_baseline(chains, args.cpu_seconds,
+                                                "every 8th chain of the scan, in order of temperature, SSE worker",
+                                                sse=True)
+        out["gpu_over_cpu"] = out["value"] / out["cpu_baseline"]["value"]
     return out
 
 
